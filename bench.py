@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""Benchmark of the imaging major cycle's grid+degrid hot path on one MI355X.
+
+A step is one pass of the major cycle's visibility-domain work over the whole
+synthetic data set, inputs resident in HBM:
+    degrid the model grid (kernel-weighted, w-projection) and subtract from the
+    measured visibilities (std_degridding + subtract_from_measurements_finegrid)
+    -> grid the residual visibilities (std_gridding + merge).
+Workload (default): 256 antennas log-radial to 5 km, 512 time steps
+(16,711,680 visibilities), 4096^2 grid, 32 w-planes with half support 4-16,
+oversampling 8.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "simsdp-generic-imaging-pipeline_amd"))
+
+METRIC = "Mvis/s grid+degrid; major-cycle s to fixed residual RMS, learned vs Hogbom"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(d, seconds):
+    """Oracle std_gridding + std_degridding (the reference's CPU actors,
+    restated) on a random sample of the same visibilities, one core."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import sdp_hip
+    import sdp_oracle
+
+    cfg = sdp_hip.make_config(d.grid_size, d.cell_size, oversampling=d.oversampling,
+                              num_kernels=d.supports.shape[0], max_w=d.max_w)
+    rng = np.random.default_rng(5)
+    grid = np.zeros((d.grid_size ** 2, 2), np.float32)
+
+    def run(n):
+        idx = np.sort(rng.choice(d.n_vis, n, replace=False))
+        uvw, vis = d.uvw[idx], d.vis[idx]
+        t0 = time.perf_counter()
+        g = sdp_oracle.std_gridding(d.grid_size, d.oversampling, d.kernels, d.supports, uvw, vis, cfg)
+        t1 = time.perf_counter()
+        sdp_oracle.std_degridding(d.grid_size, d.oversampling, d.degrid_kernels, d.supports, g, uvw, cfg)
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1
+
+    tg, td = run(2000)
+    per_vis = (tg + td) / 2000
+    n = int(max(2000, min(d.n_vis, seconds / max(per_vis, 1e-9))))
+    tg, td = run(n)
+    del grid
+    return {"value": round(n / (tg + td) / 1e6, 4), "unit": "Mvis/s", "cores": 1, "kind": "port",
+            "sample": f"{n} of {d.n_vis} visibilities (uniform random), {d.grid_size}^2 grid: oracle std_gridding "
+                      f"{tg:.2f} s + std_degridding {td:.2f} s, -O3 C, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="config2", choices=["config1", "config2"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import sdp_hip as hip
+    import sdp_synth
+
+    t = time.perf_counter()
+    d = sdp_synth.config2() if args.config == "config2" else sdp_synth.config1(with_vis=False)
+    log(f"dataset {d.name}: {d.n_vis} vis, G={d.grid_size}, planes={d.supports.shape[0]} "
+        f"h={d.supports[:, 0].min()}..{d.supports[:, 0].max()}, OS={d.oversampling} ({time.perf_counter() - t:.1f} s)")
+    G, N = d.grid_size, d.n_vis
+    hip.check(hip.load().sdp_init())
+
+    d_uvw = hip.DeviceBuffer.from_array(d.uvw)
+    d_gk = hip.DeviceBuffer.from_array(d.kernels)
+    d_dk = hip.DeviceBuffer.from_array(d.degrid_kernels)
+    d_src = hip.DeviceBuffer.from_array(d.sources)
+    d_meas = hip.DeviceBuffer(8 * N)
+    ev = [hip.Event() for _ in range(6)]
+
+    # measured visibilities: FP64-phase DFT predict of the sky (also timed)
+    ev[0].record()
+    hip.dft_predict(N, d.sources.shape[0], d_src, d_uvw, d_meas, hip.DFT_FP64_PHASE)
+    ev[1].record()
+    ev[1].sync()
+    dft_ms = ev[0].elapsed_ms(ev[1])
+    pairs = N * d.sources.shape[0]
+    log(f"DFT predict {d.sources.shape[0]} src x {N} vis: {dft_ms:.1f} ms ({pairs / dft_ms / 1e6:.1f} G pairs/s)")
+    d.vis = d_meas.download((N, 2), np.float32)
+
+    t = time.perf_counter()
+    plan = hip.GridPlan(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, d_uvw, N)
+    hip.synchronize()
+    plan_s = time.perf_counter() - t
+    info = plan.info()
+    log(f"plan {plan_s * 1e3:.0f} ms: {info}")
+
+    # model grid = FFT of a model image holding the sky's point sources
+    img = np.zeros((G, G), np.float32)
+    px = np.clip(np.round(d.sources[:, 0] / d.cell_size).astype(int) + G // 2, 0, G - 1)
+    py = np.clip(np.round(d.sources[:, 1] / d.cell_size).astype(int) + G // 2, 0, G - 1)
+    np.add.at(img, (py, px), d.sources[:, 2])
+    d_img = hip.DeviceBuffer.from_array(img)
+    d_model = hip.DeviceBuffer(8 * G * G)
+    d_work = hip.DeviceBuffer(hip.fft_work_bytes(G))
+    hip.fft2d(d_img, d_model, G, hip.FFT_FORWARD, hip.FFT_REAL_IN | hip.FFT_SHIFT_IN | hip.FFT_SHIFT_OUT, None,
+              d_work)
+
+    d_meas_p = hip.DeviceBuffer(8 * N)
+    plan.to_plan_order(d_meas, d_meas_p)
+    d_delta = hip.DeviceBuffer(8 * N)
+    d_grid = hip.DeviceBuffer(8 * G * G)
+
+    def step(timed):
+        if timed:
+            ev[0].record()
+        plan.degrid(d_dk, d_model, d_delta, d_measured=d_meas_p, order=hip.ORDER_PLAN)
+        if timed:
+            ev[1].record()
+        plan.accumulate(d_gk, d_delta, order=hip.ORDER_PLAN)
+        if timed:
+            ev[2].record()
+        plan.merge(d_grid)
+        if timed:
+            ev[3].record()
+
+    for _ in range(args.warmup):
+        step(False)
+    hip.synchronize()
+    acc = {"degrid": 0.0, "grid_accumulate": 0.0, "grid_merge": 0.0}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+        ev[3].sync()
+        acc["degrid"] += ev[0].elapsed_ms(ev[1])
+        acc["grid_accumulate"] += ev[1].elapsed_ms(ev[2])
+        acc["grid_merge"] += ev[2].elapsed_ms(ev[3])
+    hip.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    k_ms = {k: v / args.steps for k, v in acc.items()}
+    ms_per_step = wall_ms
+    log(f"step {ms_per_step:.3f} ms (events: {k_ms})")
+
+    # algorithmic bytes per launch
+    alg = {
+        "degrid": 8 * G * G + N * (12 + 8 + 8),     # grid read, uvw read, measured read, residual written
+        "grid_accumulate": N * (12 + 8),           # uvw + residual visibility read
+        "grid_merge": 8 * G * G,                   # grid written once
+    }
+    dom = max(k_ms, key=k_ms.get)
+    achieved = alg[dom] / (k_ms[dom] * 1e-3) / 1e9
+    b_gd = N * 40 + 16 * G * G
+    gd_ms = sum(k_ms.values())
+
+    extras = {}
+    if not args.no_extras:
+        # FFT: inverse with the shift / real / taper epilogue, as in the major cycle
+        d_tap = hip.DeviceBuffer.from_array(sdp_synth.taper_padded(G))
+        d_dirty = hip.DeviceBuffer(4 * G * G)
+        flags = hip.FFT_SHIFT_IN | hip.FFT_SHIFT_OUT | hip.FFT_REAL_OUT | hip.FFT_CORRECT_OUT
+        hip.fft2d(d_grid, d_dirty, G, hip.FFT_INVERSE, flags, d_tap, d_work)
+        ev[0].record()
+        for _ in range(5):
+            hip.fft2d(d_grid, d_dirty, G, hip.FFT_INVERSE, flags, d_tap, d_work)
+        ev[1].record()
+        ev[1].sync()
+        fft_ms = ev[0].elapsed_ms(ev[1]) / 5
+        extras["fft_ms"] = round(fft_ms, 4)
+        extras["fft_gflops"] = round(5 * G * G * np.log2(G * G) / (fft_ms * 1e-3) / 1e9, 1)
+        extras["dft_predict_gpairs_per_s"] = round(pairs / dft_ms / 1e6, 2)
+        extras["plan_ms"] = round(plan_s * 1e3, 1)
+        extras["grid_degrid_hbm_frac"] = round(b_gd / (gd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
+        extras["kernel_ms"] = {k: round(v, 4) for k, v in k_ms.items()}
+        extras["plan"] = info
+
+    out = {
+        "metric": METRIC,
+        "value": round(N / (ms_per_step * 1e-3) / 1e6, 2),
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{d.name}: degrid(w-proj, weighted)+subtract+grid of {N} vis on a {G}^2 grid, "
+                               f"{d.supports.shape[0]} w-planes h {d.supports[:, 0].min()}-{d.supports[:, 0].max()}, "
+                               f"OS {d.oversampling}",
+                   "n_vis": N, "grid": G, "w_planes": int(d.supports.shape[0]), "oversampling": d.oversampling},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "alg_bytes": alg[dom], "avg_ms": round(k_ms[dom], 4)},
+    }
+    out.update(extras)
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(d, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,192 @@
+/*
+ * sdp_device.h -- device-resident C-ABI of the MI355X imaging library.
+ *
+ * The actor ABI (sdp_actors.h) takes host pointers and copies on every call,
+ * like the reference's GPU actors (src/GPU/dft_run.cu:25-73).  A caller that
+ * keeps data in HBM across a major cycle uses these entry points instead:
+ * plain device pointers, sizes, and an optional HIP stream (NULL = the
+ * library's default stream).  No function here allocates per call except
+ * the plan constructors, which allocate once.
+ *
+ * Complex data are interleaved float pairs (re, im), row-major G x G for
+ * grids and images, index (v + G/2) * G + (u + G/2) for grid cell (u, v).
+ * Coordinates are float triples (u, v, w) in wavelengths.
+ *
+ * Every function returns 0 on success and a negative SDP_E* code on error;
+ * sdp_last_error() gives the message.  These have no counterpart in the
+ * reference (which only has host-pointer actors); each cites the actor whose
+ * arithmetic it implements.
+ */
+#ifndef SDP_DEVICE_H
+#define SDP_DEVICE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *sdp_stream;   /* hipStream_t */
+typedef void *sdp_event;    /* hipEvent_t  */
+
+enum {
+    SDP_OK = 0,
+    SDP_E_HIP = -1,        /* a HIP runtime call failed */
+    SDP_E_ARG = -2,        /* invalid argument (size, pointer, mode) */
+    SDP_E_NOMEM = -3,      /* device allocation failed */
+    SDP_E_UNSUPPORTED = -4 /* shape outside what the kernels support */
+};
+
+/* ---- runtime ------------------------------------------------------------ */
+int sdp_init(void);
+const char *sdp_last_error(void);
+int sdp_device_malloc(void **ptr, size_t bytes);
+int sdp_device_free(void *ptr);
+int sdp_memcpy_htod(void *dst, const void *src, size_t bytes, sdp_stream s);
+int sdp_memcpy_dtoh(void *dst, const void *src, size_t bytes, sdp_stream s);
+int sdp_memcpy_dtod(void *dst, const void *src, size_t bytes, sdp_stream s);
+int sdp_memset_zero(void *dst, size_t bytes, sdp_stream s);   /* fill kernel, graph safe */
+int sdp_stream_create(sdp_stream *s);
+int sdp_stream_destroy(sdp_stream s);
+int sdp_stream_sync(sdp_stream s);
+int sdp_device_sync(void);
+int sdp_event_create(sdp_event *e);
+int sdp_event_destroy(sdp_event e);
+int sdp_event_record(sdp_event e, sdp_stream s);
+int sdp_event_sync(sdp_event e);
+int sdp_event_elapsed_ms(sdp_event start, sdp_event stop, float *ms);
+
+/* ---- actor-level switches ---------------------------------------------- */
+enum { SDP_DEGRID_NEAREST = 0, SDP_DEGRID_WEIGHTED = 1 };
+/* std_degridding output: nearest grid cell (reference default,
+ * degridgrid.c:194-200) or kernel-weighted sum (pre-override loop
+ * degridgrid.c:147-186 with zero padding, no comm_norm division). */
+void sdp_set_degrid_mode(int mode);
+int sdp_get_degrid_mode(void);
+
+enum { SDP_DFT_REFERENCE_FP32 = 0, SDP_DFT_FP64_PHASE = 1, SDP_DFT_MFMA = 2 };
+/* dft_actor arithmetic: the reference's FP32 phase (dft_run.c:21-32), the
+ * same formula with the phase formed and range-reduced in FP64, or the
+ * blocked MFMA contraction (see sdp_dft_predict). */
+void sdp_set_dft_mode(int mode);
+int sdp_get_dft_mode(void);
+
+/* ---- fine-grid snapping (degridgrid.c:12-65, PERFORM_SIMPLIFICATION = 0) */
+/* uvw_out[i] = uvw snapped to the oversampled fine grid; w copied. */
+int sdp_finegrid_correct(int n_vis, int grid_size, int oversampling, double uv_scale,
+                         const float *d_uvw_in, float *d_uvw_out, sdp_stream s);
+/* PERFORM_SIMPLIFICATION = 1: visibilities that fall in the same fine-grid
+ * cell and w-plane are summed into the first occurrence and compacted in
+ * order of first occurrence.  *n_out receives the number kept (host int). */
+int sdp_finegrid_simplify(int n_vis, int grid_size, int oversampling, double uv_scale, double w_scale,
+                          const float *d_uvw_in, const float *d_vis_in,
+                          float *d_uvw_out, float *d_vis_out, int *n_out, sdp_stream s);
+
+/* ---- gridding / degridding plans --------------------------------------- */
+enum { SDP_SNAP_STD = 0,        /* std_gridding: half-open [ceil(p-h), ceil(p+h)) */
+       SDP_SNAP_GRIDDING_ACTOR  /* gridding_actor: snapped, inclusive [ceil(s-h), floor(s+h)],
+                                   plane = round(sqrt(|w| w_scale)) */ };
+enum { SDP_ORDER_ORIGINAL = 0, SDP_ORDER_PLAN = 1 };
+
+typedef struct sdp_grid_params {
+    int grid_size;        /* G, even, <= 32768 */
+    int oversampling;     /* OS */
+    int n_kernels;        /* number of w-planes in the support table */
+    int snap_mode;        /* SDP_SNAP_* */
+    double uv_scale;      /* cell_size * G */
+    double w_scale;       /* (n_kernels - 1)^2 / max_w */
+    int tile;             /* uv tile edge in cells, 0 = autotuned */
+    int chunk;            /* max visibilities per work item, 0 = autotuned */
+} sdp_grid_params;
+
+typedef struct sdp_grid_plan sdp_grid_plan;
+
+/* Builds the per-visibility records (exact reference index arithmetic) and
+ * the tile-sorted order for n_vis coordinates.  supports is a HOST array of
+ * n_kernels (half_support, sample_offset) int pairs (kernel_host_set_up
+ * layout, top.c:252-257). */
+int sdp_grid_plan_create(sdp_grid_plan **plan, const sdp_grid_params *p, const int *supports,
+                         const float *d_uvw, int n_vis, sdp_stream s);
+int sdp_grid_plan_destroy(sdp_grid_plan *plan);
+/* plan statistics: [0] visibilities kept, [1] dropped (outside grid or
+ * beyond the support table), [2] tiles used, [3] work items, [4] tile edge,
+ * [5] chunk size, [6] max half support, [7] scratch bytes */
+int sdp_grid_plan_info(const sdp_grid_plan *plan, int64_t *info8);
+/* reorder n_vis complex values between caller order and plan order */
+int sdp_grid_plan_to_plan_order(const sdp_grid_plan *plan, const float *d_in, float *d_out, sdp_stream s);
+int sdp_grid_plan_to_original_order(const sdp_grid_plan *plan, const float *d_in, float *d_out, sdp_stream s);
+
+/* grid[G*G] = sum_vis vis * kernel (std_gridding, degridgrid.c:83-124);
+ * the whole grid is written.  vis is in `order`. */
+int sdp_grid(const sdp_grid_plan *plan, const float *d_kernels, const float *d_vis, int order,
+             float *d_grid, sdp_stream s);
+/* the two phases of sdp_grid: per-work-item LDS accumulation into the
+ * plan's slabs, then the fixed-order merge of the slabs into the grid */
+int sdp_grid_accumulate(const sdp_grid_plan *plan, const float *d_kernels, const float *d_vis, int order,
+                        sdp_stream s);
+int sdp_grid_merge(const sdp_grid_plan *plan, float *d_grid, sdp_stream s);
+/* kernel-weighted degridding (adjoint of sdp_grid when d_kernels holds the
+ * conjugate table): vis_out = sum_taps grid * kernel.  If d_vis_measured is
+ * not NULL the output is measured - predicted (subtract_from_measurements_finegrid
+ * fused, gains_apply_run.c:133-141).  Input and output in `order`. */
+int sdp_degrid(const sdp_grid_plan *plan, const float *d_kernels, const float *d_grid,
+               const float *d_vis_measured, int order, float *d_vis_out, sdp_stream s);
+
+/* nearest-cell degridding (std_degridding output, degridgrid.c:194-200);
+ * writes n_out (>= n_vis) values, zero beyond n_vis. */
+int sdp_degrid_nearest(int n_vis, int n_out, int grid_size, double uv_scale, const float *d_grid,
+                       const float *d_uvw, float *d_vis_out, sdp_stream s);
+
+/* ---- FFT (fft_run.c:5-111) ---------------------------------------------- */
+enum { SDP_FFT_FORWARD = -1, SDP_FFT_INVERSE = +1 };
+enum { SDP_FFT_SHIFT_IN = 1,      /* multiply input by (-1)^(r+c) */
+       SDP_FFT_SHIFT_OUT = 2,     /* multiply output by (-1)^(r+c) */
+       SDP_FFT_REAL_IN = 4,       /* input is a real G x G image */
+       SDP_FFT_REAL_OUT = 8,      /* output only the real part as a G x G image */
+       SDP_FFT_CORRECT_OUT = 16   /* divide the real output by the PSWF taper */ };
+/* bytes of device work memory sdp_fft2d needs for grid size G */
+size_t sdp_fft_work_bytes(int grid_size);
+/* unnormalised 2D C2C DFT with sign `direction`; in == out allowed.
+ * d_taper: G/2+1 floats (taper with a trailing 0) when SDP_FFT_CORRECT_OUT. */
+int sdp_fft2d(const void *d_in, void *d_out, int grid_size, int direction, int flags,
+              const float *d_taper, void *d_work, sdp_stream s);
+
+/* ---- element-wise actors ------------------------------------------------ */
+int sdp_shift_c2c(int grid_size, const float *d_in, float *d_out, sdp_stream s);
+int sdp_shift_c2r(int grid_size, const float *d_in, float *d_out, sdp_stream s);
+int sdp_shift_r2c(int grid_size, const float *d_in, float *d_out, sdp_stream s);
+/* d_taper: G/2+1 floats, element G/2 must be 0 (convolution_correction_run.c:23) */
+int sdp_convolution_correct(int grid_size, const float *d_in, const float *d_taper, float *d_out, sdp_stream s);
+int sdp_vis_subtract(int n, const float *d_a, const float *d_b, float *d_out, sdp_stream s);
+int sdp_vis_add(int n, const float *d_a, const float *d_b, float *d_out, sdp_stream s);
+int sdp_vis_subtract_gains(int n_vis, int n_baselines, const float *d_measured, const float *d_pred,
+                           const float *d_gains, const int *d_pairs, int apply_only,
+                           float *d_out, sdp_stream s);
+/* out[i] = sum_{j<slices} grids[j*G*G + i] in slice order */
+int sdp_merge_grids(int grid_size, int slices, const float *d_grids, float *d_out, sdp_stream s);
+
+/* ---- Hogbom CLEAN (deconvolution_run.c:197-312) ------------------------ */
+/* d_psf: G*G+1 floats (peak-normalised PSF followed by one 0).
+ * d_residual is updated in place (it ends as the residual after the last
+ * subtraction); d_model_out = d_model_in + components; d_sources holds
+ * max_sources (x, y, flux) triples in radians after the conversion
+ * (deconvolution_run.c:297-306); *d_num_sources is a device int.
+ * d_work: sdp_hogbom_work_bytes(G) bytes. */
+size_t sdp_hogbom_work_bytes(int grid_size);
+int sdp_hogbom(int grid_size, int n_minor, int max_sources, float *d_residual, const float *d_psf,
+               double psf_max, double loop_gain, double noise_factor, double weak_source_percent,
+               double cell_size, const float *d_model_in, float *d_model_out, float *d_sources,
+               int *d_num_sources, void *d_work, sdp_stream s);
+
+/* ---- DFT predict (dft_run.c:3-47) --------------------------------------- */
+/* vis[j] = sum_s flux_s / (1 - r^2/2) * exp(-2 pi i (u l + v m - w r^2/2)),
+ * sources (l, m, flux) in radians.  mode: SDP_DFT_*. */
+int sdp_dft_predict(int n_vis, int n_sources, const float *d_sources, const float *d_uvw,
+                    float *d_vis_out, int mode, sdp_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDP_DEVICE_H */

@@ -1,0 +1,225 @@
+// sdp_elementwise.hip -- HBM-bound element-wise actors: checkerboard
+// shifts, gridding correction, visibility arithmetic and the slice merge.
+//
+// All kernels use 16-byte accesses and grid-stride loops.  The library is
+// built with -ffp-contract=off, so every expression rounds exactly as the
+// reference's C does (bit-identical outputs).
+#include "sdp_internal.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+unsigned int grid_for(size_t n)
+{
+    size_t b = (n + kBlock - 1) / kBlock;
+    if (b > 8192) b = 8192;
+    return b ? (unsigned int)b : 1u;
+}
+
+// (-1)^(r+c) for an even grid: r*G + c has the parity of r + c only if G is
+// even, which every supported G is.
+__device__ __forceinline__ float checker(size_t row, size_t col) { return ((row + col) & 1) ? -1.0f : 1.0f; }
+
+// fft_shift_complex_to_complex_actor (fft_run.c:5-20): 2 cells per thread.
+__global__ void shift_c2c_kernel(const float4 *in, float4 *out, int G)
+{
+    const size_t n2 = (size_t)G * G / 2;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+        size_t cell = 2 * i, row = cell / G, col = cell % G;
+        float a = checker(row, col);
+        float4 v = in[i];
+        v.x *= a; v.y *= a; v.z *= -a; v.w *= -a;
+        out[i] = v;
+    }
+}
+
+// fft_shift_complex_to_real_actor (fft_run.c:76-92)
+__global__ void shift_c2r_kernel(const float4 *in, float2 *out, int G)
+{
+    const size_t n2 = (size_t)G * G / 2;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+        size_t cell = 2 * i, row = cell / G, col = cell % G;
+        float a = checker(row, col);
+        float4 v = in[i];
+        out[i] = make_float2(v.x * a, v.z * -a);
+    }
+}
+
+// fft_shift_real_to_complex_actor (fft_run.c:94-111)
+__global__ void shift_r2c_kernel(const float2 *in, float4 *out, int G)
+{
+    const size_t n2 = (size_t)G * G / 2;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+        size_t cell = 2 * i, row = cell / G, col = cell % G;
+        float a = checker(row, col);
+        float2 v = in[i];
+        out[i] = make_float4(v.x * a, 0.0f, v.y * -a, 0.0f);
+    }
+}
+
+// execute_convolution_correction_actor (convolution_correction_run.c:17-31).
+// taper has G/2+1 entries, taper[G/2] == 0 stands for the element the
+// reference reads past the end for row or column 0.
+__device__ __forceinline__ float correct_one(float x, const float *taper, int r, int c, int half)
+{
+    int dc = abs(c - half), dr = abs(r - half);
+    float t = taper[dc] * taper[dr];
+    if (dc == 1229 || dr == 1229) t = 0.0f;
+    return ((double)fabsf(t) > 1e-10) ? x / t : 0.0f;
+}
+
+__global__ void correct_kernel(const float2 *in, const float *taper, float2 *out, int G)
+{
+    const size_t n2 = (size_t)G * G / 2;
+    const int half = G / 2;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+        size_t cell = 2 * i;
+        int r = (int)(cell / G), c = (int)(cell % G);
+        float2 v = in[i];
+        out[i] = make_float2(correct_one(v.x, taper, r, c, half), correct_one(v.y, taper, r, c + 1, half));
+    }
+}
+
+__global__ void vis_sub_kernel(const float2 *a, const float2 *b, float2 *out, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float2 x = a[i], y = b[i];
+        out[i] = make_float2(x.x - y.x, x.y - y.y);
+    }
+}
+
+__global__ void vis_add_kernel(const float2 *a, const float2 *b, float2 *out, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float2 x = a[i], y = b[i];
+        out[i] = make_float2(x.x + y.x, x.y + y.y);
+    }
+}
+
+// complex_multiply_apply (gains_apply_run.c:146-149)
+__device__ __forceinline__ float2 cmul_apply(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// subtract_from_measurements / apply_gains_only (gains_apply_run.c:79-92,105-115)
+__global__ void gains_kernel(const float2 *measured, const float2 *pred, const float2 *gains, const int2 *pairs,
+                             int n, int n_baselines, int apply_only, float2 *out)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int b = i % n_baselines;
+        int2 p = pairs[b];
+        float2 ga = gains[p.x], gb = gains[p.y];
+        gb.y = -gb.y;
+        float2 m = cmul_apply(measured[i], cmul_apply(ga, gb));
+        if (!apply_only) {
+            float2 q = pred[i];
+            m = make_float2(m.x - q.x, m.y - q.y);
+        }
+        out[i] = m;
+    }
+}
+
+// merge_gridded_visibilities (splitmerge.c:14-24): slice-ordered float sum.
+__global__ void merge_kernel(const float4 *grids, float4 *out, size_t n2, int slices)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x) {
+        float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int j = 0; j < slices; ++j) {
+            float4 g = grids[(size_t)j * n2 + i];
+            s.x += g.x; s.y += g.y; s.z += g.z; s.w += g.w;
+        }
+        out[i] = s;
+    }
+}
+
+int check_grid(int G)
+{
+    if (G < 2 || (G & 1)) return sdp::set_error(SDP_E_ARG, "grid size %d must be even and >= 2", G);
+    return SDP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sdp_shift_c2c(int G, const float *d_in, float *d_out, sdp_stream s)
+{
+    if (int e = check_grid(G)) return e;
+    size_t n2 = (size_t)G * G / 2;
+    shift_c2c_kernel<<<grid_for(n2), kBlock, 0, sdp::stream_of(s)>>>((const float4 *)d_in, (float4 *)d_out, G);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_shift_c2r(int G, const float *d_in, float *d_out, sdp_stream s)
+{
+    if (int e = check_grid(G)) return e;
+    SDP_REQUIRE((const void *)d_in != (const void *)d_out, "sdp_shift_c2r: in and out must differ");
+    size_t n2 = (size_t)G * G / 2;
+    shift_c2r_kernel<<<grid_for(n2), kBlock, 0, sdp::stream_of(s)>>>((const float4 *)d_in, (float2 *)d_out, G);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_shift_r2c(int G, const float *d_in, float *d_out, sdp_stream s)
+{
+    if (int e = check_grid(G)) return e;
+    SDP_REQUIRE((const void *)d_in != (const void *)d_out, "sdp_shift_r2c: in and out must differ");
+    size_t n2 = (size_t)G * G / 2;
+    shift_r2c_kernel<<<grid_for(n2), kBlock, 0, sdp::stream_of(s)>>>((const float2 *)d_in, (float4 *)d_out, G);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_convolution_correct(int G, const float *d_in, const float *d_taper, float *d_out, sdp_stream s)
+{
+    if (int e = check_grid(G)) return e;
+    size_t n2 = (size_t)G * G / 2;
+    correct_kernel<<<grid_for(n2), kBlock, 0, sdp::stream_of(s)>>>((const float2 *)d_in, d_taper, (float2 *)d_out, G);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_vis_subtract(int n, const float *d_a, const float *d_b, float *d_out, sdp_stream s)
+{
+    if (n <= 0) return SDP_OK;
+    vis_sub_kernel<<<grid_for(n), kBlock, 0, sdp::stream_of(s)>>>((const float2 *)d_a, (const float2 *)d_b,
+                                                                  (float2 *)d_out, n);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_vis_add(int n, const float *d_a, const float *d_b, float *d_out, sdp_stream s)
+{
+    if (n <= 0) return SDP_OK;
+    vis_add_kernel<<<grid_for(n), kBlock, 0, sdp::stream_of(s)>>>((const float2 *)d_a, (const float2 *)d_b,
+                                                                  (float2 *)d_out, n);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_vis_subtract_gains(int n, int n_baselines, const float *d_measured, const float *d_pred,
+                           const float *d_gains, const int *d_pairs, int apply_only, float *d_out, sdp_stream s)
+{
+    if (n <= 0) return SDP_OK;
+    SDP_REQUIRE(n_baselines > 0, "n_baselines must be positive");
+    gains_kernel<<<grid_for(n), kBlock, 0, sdp::stream_of(s)>>>((const float2 *)d_measured, (const float2 *)d_pred,
+                                                                (const float2 *)d_gains, (const int2 *)d_pairs, n,
+                                                                n_baselines, apply_only, (float2 *)d_out);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_merge_grids(int G, int slices, const float *d_grids, float *d_out, sdp_stream s)
+{
+    if (int e = check_grid(G)) return e;
+    SDP_REQUIRE(slices >= 1, "slices must be >= 1");
+    size_t n2 = (size_t)G * G / 2;
+    merge_kernel<<<grid_for(n2), kBlock, 0, sdp::stream_of(s)>>>((const float4 *)d_grids, (float4 *)d_out, n2, slices);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+}  // extern "C"

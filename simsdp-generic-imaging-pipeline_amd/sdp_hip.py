@@ -1,0 +1,508 @@
+"""ctypes binding of libsdp_actors.so (the MI355X imaging library).
+
+Two layers, mirroring include/sdp_actors.h and include/sdp_device.h:
+
+* ``Actors``: the reference actor ABI (host numpy arrays in, host arrays
+  out), the same names and argument meaning as the C actors in
+  preesm_pipelines/simsdp_fft_imaging_pipeline/Code/include/*.h, so tests read
+  like calls made by the PREESM schedule.
+* ``Device`` / ``DeviceBuffer``: device-resident entry points for callers
+  that keep data in HBM (bench, major-cycle driver).
+
+The library must be built (``__graft_entry__.build()``); importing this module
+never falls back to any other implementation.  Failing C calls raise
+``SdpError`` with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsdp_actors.so")
+REPO = os.path.dirname(HERE)
+INCLUDE_DIR = os.path.join(REPO, "include")
+
+c_int, c_double, c_float, c_size_t = ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_size_t
+c_void_p, c_char_p = ctypes.c_void_p, ctypes.c_char_p
+
+# device-layer enums (sdp_device.h)
+DEGRID_NEAREST, DEGRID_WEIGHTED = 0, 1
+DFT_REFERENCE_FP32, DFT_FP64_PHASE, DFT_MFMA = 0, 1, 2
+SNAP_STD, SNAP_GRIDDING_ACTOR = 0, 1
+ORDER_ORIGINAL, ORDER_PLAN = 0, 1
+FFT_FORWARD, FFT_INVERSE = -1, 1
+FFT_SHIFT_IN, FFT_SHIFT_OUT, FFT_REAL_IN, FFT_REAL_OUT, FFT_CORRECT_OUT = 1, 2, 4, 8, 16
+
+
+class SdpError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    """The reference ``Config`` record (common.h:187-265), 360 bytes."""
+
+    _fields_ = [
+        ("retain_device_mem", ctypes.c_bool), ("num_major_cycles", c_int), ("num_recievers", c_int),
+        ("num_baselines", c_int), ("grid_size", c_int), ("cell_size", c_double), ("output_path", c_char_p),
+        ("dirty_image_output", c_char_p), ("gpu_max_threads_per_block", c_int),
+        ("gpu_max_threads_per_block_dimension", c_int), ("right_ascension", ctypes.c_bool),
+        ("save_dirty_image", ctypes.c_bool), ("save_residual_image", ctypes.c_bool),
+        ("save_extracted_sources", ctypes.c_bool), ("save_predicted_visibilities", ctypes.c_bool),
+        ("save_estimated_gains", ctypes.c_bool), ("perform_system_test", ctypes.c_bool),
+        ("system_test_image", c_char_p), ("system_test_sources", c_char_p),
+        ("system_test_visibilities", c_char_p), ("system_test_gains", c_char_p),
+        ("default_gains_file", c_char_p), ("output_gains_file", c_char_p),
+        ("perform_gain_calibration", ctypes.c_bool), ("use_default_gains", ctypes.c_bool),
+        ("max_calibration_cycles", c_int), ("number_cal_major_cycles", c_int),
+        ("force_weight_to_one", ctypes.c_bool), ("frequency_hz", c_double), ("oversampling", c_int),
+        ("uv_scale", c_double), ("num_visibilities", c_int), ("num_kernels", c_int),
+        ("total_kernel_samples", c_int), ("max_w", c_double), ("w_scale", c_double),
+        ("kernel_real_file", c_char_p), ("kernel_imag_file", c_char_p), ("kernel_support_file", c_char_p),
+        ("degridding_kernel_real_file", c_char_p), ("degridding_kernel_imag_file", c_char_p),
+        ("degridding_kernel_support_file", c_char_p), ("visibility_source_file", c_char_p),
+        ("number_minor_cycles_cal", ctypes.c_uint), ("number_minor_cycles_img", ctypes.c_uint),
+        ("loop_gain", c_double), ("weak_source_percent_gc", c_double), ("weak_source_percent_img", c_double),
+        ("noise_factor", c_double), ("psf_max_value", c_double), ("model_sources_output", c_char_p),
+        ("residual_image_output", c_char_p), ("psf_input_file", c_char_p), ("psf_image_output", c_char_p),
+        ("clean_psf_image_output", c_char_p), ("model_image_output", c_char_p),
+        ("final_image_output", c_char_p), ("num_sources", c_int), ("predicted_vis_output", c_char_p),
+    ]
+
+
+assert ctypes.sizeof(Config) == 360
+
+
+def make_config(grid_size: int, cell_size: float, oversampling: int = 16, num_kernels: int = 1,
+                max_w: float = 1895.410847844, loop_gain: float = 0.1, weak_source_percent_img: float = 2e-4,
+                noise_factor: float = 1.5, psf_max_value: float = 1.0) -> Config:
+    """A Config filled like config_struct_set_up (top.c:9-69) for a given
+    grid, cell size and w-plane count."""
+    c = Config()
+    c.grid_size = grid_size
+    c.cell_size = cell_size
+    c.frequency_hz = 299792458.0
+    c.oversampling = oversampling
+    c.uv_scale = cell_size * grid_size
+    c.num_kernels = num_kernels
+    c.max_w = max_w
+    c.w_scale = float((num_kernels - 1) ** 2) / max_w
+    c.loop_gain = loop_gain
+    c.weak_source_percent_gc = 0.005
+    c.weak_source_percent_img = weak_source_percent_img
+    c.noise_factor = noise_factor
+    c.psf_max_value = psf_max_value
+    c.use_default_gains = True
+    c.force_weight_to_one = True
+    return c
+
+
+_lib = None
+
+
+def header_functions(header: str) -> list[str]:
+    """Names of the functions a public header declares."""
+    text = open(os.path.join(INCLUDE_DIR, header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", text,
+                       flags=re.M)
+    return [n for n in names if n not in ("if", "while", "for", "sizeof", "return")]
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SdpError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+    P = c_void_p
+    sig = {
+        "sdp_init": (c_int, []), "sdp_last_error": (c_char_p, []),
+        "sdp_device_malloc": (c_int, [ctypes.POINTER(c_void_p), c_size_t]), "sdp_device_free": (c_int, [P]),
+        "sdp_memcpy_htod": (c_int, [P, P, c_size_t, P]), "sdp_memcpy_dtoh": (c_int, [P, P, c_size_t, P]),
+        "sdp_memcpy_dtod": (c_int, [P, P, c_size_t, P]), "sdp_memset_zero": (c_int, [P, c_size_t, P]),
+        "sdp_stream_create": (c_int, [ctypes.POINTER(c_void_p)]), "sdp_stream_destroy": (c_int, [P]),
+        "sdp_stream_sync": (c_int, [P]), "sdp_device_sync": (c_int, []),
+        "sdp_event_create": (c_int, [ctypes.POINTER(c_void_p)]), "sdp_event_destroy": (c_int, [P]),
+        "sdp_event_record": (c_int, [P, P]), "sdp_event_sync": (c_int, [P]),
+        "sdp_event_elapsed_ms": (c_int, [P, P, ctypes.POINTER(c_float)]),
+        "sdp_set_degrid_mode": (None, [c_int]), "sdp_get_degrid_mode": (c_int, []),
+        "sdp_set_dft_mode": (None, [c_int]), "sdp_get_dft_mode": (c_int, []),
+        "sdp_finegrid_correct": (c_int, [c_int, c_int, c_int, c_double, P, P, P]),
+        "sdp_finegrid_simplify": (c_int, [c_int, c_int, c_int, c_double, c_double, P, P, P, P,
+                                          ctypes.POINTER(c_int), P]),
+        "sdp_grid_plan_create": (c_int, [ctypes.POINTER(c_void_p), P, P, P, c_int, P]),
+        "sdp_grid_plan_destroy": (c_int, [P]), "sdp_grid_plan_info": (c_int, [P, P]),
+        "sdp_grid_plan_to_plan_order": (c_int, [P, P, P, P]),
+        "sdp_grid_plan_to_original_order": (c_int, [P, P, P, P]),
+        "sdp_grid": (c_int, [P, P, P, c_int, P, P]),
+        "sdp_grid_accumulate": (c_int, [P, P, P, c_int, P]),
+        "sdp_grid_merge": (c_int, [P, P, P]),
+        "sdp_degrid": (c_int, [P, P, P, P, c_int, P, P]),
+        "sdp_degrid_nearest": (c_int, [c_int, c_int, c_int, c_double, P, P, P, P]),
+        "sdp_fft_work_bytes": (c_size_t, [c_int]),
+        "sdp_fft2d": (c_int, [P, P, c_int, c_int, c_int, P, P, P]),
+        "sdp_shift_c2c": (c_int, [c_int, P, P, P]), "sdp_shift_c2r": (c_int, [c_int, P, P, P]),
+        "sdp_shift_r2c": (c_int, [c_int, P, P, P]),
+        "sdp_convolution_correct": (c_int, [c_int, P, P, P, P]),
+        "sdp_vis_subtract": (c_int, [c_int, P, P, P, P]), "sdp_vis_add": (c_int, [c_int, P, P, P, P]),
+        "sdp_vis_subtract_gains": (c_int, [c_int, c_int, P, P, P, P, c_int, P, P]),
+        "sdp_merge_grids": (c_int, [c_int, c_int, P, P, P]),
+        "sdp_hogbom_work_bytes": (c_size_t, [c_int]),
+        "sdp_hogbom": (c_int, [c_int, c_int, c_int, P, P, c_double, c_double, c_double, c_double, c_double,
+                               P, P, P, P, P, P]),
+        "sdp_dft_predict": (c_int, [c_int, c_int, P, P, P, c_int, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise SdpError(load().sdp_last_error().decode(errors="replace") or f"error {rc}")
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, DeviceBuffer):
+        return a.ptr
+    if isinstance(a, int):
+        return a
+    if not (a.flags["C_CONTIGUOUS"]):
+        raise SdpError("host arrays must be C-contiguous")
+    return a.ctypes.data
+
+
+# ---------------------------------------------------------------- device layer
+
+class DeviceBuffer:
+    """A device allocation owned by Python (freed on close / garbage collection)."""
+
+    def __init__(self, nbytes: int):
+        lib = load()
+        check(lib.sdp_init())
+        p = c_void_p()
+        check(lib.sdp_device_malloc(ctypes.byref(p), c_size_t(max(int(nbytes), 16))))
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_array(cls, a: np.ndarray, stream=None) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a, stream)
+        return b
+
+    def upload(self, a: np.ndarray, stream=None) -> None:
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise SdpError("upload larger than the buffer")
+        check(load().sdp_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes, stream))
+
+    def download(self, shape, dtype, stream=None) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise SdpError("download larger than the buffer")
+        check(load().sdp_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes, stream))
+        return out
+
+    def zero(self, stream=None) -> None:
+        check(load().sdp_memset_zero(self.ptr, self.nbytes, stream))
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            load().sdp_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        e = c_void_p()
+        check(load().sdp_event_create(ctypes.byref(e)))
+        self.ptr = e.value
+
+    def record(self, stream=None):
+        check(load().sdp_event_record(self.ptr, stream))
+
+    def sync(self):
+        check(load().sdp_event_sync(self.ptr))
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = c_float()
+        check(load().sdp_event_elapsed_ms(self.ptr, later.ptr, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            load().sdp_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class GridPlan:
+    """Owner of an ``sdp_grid_plan`` (tile-sorted visibility records)."""
+
+    def __init__(self, grid_size, oversampling, supports, uv_scale, w_scale, d_uvw, n_vis,
+                 snap_mode=SNAP_STD, tile=0, chunk=0, stream=None):
+        lib = load()
+        check(lib.sdp_init())
+        sup = np.ascontiguousarray(np.asarray(supports, dtype=np.int32).reshape(-1, 2))
+
+        class Params(ctypes.Structure):
+            _fields_ = [("grid_size", c_int), ("oversampling", c_int), ("n_kernels", c_int),
+                        ("snap_mode", c_int), ("uv_scale", c_double), ("w_scale", c_double),
+                        ("tile", c_int), ("chunk", c_int)]
+
+        p = Params(grid_size, oversampling, sup.shape[0], snap_mode, uv_scale, w_scale, tile, chunk)
+        h = c_void_p()
+        check(lib.sdp_grid_plan_create(ctypes.byref(h), ctypes.byref(p), sup.ctypes.data, _ptr(d_uvw),
+                                       int(n_vis), stream))
+        self.ptr = h.value
+        self.grid_size = grid_size
+        self.n_vis = int(n_vis)
+
+    def info(self) -> dict:
+        a = np.zeros(8, dtype=np.int64)
+        check(load().sdp_grid_plan_info(self.ptr, a.ctypes.data))
+        keys = ["kept", "dropped", "tiles", "chunks", "tile", "chunk_taps", "max_half_support", "scratch_bytes"]
+        return dict(zip(keys, (int(x) for x in a)))
+
+    def grid(self, d_kernels, d_vis, d_grid, order=ORDER_ORIGINAL, stream=None):
+        check(load().sdp_grid(self.ptr, _ptr(d_kernels), _ptr(d_vis), order, _ptr(d_grid), stream))
+
+    def accumulate(self, d_kernels, d_vis, order=ORDER_ORIGINAL, stream=None):
+        check(load().sdp_grid_accumulate(self.ptr, _ptr(d_kernels), _ptr(d_vis), order, stream))
+
+    def merge(self, d_grid, stream=None):
+        check(load().sdp_grid_merge(self.ptr, _ptr(d_grid), stream))
+
+    def degrid(self, d_kernels, d_grid, d_vis_out, d_measured=None, order=ORDER_ORIGINAL, stream=None):
+        check(load().sdp_degrid(self.ptr, _ptr(d_kernels), _ptr(d_grid), _ptr(d_measured), order,
+                                _ptr(d_vis_out), stream))
+
+    def to_plan_order(self, d_in, d_out, stream=None):
+        check(load().sdp_grid_plan_to_plan_order(self.ptr, _ptr(d_in), _ptr(d_out), stream))
+
+    def to_original_order(self, d_in, d_out, stream=None):
+        check(load().sdp_grid_plan_to_original_order(self.ptr, _ptr(d_in), _ptr(d_out), stream))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            load().sdp_grid_plan_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synchronize(stream=None):
+    check(load().sdp_stream_sync(stream))
+
+
+def fft2d(d_in, d_out, grid_size, direction, flags=0, d_taper=None, d_work=None, stream=None):
+    check(load().sdp_fft2d(_ptr(d_in), _ptr(d_out), grid_size, direction, flags, _ptr(d_taper), _ptr(d_work),
+                           stream))
+
+
+def fft_work_bytes(grid_size: int) -> int:
+    return int(load().sdp_fft_work_bytes(grid_size))
+
+
+def hogbom_work_bytes(grid_size: int) -> int:
+    return int(load().sdp_hogbom_work_bytes(grid_size))
+
+
+def hogbom(grid_size, n_minor, max_sources, d_residual, d_psf, psf_max, loop_gain, noise_factor, weak,
+           cell_size, d_model_in, d_model_out, d_sources, d_num_sources, d_work, stream=None):
+    check(load().sdp_hogbom(grid_size, n_minor, max_sources, _ptr(d_residual), _ptr(d_psf), psf_max, loop_gain,
+                            noise_factor, weak, cell_size, _ptr(d_model_in), _ptr(d_model_out), _ptr(d_sources),
+                            _ptr(d_num_sources), _ptr(d_work), stream))
+
+
+def dft_predict(n_vis, n_sources, d_sources, d_uvw, d_out, mode=DFT_FP64_PHASE, stream=None):
+    check(load().sdp_dft_predict(n_vis, n_sources, _ptr(d_sources), _ptr(d_uvw), _ptr(d_out), mode, stream))
+
+
+def degrid_nearest(n_vis, n_out, grid_size, uv_scale, d_grid, d_uvw, d_out, stream=None):
+    check(load().sdp_degrid_nearest(n_vis, n_out, grid_size, uv_scale, _ptr(d_grid), _ptr(d_uvw), _ptr(d_out),
+                                    stream))
+
+
+# ---------------------------------------------------------------- actor layer
+
+class Actors:
+    """The reference actor ABI, called with numpy host arrays."""
+
+    def __init__(self):
+        self.lib = load()
+        P = c_void_p
+        cfg = ctypes.POINTER(Config)
+        ip = ctypes.POINTER(c_int)
+        sigs = {
+            "correct_to_finegrid": [c_int, c_int, c_int, c_int, P, P, cfg, P, P, ip],
+            "std_gridding": [c_int, c_int, c_int, c_int, c_int, c_int, ip, ip, P, P, P, P, cfg, P, P],
+            "std_degridding": [c_int, c_int, c_int, c_int, c_int, P, P, P, P, ip, cfg, P],
+            "gridding_actor": [c_int, c_int, c_int, c_int, P, P, P, P, cfg, P],
+            "add_visibilities": [c_int, P, P, P],
+            "dft_actor": [c_int, c_int, P, P, ip, cfg, P],
+            "hogbom_clean": [c_int, c_int, c_int, P, P, P, cfg, P, ip, P, P],
+            "fft_shift_complex_to_complex_actor": [c_int, P, cfg, P],
+            "CUFFT_EXECUTE_FORWARD_C2C_actor": [c_int, P, P],
+            "CUFFT_EXECUTE_INVERSE_C2C_actor": [c_int, P, P],
+            "fft_shift_complex_to_real_actor": [c_int, P, cfg, P],
+            "fft_shift_real_to_complex_actor": [c_int, P, cfg, P],
+            "execute_convolution_correction_actor": [c_int, P, P, cfg, P],
+            "execute_inv_convolution_correction_actor": [c_int, P, P, cfg, P],
+            "subtract_from_measurements": [c_int, c_int, c_int, P, P, P, P, cfg, P],
+            "apply_gains_only": [c_int, c_int, c_int, P, P, P, cfg, P],
+            "subtract_from_measurements_finegrid": [c_int, P, P, ip, P],
+            "merge_gridded_visibilities": [c_int, c_int, P, P],
+            "split_uvgrid": [c_int, c_int, P, P],
+            "correction_set_up": [c_int, P],
+            "prolate_spheroidal": [c_double],
+        }
+        for name, args in sigs.items():
+            f = getattr(self.lib, name)
+            f.argtypes = args
+            f.restype = c_double if name == "prolate_spheroidal" else None
+
+    @staticmethod
+    def _i(v):
+        return ctypes.byref(c_int(int(v)))
+
+    def correct_to_finegrid(self, G, OS, simplify, uvw, vis, cfg):
+        n = uvw.shape[0]
+        out_vis = np.zeros((n, 2), np.float32)
+        out_uvw = np.zeros((n, 3), np.float32)
+        kept = c_int(0)
+        self.lib.correct_to_finegrid(n, G, OS, simplify, _ptr(uvw), _ptr(vis), ctypes.byref(cfg), _ptr(out_vis),
+                                     _ptr(out_uvw), ctypes.byref(kept))
+        return out_vis, out_uvw, kept.value
+
+    def std_gridding(self, G, OS, kernels, supports, uvw, vis, cfg, n=None, bypass=0, maj_iter=0, prev_grid=None):
+        n = uvw.shape[0] if n is None else n
+        out = np.empty((G * G, 2), np.float32)
+        prev = prev_grid if prev_grid is not None else out
+        self.lib.std_gridding(G, uvw.shape[0], supports.shape[0], kernels.shape[0], OS, bypass, self._i(maj_iter),
+                              self._i(n), _ptr(kernels), _ptr(supports), _ptr(uvw), _ptr(vis), ctypes.byref(cfg),
+                              _ptr(prev), _ptr(out))
+        return out
+
+    def std_degridding(self, G, OS, kernels, supports, grid, uvw, cfg, n=None, n_total=None):
+        n = uvw.shape[0] if n is None else n
+        n_total = uvw.shape[0] if n_total is None else n_total
+        out = np.empty((n_total, 2), np.float32)
+        self.lib.std_degridding(G, n_total, supports.shape[0], kernels.shape[0], OS, _ptr(kernels), _ptr(supports),
+                                _ptr(grid), _ptr(uvw), self._i(n), ctypes.byref(cfg), _ptr(out))
+        return out
+
+    def gridding_actor(self, G, kernels, supports, uvw, vis, cfg):
+        out = np.empty((G * G, 2), np.float32)
+        self.lib.gridding_actor(G, uvw.shape[0], supports.shape[0], kernels.shape[0], _ptr(kernels),
+                                _ptr(supports), _ptr(uvw), _ptr(vis), ctypes.byref(cfg), _ptr(out))
+        return out
+
+    def add_visibilities(self, a, b):
+        out = np.empty_like(a)
+        self.lib.add_visibilities(a.shape[0], _ptr(a), _ptr(b), _ptr(out))
+        return out
+
+    def dft_actor(self, sources, uvw, cfg, max_sources=None):
+        n = uvw.shape[0]
+        out = np.empty((n, 2), np.float32)
+        ns = sources.shape[0]
+        self.lib.dft_actor(n, max_sources or ns, _ptr(sources), _ptr(uvw), self._i(ns), ctypes.byref(cfg),
+                           _ptr(out))
+        return out
+
+    def hogbom_clean(self, G, n_minor, max_sources, residual, psf, cfg, model):
+        out_model = np.empty(G * G, np.float32)
+        sources = np.zeros((max_sources, 3), np.float32)
+        ns = c_int(0)
+        halfdims = np.zeros(2, np.int32)
+        self.lib.hogbom_clean(G, n_minor, max_sources, _ptr(residual), _ptr(psf), _ptr(halfdims),
+                              ctypes.byref(cfg), _ptr(model), ctypes.byref(ns), _ptr(sources), _ptr(out_model))
+        return ns.value, sources[: ns.value], out_model
+
+    def shift_c2c(self, G, grid, cfg):
+        out = np.empty_like(grid)
+        self.lib.fft_shift_complex_to_complex_actor(G, _ptr(grid), ctypes.byref(cfg), _ptr(out))
+        return out
+
+    def shift_c2r(self, G, grid, cfg):
+        out = np.empty(G * G, np.float32)
+        self.lib.fft_shift_complex_to_real_actor(G, _ptr(grid), ctypes.byref(cfg), _ptr(out))
+        return out
+
+    def shift_r2c(self, G, image, cfg):
+        out = np.empty((G * G, 2), np.float32)
+        self.lib.fft_shift_real_to_complex_actor(G, _ptr(image), ctypes.byref(cfg), _ptr(out))
+        return out
+
+    def fft_forward(self, G, grid):
+        out = np.empty_like(grid)
+        self.lib.CUFFT_EXECUTE_FORWARD_C2C_actor(G, _ptr(grid), _ptr(out))
+        return out
+
+    def fft_inverse(self, G, grid):
+        out = np.empty_like(grid)
+        self.lib.CUFFT_EXECUTE_INVERSE_C2C_actor(G, _ptr(grid), _ptr(out))
+        return out
+
+    def convolution_correction(self, G, image, prolate, cfg):
+        out = np.empty_like(image)
+        self.lib.execute_convolution_correction_actor(G, _ptr(image), _ptr(prolate), ctypes.byref(cfg), _ptr(out))
+        return out
+
+    def subtract_from_measurements(self, measured, pred, gains, pairs, cfg):
+        out = np.empty_like(measured)
+        self.lib.subtract_from_measurements(gains.shape[0], pairs.shape[0], measured.shape[0], _ptr(measured),
+                                            _ptr(pred), _ptr(gains), _ptr(pairs), ctypes.byref(cfg), _ptr(out))
+        return out
+
+    def apply_gains_only(self, measured, gains, pairs, cfg):
+        out = np.empty_like(measured)
+        self.lib.apply_gains_only(gains.shape[0], pairs.shape[0], measured.shape[0], _ptr(measured), _ptr(gains),
+                                  _ptr(pairs), ctypes.byref(cfg), _ptr(out))
+        return out
+
+    def subtract_finegrid(self, measured, pred, n=None):
+        out = np.zeros_like(measured)
+        n = measured.shape[0] if n is None else n
+        self.lib.subtract_from_measurements_finegrid(measured.shape[0], _ptr(measured), _ptr(pred), self._i(n),
+                                                     _ptr(out))
+        return out
+
+    def merge(self, G, slices, grids):
+        out = np.empty((G * G, 2), np.float32)
+        self.lib.merge_gridded_visibilities(G, slices, _ptr(grids), _ptr(out))
+        return out
+
+    def correction_set_up(self, G):
+        out = np.empty(G // 2, np.float32)
+        self.lib.correction_set_up(G, _ptr(out))
+        return out
+
+    def set_degrid_mode(self, mode):
+        self.lib.sdp_set_degrid_mode(mode)
+
+    def set_dft_mode(self, mode):
+        self.lib.sdp_set_dft_mode(mode)

@@ -1,0 +1,278 @@
+"""Seeded synthetic inputs for the imaging major cycle.
+
+The reference reads GLEAM visibilities, kernel CSVs and a PSF CSV that are not
+shipped with it (README "Download GLEAM" step, top.c:19-60).  This module
+builds inputs of the same shapes and layouts from seeded random numbers
+(numpy PCG64):
+
+* antenna layouts (uniform disk, log-radial), Earth-rotation uvw tracks in
+  wavelengths, ordered time-major (visibility index = time * n_baselines +
+  baseline, the order subtract_from_measurements assumes,
+  gains_apply_run.c:81), baselines in calculate_receiver_pairs order
+  (top.c:119-137);
+* point-source skies and their visibilities (exact FP64 DFT plus noise);
+* gridding-kernel tables in the kernel_host_set_up layout (top.c:252-257):
+  per w-plane a ((h+1)*OS)^2 quadrant indexed [ky*(h+1)*OS + kx], the
+  support table holding (h, sample offset).  Plane 0 is the prolate
+  spheroidal kernel; plane p>0 is the w-projection kernel for
+  w_p = p^2 / w_scale (the sqrt spacing implied by the plane index
+  round(sqrt(|w| w_scale)), degridgrid.c:84).
+
+The generator is input preparation, run once per configuration; it is not
+part of the major-cycle hot path.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+C_LIGHT = 299792458.0
+
+
+def prolate_spheroidal(nu: np.ndarray) -> np.ndarray:
+    """Schwab's rational approximation used by the reference taper
+    (top.c:408-447), vectorised; 0 outside [0, 1)."""
+    p = np.array([0.08203343, -0.3644705, 0.627866, -0.5335581, 0.2312756,
+                  0.004028559, -0.03697768, 0.1021332, -0.1201436, 0.06412774])
+    q = np.array([1.0, 0.8212018, 0.2078043, 1.0, 0.9599102, 0.2918724])
+    nu = np.abs(np.asarray(nu, dtype=np.float64))
+    out = np.zeros_like(nu)
+    for part, lo, hi in ((0, 0.0, 0.75), (1, 0.75, 1.0)):
+        m = (nu >= lo) & (nu < hi)
+        d = nu[m] ** 2 - hi ** 2
+        top = sum(p[part * 5 + i] * d ** i for i in range(5))
+        bot = sum(q[part * 3 + i] * d ** i for i in range(3))
+        out[m] = top / bot
+    return out
+
+
+def taper(grid_size: int) -> np.ndarray:
+    """create_1D_half_prolate (top.c:395-404): G/2 floats."""
+    half = grid_size // 2
+    return prolate_spheroidal(np.arange(half) / half).astype(np.float32)
+
+
+def taper_padded(grid_size: int) -> np.ndarray:
+    """The taper with the trailing 0 the correction reads for row/column 0."""
+    return np.concatenate([taper(grid_size), np.zeros(1, np.float32)])
+
+
+# ------------------------------------------------------------------ geometry
+
+def disk_layout(n_ant: int, radius_m: float, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    r = radius_m * np.sqrt(rng.random(n_ant))
+    th = 2 * np.pi * rng.random(n_ant)
+    return np.stack([r * np.cos(th), r * np.sin(th), np.zeros(n_ant)], axis=1)
+
+
+def log_radial_layout(n_ant: int, r_min: float, r_max: float, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    r = np.exp(rng.uniform(np.log(r_min), np.log(r_max), n_ant))
+    th = 2 * np.pi * rng.random(n_ant)
+    return np.stack([r * np.cos(th), r * np.sin(th), np.zeros(n_ant)], axis=1)
+
+
+def receiver_pairs(n_ant: int) -> np.ndarray:
+    """calculate_receiver_pairs order (top.c:119-137)."""
+    a, b = np.triu_indices(n_ant, k=1)
+    return np.stack([a, b], axis=1).astype(np.int32)
+
+
+def uvw_tracks(enu: np.ndarray, lat_deg: float, dec_deg: float, ha_hours: np.ndarray,
+               freq_hz: float) -> np.ndarray:
+    """uvw (wavelengths, float64) for every (time, baseline), time-major."""
+    pairs = receiver_pairs(enu.shape[0])
+    d = enu[pairs[:, 1]] - enu[pairs[:, 0]]
+    lat, dec = np.radians(lat_deg), np.radians(dec_deg)
+    # ENU -> local equatorial XYZ
+    X = -np.sin(lat) * d[:, 1] + np.cos(lat) * d[:, 2]
+    Y = d[:, 0]
+    Z = np.cos(lat) * d[:, 1] + np.sin(lat) * d[:, 2]
+    H = np.radians(15.0 * np.asarray(ha_hours))[:, None]
+    u = np.sin(H) * X + np.cos(H) * Y
+    v = -np.sin(dec) * np.cos(H) * X + np.sin(dec) * np.sin(H) * Y + np.cos(dec) * Z
+    w = np.cos(dec) * np.cos(H) * X - np.cos(dec) * np.sin(H) * Y + np.sin(dec) * Z
+    lam = C_LIGHT / freq_hz
+    return np.stack([u, v, w], axis=-1).reshape(-1, 3) / lam
+
+
+def dft_f64(uvw: np.ndarray, sources: np.ndarray, block: int = 65536) -> np.ndarray:
+    """Exact FP64 predict with the reference's formula (dft_run.c:21-32)."""
+    out = np.zeros((uvw.shape[0], 2), np.float64)
+    l, m, f = (sources[:, k].astype(np.float64) for k in range(3))
+    term = 0.5 * (l * l + m * m)
+    corr = f / (1.0 - term)
+    for s in range(0, uvw.shape[0], block):
+        u = uvw[s:s + block].astype(np.float64)
+        ph = u[:, 0:1] * l + u[:, 1:2] * m - u[:, 2:3] * term
+        ph -= np.rint(ph)
+        out[s:s + block, 0] = (np.cos(2 * np.pi * ph) * corr).sum(axis=1)
+        out[s:s + block, 1] = -(np.sin(2 * np.pi * ph) * corr).sum(axis=1)
+    return out
+
+
+# ------------------------------------------------------------------ kernels
+
+def kernel_tables(supports_h: list[int], oversampling: int, fov: float, w_scale: float,
+                  q_min: int = 0) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Gridding kernel table, its conjugate (degridding) table and the
+    support table (h, offset) for planes with half supports supports_h.
+
+    Plane p images the w-term of w_p = p^2 / w_scale: the uv kernel is the
+    Fourier transform of psi(l)psi(m) exp(-i pi w_p (l^2 + m^2)) over the
+    field of view (l, m in units of the half field), sampled at 1/OS cell.
+    All planes are scaled by the sum of plane 0 over integer offsets, so the
+    w = 0 kernel sums to 1."""
+    n_planes = len(supports_h)
+    h_max = max(supports_h)
+    q = max(q_min, 2 * (h_max + 4))
+    q += q % 2
+    n_pad = q * oversampling
+    x = (np.arange(q) - q // 2) / (q / 2)          # image coordinate in half fields, [-1, 1)
+    xx, yy = np.meshgrid(x, x, indexing="ij")
+    base = prolate_spheroidal(xx) * prolate_spheroidal(yy)
+    l = xx * fov / 2
+    m = yy * fov / 2
+    tables, offsets = [], []
+    norm = None
+    off = 0
+    for p, h in enumerate(supports_h):
+        w_p = (p * p) / w_scale if w_scale > 0 else 0.0
+        screen = base * np.exp(-1j * np.pi * w_p * (l * l + m * m))
+        pad = np.zeros((n_pad, n_pad), np.complex128)
+        c = n_pad // 2
+        pad[c - q // 2:c + q // 2, c - q // 2:c + q // 2] = screen
+        k = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(pad)))   # uv kernel, 1/OS-cell samples, centred
+        if norm is None:
+            ints = k[c % oversampling::oversampling, c % oversampling::oversampling]
+            norm = ints.sum().real
+        k = k / norm
+        width = (h + 1) * oversampling
+        quad = k[c:c + width, c:c + width]                 # offsets (ky, kx) >= 0
+        tables.append(quad.reshape(-1))
+        offsets.append(off)
+        off += width * width
+    kern = np.concatenate(tables)
+    grid_k = np.stack([kern.real, kern.imag], axis=1).astype(np.float32)
+    degrid_k = np.stack([kern.real, -kern.imag], axis=1).astype(np.float32)
+    sup = np.stack([np.asarray(supports_h), np.asarray(offsets)], axis=1).astype(np.int32)
+    return grid_k, degrid_k, sup
+
+
+# ------------------------------------------------------------------ configs
+
+@dataclass
+class Dataset:
+    name: str
+    grid_size: int
+    cell_size: float
+    oversampling: int
+    uvw: np.ndarray            # (N, 3) float32 wavelengths
+    vis: np.ndarray            # (N, 2) float32
+    kernels: np.ndarray        # (S, 2) float32 gridding table
+    degrid_kernels: np.ndarray # (S, 2) float32 conjugate table
+    supports: np.ndarray       # (P, 2) int32 (h, offset)
+    sources: np.ndarray        # (n_src, 3) float32 (l, m, flux)
+    max_w: float
+    n_minor: int
+    n_major: int
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def uv_scale(self) -> float:
+        return self.cell_size * self.grid_size
+
+    @property
+    def w_scale(self) -> float:
+        n = self.supports.shape[0]
+        return float((n - 1) ** 2) / self.max_w if n > 1 else 0.0
+
+    @property
+    def n_vis(self) -> int:
+        return self.uvw.shape[0]
+
+
+def config1(n_time: int = 256, grid_size: int = 1024, with_vis: bool = True) -> Dataset:
+    """Plain FFT imaging, single channel: 64 antennas uniform in a 2 km
+    diameter disk (seed 1), latitude -30 deg, dec -30 deg, HA -3..+3 h in
+    n_time steps at 150 MHz -> 2016 * n_time visibilities; 50 sources with
+    flux U[0.1, 1] Jy in a 2 x 2 deg box (seed 2), noise sigma 0.01 Jy per
+    component (seed 3); PSWF support 7 (h = 3), OS 16, one plane."""
+    enu = disk_layout(64, 1000.0, seed=1)
+    ha = np.linspace(-3.0, 3.0, n_time)
+    uvw = uvw_tracks(enu, -30.0, -30.0, ha, 150e6)
+    rng = np.random.default_rng(2)
+    half_box = np.radians(1.0)
+    src = np.stack([rng.uniform(-half_box, half_box, 50), rng.uniform(-half_box, half_box, 50),
+                    rng.uniform(0.1, 1.0, 50)], axis=1).astype(np.float32)
+    cell = 4.0e-4 * 1024 / grid_size
+    gk, dk, sup = kernel_tables([3], 16, fov=cell * grid_size, w_scale=0.0)
+    uvw32 = uvw.astype(np.float32)
+    if with_vis:
+        vis = dft_f64(uvw32, src)
+        vis += np.random.default_rng(3).normal(0.0, 0.01, vis.shape)
+        vis32 = vis.astype(np.float32)
+    else:
+        vis32 = np.zeros((uvw32.shape[0], 2), np.float32)
+    return Dataset("config1", grid_size, cell, 16, uvw32, vis32, gk, dk, sup, src, max_w=1.0,
+                   n_minor=200, n_major=1)
+
+
+def config2(n_ant: int = 256, n_time: int = 512, grid_size: int = 4096, n_planes: int = 32) -> Dataset:
+    """Plain full major cycle at real size: 256 antennas log-radial out to
+    5 km radius (10 km across, seed 11), 512 time steps (HA -4..+4 h) at
+    150 MHz -> 32,640 * 512 = 16,711,680 visibilities; 1000 sources with
+    dN/dS ~ S^-1.5 on [1 mJy, 1 Jy] (seed 12) in the central half of a
+    3.5 deg field; 4096^2 grid with w-projection, 32 planes, half support
+    growing 4 -> 16 with |w|, OS 8.  Measured visibilities are left zero
+    here: the major-cycle driver predicts them on the GPU."""
+    enu = log_radial_layout(n_ant, 50.0, 5000.0, seed=11)
+    ha = np.linspace(-4.0, 4.0, n_time)
+    uvw = uvw_tracks(enu, -30.0, -30.0, ha, 150e6).astype(np.float32)
+    cell = 1.5e-5
+    fov = cell * grid_size
+    max_w = float(np.abs(uvw[:, 2]).max()) * 1.0001
+    w_scale = (n_planes - 1) ** 2 / max_w
+    # full support 8 -> 32 cells across the planes; every plane's w-term
+    # spread (about 3 + w_p fov^2 / 2 cells) stays inside its support
+    sup_h = [4 + int(round(12 * p / (n_planes - 1))) for p in range(n_planes)]
+    assert all(3 + (p * p / w_scale) * fov * fov / 2 <= h for p, h in enumerate(sup_h))
+    gk, dk, sup = kernel_tables(sup_h, 8, fov=fov, w_scale=w_scale)
+    rng = np.random.default_rng(12)
+    n_src = 1000
+    a = -1.5
+    u01 = rng.random(n_src)
+    s_lo, s_hi = 1e-3, 1.0
+    flux = (s_lo ** (a + 1) + u01 * (s_hi ** (a + 1) - s_lo ** (a + 1))) ** (1.0 / (a + 1))
+    lm = rng.uniform(-fov / 4, fov / 4, (n_src, 2))
+    src = np.concatenate([lm, flux[:, None]], axis=1).astype(np.float32)
+    vis = np.zeros((uvw.shape[0], 2), np.float32)
+    return Dataset("config2", grid_size, cell, 8, uvw, vis, gk, dk, sup, src, max_w=max_w,
+                   n_minor=1000, n_major=5)
+
+
+def small_case(n_vis: int = 4000, grid_size: int = 128, n_planes: int = 4, oversampling: int = 16,
+               seed: int = 7, h_list=None, fill: float = 0.98, w_max: float = 200.0) -> Dataset:
+    """A small random case for parity tests: uvw uniform so that footprints
+    cover most of the grid (some reach past the edge), w uniform in
+    [-w_max, w_max], complex Gaussian visibilities."""
+    rng = np.random.default_rng(seed)
+    cell = 1.0 / grid_size / 10.0
+    uv_scale = cell * grid_size
+    half = grid_size / 2
+    u = rng.uniform(-half * fill, half * fill, n_vis) / uv_scale
+    v = rng.uniform(-half * fill, half * fill, n_vis) / uv_scale
+    w = rng.uniform(-w_max, w_max, n_vis)
+    uvw = np.stack([u, v, w], axis=1).astype(np.float32)
+    vis = rng.normal(size=(n_vis, 2)).astype(np.float32)
+    if h_list is None:
+        h_list = [3 + (p % 3) for p in range(n_planes)]
+    max_w = w_max * 1.0001
+    w_scale = (len(h_list) - 1) ** 2 / max_w if len(h_list) > 1 else 0.0
+    gk, dk, sup = kernel_tables(h_list, oversampling, fov=cell * grid_size, w_scale=w_scale if w_scale else 0.0)
+    src = np.zeros((0, 3), np.float32)
+    return Dataset("small", grid_size, cell, oversampling, uvw, vis, gk, dk, sup, src, max_w=max_w,
+                   n_minor=50, n_major=1)

@@ -1,0 +1,77 @@
+"""The drop-in boundary: the shared library loads on a host without a GPU
+and exports every function the public headers declare (no compute calls)."""
+import ctypes
+import os
+import subprocess
+
+import pytest
+
+import sdp_hip
+
+
+def test_library_exports_every_declared_function():
+    lib = ctypes.CDLL(sdp_hip.LIB_PATH)
+    missing = []
+    for header in ("sdp_actors.h", "sdp_device.h"):
+        names = sdp_hip.header_functions(header)
+        assert len(names) > 10, header
+        missing += [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_actor_prototypes_cover_reference_symbols():
+    names = set(sdp_hip.header_functions("sdp_actors.h"))
+    for n in ("correct_to_finegrid", "std_gridding", "std_degridding", "gridding_actor", "add_visibilities",
+              "dft_actor", "hogbom_clean", "fft_shift_complex_to_complex_actor", "CUFFT_EXECUTE_FORWARD_C2C_actor",
+              "CUFFT_EXECUTE_INVERSE_C2C_actor", "fft_shift_complex_to_real_actor",
+              "fft_shift_real_to_complex_actor", "execute_convolution_correction_actor",
+              "execute_inv_convolution_correction_actor", "subtract_from_measurements_finegrid",
+              "subtract_from_measurements", "split_fftdegrid_visibilities", "split_uvgrid",
+              "merge_gridded_visibilities", "correction_set_up", "psf_host_set_up", "config_struct_set_up_sequel",
+              "kernel_host_set_up"):
+        assert n in names, n
+
+
+def test_config_layout_matches_reference():
+    C = sdp_hip.Config
+    assert ctypes.sizeof(C) == 360
+    off = {f: getattr(C, f).offset for f, _ in C._fields_}
+    assert off["cell_size"] == 24 and off["frequency_hz"] == 128 and off["oversampling"] == 136
+    assert off["uv_scale"] == 144 and off["max_w"] == 168 and off["w_scale"] == 176
+    assert off["loop_gain"] == 248 and off["weak_source_percent_img"] == 264
+    assert off["noise_factor"] == 272 and off["psf_max_value"] == 280
+
+
+def test_header_compiles_as_c(tmp_path):
+    src = tmp_path / "t.c"
+    src.write_text('#include "sdp_actors.h"\n#include "sdp_device.h"\nint main(void){return (int)sizeof(Config) - 360;}\n')
+    exe = tmp_path / "t"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", sdp_hip.INCLUDE_DIR, str(src), "-o", str(exe)],
+                   check=True)
+    assert subprocess.run([str(exe)]).returncode == 0
+
+
+def test_setup_actors_on_host(tmp_path):
+    """Set-up actors that stay on the host (top.c): no GPU needed."""
+    lib = ctypes.CDLL(sdp_hip.LIB_PATH)
+    import numpy as np
+    import sdp_oracle
+    G = 64
+    out = np.zeros(G // 2, np.float32)
+    lib.correction_set_up.argtypes = [ctypes.c_int, ctypes.c_void_p]
+    lib.correction_set_up(G, out.ctypes.data)
+    np.testing.assert_array_equal(out, sdp_oracle.half_prolate(G))
+    # psf_host_set_up: crop a 6x6 file to 4x4 and normalise by the file's max
+    psf = np.arange(36, dtype=np.float64).reshape(6, 6) / 7.0
+    f = tmp_path / "psf.csv"
+    f.write_text(" ".join(f"{v:.9f}" for v in psf.ravel()))
+    cfg = sdp_hip.make_config(4, 0.01)
+    cfg.psf_input_file = str(f).encode()
+    cfg.psf_max_value = 0.0
+    lib.psf_host_set_up.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(sdp_hip.Config), ctypes.c_void_p,
+                                    ctypes.c_void_p]
+    got = np.zeros(16, np.float32)
+    mx = ctypes.c_double(0.0)
+    lib.psf_host_set_up(4, 6, ctypes.byref(cfg), got.ctypes.data, ctypes.byref(mx))
+    assert mx.value == pytest.approx(35 / 7.0, rel=1e-6)
+    np.testing.assert_allclose(got.reshape(4, 4), (psf[1:5, 1:5] / (35 / 7.0)).astype(np.float32), rtol=1e-6)

@@ -1,0 +1,345 @@
+"""HIP path against the CPU oracle, through the C-ABI (actor and device layers).
+
+Bars: integer / index / copy work is bit-exact; floating-point work is within
+the tolerance written in each test, relative to the data's peak.
+"""
+import numpy as np
+import pytest
+
+import sdp_synth
+
+pytestmark = pytest.mark.gpu
+
+
+def cfg_of(hip, d, **kw):
+    return hip.make_config(d.grid_size, d.cell_size, oversampling=d.oversampling,
+                           num_kernels=d.supports.shape[0], max_w=d.max_w, **kw)
+
+
+def integer_kernels(n):
+    k = np.arange(n)
+    return np.stack([(k % 251 + 1), (k % 13) - 6], axis=1).astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def case():
+    return sdp_synth.small_case(n_vis=3000, grid_size=128, n_planes=4, oversampling=16, seed=7)
+
+
+# ---------------------------------------------------------------- gridding
+
+@pytest.mark.parametrize("snap", ["std", "actor"])
+def test_grid_indices_bit_exact(hip, actors, oracle, case, rng, snap):
+    """Integer-valued kernels and visibilities make every sum exact in float
+    whatever the order, so any index difference shows as a mismatch."""
+    d = case
+    kern = integer_kernels(d.kernels.shape[0])
+    vis = rng.integers(-3, 4, size=(d.n_vis, 2)).astype(np.float32)
+    cfg = cfg_of(hip, d)
+    if snap == "std":
+        got = actors.std_gridding(d.grid_size, d.oversampling, kern, d.supports, d.uvw, vis, cfg)
+        ref = oracle.std_gridding(d.grid_size, d.oversampling, kern, d.supports, d.uvw, vis, cfg)
+    else:
+        d16 = sdp_synth.small_case(n_vis=3000, grid_size=128, n_planes=4, oversampling=16, seed=8)
+        kern = integer_kernels(d16.kernels.shape[0])
+        got = actors.gridding_actor(d16.grid_size, kern, d16.supports, d16.uvw, vis, cfg_of(hip, d16))
+        ref = oracle.gridding_actor(d16.grid_size, kern, d16.supports, d16.uvw, vis, cfg_of(hip, d16))
+    assert np.count_nonzero(ref) > 1000
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_grid_float_accuracy(hip, actors, oracle, case):
+    """Compensated accumulation: the GPU grid is at least as close to the
+    FP64 grid as the reference's sequential float sum, and within 1e-6 of
+    the peak."""
+    d = case
+    cfg = cfg_of(hip, d)
+    got = actors.std_gridding(d.grid_size, d.oversampling, d.kernels, d.supports, d.uvw, d.vis, cfg)
+    ref32 = oracle.std_gridding(d.grid_size, d.oversampling, d.kernels, d.supports, d.uvw, d.vis, cfg)
+    ref64 = oracle.std_gridding_f64(d.grid_size, d.oversampling, d.kernels, d.supports, d.uvw, d.vis,
+                                    d.uv_scale, d.w_scale)
+    peak = np.abs(ref64).max()
+    err_gpu = np.abs(got - ref64).max()
+    err_ref = np.abs(ref32 - ref64).max()
+    assert err_gpu <= 1e-6 * peak
+    assert err_gpu <= err_ref + 1e-7 * peak
+    np.testing.assert_allclose(got, ref32, atol=2e-6 * peak)
+
+
+def test_grid_device_api_plan_order_and_determinism(hip, case):
+    d = case
+    G = d.grid_size
+    uvw = hip.DeviceBuffer.from_array(d.uvw)
+    vis = hip.DeviceBuffer.from_array(d.vis)
+    kern = hip.DeviceBuffer.from_array(d.kernels)
+    plan = hip.GridPlan(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, uvw, d.n_vis)
+    info = plan.info()
+    assert info["kept"] + info["dropped"] == d.n_vis
+    g1 = hip.DeviceBuffer(8 * G * G)
+    g2 = hip.DeviceBuffer(8 * G * G)
+    plan.grid(kern, vis, g1)
+    vis_p = hip.DeviceBuffer(8 * d.n_vis)
+    plan.to_plan_order(vis, vis_p)
+    plan.grid(kern, vis_p, g2, order=hip.ORDER_PLAN)
+    a = g1.download((G * G, 2), np.float32)
+    b = g2.download((G * G, 2), np.float32)
+    np.testing.assert_array_equal(a, b)
+    plan.grid(kern, vis, g2)
+    np.testing.assert_array_equal(a, g2.download((G * G, 2), np.float32))
+    back = hip.DeviceBuffer(8 * d.n_vis)
+    plan.to_original_order(vis_p, back)
+    np.testing.assert_array_equal(back.download((d.n_vis, 2), np.float32), d.vis)
+
+
+@pytest.mark.parametrize("tile,chunk", [(16, 2000), (40, 1 << 16), (64, 5000)])
+def test_grid_tilings_agree(hip, oracle, case, tile, chunk):
+    d = case
+    G = d.grid_size
+    uvw = hip.DeviceBuffer.from_array(d.uvw)
+    vis = hip.DeviceBuffer.from_array(d.vis)
+    kern = hip.DeviceBuffer.from_array(d.kernels)
+    plan = hip.GridPlan(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, uvw, d.n_vis, tile=tile, chunk=chunk)
+    g = hip.DeviceBuffer(8 * G * G)
+    plan.grid(kern, vis, g)
+    ref64 = oracle.std_gridding_f64(G, d.oversampling, d.kernels, d.supports, d.uvw, d.vis, d.uv_scale, d.w_scale)
+    assert np.abs(g.download((G * G, 2), np.float32) - ref64).max() <= 1e-6 * np.abs(ref64).max()
+
+
+# ---------------------------------------------------------------- degridding
+
+def test_degrid_nearest_bit_exact(hip, actors, oracle, case, rng):
+    d = case
+    cfg = cfg_of(hip, d)
+    grid = rng.normal(size=(d.grid_size ** 2, 2)).astype(np.float32)
+    n = d.n_vis - 100
+    actors.set_degrid_mode(hip.DEGRID_NEAREST)
+    got = actors.std_degridding(d.grid_size, d.oversampling, d.kernels, d.supports, grid, d.uvw, cfg, n=n)
+    ref = oracle.std_degridding(d.grid_size, d.oversampling, d.kernels, d.supports, grid, d.uvw, cfg, n=n)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_degrid_weighted_accuracy_and_adjointness(hip, actors, oracle, case, rng):
+    d = case
+    G = d.grid_size
+    cfg = cfg_of(hip, d)
+    grid = rng.normal(size=(G * G, 2)).astype(np.float32)
+    actors.set_degrid_mode(hip.DEGRID_WEIGHTED)
+    try:
+        got = actors.std_degridding(G, d.oversampling, d.degrid_kernels, d.supports, grid, d.uvw, cfg)
+    finally:
+        actors.set_degrid_mode(hip.DEGRID_NEAREST)
+    ref = oracle.degrid_weighted_f64(G, d.oversampling, d.degrid_kernels, d.supports, grid, d.uvw, d.uv_scale,
+                                     d.w_scale)
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 2e-6 * scale
+    # <G x, y> == <x, D y> with D using the conjugate table
+    x = d.vis
+    gx = actors.std_gridding(G, d.oversampling, d.kernels, d.supports, d.uvw, x, cfg).astype(np.float64)
+    c = lambda a: a[..., 0] + 1j * a[..., 1]
+    lhs = np.vdot(c(gx), c(grid.astype(np.float64)))
+    rhs = np.vdot(c(x.astype(np.float64)), c(got.astype(np.float64)))
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+def test_degrid_fused_subtract(hip, case, rng):
+    d = case
+    G = d.grid_size
+    uvw = hip.DeviceBuffer.from_array(d.uvw)
+    kern = hip.DeviceBuffer.from_array(d.degrid_kernels)
+    grid = hip.DeviceBuffer.from_array(rng.normal(size=(G * G, 2)).astype(np.float32))
+    meas = hip.DeviceBuffer.from_array(d.vis)
+    plan = hip.GridPlan(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, uvw, d.n_vis)
+    pred = hip.DeviceBuffer(8 * d.n_vis)
+    delta = hip.DeviceBuffer(8 * d.n_vis)
+    plan.degrid(kern, grid, pred)
+    plan.degrid(kern, grid, delta, d_measured=meas)
+    p = pred.download((d.n_vis, 2), np.float32)
+    np.testing.assert_array_equal(delta.download((d.n_vis, 2), np.float32), d.vis - p)
+
+
+# ---------------------------------------------------------------- fine grid
+
+def test_finegrid_bit_exact(hip, actors, oracle, case):
+    d = case
+    cfg = cfg_of(hip, d)
+    uvw = np.concatenate([d.uvw, d.uvw[:500] + np.float32(1e-4), d.uvw[:50]], axis=0)   # near and exact repeats
+    vis = np.concatenate([d.vis, d.vis[:500], d.vis[:50] * 2], axis=0)
+    for simplify in (0, 1):
+        gv, gu, gn = actors.correct_to_finegrid(d.grid_size, d.oversampling, simplify, uvw, vis, cfg)
+        rv, ru, rn = oracle.correct_to_finegrid(d.grid_size, d.oversampling, simplify, uvw, vis, cfg)
+        assert gn == rn
+        if simplify:
+            assert rn < uvw.shape[0]
+        np.testing.assert_array_equal(gv, rv)
+        np.testing.assert_array_equal(gu[:rn], ru[:rn])
+
+
+# ---------------------------------------------------------------- FFT
+
+@pytest.mark.parametrize("G", [16, 32, 64, 128, 256, 512, 1024, 2048])
+def test_fft_against_fp64(actors, rng, G):
+    x = rng.normal(size=(G * G, 2)).astype(np.float32)
+    z = (x[:, 0] + 1j * x[:, 1]).astype(np.complex128).reshape(G, G)
+    for got, ref in ((actors.fft_forward(G, x), np.fft.fft2(z)), (actors.fft_inverse(G, x), np.fft.ifft2(z) * G * G)):
+        ref = ref.ravel()
+        err = np.abs((got[:, 0] + 1j * got[:, 1]) - ref).max()
+        assert err <= 3e-7 * np.log2(G) * np.abs(ref).max(), (G, err)
+
+
+def test_fft_fused_equals_actor_chain(hip, actors, oracle, rng):
+    G = 256
+    cfg = hip.make_config(G, 1e-3)
+    grid = rng.normal(size=(G * G, 2)).astype(np.float32)
+    taper = sdp_synth.taper_padded(G)
+    chain = actors.convolution_correction(
+        G, actors.shift_c2r(G, actors.fft_inverse(G, actors.shift_c2c(G, grid, cfg)), cfg), taper[:-1], cfg)
+    d_in = hip.DeviceBuffer.from_array(grid)
+    d_out = hip.DeviceBuffer(4 * G * G)
+    d_tap = hip.DeviceBuffer.from_array(taper)
+    d_w = hip.DeviceBuffer(hip.fft_work_bytes(G))
+    hip.fft2d(d_in, d_out, G, hip.FFT_INVERSE,
+              hip.FFT_SHIFT_IN | hip.FFT_SHIFT_OUT | hip.FFT_REAL_OUT | hip.FFT_CORRECT_OUT, d_tap, d_w)
+    np.testing.assert_array_equal(d_out.download(G * G, np.float32), chain)
+    img = rng.normal(size=G * G).astype(np.float32)
+    chain2 = actors.shift_c2c(G, actors.fft_forward(G, actors.shift_r2c(G, img, cfg)), cfg)
+    d_img = hip.DeviceBuffer.from_array(img)
+    d_g = hip.DeviceBuffer(8 * G * G)
+    hip.fft2d(d_img, d_g, G, hip.FFT_FORWARD, hip.FFT_REAL_IN | hip.FFT_SHIFT_IN | hip.FFT_SHIFT_OUT, None, d_w)
+    np.testing.assert_array_equal(d_g.download((G * G, 2), np.float32), chain2)
+
+
+def test_fft_in_place_and_errors(hip, rng):
+    G = 64
+    x = rng.normal(size=(G * G, 2)).astype(np.float32)
+    a = hip.DeviceBuffer.from_array(x)
+    w = hip.DeviceBuffer(hip.fft_work_bytes(G))
+    hip.fft2d(a, a, G, hip.FFT_FORWARD, 0, None, w)
+    hip.fft2d(a, a, G, hip.FFT_INVERSE, 0, None, w)
+    np.testing.assert_allclose(a.download((G * G, 2), np.float32) / (G * G), x, atol=1e-5)
+    with pytest.raises(hip.SdpError, match="unsupported"):
+        hip.fft2d(a, a, 96, hip.FFT_FORWARD, 0, None, w)
+
+
+# ---------------------------------------------------------------- element-wise
+
+def test_elementwise_bit_exact(hip, actors, oracle, rng):
+    G = 128
+    cfg = hip.make_config(G, 1e-3)
+    grid = rng.normal(size=(G * G, 2)).astype(np.float32)
+    img = rng.normal(size=G * G).astype(np.float32)
+    np.testing.assert_array_equal(actors.shift_c2c(G, grid, cfg), oracle.shift_c2c(G, grid))
+    np.testing.assert_array_equal(actors.shift_c2r(G, grid, cfg), oracle.shift_c2r(G, grid))
+    np.testing.assert_array_equal(actors.shift_r2c(G, img, cfg), oracle.shift_r2c(G, img))
+    taper = oracle.half_prolate(G)
+    np.testing.assert_array_equal(actors.convolution_correction(G, img, taper, cfg),
+                                  oracle.convolution_correction(G, img, np.concatenate([taper, [0]])))
+    grids = rng.normal(size=(4 * G * G, 2)).astype(np.float32)
+    np.testing.assert_array_equal(actors.merge(G, 4, grids), oracle.merge(G, 4, grids))
+    m = rng.normal(size=(5000, 2)).astype(np.float32)
+    p = rng.normal(size=(5000, 2)).astype(np.float32)
+    gains = rng.normal(size=(10, 2)).astype(np.float32)
+    pairs = sdp_synth.receiver_pairs(10)
+    np.testing.assert_array_equal(actors.subtract_from_measurements(m, p, gains, pairs, cfg),
+                                  oracle.subtract_from_measurements(m, p, gains, pairs))
+    np.testing.assert_array_equal(actors.apply_gains_only(m, gains, pairs, cfg),
+                                  oracle.apply_gains_only(m, gains, pairs))
+    np.testing.assert_array_equal(actors.add_visibilities(m, p), oracle.add_visibilities(m, p))
+    np.testing.assert_array_equal(actors.subtract_finegrid(m, p, n=4000), oracle.subtract_finegrid(m, p, n=4000))
+
+
+# ---------------------------------------------------------------- Hogbom
+
+def dirty_case(G, rng, n_src=6):
+    yy, xx = np.mgrid[0:G, 0:G]
+    c = G // 2
+    r2 = (yy - c) ** 2 + (xx - c) ** 2
+    psf = (np.exp(-r2 / 8.0) * (1 + 0.3 * np.cos(0.7 * (xx - c)))).astype(np.float32)
+    psf /= psf.max()
+    psf_padded = np.concatenate([psf.ravel(), [0.0]]).astype(np.float32)
+    res = np.zeros((G, G), np.float64)
+    for _ in range(n_src):
+        y, x = rng.integers(G // 4, 3 * G // 4, 2)
+        f = rng.uniform(0.2, 2.0)
+        res += f * np.roll(np.roll(psf, y - c, 0), x - c, 1)
+    res += rng.normal(0, 0.003, res.shape)
+    return res.astype(np.float32).ravel(), psf_padded
+
+
+@pytest.mark.parametrize("G,n_minor", [(64, 60), (128, 200)])
+def test_hogbom_bit_exact(hip, actors, oracle, rng, G, n_minor):
+    res, psf = dirty_case(G, rng)
+    cfg = hip.make_config(G, 1e-4, psf_max_value=1.0)
+    model = np.zeros(G * G, np.float32)
+    ns, src, out_model = actors.hogbom_clean(G, n_minor, 400, res, psf[:-1].copy(), cfg, model)
+    rns, rsrc, rmodel, _ = oracle.hogbom_clean(G, n_minor, 400, res, psf, cfg, model)
+    assert ns == rns and ns > 10
+    np.testing.assert_array_equal(src, rsrc)
+    np.testing.assert_array_equal(out_model, rmodel)
+
+
+def test_hogbom_quirks(hip, oracle, rng):
+    """Mirrored PSF read, the padded PSF element, ties, both stop tests: the
+    device path against the oracle, residual included."""
+    G = 16
+    cases = []
+    psf = np.zeros(G * G + 1, np.float32)
+    psf[(G // 2) * G + G // 2] = 1.0
+    psf[(G // 2) * G + G // 2 + 1] = 0.5
+    psf[(G - 1) * G + 3] = 0.25
+    r = np.zeros(G * G, np.float32)
+    r[7 * G + 12] = 2.0          # mx >= G/2 with my <= G/2 reads the padded element
+    r[9 * G + 2] = -1.5
+    cases.append((r, psf, dict(psf_max_value=1.0), 5))
+    r = np.zeros(G * G, np.float32)
+    r[20] = -1.0
+    r[40] = 1.0
+    cases.append((r, psf, dict(psf_max_value=2.0), 3))
+    r = np.ones(G * G, np.float32)
+    r[0] = 0.0
+    r[50] = 1.5
+    cases.append((r, psf, dict(psf_max_value=1.0, noise_factor=20.0), 5))
+    r = np.zeros(G * G, np.float32)
+    r[30] = 4.0
+    r[90] = 0.9
+    cases.append((r, psf, dict(psf_max_value=1.0, loop_gain=0.5, weak_source_percent_img=0.5), 10))
+    cases.append((np.zeros(G * G, np.float32), psf, dict(psf_max_value=1.0), 3))
+    for r, p, kw, n_minor in cases:
+        cfg = hip.make_config(G, 1e-3, **kw)
+        rns, rsrc, rmodel, rres = oracle.hogbom_clean(G, n_minor, 8, r, p, cfg, np.zeros(G * G, np.float32))
+        d_res = hip.DeviceBuffer.from_array(r)
+        d_psf = hip.DeviceBuffer.from_array(p)
+        d_m = hip.DeviceBuffer.from_array(np.zeros(G * G, np.float32))
+        d_mo = hip.DeviceBuffer(4 * G * G)
+        d_src = hip.DeviceBuffer(12 * 8)
+        d_n = hip.DeviceBuffer(16)
+        d_w = hip.DeviceBuffer(hip.hogbom_work_bytes(G))
+        hip.hogbom(G, n_minor, 8, d_res, d_psf, cfg.psf_max_value, cfg.loop_gain, cfg.noise_factor,
+                   cfg.weak_source_percent_img, cfg.cell_size, d_m, d_mo, d_src, d_n, d_w)
+        ns = int(d_n.download(1, np.int32)[0])
+        assert ns == rns, kw
+        np.testing.assert_array_equal(d_src.download((8, 3), np.float32)[:ns], rsrc)
+        np.testing.assert_array_equal(d_mo.download(G * G, np.float32), rmodel)
+        np.testing.assert_array_equal(d_res.download(G * G, np.float32), rres)
+
+
+# ---------------------------------------------------------------- DFT
+
+def test_dft_modes(hip, actors, oracle, rng):
+    d = sdp_synth.config1(n_time=8, with_vis=False)
+    cfg = cfg_of(hip, d)
+    src = d.sources
+    ref32 = oracle.dft_actor(src, d.uvw, cfg)
+    ref64 = oracle.dft_f64(src, d.uvw)
+    scale = np.abs(src[:, 2]).sum()
+    actors.set_dft_mode(hip.DFT_REFERENCE_FP32)
+    try:
+        got_ref = actors.dft_actor(src, d.uvw, cfg)
+    finally:
+        actors.set_dft_mode(hip.DFT_FP64_PHASE)
+    got64 = actors.dft_actor(src, d.uvw, cfg)
+    # same float phase as the reference: only sinf/cosf ulps and summation differ
+    assert np.abs(got_ref - ref32).max() <= 2e-6 * scale
+    # FP64 phase: float rounding of the reduced phase and the float sum only
+    assert np.abs(got64 - ref64).max() <= 2e-6 * scale
+    # the reference's float phase is itself this far from FP64 (stated, not a bar)
+    assert np.abs(ref32 - ref64).max() < 1e-2 * scale

@@ -1,0 +1,344 @@
+"""Pins the CPU oracle (oracle/sdp_oracle.c) without the reference binary.
+
+The reference ships no tests, golden vectors or fixtures for this path and
+its C actors cannot be compiled here (common.h:31 includes cblas.h, fft_run.c
+includes fftw3.h; neither exists in this image).  Each oracle function is
+therefore checked against something fixed independently of its code:
+hand-derived known answers, closed forms, invariants, an independent FFT
+(numpy) and brute force on tiny inputs.  A dropped term, swapped index or
+wrong sign in the oracle fails at least one of these.
+"""
+import numpy as np
+import pytest
+
+import sdp_hip  # only for the Config record layout (no GPU calls)
+
+
+def cfg_for(G, cell=None, n_kernels=1, max_w=1.0, **kw):
+    cell = (1.0 / G) if cell is None else cell
+    return sdp_hip.make_config(G, cell, num_kernels=n_kernels, max_w=max_w, **kw)
+
+
+def index_kernel(h, OS):
+    """kernel[k] = (k, 0.5): every tap reveals the table index it used."""
+    n = ((h + 1) * OS) ** 2
+    k = np.stack([np.arange(n, dtype=np.float32), np.full(n, 0.5, np.float32)], axis=1)
+    return k, np.array([[h, 0]], np.int32)
+
+
+# ---------------------------------------------------------------- gridding
+
+def test_gridding_single_visibility_known_answer(oracle):
+    G, OS, h = 64, 16, 3
+    kern, sup = index_kernel(h, OS)
+    cfg = cfg_for(G)                       # uv_scale = 1: grid position = u
+    uvw = np.array([[10.25, -3.5, 0.0]], np.float32)
+    vis = np.array([[1.0, 0.0]], np.float32)
+    grid = oracle.std_gridding(G, OS, kern, sup, uvw, vis, cfg).reshape(G, G, 2)
+    # u in [ceil(7.25), ceil(13.25)) = 8..13, |u - 10.25| * 16 = 36,20,4,12,28,44
+    kx = {8: 36, 9: 20, 10: 4, 11: 12, 12: 28, 13: 44}
+    # v in [ceil(-6.5), ceil(-0.5)) = -6..-1, |v + 3.5| * 16 = 40,24,8,8,24,40
+    ky = {-6: 40, -5: 24, -4: 8, -3: 8, -2: 24, -1: 40}
+    expect = np.zeros((G, G, 2), np.float32)
+    for v, a in ky.items():
+        for u, b in kx.items():
+            expect[v + 32, u + 32] = (a * 64 + b, 0.5)
+    np.testing.assert_array_equal(grid, expect)
+
+
+def test_gridding_rounds_half_away_from_zero(oracle):
+    G, OS, h = 64, 16, 3
+    kern, sup = index_kernel(h, OS)
+    cfg = cfg_for(G)
+    # (10 - 10.03125) * 16 = -0.5 -> roundf gives -1 -> kx = 1 (banker's rounding would give 0)
+    uvw = np.array([[10.03125, 0.0, 0.0]], np.float32)
+    grid = oracle.std_gridding(G, OS, kern, sup, uvw, np.array([[1.0, 0.0]], np.float32), cfg).reshape(G, G, 2)
+    assert grid[32, 42, 0] == 0 * 64 + 1
+
+
+def test_gridding_conjugates_for_negative_w_and_skips_edge(oracle):
+    G, OS, h = 64, 16, 3
+    kern, sup = index_kernel(h, OS)
+    cfg = cfg_for(G)
+    uvw = np.array([[30.5, 0.0, -1.0]], np.float32)   # footprint 28..33, cells >= 32 dropped
+    grid = oracle.std_gridding(G, OS, kern, sup, uvw, np.array([[1.0, 0.0]], np.float32), cfg).reshape(G, G, 2)
+    cols = np.nonzero(grid[32, :, 0])[0] - 32
+    assert list(cols) == [28, 29, 30, 31]
+    assert np.all(grid[grid[..., 0] != 0][:, 1] == -0.5)
+
+
+def test_gridding_plane_selection(oracle):
+    G, OS = 64, 4
+    sup = np.array([[1, 0], [2, 64], [3, 64 + 144]], np.int32)   # offsets sum ((h+1)*OS)^2
+    n = 64 + 144 + 256
+    kern = np.stack([np.arange(n, dtype=np.float32), np.zeros(n, np.float32)], axis=1)
+    # w_scale = (3 - 1)^2 / max_w = 1 with max_w = 4
+    cfg = cfg_for(G, n_kernels=3, max_w=4.0)
+    for w, plane in ((2.25, 2), (2.2, 1), (0.2, 0), (0.3, 1)):
+        g = oracle.std_gridding(G, OS, kern, sup, np.array([[0.0, 0.0, w]], np.float32),
+                                np.array([[1.0, 0.0]], np.float32), cfg).reshape(G, G, 2)
+        nz = g[..., 0][g[..., 0] != 0]
+        lo, hi = sup[plane, 1], sup[plane, 1] + ((sup[plane, 0] + 1) * OS) ** 2
+        assert np.all((nz >= lo) & (nz < hi)), (w, plane)
+        assert np.count_nonzero(g[..., 0]) + (1 if g[32, 32, 0] == 0 else 0) == (2 * sup[plane, 0]) ** 2
+
+
+def test_gridding_linear_and_bypass(oracle, rng):
+    G, OS = 64, 16
+    from sdp_synth import kernel_tables
+    kern, _, sup = kernel_tables([3], OS, fov=1.0, w_scale=0.0)
+    cfg = cfg_for(G)
+    uvw = np.concatenate([rng.uniform(-25, 25, (50, 2)), rng.uniform(-1, 1, (50, 1))], axis=1).astype(np.float32)
+    a = rng.normal(size=(50, 2)).astype(np.float32)
+    b = rng.normal(size=(50, 2)).astype(np.float32)
+    ga = oracle.std_gridding_f64(G, OS, kern, sup, uvw, a, 1.0, 0.0)
+    gb = oracle.std_gridding_f64(G, OS, kern, sup, uvw, b, 1.0, 0.0)
+    gab = oracle.std_gridding_f64(G, OS, kern, sup, uvw, (a + b).astype(np.float32), 1.0, 0.0)
+    # the float a+b rounds, so compare against that rounding's size
+    np.testing.assert_allclose(gab, ga + gb, atol=1e-5 * np.abs(ga + gb).max())
+    f32 = oracle.std_gridding(G, OS, kern, sup, uvw, a, cfg)
+    np.testing.assert_allclose(f32, ga, atol=1e-5 * np.abs(ga).max())
+    prev = rng.normal(size=(G * G, 2)).astype(np.float32)
+    out = oracle.std_gridding(G, OS, kern, sup, uvw, a, cfg, bypass=1, maj_iter=1, prev_grid=prev)
+    np.testing.assert_array_equal(out, prev)
+
+
+# ---------------------------------------------------------------- degridding
+
+def test_degridding_nearest_known_answer(oracle, rng):
+    G, OS = 64, 16
+    kern, sup = index_kernel(3, OS)
+    cfg = cfg_for(G)
+    grid = rng.normal(size=(G * G, 2)).astype(np.float32)
+    uvw = np.array([[5.4, -3.6, 0.0], [0.0, 0.0, 0.0]], np.float32)
+    out = oracle.std_degridding(G, OS, kern, sup, grid, uvw, cfg, n=2, n_total=3)
+    # (int)((5.4f + 32) + 0.5) = 37, (int)((-3.6f + 32) + 0.5) = 28
+    np.testing.assert_array_equal(out[0], grid[37 + 28 * G])
+    np.testing.assert_array_equal(out[1], grid[32 + 32 * G])
+    np.testing.assert_array_equal(out[2], [0.0, 0.0])
+
+
+def test_weighted_degridding_is_adjoint_of_gridding(oracle, rng):
+    from sdp_synth import kernel_tables
+    G, OS = 64, 8
+    gk, dk, sup = kernel_tables([3, 4, 5], OS, fov=0.2, w_scale=(3 - 1) ** 2 / 300.0)
+    n = 300
+    uvw = np.concatenate([rng.uniform(-30, 30, (n, 2)) / 0.2, rng.uniform(-300, 300, (n, 1))],
+                         axis=1).astype(np.float32)
+    x = rng.normal(size=(n, 2)).astype(np.float32)
+    y = rng.normal(size=(G * G, 2)).astype(np.float32)
+    gx = oracle.std_gridding_f64(G, OS, gk, sup, uvw, x, 0.2, (3 - 1) ** 2 / 300.0)
+    dy = oracle.degrid_weighted_f64(G, OS, dk, sup, y, uvw, 0.2, (3 - 1) ** 2 / 300.0)
+    c = lambda a: a[..., 0] + 1j * a[..., 1]
+    lhs = np.vdot(c(gx), c(y.astype(np.float64)))
+    rhs = np.vdot(c(x.astype(np.float64)), c(dy))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    # a wrong conjugation rule breaks it
+    dy_bad = oracle.degrid_weighted_f64(G, OS, gk, sup, y, uvw, 0.2, (3 - 1) ** 2 / 300.0)
+    assert abs(np.vdot(c(x.astype(np.float64)), c(dy_bad)) - lhs) > 1e-3 * abs(lhs)
+
+
+# ---------------------------------------------------------------- fine grid
+
+def test_finegrid_known_answer_and_simplification(oracle):
+    G, OS = 64, 16
+    cfg = cfg_for(G)
+    uvw = np.array([[3.3, -3.3, 1.0], [3.31, -3.3, 1.0], [-7.0, 2.0, 1.0]], np.float32)
+    vis = np.array([[1.0, 2.0], [0.5, 0.25], [3.0, 4.0]], np.float32)
+    ov, ou, n = oracle.correct_to_finegrid(G, OS, 0, uvw, vis, cfg)
+    assert n == 3
+    np.testing.assert_array_equal(ou[0], [3.3125, -3.3125, 1.0])
+    np.testing.assert_array_equal(ov, vis)
+    ov, ou, n = oracle.correct_to_finegrid(G, OS, 1, uvw, vis, cfg)
+    assert n == 2
+    np.testing.assert_array_equal(ov[:2], [[1.5, 2.25], [3.0, 4.0]])
+    np.testing.assert_array_equal(ou[:2], [[3.3125, -3.3125, 1.0], [-7.0, 2.0, 1.0]])
+    np.testing.assert_array_equal(ov[2], [0.0, 0.0])
+
+
+# ---------------------------------------------------------------- DFT
+
+def test_dft_known_answers(oracle, rng):
+    cfg = cfg_for(64)
+    uvw = rng.uniform(-100, 100, (64, 3)).astype(np.float32)
+    out = oracle.dft_actor(np.array([[0.0, 0.0, 0.7]], np.float32), uvw, cfg)
+    np.testing.assert_array_equal(out, np.tile([0.7, 0.0], (64, 1)).astype(np.float32))
+    # w = 0: V = f / (1 - r^2/2) exp(-2 pi i (u l + v m))
+    l, m, f = 0.01, -0.02, 0.9
+    uvw0 = uvw.copy()
+    uvw0[:, 2] = 0
+    expect = f / (1 - 0.5 * (l * l + m * m)) * np.exp(-2j * np.pi * (uvw0[:, 0] * l + uvw0[:, 1] * m))
+    for fn in (lambda: oracle.dft_actor(np.array([[l, m, f]], np.float32), uvw0, cfg),
+               lambda: oracle.dft_f64(np.array([[l, m, f]], np.float32), uvw0)):
+        got = fn()
+        np.testing.assert_allclose(got[:, 0] + 1j * got[:, 1], expect, atol=2e-5)
+    # u = v = 0: V = f / (1 - r^2/2) exp(+i pi w r^2)
+    uvww = np.zeros((16, 3), np.float32)
+    uvww[:, 2] = np.linspace(-500, 500, 16)
+    r2 = l * l + m * m
+    expect = f / (1 - r2 / 2) * np.exp(1j * np.pi * uvww[:, 2] * r2)
+    got = oracle.dft_f64(np.array([[l, m, f]], np.float32), uvww)
+    np.testing.assert_allclose(got[:, 0] + 1j * got[:, 1], expect, atol=1e-6)
+    # superposition
+    s = np.array([[l, m, f], [0.003, 0.004, 0.2]], np.float32)
+    both = oracle.dft_f64(s, uvw)
+    sep = oracle.dft_f64(s[:1], uvw) + oracle.dft_f64(s[1:], uvw)
+    np.testing.assert_allclose(both, sep, atol=1e-12)
+
+
+# ---------------------------------------------------------------- Hogbom
+
+def delta_psf(G, extra=None):
+    psf = np.zeros(G * G + 1, np.float32)
+    psf[(G // 2) * G + G // 2] = 1.0
+    if extra:
+        for (dy, dx), v in extra.items():
+            psf[(G // 2 + dy) * G + G // 2 + dx] = v
+    return psf
+
+
+def test_hogbom_delta_psf_geometric_fluxes(oracle):
+    G = 16
+    cfg = cfg_for(G, cell=0.01, psf_max_value=1.0)
+    res = np.zeros(G * G, np.float32)
+    res[7 * G + 5] = 2.0
+    ns, src, model, r = oracle.hogbom_clean(G, 3, 10, res, delta_psf(G), cfg, np.zeros(G * G, np.float32))
+    assert ns == 3
+    np.testing.assert_allclose(src[:, 2], [0.2, 0.18, 0.162], rtol=1e-6)
+    np.testing.assert_allclose(src[:, 0], (5 - 8) * 0.01, rtol=1e-6)
+    np.testing.assert_allclose(src[:, 1], (7 - 8) * 0.01, rtol=1e-6)
+    np.testing.assert_allclose(model[7 * G + 5], 0.542, rtol=1e-6)
+    np.testing.assert_allclose(r[7 * G + 5], 2.0 * 0.9 ** 3, rtol=1e-6)
+
+
+def test_hogbom_psf_is_read_mirrored_in_x(oracle):
+    G = 16
+    cfg = cfg_for(G, psf_max_value=1.0)
+    res = np.zeros(G * G, np.float32)
+    res[7 * G + 5] = 2.0
+    psf = delta_psf(G, {(0, 1): 0.5})         # lobe one pixel right of the centre
+    _, _, _, r = oracle.hogbom_clean(G, 1, 10, res, psf, cfg, np.zeros(G * G, np.float32))
+    # column index G - (x - mx + G/2) puts it one pixel LEFT of the peak
+    np.testing.assert_allclose(r[7 * G + 4], -0.1, rtol=1e-6)
+    assert r[7 * G + 6] == 0.0
+
+
+def test_hogbom_tie_break_and_stops(oracle):
+    G = 16
+    cfg = cfg_for(G, psf_max_value=1.0)
+    res = np.zeros(G * G, np.float32)
+    res[20] = -1.0
+    res[40] = 1.0
+    ns, src, _, _ = oracle.hogbom_clean(G, 1, 4, res, delta_psf(G), cfg, np.zeros(G * G, np.float32))
+    assert ns == 1 and src[0, 2] == pytest.approx(-0.1)
+    # noise stop (noise_factor * gain > 1): peak 1.5 below 2 * mean(|r[1:]|) ~ 2.0
+    cfgn = cfg_for(G, psf_max_value=1.0, noise_factor=20.0)
+    res = np.ones(G * G, np.float32)
+    res[0] = 0.0
+    res[50] = 1.5
+    ns, _, _, _ = oracle.hogbom_clean(G, 5, 4, res, delta_psf(G), cfgn, np.zeros(G * G, np.float32))
+    assert ns == 0
+    # weak-source stop: |max| < |first flux| * weak
+    cfgw = cfg_for(G, psf_max_value=1.0, loop_gain=0.5, weak_source_percent_img=0.5)
+    res = np.zeros(G * G, np.float32)
+    res[30] = 4.0
+    res[90] = 0.9
+    ns, src, _, _ = oracle.hogbom_clean(G, 10, 10, res, delta_psf(G), cfgw, np.zeros(G * G, np.float32))
+    assert ns == 3
+    np.testing.assert_allclose(src[:, 2], [2.0, 1.0, 0.5])
+
+
+# ---------------------------------------------------------------- FFT
+
+@pytest.mark.parametrize("G", [8, 16])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft_matches_definition(oracle, rng, G, sign):
+    x = rng.normal(size=(G * G, 2))
+    np.testing.assert_allclose(oracle.fft2d_f64(G, x, sign), oracle.dft2d_direct_f64(G, x, sign), atol=1e-9)
+
+
+def test_fft_matches_numpy_and_known_answers(oracle, rng):
+    G = 64
+    x = rng.normal(size=(G * G, 2))
+    z = (x[:, 0] + 1j * x[:, 1]).reshape(G, G)
+    fwd = oracle.fft2d_f64(G, x, -1)
+    np.testing.assert_allclose(fwd[:, 0] + 1j * fwd[:, 1], np.fft.fft2(z).ravel(), atol=1e-9)
+    inv = oracle.fft2d_f64(G, x, +1)
+    np.testing.assert_allclose(inv[:, 0] + 1j * inv[:, 1], (np.fft.ifft2(z) * G * G).ravel(), atol=1e-9)
+    # impulse -> ones; plane wave -> G^2 delta; Parseval; round trip
+    imp = np.zeros((G * G, 2))
+    imp[0, 0] = 1
+    np.testing.assert_allclose(oracle.fft2d_f64(G, imp, -1), np.tile([1.0, 0.0], (G * G, 1)), atol=1e-12)
+    yy, xx = np.mgrid[0:G, 0:G]
+    pw = np.exp(2j * np.pi * (3 * yy + 5 * xx) / G).ravel()
+    X = oracle.fft2d_f64(G, np.stack([pw.real, pw.imag], 1), -1)
+    mag = np.hypot(X[:, 0], X[:, 1]).reshape(G, G)
+    assert mag[3, 5] == pytest.approx(G * G) and np.sum(mag > 1e-6) == 1
+    assert np.sum(fwd ** 2) == pytest.approx(G * G * np.sum(x ** 2), rel=1e-12)
+    np.testing.assert_allclose(oracle.fft2d_f64(G, fwd, +1) / (G * G), x, atol=1e-12)
+
+
+# ---------------------------------------------------------------- element-wise
+
+def test_shifts_and_correction(oracle, rng):
+    G = 32
+    g = rng.normal(size=(G * G, 2)).astype(np.float32)
+    s = oracle.shift_c2c(G, g)
+    np.testing.assert_array_equal(oracle.shift_c2c(G, s), g)
+    assert s[0, 0] == g[0, 0] and s[1, 0] == -g[1, 0] and s[G + 1, 0] == g[G + 1, 0] and s[G, 0] == -g[G, 0]
+    np.testing.assert_array_equal(oracle.shift_c2r(G, g), s[:, 0])
+    img = rng.normal(size=G * G).astype(np.float32)
+    rc = oracle.shift_r2c(G, img)
+    np.testing.assert_array_equal(rc[:, 0], oracle.shift_c2r(G, np.stack([img, np.zeros_like(img)], 1)))
+    assert np.all(rc[:, 1] == 0)
+    taper = np.concatenate([oracle.half_prolate(G), [0.0]]).astype(np.float32)
+    out = oracle.convolution_correction(G, img, taper).reshape(G, G)
+    t2 = np.outer(taper[np.abs(np.arange(G) - G // 2)], taper[np.abs(np.arange(G) - G // 2)])
+    ok = t2 > 1e-10
+    np.testing.assert_allclose(out[ok] * t2[ok], img.reshape(G, G)[ok], rtol=1e-6)
+    assert np.all(out[0, :] == 0) and np.all(out[:, 0] == 0)
+
+
+def test_correction_zeroes_distance_1229(oracle):
+    G = 2460
+    img = np.ones(G * G, np.float32)
+    taper = np.concatenate([oracle.half_prolate(G), [0.0]]).astype(np.float32)
+    out = oracle.convolution_correction(G, img, taper).reshape(G, G)
+    assert np.all(out[G // 2 - 1229, :] == 0) and np.all(out[:, G // 2 + 1229] == 0)
+    assert out[G // 2, G // 2] == pytest.approx(1.0 / taper[0] ** 2)
+
+
+def test_prolate_published_values(oracle):
+    # the approximation is normalised to psi(0) = 1 (to its published precision)
+    assert oracle.prolate_spheroidal(0.0) == pytest.approx(1.0, abs=1e-6)
+    assert oracle.prolate_spheroidal(1.0) == 0.0 and oracle.prolate_spheroidal(1.5) == 0.0
+    # the two rational branches meet at nu = 0.75
+    assert oracle.prolate_spheroidal(0.75 - 1e-9) == pytest.approx(oracle.prolate_spheroidal(0.75), rel=2e-3)
+    # psi decreases monotonically on [0, 1)
+    v = [oracle.prolate_spheroidal(x) for x in np.linspace(0, 0.999, 200)]
+    assert np.all(np.diff(v) < 0)
+
+
+def test_visibility_arithmetic(oracle, rng):
+    m = rng.normal(size=(12, 2)).astype(np.float32)
+    p = rng.normal(size=(12, 2)).astype(np.float32)
+    pairs = np.array([[0, 1], [0, 2], [1, 2]], np.int32)
+    ones = np.tile([1.0, 0.0], (3, 1)).astype(np.float32)
+    np.testing.assert_array_equal(oracle.subtract_from_measurements(m, p, ones, pairs), m - p)
+    gi = np.array([[0.0, 1.0], [1.0, 0.0], [1.0, 0.0]], np.float32)   # g0 = i
+    got = oracle.apply_gains_only(m, gi, pairs)
+    mc = m[:, 0] + 1j * m[:, 1]
+    expect = np.where(np.arange(12) % 3 < 2, mc * 1j, mc)               # baselines 0,1 use antenna 0
+    np.testing.assert_allclose(got[:, 0] + 1j * got[:, 1], expect, atol=1e-7)
+    np.testing.assert_array_equal(oracle.add_visibilities(m, p), m + p)
+    d = oracle.subtract_finegrid(m, p, n=5)
+    np.testing.assert_array_equal(d[:5], m[:5] - p[:5])
+    assert np.all(d[5:] == 0)
+
+
+def test_merge_and_split(oracle, rng):
+    G, S = 8, 4
+    grids = rng.normal(size=(S * G * G, 2)).astype(np.float32)
+    np.testing.assert_allclose(oracle.merge(G, S, grids), grids.reshape(S, G * G, 2).sum(0), rtol=1e-6)
+    assert list(oracle.split_counts(10, 4, 10)) == [2, 2, 2, 2]
+    assert list(oracle.split_counts(12, 4, 7)) == [3, 3, 1, 0]
