@@ -162,10 +162,13 @@ void std_degridding(int GRID_SIZE, int NUM_VISIBILITIES, int NUM_KERNELS, int TO
                                         config, d_uvw, n);
         CHECK(sdp_degrid(plan, d_k, d_grid, NULL, SDP_ORDER_ORIGINAL, d_out, NULL));
         sdp_grid_plan_destroy(plan);
-        if (n_out > n) CHECK(sdp_memset_zero(d_out + 2 * (size_t)n, sizeof(float2) * (size_t)(n_out - n), NULL));
-    } else {
-        CHECK(sdp_degrid_nearest(n, n_out, GRID_SIZE, config->uv_scale, d_grid, d_uvw, d_out, NULL));
+        download(output_visibilities, d_out, sizeof(float2) * (size_t)(n < NUM_VISIBILITIES ? n : NUM_VISIBILITIES));
+        if (NUM_VISIBILITIES > n)   /* degridgrid.c:132: entries past the corrected count stay 0 */
+            memset(output_visibilities + n, 0, sizeof(float2) * (size_t)(NUM_VISIBILITIES - n));
+        unlock();
+        return;
     }
+    CHECK(sdp_degrid_nearest(n, n_out, GRID_SIZE, config->uv_scale, d_grid, d_uvw, d_out, NULL));
     download(output_visibilities, d_out, sizeof(float2) * (size_t)NUM_VISIBILITIES);
     unlock();
 }

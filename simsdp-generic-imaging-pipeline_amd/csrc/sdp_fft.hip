@@ -21,6 +21,7 @@
 //
 // Twiddles come from one table per G of exp(2 pi i m / G) computed in double
 // precision on the host and rounded once to float.
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -302,7 +303,7 @@ int sdp_fft2d(const void *d_in, void *d_out, int G, int direction, int flags, co
     const int lg = ilog2(G);
     // pass R
     {
-        const int rpb = G >= 2048 ? 1 : 2048 / G;
+        const int rpb = G >= 2048 ? 1 : std::min(G, 2048 / G);   // rows per workgroup
         const size_t lds = sizeof(float2) * (size_t)rpb * G;
         const int maxb = butterflies_per_thread(G, rpb);
         dim3 grid(G / rpb);
