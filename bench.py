@@ -105,7 +105,11 @@ def main():
     d.vis = d_meas.download((N, 2), np.float32)
 
     t = time.perf_counter()
-    plan = hip.GridPlan(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, d_uvw, N)
+    tile, chunk, tune_ms = hip.autotune(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, d_uvw, N, d_gk, d_dk)
+    tune_s = time.perf_counter() - t
+    log(f"autotune {tune_s:.1f} s: tile {tile}, {chunk} taps per work item ({tune_ms:.2f} ms per pass)")
+    t = time.perf_counter()
+    plan = hip.GridPlan(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, d_uvw, N, tile=tile, chunk=chunk)
     hip.synchronize()
     plan_s = time.perf_counter() - t
     info = plan.info()
@@ -126,6 +130,7 @@ def main():
     plan.to_plan_order(d_meas, d_meas_p)
     d_delta = hip.DeviceBuffer(8 * N)
     d_grid = hip.DeviceBuffer(8 * G * G)
+    info["classes"] = plan.class_launches()
 
     def step(timed):
         if timed:
@@ -143,12 +148,12 @@ def main():
     for _ in range(args.warmup):
         step(False)
     hip.synchronize()
-    acc = {"degrid": 0.0, "grid_accumulate": 0.0, "grid_merge": 0.0}
+    acc = {"degrid_subtract": 0.0, "grid_accumulate": 0.0, "grid_merge": 0.0}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step(True)
         ev[3].sync()
-        acc["degrid"] += ev[0].elapsed_ms(ev[1])
+        acc["degrid_subtract"] += ev[0].elapsed_ms(ev[1])
         acc["grid_accumulate"] += ev[1].elapsed_ms(ev[2])
         acc["grid_merge"] += ev[2].elapsed_ms(ev[3])
     hip.synchronize()
@@ -157,13 +162,15 @@ def main():
     ms_per_step = wall_ms
     log(f"step {ms_per_step:.3f} ms (events: {k_ms})")
 
-    # algorithmic bytes per launch
+    # algorithmic bytes per step of each phase
     alg = {
-        "degrid": 8 * G * G + N * (12 + 8 + 8),     # grid read, uvw read, measured read, residual written
+        # model grid read, uvw read, measured visibility read, residual visibility written
+        "degrid_subtract": 8 * G * G + N * (12 + 8 + 8),
         "grid_accumulate": N * (12 + 8),           # uvw + residual visibility read
-        "grid_merge": 8 * G * G,                   # grid written once
+        "grid_merge": 8 * G * G,                   # residual grid written once
     }
     dom = max(k_ms, key=k_ms.get)
+    launches = int(info["classes"]) if dom != "grid_merge" else 1 + (1 if info["chunks"] > info["tiles"] else 0)
     achieved = alg[dom] / (k_ms[dom] * 1e-3) / 1e9
     b_gd = N * 40 + 16 * G * G
     gd_ms = sum(k_ms.values())
@@ -187,6 +194,16 @@ def main():
         extras["plan_ms"] = round(plan_s * 1e3, 1)
         extras["grid_degrid_hbm_frac"] = round(b_gd / (gd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
         extras["kernel_ms"] = {k: round(v, 4) for k, v in k_ms.items()}
+        # the fused degrid + subtract + grid kernel on the same data, for comparison
+        ev[0].record()
+        plan.degrid_grid_accumulate(d_dk, d_gk, d_model, d_meas_p, order=hip.ORDER_PLAN)
+        ev[1].record()
+        ev[1].sync()
+        extras["fused_degrid_grid_ms"] = round(ev[0].elapsed_ms(ev[1]), 4)
+        extras["autotune"] = {"tile": tile, "chunk_taps": chunk, "pass_ms": round(tune_ms, 3), "s": round(tune_s, 2)}
+        taps = plan.total_taps()
+        extras["taps_per_step"] = taps
+        extras["gtaps_per_s"] = round(2 * taps / ((k_ms["degrid_subtract"] + k_ms["grid_accumulate"]) * 1e-3) / 1e9, 1)
         extras["plan"] = info
 
     out = {
@@ -208,7 +225,8 @@ def main():
                    "n_vis": N, "grid": G, "w_planes": int(d.supports.shape[0]), "oversampling": d.oversampling},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "alg_bytes": alg[dom], "avg_ms": round(k_ms[dom], 4)},
+                     "alg_bytes_per_step": alg[dom], "launches_per_step": launches,
+                     "ms_per_step": round(k_ms[dom], 4)},
     }
     out.update(extras)
     if not args.no_cpu:

@@ -47,6 +47,8 @@ int sdp_memcpy_htod(void *dst, const void *src, size_t bytes, sdp_stream s);
 int sdp_memcpy_dtoh(void *dst, const void *src, size_t bytes, sdp_stream s);
 int sdp_memcpy_dtod(void *dst, const void *src, size_t bytes, sdp_stream s);
 int sdp_memset_zero(void *dst, size_t bytes, sdp_stream s);   /* fill kernel, graph safe */
+/* copy kernel (graph safe); pointers and size multiples of 16 bytes */
+int sdp_device_copy(void *dst, const void *src, size_t bytes, sdp_stream s);
 int sdp_stream_create(sdp_stream *s);
 int sdp_stream_destroy(sdp_stream s);
 int sdp_stream_sync(sdp_stream s);
@@ -96,8 +98,8 @@ typedef struct sdp_grid_params {
     int snap_mode;        /* SDP_SNAP_* */
     double uv_scale;      /* cell_size * G */
     double w_scale;       /* (n_kernels - 1)^2 / max_w */
-    int tile;             /* uv tile edge in cells, 0 = autotuned */
-    int chunk;            /* max visibilities per work item, 0 = autotuned */
+    int tile;             /* uv tile edge in cells, 0 = default */
+    int chunk;            /* taps per work item, 0 = default */
 } sdp_grid_params;
 
 typedef struct sdp_grid_plan sdp_grid_plan;
@@ -109,10 +111,11 @@ typedef struct sdp_grid_plan sdp_grid_plan;
 int sdp_grid_plan_create(sdp_grid_plan **plan, const sdp_grid_params *p, const int *supports,
                          const float *d_uvw, int n_vis, sdp_stream s);
 int sdp_grid_plan_destroy(sdp_grid_plan *plan);
-/* plan statistics: [0] visibilities kept, [1] dropped (outside grid or
- * beyond the support table), [2] tiles used, [3] work items, [4] tile edge,
- * [5] chunk size, [6] max half support, [7] scratch bytes */
-int sdp_grid_plan_info(const sdp_grid_plan *plan, int64_t *info8);
+/* plan statistics (10 values): [0] visibilities kept, [1] dropped (outside
+ * grid or beyond the support table), [2] tiles used, [3] work items, [4] tile
+ * edge, [5] taps per work item, [6] max half support, [7] scratch bytes,
+ * [8] kernel launches per pass (support classes present), [9] total taps */
+int sdp_grid_plan_info(const sdp_grid_plan *plan, int64_t *info10);
 /* reorder n_vis complex values between caller order and plan order */
 int sdp_grid_plan_to_plan_order(const sdp_grid_plan *plan, const float *d_in, float *d_out, sdp_stream s);
 int sdp_grid_plan_to_original_order(const sdp_grid_plan *plan, const float *d_in, float *d_out, sdp_stream s);
@@ -132,6 +135,29 @@ int sdp_grid_merge(const sdp_grid_plan *plan, float *d_grid, sdp_stream s);
  * fused, gains_apply_run.c:133-141).  Input and output in `order`. */
 int sdp_degrid(const sdp_grid_plan *plan, const float *d_kernels, const float *d_grid,
                const float *d_vis_measured, int order, float *d_vis_out, sdp_stream s);
+
+/* the major cycle's visibility-domain step in one pass: predict each
+ * visibility from d_model_grid (as sdp_degrid with d_degrid_kernels),
+ * subtract it from d_vis_measured, and grid the residual into
+ * d_residual_grid (as sdp_grid with d_grid_kernels); the residual
+ * visibilities are also written to d_residual_vis when it is not NULL.
+ * Arithmetic of std_degridding + subtract_from_measurements_finegrid +
+ * std_gridding (degridgrid.c:69-186, gains_apply_run.c:133-141). */
+int sdp_degrid_grid(const sdp_grid_plan *plan, const float *d_degrid_kernels, const float *d_grid_kernels,
+                    const float *d_model_grid, const float *d_vis_measured, int order, float *d_residual_vis,
+                    float *d_residual_grid, sdp_stream s);
+/* sdp_degrid_grid without the final sdp_grid_merge */
+int sdp_degrid_grid_accumulate(const sdp_grid_plan *plan, const float *d_degrid_kernels,
+                               const float *d_grid_kernels, const float *d_model_grid, const float *d_vis_measured,
+                               int order, float *d_residual_vis, sdp_stream s);
+
+/* Autotuner: builds a plan for every (tile, taps-per-work-item) candidate
+ * on these coordinates, times one degrid + grid + merge pass of each with
+ * HIP events and returns the fastest pair (and its time).  Candidates whose
+ * windows do not fit LDS are skipped.  Allocates scratch for the timing. */
+int sdp_grid_autotune(const sdp_grid_params *p, const int *supports, const float *d_uvw, int n_vis,
+                      const float *d_grid_kernels, const float *d_degrid_kernels, const int *tiles, int n_tiles,
+                      const int *chunks, int n_chunk, int *best_tile, int *best_chunk, float *best_ms, sdp_stream s);
 
 /* nearest-cell degridding (std_degridding output, degridgrid.c:194-200);
  * writes n_out (>= n_vis) values, zero beyond n_vis. */

@@ -23,6 +23,13 @@ __global__ void zero_kernel(uint4 *p, size_t n16, unsigned char *tail, size_t nt
     for (; i < n16; i += stride) p[i] = make_uint4(0u, 0u, 0u, 0u);
     if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0;
 }
+// Plain 16-byte copy (graph safe, and the reference copy for HBM counters).
+__global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
 }  // namespace
 
 namespace sdp {
@@ -111,6 +118,19 @@ int sdp_memset_zero(void *dst, size_t bytes, sdp_stream s)
     if (blocks > 65536) blocks = 65536;
     zero_kernel<<<blocks, 256, 0, sdp::stream_of(s)>>>((uint4 *)dst, n16, (unsigned char *)dst + n16 * 16,
                                                       bytes - n16 * 16);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_device_copy(void *dst, const void *src, size_t bytes, sdp_stream s)
+{
+    if (!bytes || dst == src) return SDP_OK;
+    SDP_REQUIRE(((uintptr_t)dst & 15) == 0 && ((uintptr_t)src & 15) == 0 && (bytes & 15) == 0,
+                "sdp_device_copy: pointers and size must be multiples of 16 bytes");
+    const size_t n16 = bytes / 16;
+    unsigned int blocks = sdp_div_up(n16, 256);
+    if (blocks > 16384) blocks = 16384;
+    copy_kernel<<<blocks, 256, 0, sdp::stream_of(s)>>>((const uint4 *)src, (uint4 *)dst, n16);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }

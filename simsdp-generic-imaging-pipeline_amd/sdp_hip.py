@@ -125,6 +125,7 @@ def load() -> ctypes.CDLL:
         "sdp_device_malloc": (c_int, [ctypes.POINTER(c_void_p), c_size_t]), "sdp_device_free": (c_int, [P]),
         "sdp_memcpy_htod": (c_int, [P, P, c_size_t, P]), "sdp_memcpy_dtoh": (c_int, [P, P, c_size_t, P]),
         "sdp_memcpy_dtod": (c_int, [P, P, c_size_t, P]), "sdp_memset_zero": (c_int, [P, c_size_t, P]),
+        "sdp_device_copy": (c_int, [P, P, c_size_t, P]),
         "sdp_stream_create": (c_int, [ctypes.POINTER(c_void_p)]), "sdp_stream_destroy": (c_int, [P]),
         "sdp_stream_sync": (c_int, [P]), "sdp_device_sync": (c_int, []),
         "sdp_event_create": (c_int, [ctypes.POINTER(c_void_p)]), "sdp_event_destroy": (c_int, [P]),
@@ -143,6 +144,10 @@ def load() -> ctypes.CDLL:
         "sdp_grid_accumulate": (c_int, [P, P, P, c_int, P]),
         "sdp_grid_merge": (c_int, [P, P, P]),
         "sdp_degrid": (c_int, [P, P, P, P, c_int, P, P]),
+        "sdp_degrid_grid": (c_int, [P, P, P, P, P, c_int, P, P, P]),
+        "sdp_degrid_grid_accumulate": (c_int, [P, P, P, P, P, c_int, P, P]),
+        "sdp_grid_autotune": (c_int, [P, P, P, c_int, P, P, P, c_int, P, c_int, ctypes.POINTER(c_int),
+                                      ctypes.POINTER(c_int), ctypes.POINTER(c_float), P]),
         "sdp_degrid_nearest": (c_int, [c_int, c_int, c_int, c_double, P, P, P, P]),
         "sdp_fft_work_bytes": (c_size_t, [c_int]),
         "sdp_fft2d": (c_int, [P, P, c_int, c_int, c_int, P, P, P]),
@@ -254,6 +259,25 @@ class Event:
             pass
 
 
+class _Params(ctypes.Structure):
+    _fields_ = [("grid_size", c_int), ("oversampling", c_int), ("n_kernels", c_int), ("snap_mode", c_int),
+                ("uv_scale", c_double), ("w_scale", c_double), ("tile", c_int), ("chunk", c_int)]
+
+
+def autotune(grid_size, oversampling, supports, uv_scale, w_scale, d_uvw, n_vis, d_grid_kernels, d_degrid_kernels,
+             tiles=(4, 8, 16), chunks=(32768, 65536, 131072, 262144), snap_mode=SNAP_STD, stream=None):
+    """Fastest (tile, taps per work item) for these coordinates: (tile, chunk, ms)."""
+    sup = np.ascontiguousarray(np.asarray(supports, dtype=np.int32).reshape(-1, 2))
+    p = _Params(grid_size, oversampling, sup.shape[0], snap_mode, uv_scale, w_scale, 0, 0)
+    t = np.asarray(tiles, np.int32)
+    c = np.asarray(chunks, np.int32)
+    bt, bc, ms = c_int(0), c_int(0), c_float(0)
+    check(load().sdp_grid_autotune(ctypes.byref(p), sup.ctypes.data, _ptr(d_uvw), int(n_vis), _ptr(d_grid_kernels),
+                                   _ptr(d_degrid_kernels), t.ctypes.data, t.size, c.ctypes.data, c.size,
+                                   ctypes.byref(bt), ctypes.byref(bc), ctypes.byref(ms), stream))
+    return bt.value, bc.value, ms.value
+
+
 class GridPlan:
     """Owner of an ``sdp_grid_plan`` (tile-sorted visibility records)."""
 
@@ -262,13 +286,7 @@ class GridPlan:
         lib = load()
         check(lib.sdp_init())
         sup = np.ascontiguousarray(np.asarray(supports, dtype=np.int32).reshape(-1, 2))
-
-        class Params(ctypes.Structure):
-            _fields_ = [("grid_size", c_int), ("oversampling", c_int), ("n_kernels", c_int),
-                        ("snap_mode", c_int), ("uv_scale", c_double), ("w_scale", c_double),
-                        ("tile", c_int), ("chunk", c_int)]
-
-        p = Params(grid_size, oversampling, sup.shape[0], snap_mode, uv_scale, w_scale, tile, chunk)
+        p = _Params(grid_size, oversampling, sup.shape[0], snap_mode, uv_scale, w_scale, tile, chunk)
         h = c_void_p()
         check(lib.sdp_grid_plan_create(ctypes.byref(h), ctypes.byref(p), sup.ctypes.data, _ptr(d_uvw),
                                        int(n_vis), stream))
@@ -277,10 +295,17 @@ class GridPlan:
         self.n_vis = int(n_vis)
 
     def info(self) -> dict:
-        a = np.zeros(8, dtype=np.int64)
+        a = np.zeros(10, dtype=np.int64)
         check(load().sdp_grid_plan_info(self.ptr, a.ctypes.data))
-        keys = ["kept", "dropped", "tiles", "chunks", "tile", "chunk_taps", "max_half_support", "scratch_bytes"]
+        keys = ["kept", "dropped", "tiles", "chunks", "tile", "chunk_taps", "max_half_support", "scratch_bytes",
+                "class_launches", "taps"]
         return dict(zip(keys, (int(x) for x in a)))
+
+    def class_launches(self) -> int:
+        return self.info()["class_launches"]
+
+    def total_taps(self) -> int:
+        return self.info()["taps"]
 
     def grid(self, d_kernels, d_vis, d_grid, order=ORDER_ORIGINAL, stream=None):
         check(load().sdp_grid(self.ptr, _ptr(d_kernels), _ptr(d_vis), order, _ptr(d_grid), stream))
@@ -294,6 +319,17 @@ class GridPlan:
     def degrid(self, d_kernels, d_grid, d_vis_out, d_measured=None, order=ORDER_ORIGINAL, stream=None):
         check(load().sdp_degrid(self.ptr, _ptr(d_kernels), _ptr(d_grid), _ptr(d_measured), order,
                                 _ptr(d_vis_out), stream))
+
+    def degrid_grid(self, d_degrid_kernels, d_grid_kernels, d_model, d_measured, d_residual_grid,
+                    d_residual_vis=None, order=ORDER_ORIGINAL, stream=None):
+        check(load().sdp_degrid_grid(self.ptr, _ptr(d_degrid_kernels), _ptr(d_grid_kernels), _ptr(d_model),
+                                     _ptr(d_measured), order, _ptr(d_residual_vis), _ptr(d_residual_grid), stream))
+
+    def degrid_grid_accumulate(self, d_degrid_kernels, d_grid_kernels, d_model, d_measured, d_residual_vis=None,
+                               order=ORDER_ORIGINAL, stream=None):
+        check(load().sdp_degrid_grid_accumulate(self.ptr, _ptr(d_degrid_kernels), _ptr(d_grid_kernels),
+                                                _ptr(d_model), _ptr(d_measured), order, _ptr(d_residual_vis),
+                                                stream))
 
     def to_plan_order(self, d_in, d_out, stream=None):
         check(load().sdp_grid_plan_to_plan_order(self.ptr, _ptr(d_in), _ptr(d_out), stream))

@@ -91,7 +91,7 @@ def test_grid_device_api_plan_order_and_determinism(hip, case):
     np.testing.assert_array_equal(back.download((d.n_vis, 2), np.float32), d.vis)
 
 
-@pytest.mark.parametrize("tile,chunk", [(16, 2000), (40, 1 << 16), (64, 5000)])
+@pytest.mark.parametrize("tile,chunk", [(8, 2000), (24, 1 << 16), (32, 5000)])
 def test_grid_tilings_agree(hip, oracle, case, tile, chunk):
     d = case
     G = d.grid_size
@@ -155,6 +155,35 @@ def test_degrid_fused_subtract(hip, case, rng):
     plan.degrid(kern, grid, delta, d_measured=meas)
     p = pred.download((d.n_vis, 2), np.float32)
     np.testing.assert_array_equal(delta.download((d.n_vis, 2), np.float32), d.vis - p)
+
+
+@pytest.mark.parametrize("order", ["original", "plan"])
+def test_degrid_grid_fused_equals_separate(hip, case, rng, order):
+    """The fused major-cycle pass gives bit-identical residual visibilities
+    and residual grid to degrid(+subtract) followed by grid."""
+    d = case
+    G = d.grid_size
+    o = hip.ORDER_ORIGINAL if order == "original" else hip.ORDER_PLAN
+    uvw = hip.DeviceBuffer.from_array(d.uvw)
+    gk = hip.DeviceBuffer.from_array(d.kernels)
+    dk = hip.DeviceBuffer.from_array(d.degrid_kernels)
+    model = hip.DeviceBuffer.from_array(rng.normal(size=(G * G, 2)).astype(np.float32))
+    meas = hip.DeviceBuffer.from_array(d.vis)
+    plan = hip.GridPlan(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, uvw, d.n_vis)
+    if o == hip.ORDER_PLAN:
+        mp = hip.DeviceBuffer(8 * d.n_vis)
+        plan.to_plan_order(meas, mp)
+        meas = mp
+    res_sep = hip.DeviceBuffer(8 * d.n_vis)
+    grid_sep = hip.DeviceBuffer(8 * G * G)
+    plan.degrid(dk, model, res_sep, d_measured=meas, order=o)
+    plan.grid(gk, res_sep, grid_sep, order=o)
+    res_f = hip.DeviceBuffer(8 * d.n_vis)
+    grid_f = hip.DeviceBuffer(8 * G * G)
+    plan.degrid_grid(dk, gk, model, meas, grid_f, d_residual_vis=res_f, order=o)
+    np.testing.assert_array_equal(res_f.download((d.n_vis, 2), np.float32),
+                                  res_sep.download((d.n_vis, 2), np.float32))
+    np.testing.assert_array_equal(grid_f.download((G * G, 2), np.float32), grid_sep.download((G * G, 2), np.float32))
 
 
 # ---------------------------------------------------------------- fine grid
@@ -343,3 +372,20 @@ def test_dft_modes(hip, actors, oracle, rng):
     assert np.abs(got64 - ref64).max() <= 2e-6 * scale
     # the reference's float phase is itself this far from FP64 (stated, not a bar)
     assert np.abs(ref32 - ref64).max() < 1e-2 * scale
+
+
+def test_autotune_picks_a_working_candidate(hip, oracle, case):
+    d = case
+    G = d.grid_size
+    uvw = hip.DeviceBuffer.from_array(d.uvw)
+    gk = hip.DeviceBuffer.from_array(d.kernels)
+    dk = hip.DeviceBuffer.from_array(d.degrid_kernels)
+    tile, chunk, ms = hip.autotune(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, uvw, d.n_vis, gk, dk,
+                                   tiles=(4, 8, 16), chunks=(4096, 65536))
+    assert tile in (4, 8, 16) and chunk in (4096, 65536) and ms > 0
+    plan = hip.GridPlan(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, uvw, d.n_vis, tile=tile, chunk=chunk)
+    vis = hip.DeviceBuffer.from_array(d.vis)
+    g = hip.DeviceBuffer(8 * G * G)
+    plan.grid(gk, vis, g)
+    ref64 = oracle.std_gridding_f64(G, d.oversampling, d.kernels, d.supports, d.uvw, d.vis, d.uv_scale, d.w_scale)
+    assert np.abs(g.download((G * G, 2), np.float32) - ref64).max() <= 1e-6 * np.abs(ref64).max()
