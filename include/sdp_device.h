@@ -41,6 +41,9 @@ enum {
 /* ---- runtime ------------------------------------------------------------ */
 int sdp_init(void);
 const char *sdp_last_error(void);
+/* records `message` as this thread's last error and returns `code` (for
+ * host layers built on this ABI) */
+int sdp_set_last_error(int code, const char *message);
 int sdp_device_malloc(void **ptr, size_t bytes);
 int sdp_device_free(void *ptr);
 int sdp_memcpy_htod(void *dst, const void *src, size_t bytes, sdp_stream s);
@@ -58,6 +61,14 @@ int sdp_event_destroy(sdp_event e);
 int sdp_event_record(sdp_event e, sdp_stream s);
 int sdp_event_sync(sdp_event e);
 int sdp_event_elapsed_ms(sdp_event start, sdp_event stop, float *ms);
+
+/* stream capture into an executable graph (replayed with sdp_graph_launch);
+ * every call between begin and end must be enqueue-only on that stream */
+typedef void *sdp_graph;
+int sdp_graph_begin_capture(sdp_stream s);
+int sdp_graph_end_capture(sdp_stream s, sdp_graph *graph);
+int sdp_graph_launch(sdp_graph graph, sdp_stream s);
+int sdp_graph_destroy(sdp_graph graph);
 
 /* ---- actor-level switches ---------------------------------------------- */
 enum { SDP_DEGRID_NEAREST = 0, SDP_DEGRID_WEIGHTED = 1 };
@@ -170,7 +181,9 @@ enum { SDP_FFT_SHIFT_IN = 1,      /* multiply input by (-1)^(r+c) */
        SDP_FFT_SHIFT_OUT = 2,     /* multiply output by (-1)^(r+c) */
        SDP_FFT_REAL_IN = 4,       /* input is a real G x G image */
        SDP_FFT_REAL_OUT = 8,      /* output only the real part as a G x G image */
-       SDP_FFT_CORRECT_OUT = 16   /* divide the real output by the PSWF taper */ };
+       SDP_FFT_CORRECT_OUT = 16,  /* divide the real output by the PSWF taper */
+       SDP_FFT_CORRECT_IN = 32    /* divide the real input by the taper (the inverse correction
+                                     the reference leaves as a copy, convolution_correction_run.c:65) */ };
 /* bytes of device work memory sdp_fft2d needs for grid size G */
 size_t sdp_fft_work_bytes(int grid_size);
 /* unnormalised 2D C2C DFT with sign `direction`; in == out allowed.
@@ -192,6 +205,17 @@ int sdp_vis_subtract_gains(int n_vis, int n_baselines, const float *d_measured, 
 /* out[i] = sum_{j<slices} grids[j*G*G + i] in slice order */
 int sdp_merge_grids(int grid_size, int slices, const float *d_grids, float *d_out, sdp_stream s);
 
+/* image statistics: sum of squares (double), largest value, largest |value|;
+ * synchronous (results on the host) */
+int sdp_image_stats(size_t n, const float *d_img, double *sum_sq, float *max_value, float *max_abs, sdp_stream s);
+/* the same, enqueued only: d_stats[0..2] = {sum of squares, max, max |value|}
+ * as doubles, d_work of sdp_image_stats_work_bytes() bytes */
+size_t sdp_image_stats_work_bytes(void);
+int sdp_image_stats_async(size_t n, const float *d_img, double *d_stats, void *d_work, sdp_stream s);
+int sdp_fill_complex(size_t n, float *d, float re, float im, sdp_stream s);
+/* d[i] = (float)((double)d[i] / by): psf_host_set_up's normalisation (top.c:505-507) */
+int sdp_divide(size_t n, float *d, double by, sdp_stream s);
+
 /* ---- Hogbom CLEAN (deconvolution_run.c:197-312) ------------------------ */
 /* d_psf: G*G+1 floats (peak-normalised PSF followed by one 0).
  * d_residual is updated in place (it ends as the residual after the last
@@ -204,6 +228,14 @@ int sdp_hogbom(int grid_size, int n_minor, int max_sources, float *d_residual, c
                double psf_max, double loop_gain, double noise_factor, double weak_source_percent,
                double cell_size, const float *d_model_in, float *d_model_out, float *d_sources,
                int *d_num_sources, void *d_work, sdp_stream s);
+/* flags for sdp_hogbom_ex.  By default the PSF is read x-mirrored about its
+ * centre exactly as the reference indexes it (deconvolution_run.c:269);
+ * SDP_HOGBOM_PSF_AS_STORED subtracts it in its stored orientation. */
+enum { SDP_HOGBOM_PSF_AS_STORED = 1 };
+int sdp_hogbom_ex(int grid_size, int n_minor, int max_sources, float *d_residual, const float *d_psf,
+                  double psf_max, double loop_gain, double noise_factor, double weak_source_percent,
+                  double cell_size, const float *d_model_in, float *d_model_out, float *d_sources,
+                  int *d_num_sources, void *d_work, int flags, sdp_stream s);
 
 /* ---- DFT predict (dft_run.c:3-47) --------------------------------------- */
 /* vis[j] = sum_s flux_s / (1 - r^2/2) * exp(-2 pi i (u l + v m - w r^2/2)),

@@ -1,0 +1,102 @@
+/*
+ * sdp_pipeline.h -- the imaging major cycle on one GPU, everything resident
+ * in device memory.
+ *
+ * One major cycle follows the reference's FFT imaging graph
+ * (preesm_pipelines/simsdp_fft_imaging_pipeline/Algo/fft_delta.pi:61-320:
+ * fft_run, fft_degrid, subtract_from_measurements_finegrid, grid,
+ * inverse_fft_run, convolution_correction_run, then the Hogbom actor):
+ *
+ *   model image --(taper divide, shifts, forward FFT)--> model grid
+ *   model grid  --(kernel-weighted degrid, minus measured)--> residual vis
+ *   residual vis --(grid)--> residual grid
+ *   residual grid --(shifts, inverse FFT, real part, taper divide)--> residual image
+ *   residual image --(Hogbom CLEAN, deconvolution_run.c:197-312)--> model image += components
+ *
+ * The PSF is made the same way from unit visibilities and normalised by its
+ * peak, as psf_host_set_up does with the PSF it reads (top.c:449-512).
+ *
+ * Residual RMS is reported in Jy: sqrt(mean(residual^2)) / psf_max.
+ *
+ * Differences from the reference graph, each selectable:
+ *  - the predict step divides the model by the taper before the forward FFT
+ *    (the reference's execute_inv_convolution_correction_actor is an
+ *    identity copy, convolution_correction_run.c:41-68);
+ *    SDP_PIPE_REFERENCE_PREDICT restores the identity;
+ *  - degridding is kernel-weighted (std_degridding's nearest-cell override,
+ *    degridgrid.c:194-200, is available through the actor ABI);
+ *  - Hogbom subtracts the PSF in its stored orientation;
+ *    SDP_PIPE_PSF_MIRRORED restores the reference's x-mirrored read
+ *    (deconvolution_run.c:269).
+ */
+#ifndef SDP_PIPELINE_H
+#define SDP_PIPELINE_H
+
+#include "sdp_device.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SDP_PIPE_GRAPH = 1,              /* replay each cycle from a captured graph */
+    SDP_PIPE_REFERENCE_PREDICT = 2,  /* no taper division before the forward FFT */
+    SDP_PIPE_PSF_MIRRORED = 4        /* Hogbom reads the PSF x-mirrored */
+};
+
+typedef struct sdp_pipeline_params {
+    int grid_size;          /* G, power of two in [16, 8192] */
+    int oversampling;       /* kernel oversampling OS */
+    int n_kernels;          /* w-planes */
+    int n_vis;
+    double uv_scale;        /* G * cell_size */
+    double w_scale;         /* (n_kernels - 1)^2 / max_w */
+    double cell_size;       /* radians per pixel */
+    int n_minor;            /* Hogbom iterations per cycle */
+    int max_sources;        /* capacity of the per-cycle source list */
+    double loop_gain;
+    double noise_factor;
+    double weak_source_percent;
+    int flags;              /* SDP_PIPE_* */
+    int tile, chunk;        /* gridder tiling, 0 for the defaults */
+} sdp_pipeline_params;
+
+typedef struct sdp_pipeline sdp_pipeline;
+
+/* Builds the gridding plan, the PSF and all work buffers.  d_uvw (n_vis
+ * float3) and d_vis (n_vis measured float2) are only read here.  The kernel
+ * tables (supports: host int2 {h, offset} per plane as kernel_host_set_up
+ * lays them out, top.c:235-379; d_degrid_kernels the conjugate table) and
+ * d_taper (correction_set_up's prolate spheroidal, G/2 + 1 floats with the
+ * last one 0) are used by every cycle and must outlive the pipeline. */
+int sdp_pipeline_create(sdp_pipeline **out, const sdp_pipeline_params *params, const int *supports,
+                        const float *d_uvw, const float *d_vis, const float *d_grid_kernels,
+                        const float *d_degrid_kernels, const float *d_taper, sdp_stream s);
+int sdp_pipeline_destroy(sdp_pipeline *p);
+
+/* One major cycle.  rms_residual: RMS (Jy) of the residual image formed in
+ * this cycle, before CLEAN; rms_after: RMS after this cycle's CLEAN
+ * subtraction; ms: wall time of the cycle on the stream.  Any output may be
+ * NULL.  Synchronous. */
+int sdp_pipeline_cycle(sdp_pipeline *p, double *rms_residual, double *rms_after, float *ms);
+
+/* Restarts from an empty model (PSF and plan kept). */
+int sdp_pipeline_reset(sdp_pipeline *p);
+/* Replaces the model image with G x G floats from device memory d_model
+ * (a prior model, or a minor-cycle proposal made elsewhere). */
+int sdp_pipeline_load_model(sdp_pipeline *p, const float *d_model);
+
+enum { SDP_IMAGE_RESIDUAL = 0, SDP_IMAGE_MODEL = 1, SDP_IMAGE_PSF = 2 };
+/* Device pointer to one of the pipeline's G x G float images. */
+const float *sdp_pipeline_image(const sdp_pipeline *p, int which);
+double sdp_pipeline_psf_max(const sdp_pipeline *p);
+/* Components found by the last cycle: up to max_sources float3 (l, m,
+ * flux) copied to the host, count in *n. */
+int sdp_pipeline_sources(const sdp_pipeline *p, float *h_sources, int *n);
+/* Gridding plan used by the pipeline (for its info). */
+const sdp_grid_plan *sdp_pipeline_plan(const sdp_pipeline *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

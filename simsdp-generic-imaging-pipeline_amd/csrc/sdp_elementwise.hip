@@ -4,6 +4,10 @@
 // All kernels use 16-byte accesses and grid-stride loops.  The library is
 // built with -ffp-contract=off, so every expression rounds exactly as the
 // reference's C does (bit-identical outputs).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "sdp_internal.h"
 
 namespace {
@@ -134,6 +138,68 @@ __global__ void merge_kernel(const float4 *grids, float4 *out, size_t n2, int sl
     }
 }
 
+// sum of squares and largest value / |value| of an image, one partial per block
+__global__ void image_stats_kernel(const float *img, size_t n, double *part)
+{
+    __shared__ double s_sq[256];
+    __shared__ float s_max[256], s_amax[256];
+    double sq = 0.0;
+    float mx = -INFINITY, amx = 0.f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float v = img[i];
+        sq += (double)v * (double)v;
+        mx = fmaxf(mx, v);
+        amx = fmaxf(amx, fabsf(v));
+    }
+    s_sq[threadIdx.x] = sq;
+    s_max[threadIdx.x] = mx;
+    s_amax[threadIdx.x] = amx;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            s_sq[threadIdx.x] += s_sq[threadIdx.x + off];
+            s_max[threadIdx.x] = fmaxf(s_max[threadIdx.x], s_max[threadIdx.x + off]);
+            s_amax[threadIdx.x] = fmaxf(s_amax[threadIdx.x], s_amax[threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = s_sq[0];
+        part[3 * blockIdx.x + 1] = s_max[0];
+        part[3 * blockIdx.x + 2] = s_amax[0];
+    }
+}
+
+// combines the per-block partials in block order: out = {sum_sq, max, max_abs}
+__global__ void image_stats_final_kernel(const double *part, int nparts, double *out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double sq = 0.0, mx = -INFINITY, amx = 0.0;
+    for (int b = 0; b < nparts; ++b) {
+        sq += part[3 * b];
+        mx = fmax(mx, part[3 * b + 1]);
+        amx = fmax(amx, part[3 * b + 2]);
+    }
+    out[0] = sq;
+    out[1] = mx;
+    out[2] = amx;
+}
+
+constexpr unsigned int kStatsBlocks = 1024;
+
+__global__ void fill_complex_kernel(float2 *d, size_t n, float re, float im)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        d[i] = make_float2(re, im);
+}
+
+// psf_host_set_up's normalisation (top.c:505-507): float / double, rounded to float
+__global__ void divide_kernel(float *d, size_t n, double by)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        d[i] = (float)((double)d[i] / by);
+}
+
 int check_grid(int G)
 {
     if (G < 2 || (G & 1)) return sdp::set_error(SDP_E_ARG, "grid size %d must be even and >= 2", G);
@@ -218,6 +284,59 @@ int sdp_merge_grids(int G, int slices, const float *d_grids, float *d_out, sdp_s
     SDP_REQUIRE(slices >= 1, "slices must be >= 1");
     size_t n2 = (size_t)G * G / 2;
     merge_kernel<<<grid_for(n2), kBlock, 0, sdp::stream_of(s)>>>((const float4 *)d_grids, (float4 *)d_out, n2, slices);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+size_t sdp_image_stats_work_bytes(void) { return sizeof(double) * 3 * kStatsBlocks; }
+
+int sdp_image_stats_async(size_t n, const float *d_img, double *d_stats, void *d_work, sdp_stream s)
+{
+    SDP_REQUIRE(d_img && n > 0, "sdp_image_stats: empty image");
+    SDP_REQUIRE(d_stats && d_work, "sdp_image_stats: null output or work buffer");
+    hipStream_t st = sdp::stream_of(s);
+    const unsigned int blocks = std::min<unsigned int>(kStatsBlocks, sdp_div_up(n, 256));
+    image_stats_kernel<<<blocks, 256, 0, st>>>(d_img, n, (double *)d_work);
+    SDP_LAUNCH_CHECK();
+    image_stats_final_kernel<<<1, 64, 0, st>>>((const double *)d_work, (int)blocks, d_stats);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_image_stats(size_t n, const float *d_img, double *sum_sq, float *max_value, float *max_abs, sdp_stream s)
+{
+    SDP_REQUIRE(d_img && n > 0, "sdp_image_stats: empty image");
+    hipStream_t st = sdp::stream_of(s);
+    double *buf = nullptr;
+    SDP_HIP_TRY(hipMalloc(&buf, sdp_image_stats_work_bytes() + 4 * sizeof(double)));
+    double h[3] = {0.0, 0.0, 0.0};
+    int rc = sdp_image_stats_async(n, d_img, buf, buf + 4, s);
+    hipError_t e = hipSuccess;
+    if (rc == SDP_OK) {
+        e = hipMemcpyAsync(h, buf, sizeof h, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    (void)hipFree(buf);
+    if (rc != SDP_OK) return rc;
+    if (e != hipSuccess) return sdp::set_error(SDP_E_HIP, "sdp_image_stats: %s", hipGetErrorString(e));
+    if (sum_sq) *sum_sq = h[0];
+    if (max_value) *max_value = (float)h[1];
+    if (max_abs) *max_abs = (float)h[2];
+    return SDP_OK;
+}
+
+int sdp_fill_complex(size_t n, float *d, float re, float im, sdp_stream s)
+{
+    if (!n) return SDP_OK;
+    fill_complex_kernel<<<grid_for(n), kBlock, 0, sdp::stream_of(s)>>>((float2 *)d, n, re, im);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_divide(size_t n, float *d, double by, sdp_stream s)
+{
+    if (!n) return SDP_OK;
+    divide_kernel<<<grid_for(n), kBlock, 0, sdp::stream_of(s)>>>(d, n, by);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }

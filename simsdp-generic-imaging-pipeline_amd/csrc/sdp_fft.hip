@@ -138,11 +138,20 @@ __device__ void lds_fft(float2 *buf, Line L, float sign, const float2 *__restric
 
 __device__ __forceinline__ float checker(int r, int c) { return ((r + c) & 1) ? -1.0f : 1.0f; }
 
+__device__ __forceinline__ float taper_divide(float x, const float *taper, int r, int c, int half)
+{
+    const int dc = abs(c - half), dr = abs(r - half);
+    float t = taper[dc] * taper[dr];
+    if (dc == 1229 || dr == 1229) t = 0.0f;
+    return ((double)fabsf(t) > 1e-10) ? x / t : 0.0f;
+}
+
+
 // Pass R: rows_per_block rows of length G.
 template <int MAXB>
 __global__ void __launch_bounds__(kFftThreads)
 fft_rows_kernel(const void *__restrict__ in, float2 *__restrict__ out, int G, int rows_per_block, int flags,
-                float sign, const float2 *__restrict__ tw)
+                float sign, const float2 *__restrict__ tw, const float *__restrict__ taper)
 {
     extern __shared__ float2 lds[];
     const int row0 = blockIdx.x * rows_per_block;
@@ -154,6 +163,7 @@ fft_rows_kernel(const void *__restrict__ in, float2 *__restrict__ out, int G, in
             v = make_float2(((const float *)in)[(size_t)r * G + c], 0.0f);
         else
             v = ((const float2 *)in)[(size_t)r * G + c];
+        if (flags & SDP_FFT_CORRECT_IN) v.x = taper_divide(v.x, taper, r, c, G / 2);
         if (flags & SDP_FFT_SHIFT_IN) {
             const float a = checker(r, c);
             v.x *= a;
@@ -195,14 +205,6 @@ struct OutArgs {
     int flags;
     const float *taper;   // G/2+1 entries, last one 0
 };
-
-__device__ __forceinline__ float taper_divide(float x, const float *taper, int r, int c, int half)
-{
-    const int dc = abs(c - half), dr = abs(r - half);
-    float t = taper[dc] * taper[dr];
-    if (dc == 1229 || dr == 1229) t = 0.0f;
-    return ((double)fabsf(t) > 1e-10) ? x / t : 0.0f;
-}
 
 // Pass C2: rows k1*G2 + j2 (j2 = 0..G2-1) -> output rows k1 + G1*k2.
 template <int MAXB>
@@ -293,6 +295,8 @@ int sdp_fft2d(const void *d_in, void *d_out, int G, int direction, int flags, co
     SDP_REQUIRE(d_in && d_out && d_work, "sdp_fft2d: null pointer");
     SDP_REQUIRE(!(flags & SDP_FFT_CORRECT_OUT) || ((flags & SDP_FFT_REAL_OUT) && d_taper),
                 "CORRECT_OUT needs REAL_OUT and a taper");
+    SDP_REQUIRE(!(flags & SDP_FFT_CORRECT_IN) || ((flags & SDP_FFT_REAL_IN) && d_taper),
+                "CORRECT_IN needs REAL_IN and a taper");
     SDP_REQUIRE(d_work != d_in && d_work != d_out, "work buffer must not alias input or output");
     const float2 *tw = nullptr;
     if (int e = twiddles_for(G, &tw)) return e;
@@ -308,13 +312,13 @@ int sdp_fft2d(const void *d_in, void *d_out, int G, int direction, int flags, co
         const int maxb = butterflies_per_thread(G, rpb);
         dim3 grid(G / rpb);
         if (maxb <= 4) {
-            fft_rows_kernel<4><<<grid, kFftThreads, lds, st>>>(d_in, work, G, rpb, flags, sign, tw);
+            fft_rows_kernel<4><<<grid, kFftThreads, lds, st>>>(d_in, work, G, rpb, flags, sign, tw, d_taper);
         } else if (maxb <= 8) {
             allow_lds(fft_rows_kernel<8>, lds);
-            fft_rows_kernel<8><<<grid, kFftThreads, lds, st>>>(d_in, work, G, rpb, flags, sign, tw);
+            fft_rows_kernel<8><<<grid, kFftThreads, lds, st>>>(d_in, work, G, rpb, flags, sign, tw, d_taper);
         } else {
             allow_lds(fft_rows_kernel<16>, lds);
-            fft_rows_kernel<16><<<grid, kFftThreads, lds, st>>>(d_in, work, G, rpb, flags, sign, tw);
+            fft_rows_kernel<16><<<grid, kFftThreads, lds, st>>>(d_in, work, G, rpb, flags, sign, tw, d_taper);
         }
         SDP_LAUNCH_CHECK();
     }

@@ -41,7 +41,7 @@ struct State {
 };
 
 struct HogbomArgs {
-    int G, n_minor, max_sources, nblocks, rows_per_block;
+    int G, n_minor, max_sources, nblocks, rows_per_block, mirror;
     double psf_max, gain, noise, weak;
 };
 
@@ -97,10 +97,17 @@ hogbom_step_kernel(int iter, HogbomArgs a, float *__restrict__ r, const float *_
         const double scale = a.gain * (double)s_maxval;
         for (int y = ylo; y < yhi; ++y) {
             float *row = r + (size_t)y * G;
-            const float *prow = psf + (size_t)(y - my + half) * G + (G - half + mx);
-            // psf index (y - my + half) * G + (G - (x - mx + half)) = prow - x
-            for (int x = mx - sx + (int)threadIdx.x; x < mx + ex; x += kThreads)
-                row[x] = (float)((double)row[x] - scale * (double)prow[-x] * a.psf_max);
+            if (a.mirror) {
+                // reference index (y - my + half) * G + (G - (x - mx + half)) = base - x
+                const long long base = (long long)(y - my + half) * G + (G - half + mx);
+                for (int x = mx - sx + (int)threadIdx.x; x < mx + ex; x += kThreads)
+                    row[x] = (float)((double)row[x] - scale * (double)psf[base - x] * a.psf_max);
+            } else {
+                // PSF in its own orientation: (y - my + half) * G + (x - mx + half) = base + x
+                const long long base = (long long)(y - my + half) * G + (half - mx);
+                for (int x = mx - sx + (int)threadIdx.x; x < mx + ex; x += kThreads)
+                    row[x] = (float)((double)row[x] - scale * (double)psf[base + x] * a.psf_max);
+            }
         }
     }
     if (iter >= a.n_minor) return;   // final launch: subtraction only
@@ -250,6 +257,16 @@ int sdp_hogbom(int G, int n_minor, int max_sources, float *d_residual, const flo
                const float *d_model_in, float *d_model_out, float *d_sources, int *d_num_sources, void *d_work,
                sdp_stream s)
 {
+    return sdp_hogbom_ex(G, n_minor, max_sources, d_residual, d_psf, psf_max, loop_gain, noise_factor,
+                         weak_source_percent, cell_size, d_model_in, d_model_out, d_sources, d_num_sources, d_work,
+                         0, s);
+}
+
+int sdp_hogbom_ex(int G, int n_minor, int max_sources, float *d_residual, const float *d_psf, double psf_max,
+                  double loop_gain, double noise_factor, double weak_source_percent, double cell_size,
+                  const float *d_model_in, float *d_model_out, float *d_sources, int *d_num_sources, void *d_work,
+                  int flags, sdp_stream s)
+{
     SDP_REQUIRE(G >= 2 && G % 2 == 0 && G <= 32768, "bad grid size %d", G);
     SDP_REQUIRE(n_minor >= 0 && max_sources >= 0, "bad iteration / source counts");
     SDP_REQUIRE(d_residual && d_psf && d_model_in && d_model_out && d_sources && d_num_sources && d_work,
@@ -262,6 +279,7 @@ int sdp_hogbom(int G, int n_minor, int max_sources, float *d_residual, const flo
     a.n_minor = n_minor;
     a.max_sources = max_sources;
     a.rows_per_block = G >= kPixelsPerBlock ? 1 : kPixelsPerBlock / G;
+    a.mirror = (flags & SDP_HOGBOM_PSF_AS_STORED) ? 0 : 1;
     a.nblocks = (G + a.rows_per_block - 1) / a.rows_per_block;
     a.psf_max = psf_max;
     a.gain = loop_gain;

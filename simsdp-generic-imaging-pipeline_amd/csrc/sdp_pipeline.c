@@ -1,0 +1,263 @@
+/*
+ * sdp_pipeline.c -- the imaging major cycle (include/sdp_pipeline.h) on top
+ * of the device C-ABI (include/sdp_device.h).
+ *
+ * Everything stays in device memory between cycles: the measured
+ * visibilities in the gridding plan's order, one complex grid (the model
+ * grid during predict, then the residual grid), the residual, model and PSF
+ * images.  A cycle is a fixed sequence of enqueued launches with no host
+ * synchronisation inside, so after a first eager cycle it can be captured
+ * once into a graph and replayed (SDP_PIPE_GRAPH).
+ *
+ * Stage order per cycle (preesm_pipelines/simsdp_fft_imaging_pipeline/
+ * Algo/fft_delta.pi:61-320; actor sources under .../Code/src/CPU/):
+ *   1. fft_run: fft_shift_real_to_complex, forward FFT,
+ *      fft_shift_complex_to_complex (fft_run.c:5-20,50-73,94-111), with the
+ *      taper division of the inverse correction fused on the way in;
+ *   2. fft_degrid + subtract_from_measurements_finegrid (degridgrid.c:127-202,
+ *      gains_apply_run.c:120-144), kernel-weighted, one fused pass;
+ *   3. grid (degridgrid.c:69-125);
+ *   4. inverse_fft_run + convolution_correction_run (fft_run.c:23-48,76-92,
+ *      convolution_correction_run.c:4-37), one fused FFT;
+ *   5. residual statistics;
+ *   6. hogbom_clean (deconvolution_run.c:197-312) on the residual image,
+ *      adding its components to the model in place;
+ *   7. residual statistics after CLEAN.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "sdp_pipeline.h"
+
+struct sdp_pipeline {
+    sdp_pipeline_params p;
+    sdp_stream s;
+    sdp_grid_plan *plan;
+    const float *gk, *dk, *taper;
+    float *vis_meas;        /* n_vis float2, plan order */
+    float *vis_resid;       /* n_vis float2, plan order */
+    float *grid;            /* G*G float2 */
+    float *residual;        /* G*G float */
+    float *model;           /* G*G float */
+    float *psf;             /* G*G + 1 float, peak-normalised, last 0 */
+    float *sources;         /* max_sources float3 */
+    int *num_sources;
+    void *fft_work, *hog_work, *stats_work;
+    double *stats;          /* {sum_sq, max, max_abs} before and after CLEAN */
+    double psf_max;
+    int cycles_done;
+    sdp_graph graph;
+    sdp_event ev0, ev1;
+};
+
+#define TRY(x) do { int rc_ = (x); if (rc_ != SDP_OK) return rc_; } while (0)
+#define FAIL(code, msg) sdp_set_last_error((code), "sdp_pipeline: " msg)
+
+static size_t pixels(const sdp_pipeline *P) { return (size_t)P->p.grid_size * (size_t)P->p.grid_size; }
+
+/* Stages 1-7 above: launches only, no synchronisation. */
+static int enqueue_cycle(sdp_pipeline *P)
+{
+    const sdp_pipeline_params *p = &P->p;
+    const int G = p->grid_size;
+    const int predict_flags = SDP_FFT_REAL_IN | SDP_FFT_SHIFT_IN | SDP_FFT_SHIFT_OUT |
+                              ((p->flags & SDP_PIPE_REFERENCE_PREDICT) ? 0 : SDP_FFT_CORRECT_IN);
+    TRY(sdp_fft2d(P->model, P->grid, G, SDP_FFT_FORWARD, predict_flags, P->taper, P->fft_work, P->s));
+    TRY(sdp_degrid(P->plan, P->dk, P->grid, P->vis_meas, SDP_ORDER_PLAN, P->vis_resid, P->s));
+    TRY(sdp_grid(P->plan, P->gk, P->vis_resid, SDP_ORDER_PLAN, P->grid, P->s));
+    TRY(sdp_fft2d(P->grid, P->residual, G, SDP_FFT_INVERSE,
+                  SDP_FFT_SHIFT_IN | SDP_FFT_SHIFT_OUT | SDP_FFT_REAL_OUT | SDP_FFT_CORRECT_OUT, P->taper,
+                  P->fft_work, P->s));
+    TRY(sdp_image_stats_async(pixels(P), P->residual, P->stats, P->stats_work, P->s));
+    TRY(sdp_hogbom_ex(G, p->n_minor, p->max_sources, P->residual, P->psf, P->psf_max, p->loop_gain,
+                      p->noise_factor, p->weak_source_percent, p->cell_size, P->model, P->model, P->sources,
+                      P->num_sources, P->hog_work,
+                      (p->flags & SDP_PIPE_PSF_MIRRORED) ? 0 : SDP_HOGBOM_PSF_AS_STORED, P->s));
+    TRY(sdp_image_stats_async(pixels(P), P->residual, P->stats + 3, P->stats_work, P->s));
+    return SDP_OK;
+}
+
+/* PSF: grid unit visibilities, inverse FFT with the same shifts and
+ * correction as the residual, record the peak and divide by it
+ * (psf_host_set_up, top.c:449-512). */
+static int make_psf(sdp_pipeline *P)
+{
+    const int G = P->p.grid_size;
+    TRY(sdp_fill_complex((size_t)P->p.n_vis, P->vis_resid, 1.0f, 0.0f, P->s));
+    TRY(sdp_grid(P->plan, P->gk, P->vis_resid, SDP_ORDER_PLAN, P->grid, P->s));
+    TRY(sdp_fft2d(P->grid, P->psf, G, SDP_FFT_INVERSE,
+                  SDP_FFT_SHIFT_IN | SDP_FFT_SHIFT_OUT | SDP_FFT_REAL_OUT | SDP_FFT_CORRECT_OUT, P->taper,
+                  P->fft_work, P->s));
+    TRY(sdp_memset_zero(P->psf + pixels(P), sizeof(float), P->s));
+    double h[3];
+    TRY(sdp_image_stats_async(pixels(P), P->psf, P->stats, P->stats_work, P->s));
+    TRY(sdp_memcpy_dtoh(h, P->stats, sizeof h, P->s));
+    if (!(h[1] > 0.0)) return FAIL(SDP_E_ARG, "PSF peak is not positive (no visibility on the grid?)");
+    P->psf_max = (double)(float)h[1];
+    TRY(sdp_divide(pixels(P), P->psf, P->psf_max, P->s));
+    return SDP_OK;
+}
+
+static void release(sdp_pipeline *P)
+{
+    if (P->graph) (void)sdp_graph_destroy(P->graph);
+    if (P->ev0) (void)sdp_event_destroy(P->ev0);
+    if (P->ev1) (void)sdp_event_destroy(P->ev1);
+    if (P->plan) (void)sdp_grid_plan_destroy(P->plan);
+    void *bufs[] = {P->vis_meas, P->vis_resid, P->grid, P->residual, P->model, P->psf, P->sources,
+                    P->num_sources, P->fft_work, P->hog_work, P->stats_work, P->stats};
+    for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; ++i) (void)sdp_device_free(bufs[i]);
+    free(P);
+}
+
+int sdp_pipeline_create(sdp_pipeline **out, const sdp_pipeline_params *params, const int *supports,
+                        const float *d_uvw, const float *d_vis, const float *d_grid_kernels,
+                        const float *d_degrid_kernels, const float *d_taper, sdp_stream s)
+{
+    if (!out || !params || !supports || !d_uvw || !d_vis || !d_grid_kernels || !d_degrid_kernels || !d_taper)
+        return FAIL(SDP_E_ARG, "null argument");
+    *out = NULL;
+    const sdp_pipeline_params *p = params;
+    const int G = p->grid_size;
+    if (G < 16 || G > 8192 || (G & (G - 1)) || p->n_vis < 1 || p->n_kernels < 1 || p->oversampling < 1 ||
+        p->n_minor < 0 || p->max_sources < 1 || !(p->cell_size > 0.0))
+        return FAIL(SDP_E_ARG, "bad parameters (grid size, counts or cell size)");
+    TRY(sdp_init());
+
+    sdp_pipeline *P = (sdp_pipeline *)calloc(1, sizeof *P);
+    if (!P) return FAIL(SDP_E_NOMEM, "out of host memory");
+    P->p = *p;
+    P->s = s;
+    P->gk = d_grid_kernels;
+    P->dk = d_degrid_kernels;
+    P->taper = d_taper;
+
+    sdp_grid_params gp;
+    memset(&gp, 0, sizeof gp);
+    gp.grid_size = G;
+    gp.oversampling = p->oversampling;
+    gp.n_kernels = p->n_kernels;
+    gp.snap_mode = SDP_SNAP_STD;
+    gp.uv_scale = p->uv_scale;
+    gp.w_scale = p->w_scale;
+    gp.tile = p->tile;
+    gp.chunk = p->chunk;
+
+    const size_t npix = (size_t)G * G;
+    const size_t vis_bytes = sizeof(float) * 2 * (size_t)p->n_vis;
+    int rc = sdp_grid_plan_create(&P->plan, &gp, supports, d_uvw, p->n_vis, s);
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->vis_meas, vis_bytes);
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->vis_resid, vis_bytes);
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->grid, sizeof(float) * 2 * npix);
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->residual, sizeof(float) * npix);
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->model, sizeof(float) * npix);
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->psf, sizeof(float) * (npix + 4));
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->sources, sizeof(float) * 3 * (size_t)p->max_sources);
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->num_sources, 16);
+    if (rc == SDP_OK) rc = sdp_device_malloc(&P->fft_work, sdp_fft_work_bytes(G));
+    if (rc == SDP_OK) rc = sdp_device_malloc(&P->hog_work, sdp_hogbom_work_bytes(G));
+    if (rc == SDP_OK) rc = sdp_device_malloc(&P->stats_work, sdp_image_stats_work_bytes());
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->stats, sizeof(double) * 8);
+    if (rc == SDP_OK) rc = sdp_event_create(&P->ev0);
+    if (rc == SDP_OK) rc = sdp_event_create(&P->ev1);
+    if (rc == SDP_OK) rc = sdp_grid_plan_to_plan_order(P->plan, d_vis, P->vis_meas, s);
+    if (rc == SDP_OK) rc = make_psf(P);
+    if (rc == SDP_OK) rc = sdp_pipeline_reset(P);
+    if (rc != SDP_OK) {
+        release(P);
+        return rc;
+    }
+    *out = P;
+    return SDP_OK;
+}
+
+int sdp_pipeline_destroy(sdp_pipeline *P)
+{
+    if (P) {
+        (void)sdp_stream_sync(P->s);
+        release(P);
+    }
+    return SDP_OK;
+}
+
+int sdp_pipeline_reset(sdp_pipeline *P)
+{
+    if (!P) return FAIL(SDP_E_ARG, "null pipeline");
+    TRY(sdp_memset_zero(P->model, sizeof(float) * pixels(P), P->s));
+    TRY(sdp_memset_zero(P->sources, sizeof(float) * 3 * (size_t)P->p.max_sources, P->s));
+    TRY(sdp_memset_zero(P->num_sources, 16, P->s));
+    P->cycles_done = 0;
+    return sdp_stream_sync(P->s);
+}
+
+int sdp_pipeline_load_model(sdp_pipeline *P, const float *d_model)
+{
+    if (!P || !d_model) return FAIL(SDP_E_ARG, "null argument");
+    TRY(sdp_memcpy_dtod(P->model, d_model, sizeof(float) * pixels(P), P->s));
+    return sdp_stream_sync(P->s);
+}
+
+int sdp_pipeline_cycle(sdp_pipeline *P, double *rms_residual, double *rms_after, float *ms)
+{
+    if (!P) return FAIL(SDP_E_ARG, "null pipeline");
+    TRY(sdp_event_record(P->ev0, P->s));
+    if ((P->p.flags & SDP_PIPE_GRAPH) && P->cycles_done > 0) {
+        if (!P->graph) {
+            /* the first cycle ran eagerly, so every lazily created resource
+             * (twiddle tables, kernel attributes) exists before capture */
+            TRY(sdp_graph_begin_capture(P->s));
+            int rc = enqueue_cycle(P);
+            sdp_graph g = NULL;
+            int rc2 = sdp_graph_end_capture(P->s, &g);
+            if (rc != SDP_OK) {
+                if (g) (void)sdp_graph_destroy(g);
+                return rc;
+            }
+            TRY(rc2);
+            P->graph = g;
+            TRY(sdp_event_record(P->ev0, P->s));
+        }
+        TRY(sdp_graph_launch(P->graph, P->s));
+    } else {
+        TRY(enqueue_cycle(P));
+    }
+    TRY(sdp_event_record(P->ev1, P->s));
+    double h[6];
+    TRY(sdp_memcpy_dtoh(h, P->stats, sizeof h, P->s));
+    float t = 0.f;
+    TRY(sdp_event_elapsed_ms(P->ev0, P->ev1, &t));
+    P->cycles_done++;
+    const double n = (double)pixels(P);
+    if (rms_residual) *rms_residual = sqrt(h[0] / n) / P->psf_max;
+    if (rms_after) *rms_after = sqrt(h[3] / n) / P->psf_max;
+    if (ms) *ms = t;
+    return SDP_OK;
+}
+
+const float *sdp_pipeline_image(const sdp_pipeline *P, int which)
+{
+    if (!P) return NULL;
+    switch (which) {
+    case SDP_IMAGE_RESIDUAL: return P->residual;
+    case SDP_IMAGE_MODEL: return P->model;
+    case SDP_IMAGE_PSF: return P->psf;
+    default: return NULL;
+    }
+}
+
+double sdp_pipeline_psf_max(const sdp_pipeline *P) { return P ? P->psf_max : 0.0; }
+
+int sdp_pipeline_sources(const sdp_pipeline *P, float *h_sources, int *n)
+{
+    if (!P || !n) return FAIL(SDP_E_ARG, "null argument");
+    int cnt = 0;
+    TRY(sdp_memcpy_dtoh(&cnt, P->num_sources, sizeof cnt, P->s));
+    if (cnt < 0) cnt = 0;
+    if (cnt > P->p.max_sources) cnt = P->p.max_sources;
+    if (h_sources && cnt) TRY(sdp_memcpy_dtoh(h_sources, P->sources, sizeof(float) * 3 * (size_t)cnt, P->s));
+    *n = cnt;
+    return SDP_OK;
+}
+
+const sdp_grid_plan *sdp_pipeline_plan(const sdp_pipeline *P) { return P ? P->plan : NULL; }

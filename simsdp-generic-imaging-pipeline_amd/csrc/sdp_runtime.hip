@@ -69,6 +69,11 @@ int sdp_init(void)
 
 const char *sdp_last_error(void) { return g_error; }
 
+int sdp_set_last_error(int code, const char *message)
+{
+    return sdp::set_error(code, "%s", message ? message : "");
+}
+
 int sdp_device_malloc(void **ptr, size_t bytes)
 {
     SDP_REQUIRE(ptr, "sdp_device_malloc: null ptr");
@@ -190,6 +195,37 @@ int sdp_event_sync(sdp_event e)
 int sdp_event_elapsed_ms(sdp_event a, sdp_event b, float *ms)
 {
     SDP_HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+    return SDP_OK;
+}
+
+int sdp_graph_begin_capture(sdp_stream s)
+{
+    SDP_HIP_TRY(hipStreamBeginCapture(sdp::stream_of(s), hipStreamCaptureModeThreadLocal));
+    return SDP_OK;
+}
+
+int sdp_graph_end_capture(sdp_stream s, sdp_graph *graph)
+{
+    SDP_REQUIRE(graph, "sdp_graph_end_capture: null output");
+    hipGraph_t g = nullptr;
+    SDP_HIP_TRY(hipStreamEndCapture(sdp::stream_of(s), &g));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return sdp::set_error(SDP_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    *graph = (sdp_graph)ex;
+    return SDP_OK;
+}
+
+int sdp_graph_launch(sdp_graph graph, sdp_stream s)
+{
+    SDP_HIP_TRY(hipGraphLaunch((hipGraphExec_t)graph, sdp::stream_of(s)));
+    return SDP_OK;
+}
+
+int sdp_graph_destroy(sdp_graph graph)
+{
+    if (graph) SDP_HIP_TRY(hipGraphExecDestroy((hipGraphExec_t)graph));
     return SDP_OK;
 }
 

@@ -35,7 +35,11 @@ DFT_REFERENCE_FP32, DFT_FP64_PHASE, DFT_MFMA = 0, 1, 2
 SNAP_STD, SNAP_GRIDDING_ACTOR = 0, 1
 ORDER_ORIGINAL, ORDER_PLAN = 0, 1
 FFT_FORWARD, FFT_INVERSE = -1, 1
-FFT_SHIFT_IN, FFT_SHIFT_OUT, FFT_REAL_IN, FFT_REAL_OUT, FFT_CORRECT_OUT = 1, 2, 4, 8, 16
+FFT_SHIFT_IN, FFT_SHIFT_OUT, FFT_REAL_IN, FFT_REAL_OUT, FFT_CORRECT_OUT, FFT_CORRECT_IN = 1, 2, 4, 8, 16, 32
+HOGBOM_PSF_AS_STORED = 1
+# major-cycle pipeline (sdp_pipeline.h)
+PIPE_GRAPH, PIPE_REFERENCE_PREDICT, PIPE_PSF_MIRRORED = 1, 2, 4
+IMAGE_RESIDUAL, IMAGE_MODEL, IMAGE_PSF = 0, 1, 2
 
 
 class SdpError(RuntimeError):
@@ -160,7 +164,28 @@ def load() -> ctypes.CDLL:
         "sdp_hogbom_work_bytes": (c_size_t, [c_int]),
         "sdp_hogbom": (c_int, [c_int, c_int, c_int, P, P, c_double, c_double, c_double, c_double, c_double,
                                P, P, P, P, P, P]),
+        "sdp_hogbom_ex": (c_int, [c_int, c_int, c_int, P, P, c_double, c_double, c_double, c_double, c_double,
+                                  P, P, P, P, P, c_int, P]),
         "sdp_dft_predict": (c_int, [c_int, c_int, P, P, P, c_int, P]),
+        "sdp_set_last_error": (c_int, [c_int, c_char_p]),
+        "sdp_image_stats": (c_int, [c_size_t, P, ctypes.POINTER(c_double), ctypes.POINTER(c_float),
+                                    ctypes.POINTER(c_float), P]),
+        "sdp_image_stats_work_bytes": (c_size_t, []),
+        "sdp_image_stats_async": (c_int, [c_size_t, P, P, P, P]),
+        "sdp_fill_complex": (c_int, [c_size_t, P, c_float, c_float, P]),
+        "sdp_divide": (c_int, [c_size_t, P, c_double, P]),
+        "sdp_graph_begin_capture": (c_int, [P]),
+        "sdp_graph_end_capture": (c_int, [P, ctypes.POINTER(c_void_p)]),
+        "sdp_graph_launch": (c_int, [P, P]), "sdp_graph_destroy": (c_int, [P]),
+        "sdp_pipeline_create": (c_int, [ctypes.POINTER(c_void_p), P, P, P, P, P, P, P, P]),
+        "sdp_pipeline_destroy": (c_int, [P]), "sdp_pipeline_reset": (c_int, [P]),
+        "sdp_pipeline_load_model": (c_int, [P, P]),
+        "sdp_pipeline_cycle": (c_int, [P, ctypes.POINTER(c_double), ctypes.POINTER(c_double),
+                                       ctypes.POINTER(c_float)]),
+        "sdp_pipeline_image": (c_void_p, [P, c_int]),
+        "sdp_pipeline_psf_max": (c_double, [P]),
+        "sdp_pipeline_sources": (c_int, [P, P, ctypes.POINTER(c_int)]),
+        "sdp_pipeline_plan": (c_void_p, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -371,6 +396,133 @@ def hogbom(grid_size, n_minor, max_sources, d_residual, d_psf, psf_max, loop_gai
     check(load().sdp_hogbom(grid_size, n_minor, max_sources, _ptr(d_residual), _ptr(d_psf), psf_max, loop_gain,
                             noise_factor, weak, cell_size, _ptr(d_model_in), _ptr(d_model_out), _ptr(d_sources),
                             _ptr(d_num_sources), _ptr(d_work), stream))
+
+
+def hogbom_ex(grid_size, n_minor, max_sources, d_residual, d_psf, psf_max, loop_gain, noise_factor, weak,
+              cell_size, d_model_in, d_model_out, d_sources, d_num_sources, d_work, flags=0, stream=None):
+    check(load().sdp_hogbom_ex(grid_size, n_minor, max_sources, _ptr(d_residual), _ptr(d_psf), psf_max, loop_gain,
+                               noise_factor, weak, cell_size, _ptr(d_model_in), _ptr(d_model_out), _ptr(d_sources),
+                               _ptr(d_num_sources), _ptr(d_work), flags, stream))
+
+
+def image_stats(n, d_img, stream=None) -> tuple[float, float, float]:
+    """(sum of squares, max, max |value|) of n floats on the device."""
+    sq, mx, amx = c_double(), c_float(), c_float()
+    check(load().sdp_image_stats(n, _ptr(d_img), ctypes.byref(sq), ctypes.byref(mx), ctypes.byref(amx), stream))
+    return sq.value, mx.value, amx.value
+
+
+def fill_complex(n, d, re, im, stream=None):
+    check(load().sdp_fill_complex(n, _ptr(d), re, im, stream))
+
+
+def divide(n, d, by, stream=None):
+    check(load().sdp_divide(n, _ptr(d), by, stream))
+
+
+class Graph:
+    """A captured stream sequence (sdp_graph_begin_capture .. end_capture)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @staticmethod
+    def begin(stream=None):
+        check(load().sdp_graph_begin_capture(stream))
+
+    @classmethod
+    def end(cls, stream=None) -> "Graph":
+        h = c_void_p()
+        check(load().sdp_graph_end_capture(stream, ctypes.byref(h)))
+        return cls(h.value)
+
+    def launch(self, stream=None):
+        check(load().sdp_graph_launch(self.handle, stream))
+
+    def close(self):
+        if self.handle:
+            check(load().sdp_graph_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _PipelineParams(ctypes.Structure):
+    _fields_ = [("grid_size", c_int), ("oversampling", c_int), ("n_kernels", c_int), ("n_vis", c_int),
+                ("uv_scale", c_double), ("w_scale", c_double), ("cell_size", c_double),
+                ("n_minor", c_int), ("max_sources", c_int), ("loop_gain", c_double), ("noise_factor", c_double),
+                ("weak_source_percent", c_double), ("flags", c_int), ("tile", c_int), ("chunk", c_int)]
+
+
+class Pipeline:
+    """The device-resident major cycle (sdp_pipeline.h)."""
+
+    def __init__(self, grid_size, oversampling, supports, uv_scale, w_scale, cell_size, d_uvw, d_vis, n_vis,
+                 d_grid_kernels, d_degrid_kernels, d_taper, n_minor, max_sources=None, loop_gain=0.1,
+                 noise_factor=0.0, weak_source_percent=0.0, flags=0, tile=0, chunk=0, stream=None):
+        self._sup = np.ascontiguousarray(supports, dtype=np.int32)
+        self.grid_size = grid_size
+        self.params = _PipelineParams(grid_size, oversampling, self._sup.shape[0], n_vis, uv_scale, w_scale,
+                                      cell_size, n_minor, max_sources or max(n_minor, 1), loop_gain, noise_factor,
+                                      weak_source_percent, flags, tile, chunk)
+        # the tables and taper are used by every cycle: keep them alive
+        self._keep = (d_grid_kernels, d_degrid_kernels, d_taper)
+        h = c_void_p()
+        check(load().sdp_pipeline_create(ctypes.byref(h), ctypes.byref(self.params), _ptr(self._sup), _ptr(d_uvw),
+                                         _ptr(d_vis), _ptr(d_grid_kernels), _ptr(d_degrid_kernels), _ptr(d_taper),
+                                         stream))
+        self.handle = h.value
+
+    def cycle(self) -> tuple[float, float, float]:
+        """One major cycle: (residual RMS before CLEAN, after CLEAN, ms)."""
+        a, b, t = c_double(), c_double(), c_float()
+        check(load().sdp_pipeline_cycle(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)))
+        return a.value, b.value, t.value
+
+    def reset(self):
+        check(load().sdp_pipeline_reset(self.handle))
+
+    def load_model(self, d_model):
+        check(load().sdp_pipeline_load_model(self.handle, _ptr(d_model)))
+
+    def image(self, which) -> np.ndarray:
+        G = self.grid_size
+        ptr = load().sdp_pipeline_image(self.handle, which)
+        out = np.empty((G, G), np.float32)
+        check(load().sdp_memcpy_dtoh(out.ctypes.data, ptr, out.nbytes, None))
+        return out
+
+    @property
+    def psf_max(self) -> float:
+        return float(load().sdp_pipeline_psf_max(self.handle))
+
+    def sources(self) -> np.ndarray:
+        buf = np.zeros((self.params.max_sources, 3), np.float32)
+        n = c_int()
+        check(load().sdp_pipeline_sources(self.handle, buf.ctypes.data, ctypes.byref(n)))
+        return buf[:n.value].copy()
+
+    def plan_info(self) -> dict:
+        info = np.zeros(10, np.int64)
+        check(load().sdp_grid_plan_info(load().sdp_pipeline_plan(self.handle), info.ctypes.data))
+        keys = ["kept", "dropped", "tiles", "chunks", "tile", "chunk_taps", "max_half_support", "scratch_bytes",
+                "class_launches", "taps"]
+        return dict(zip(keys, (int(v) for v in info)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            check(load().sdp_pipeline_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def dft_predict(n_vis, n_sources, d_sources, d_uvw, d_out, mode=DFT_FP64_PHASE, stream=None):

@@ -276,3 +276,35 @@ def small_case(n_vis: int = 4000, grid_size: int = 128, n_planes: int = 4, overs
     src = np.zeros((0, 3), np.float32)
     return Dataset("small", grid_size, cell, oversampling, uvw, vis, gk, dk, sup, src, max_w=max_w,
                    n_minor=50, n_major=1)
+
+
+def sky_case(grid_size: int = 256, n_ant: int = 48, n_time: int = 64, n_src: int = 8, oversampling: int = 8,
+             seed: int = 21, noise: float = 0.0) -> Dataset:
+    """A small end-to-end imaging case: n_ant antennas log-radial 20 m to
+    1.5 km (seed), HA -3..+3 h in n_time steps at 150 MHz, 4 w-planes
+    (half supports 4, 4, 6, 8, each wide enough for its w-term), n_src point
+    sources on pixel centres inside the central half of the field with flux
+    U[0.2, 1] Jy, visibilities from the FP64 DFT plus optional Gaussian
+    noise (sigma `noise` per component)."""
+    rng = np.random.default_rng(seed)
+    enu = log_radial_layout(n_ant, 20.0, 1500.0, seed=seed)
+    uvw = uvw_tracks(enu, -30.0, -30.0, np.linspace(-3.0, 3.0, n_time), 150e6).astype(np.float32)
+    sup_h = [4, 4, 6, 8]
+    u_max = float(np.abs(uvw[:, :2]).max())
+    cell = (grid_size / 2 - max(sup_h) - 2) / (u_max * grid_size)
+    fov = cell * grid_size
+    max_w = float(np.abs(uvw[:, 2]).max()) * 1.0001
+    w_scale = (len(sup_h) - 1) ** 2 / max_w
+    assert all(3 + (p * p / w_scale) * fov * fov / 2 <= h for p, h in enumerate(sup_h))
+    gk, dk, sup = kernel_tables(sup_h, oversampling, fov=fov, w_scale=w_scale)
+    px = rng.choice(np.arange(grid_size // 4, 3 * grid_size // 4), size=(n_src, 2))
+    lm = (px - grid_size // 2) * cell
+    flux = rng.uniform(0.2, 1.0, n_src)
+    src = np.concatenate([lm, flux[:, None]], axis=1).astype(np.float32)
+    vis = dft_f64(uvw, src)
+    if noise > 0:
+        vis += rng.normal(0.0, noise, vis.shape)
+    ds = Dataset("sky", grid_size, cell, oversampling, uvw, vis.astype(np.float32), gk, dk, sup, src,
+                 max_w=max_w, n_minor=200, n_major=4)
+    ds.extra["pixels"] = px
+    return ds

@@ -12,11 +12,33 @@ import sdp_hip
 def test_library_exports_every_declared_function():
     lib = ctypes.CDLL(sdp_hip.LIB_PATH)
     missing = []
-    for header in ("sdp_actors.h", "sdp_device.h"):
+    for header in ("sdp_actors.h", "sdp_device.h", "sdp_pipeline.h"):
         names = sdp_hip.header_functions(header)
-        assert len(names) > 10, header
+        assert len(names) > 5, header
         missing += [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_python_binding_covers_device_and_pipeline_headers():
+    """Every device-layer and pipeline function has a ctypes signature."""
+    lib = sdp_hip.load()
+    unbound = []
+    for header in ("sdp_device.h", "sdp_pipeline.h"):
+        for n in sdp_hip.header_functions(header):
+            if getattr(lib, n).argtypes is None:
+                unbound.append(n)
+    assert not unbound, unbound
+
+
+def test_pipeline_argument_errors_need_no_gpu():
+    """Argument checks run before any device call and report a message."""
+    lib = sdp_hip.load()
+    h = ctypes.c_void_p()
+    rc = lib.sdp_pipeline_create(ctypes.byref(h), None, None, None, None, None, None, None, None)
+    assert rc == -2 and h.value is None
+    assert b"null argument" in lib.sdp_last_error()
+    assert lib.sdp_pipeline_cycle(None, None, None, None) == -2
+    assert lib.sdp_pipeline_image(None, 0) is None
 
 
 def test_actor_prototypes_cover_reference_symbols():
@@ -44,7 +66,8 @@ def test_config_layout_matches_reference():
 
 def test_header_compiles_as_c(tmp_path):
     src = tmp_path / "t.c"
-    src.write_text('#include "sdp_actors.h"\n#include "sdp_device.h"\nint main(void){return (int)sizeof(Config) - 360;}\n')
+    src.write_text('#include "sdp_actors.h"\n#include "sdp_device.h"\n#include "sdp_pipeline.h"\n'
+                   'int main(void){return (int)sizeof(Config) - 360;}\n')
     exe = tmp_path / "t"
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", sdp_hip.INCLUDE_DIR, str(src), "-o", str(exe)],
                    check=True)
