@@ -68,6 +68,15 @@ def major_cycle(hip, sdp_synth, d, d_uvw, d_meas, d_gk, d_dk, tile, chunk, targe
     after the first.  Reports the residual RMS per cycle (Jy) and the time
     to bring it to target_frac of the dirty image's RMS."""
     d_tap = hip.DeviceBuffer.from_array(sdp_synth.taper_padded(d.grid_size))
+    # stage breakdown from a profiled (eager) pipeline: second cycle
+    prof = hip.Pipeline(d.grid_size, d.oversampling, d.supports, d.uv_scale, d.w_scale, d.cell_size, d_uvw, d_meas,
+                        d.n_vis, d_gk, d_dk, d_tap, d.n_minor, loop_gain=0.1, flags=hip.PIPE_PROFILE,
+                        tile=tile, chunk=chunk)
+    prof.cycle()
+    prof.cycle()
+    stage_ms = {k: round(v, 3) for k, v in prof.stage_ms().items()}
+    prof.close()
+    log(f"major-cycle stages (ms): {stage_ms}")
     t = time.perf_counter()
     pipe = hip.Pipeline(d.grid_size, d.oversampling, d.supports, d.uv_scale, d.w_scale, d.cell_size, d_uvw, d_meas,
                         d.n_vis, d_gk, d_dk, d_tap, d.n_minor, loop_gain=0.1, flags=hip.PIPE_GRAPH,
@@ -89,7 +98,14 @@ def major_cycle(hip, sdp_synth, d, d_uvw, d_meas, d_gk, d_dk, tile, chunk, targe
         log(f"major cycle {c}: residual RMS {before:.4g} Jy -> {after:.4g} after CLEAN, {ms:.2f} ms")
     n_src = pipe.sources().shape[0]
     pipe.close()
+    G = d.grid_size
+    hog_ms = stage_ms["clean"]
+    # Hogbom per iteration: scan 4 G^2 bytes + PSF window read and residual
+    # window read-modify-write 12 A_win (A_win = G^2 for a central peak)
+    hog_bytes = (4 + 12) * G * G * d.n_minor
     return {"minor": "hogbom", "n_minor": d.n_minor, "gain": 0.1, "setup_s": round(setup_s, 3),
+            "stage_ms": stage_ms, "hogbom_us_per_iter": round(hog_ms * 1e3 / max(d.n_minor, 1), 2),
+            "hogbom_gbs": round(hog_bytes / (hog_ms * 1e-3) / 1e9, 1),
             "cycles": cycles, "s_total": round(elapsed_ms * 1e-3, 4),
             "target_rms": float(f"{target:.6g}"), "target": f"{target_frac} x dirty-image RMS",
             "s_to_target": None if s_to_target is None else round(s_to_target, 4),

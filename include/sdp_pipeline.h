@@ -41,7 +41,8 @@ extern "C" {
 enum {
     SDP_PIPE_GRAPH = 1,              /* replay each cycle from a captured graph */
     SDP_PIPE_REFERENCE_PREDICT = 2,  /* no taper division before the forward FFT */
-    SDP_PIPE_PSF_MIRRORED = 4        /* Hogbom reads the PSF x-mirrored */
+    SDP_PIPE_PSF_MIRRORED = 4,       /* Hogbom reads the PSF x-mirrored */
+    SDP_PIPE_PROFILE = 8             /* time each stage with events (cycles run eagerly) */
 };
 
 typedef struct sdp_pipeline_params {
@@ -79,6 +80,12 @@ int sdp_pipeline_destroy(sdp_pipeline *p);
  * subtraction; ms: wall time of the cycle on the stream.  Any output may be
  * NULL.  Synchronous. */
 int sdp_pipeline_cycle(sdp_pipeline *p, double *rms_residual, double *rms_after, float *ms);
+
+/* With SDP_PIPE_PROFILE: milliseconds of each stage of the last cycle,
+ * ms[SDP_STAGE_COUNT]. */
+enum { SDP_STAGE_PREDICT_FFT = 0, SDP_STAGE_DEGRID_SUBTRACT, SDP_STAGE_GRID, SDP_STAGE_INVERSE_FFT,
+       SDP_STAGE_CLEAN, SDP_STAGE_COUNT };
+int sdp_pipeline_stage_ms(const sdp_pipeline *p, float *ms);
 
 /* Restarts from an empty model (PSF and plan kept). */
 int sdp_pipeline_reset(sdp_pipeline *p);

@@ -49,12 +49,21 @@ struct sdp_pipeline {
     int cycles_done;
     sdp_graph graph;
     sdp_event ev0, ev1;
+    sdp_event stage_ev[SDP_STAGE_COUNT + 1];
+    float stage_ms[SDP_STAGE_COUNT];
 };
 
 #define TRY(x) do { int rc_ = (x); if (rc_ != SDP_OK) return rc_; } while (0)
 #define FAIL(code, msg) sdp_set_last_error((code), "sdp_pipeline: " msg)
 
 static size_t pixels(const sdp_pipeline *P) { return (size_t)P->p.grid_size * (size_t)P->p.grid_size; }
+
+/* event between stages when profiling */
+static int mark(sdp_pipeline *P, int stage)
+{
+    if (!(P->p.flags & SDP_PIPE_PROFILE)) return SDP_OK;
+    return sdp_event_record(P->stage_ev[stage], P->s);
+}
 
 /* Stages 1-7 above: launches only, no synchronisation. */
 static int enqueue_cycle(sdp_pipeline *P)
@@ -63,18 +72,24 @@ static int enqueue_cycle(sdp_pipeline *P)
     const int G = p->grid_size;
     const int predict_flags = SDP_FFT_REAL_IN | SDP_FFT_SHIFT_IN | SDP_FFT_SHIFT_OUT |
                               ((p->flags & SDP_PIPE_REFERENCE_PREDICT) ? 0 : SDP_FFT_CORRECT_IN);
+    TRY(mark(P, SDP_STAGE_PREDICT_FFT));
     TRY(sdp_fft2d(P->model, P->grid, G, SDP_FFT_FORWARD, predict_flags, P->taper, P->fft_work, P->s));
+    TRY(mark(P, SDP_STAGE_DEGRID_SUBTRACT));
     TRY(sdp_degrid(P->plan, P->dk, P->grid, P->vis_meas, SDP_ORDER_PLAN, P->vis_resid, P->s));
+    TRY(mark(P, SDP_STAGE_GRID));
     TRY(sdp_grid(P->plan, P->gk, P->vis_resid, SDP_ORDER_PLAN, P->grid, P->s));
+    TRY(mark(P, SDP_STAGE_INVERSE_FFT));
     TRY(sdp_fft2d(P->grid, P->residual, G, SDP_FFT_INVERSE,
                   SDP_FFT_SHIFT_IN | SDP_FFT_SHIFT_OUT | SDP_FFT_REAL_OUT | SDP_FFT_CORRECT_OUT, P->taper,
                   P->fft_work, P->s));
+    TRY(mark(P, SDP_STAGE_CLEAN));
     TRY(sdp_image_stats_async(pixels(P), P->residual, P->stats, P->stats_work, P->s));
     TRY(sdp_hogbom_ex(G, p->n_minor, p->max_sources, P->residual, P->psf, P->psf_max, p->loop_gain,
                       p->noise_factor, p->weak_source_percent, p->cell_size, P->model, P->model, P->sources,
                       P->num_sources, P->hog_work,
                       (p->flags & SDP_PIPE_PSF_MIRRORED) ? 0 : SDP_HOGBOM_PSF_AS_STORED, P->s));
     TRY(sdp_image_stats_async(pixels(P), P->residual, P->stats + 3, P->stats_work, P->s));
+    TRY(mark(P, SDP_STAGE_COUNT));
     return SDP_OK;
 }
 
@@ -104,6 +119,8 @@ static void release(sdp_pipeline *P)
     if (P->graph) (void)sdp_graph_destroy(P->graph);
     if (P->ev0) (void)sdp_event_destroy(P->ev0);
     if (P->ev1) (void)sdp_event_destroy(P->ev1);
+    for (int i = 0; i <= SDP_STAGE_COUNT; ++i)
+        if (P->stage_ev[i]) (void)sdp_event_destroy(P->stage_ev[i]);
     if (P->plan) (void)sdp_grid_plan_destroy(P->plan);
     void *bufs[] = {P->vis_meas, P->vis_resid, P->grid, P->residual, P->model, P->psf, P->sources,
                     P->num_sources, P->fft_work, P->hog_work, P->stats_work, P->stats};
@@ -161,6 +178,8 @@ int sdp_pipeline_create(sdp_pipeline **out, const sdp_pipeline_params *params, c
     if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->stats, sizeof(double) * 8);
     if (rc == SDP_OK) rc = sdp_event_create(&P->ev0);
     if (rc == SDP_OK) rc = sdp_event_create(&P->ev1);
+    for (int i = 0; rc == SDP_OK && i <= SDP_STAGE_COUNT && (p->flags & SDP_PIPE_PROFILE); ++i)
+        rc = sdp_event_create(&P->stage_ev[i]);
     if (rc == SDP_OK) rc = sdp_grid_plan_to_plan_order(P->plan, d_vis, P->vis_meas, s);
     if (rc == SDP_OK) rc = make_psf(P);
     if (rc == SDP_OK) rc = sdp_pipeline_reset(P);
@@ -202,7 +221,8 @@ int sdp_pipeline_cycle(sdp_pipeline *P, double *rms_residual, double *rms_after,
 {
     if (!P) return FAIL(SDP_E_ARG, "null pipeline");
     TRY(sdp_event_record(P->ev0, P->s));
-    if ((P->p.flags & SDP_PIPE_GRAPH) && P->cycles_done > 0) {
+    const int use_graph = (P->p.flags & SDP_PIPE_GRAPH) && !(P->p.flags & SDP_PIPE_PROFILE);
+    if (use_graph && P->cycles_done > 0) {
         if (!P->graph) {
             /* the first cycle ran eagerly, so every lazily created resource
              * (twiddle tables, kernel attributes) exists before capture */
@@ -227,6 +247,9 @@ int sdp_pipeline_cycle(sdp_pipeline *P, double *rms_residual, double *rms_after,
     TRY(sdp_memcpy_dtoh(h, P->stats, sizeof h, P->s));
     float t = 0.f;
     TRY(sdp_event_elapsed_ms(P->ev0, P->ev1, &t));
+    if (P->p.flags & SDP_PIPE_PROFILE)
+        for (int i = 0; i < SDP_STAGE_COUNT; ++i)
+            TRY(sdp_event_elapsed_ms(P->stage_ev[i], P->stage_ev[i + 1], &P->stage_ms[i]));
     P->cycles_done++;
     const double n = (double)pixels(P);
     if (rms_residual) *rms_residual = sqrt(h[0] / n) / P->psf_max;
@@ -244,6 +267,14 @@ const float *sdp_pipeline_image(const sdp_pipeline *P, int which)
     case SDP_IMAGE_PSF: return P->psf;
     default: return NULL;
     }
+}
+
+int sdp_pipeline_stage_ms(const sdp_pipeline *P, float *ms)
+{
+    if (!P || !ms) return FAIL(SDP_E_ARG, "null argument");
+    if (!(P->p.flags & SDP_PIPE_PROFILE)) return FAIL(SDP_E_ARG, "pipeline created without SDP_PIPE_PROFILE");
+    memcpy(ms, P->stage_ms, sizeof P->stage_ms);
+    return SDP_OK;
 }
 
 double sdp_pipeline_psf_max(const sdp_pipeline *P) { return P ? P->psf_max : 0.0; }

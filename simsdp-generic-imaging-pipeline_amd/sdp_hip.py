@@ -38,7 +38,8 @@ FFT_FORWARD, FFT_INVERSE = -1, 1
 FFT_SHIFT_IN, FFT_SHIFT_OUT, FFT_REAL_IN, FFT_REAL_OUT, FFT_CORRECT_OUT, FFT_CORRECT_IN = 1, 2, 4, 8, 16, 32
 HOGBOM_PSF_AS_STORED = 1
 # major-cycle pipeline (sdp_pipeline.h)
-PIPE_GRAPH, PIPE_REFERENCE_PREDICT, PIPE_PSF_MIRRORED = 1, 2, 4
+PIPE_GRAPH, PIPE_REFERENCE_PREDICT, PIPE_PSF_MIRRORED, PIPE_PROFILE = 1, 2, 4, 8
+STAGES = ("predict_fft", "degrid_subtract", "grid", "inverse_fft", "clean")
 IMAGE_RESIDUAL, IMAGE_MODEL, IMAGE_PSF = 0, 1, 2
 
 
@@ -184,6 +185,7 @@ def load() -> ctypes.CDLL:
                                        ctypes.POINTER(c_float)]),
         "sdp_pipeline_image": (c_void_p, [P, c_int]),
         "sdp_pipeline_psf_max": (c_double, [P]),
+        "sdp_pipeline_stage_ms": (c_int, [P, P]),
         "sdp_pipeline_sources": (c_int, [P, P, ctypes.POINTER(c_int)]),
         "sdp_pipeline_plan": (c_void_p, [P]),
     }
@@ -495,6 +497,12 @@ class Pipeline:
         out = np.empty((G, G), np.float32)
         check(load().sdp_memcpy_dtoh(out.ctypes.data, ptr, out.nbytes, None))
         return out
+
+    def stage_ms(self) -> dict:
+        """Per-stage milliseconds of the last cycle (pipeline made with PIPE_PROFILE)."""
+        ms = np.zeros(len(STAGES), np.float32)
+        check(load().sdp_pipeline_stage_ms(self.handle, ms.ctypes.data))
+        return dict(zip(STAGES, (float(v) for v in ms)))
 
     @property
     def psf_max(self) -> float:

@@ -206,6 +206,23 @@ def test_pipeline_graph_replay_equals_eager(hip, sky):
     graph.close()
 
 
+def test_pipeline_profile_stages(hip, sky):
+    """Profiling records every stage and changes no result."""
+    eager = make_pipeline(hip, sky, n_minor=30)
+    prof = make_pipeline(hip, sky, n_minor=30, flags=hip.PIPE_PROFILE | hip.PIPE_GRAPH)
+    for _ in range(2):
+        e = eager.cycle()
+        p = prof.cycle()
+        assert e[:2] == p[:2]
+        st = prof.stage_ms()
+        assert set(st) == set(hip.STAGES) and all(v > 0 for v in st.values())
+        assert sum(st.values()) <= p[2] * 1.01 + 0.05
+    with pytest.raises(hip.SdpError):
+        eager.stage_ms()
+    eager.close()
+    prof.close()
+
+
 def test_pipeline_psf_orientation_matters(hip, sky):
     """Subtracting the PSF in its stored orientation cleans deeper than the
     reference's x-mirrored read (the w-term makes the PSF asymmetric)."""
