@@ -121,6 +121,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--accumulate", choices=["blocked", "compensated"], default="blocked",
+                    help="gridder accumulation inside a work item: float (blocked) or TwoSum; "
+                         "work items are merged TwoSum-compensated either way")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -129,6 +134,16 @@ def main():
     import numpy as np
     import sdp_hip as hip
     import sdp_synth
+
+    if args.copy_probe:
+        n = 1 << 30
+        a, b = hip.DeviceBuffer(n), hip.DeviceBuffer(n)
+        a.zero()
+        for _ in range(5):
+            hip.check(hip.load().sdp_device_copy(b.ptr, a.ptr, n, None))
+        hip.synchronize()
+        log(f"copied {n} bytes x 5")
+        return
 
     t = time.perf_counter()
     d = sdp_synth.config2() if args.config == "config2" else sdp_synth.config1(with_vis=False)
@@ -155,11 +170,14 @@ def main():
     d.vis = d_meas.download((N, 2), np.float32)
 
     t = time.perf_counter()
-    tile, chunk, tune_ms = hip.autotune(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, d_uvw, N, d_gk, d_dk)
+    acc_mode = hip.ACC_COMPENSATED if args.accumulate == "compensated" else hip.ACC_BLOCKED
+    tile, chunk, tune_ms = hip.autotune(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, d_uvw, N, d_gk, d_dk,
+                                        accumulate=acc_mode)
     tune_s = time.perf_counter() - t
     log(f"autotune {tune_s:.1f} s: tile {tile}, {chunk} taps per work item ({tune_ms:.2f} ms per pass)")
     t = time.perf_counter()
-    plan = hip.GridPlan(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, d_uvw, N, tile=tile, chunk=chunk)
+    plan = hip.GridPlan(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, d_uvw, N, tile=tile, chunk=chunk,
+                        accumulate=acc_mode)
     hip.synchronize()
     plan_s = time.perf_counter() - t
     info = plan.info()
@@ -273,7 +291,8 @@ def main():
         "config": {"workload": f"{d.name}: degrid(w-proj, weighted)+subtract+grid of {N} vis on a {G}^2 grid, "
                                f"{d.supports.shape[0]} w-planes h {d.supports[:, 0].min()}-{d.supports[:, 0].max()}, "
                                f"OS {d.oversampling}",
-                   "n_vis": N, "grid": G, "w_planes": int(d.supports.shape[0]), "oversampling": d.oversampling},
+                   "n_vis": N, "grid": G, "w_planes": int(d.supports.shape[0]), "oversampling": d.oversampling,
+                   "accumulate": args.accumulate},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "alg_bytes_per_step": alg[dom], "launches_per_step": launches,

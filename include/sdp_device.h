@@ -111,7 +111,14 @@ typedef struct sdp_grid_params {
     double w_scale;       /* (n_kernels - 1)^2 / max_w */
     int tile;             /* uv tile edge in cells, 0 = default */
     int chunk;            /* taps per work item, 0 = default */
+    int accumulate;       /* SDP_ACC_*: how a work item sums into its sub-grid */
 } sdp_grid_params;
+/* Grid-cell accumulation.  Every cell's value is a TwoSum-compensated sum of
+ * the work items' partial sums, merged in a fixed order.  Inside a work item
+ * (a few hundred visibilities at most per cell) the partial is a float sum
+ * (SDP_ACC_BLOCKED, the default) or itself TwoSum-compensated
+ * (SDP_ACC_COMPENSATED: about twice the arithmetic and LDS). */
+enum { SDP_ACC_BLOCKED = 0, SDP_ACC_COMPENSATED = 1 };
 
 typedef struct sdp_grid_plan sdp_grid_plan;
 

@@ -35,6 +35,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -43,7 +44,7 @@
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kBatch = 8;   // taps per lane whose kernel samples are loaded together
+constexpr int kRowBatch = 4;   // footprint rows per lane whose kernel samples are loaded together
 // half-support limits of the support classes; a class's window is
 // tile + 2 * (largest half support present in it)
 constexpr int kNumClasses = 6;
@@ -170,6 +171,11 @@ __global__ void permute_kernel(const float2 *in, const uint32_t *perm, int n, fl
 
 struct ChunkArgs {
     int G, OS, tile, ntx, chunk_base;
+    int lane_shift;   // log2 of the lanes per visibility (64 / copies)
+    int col_shift;    // log2 of the footprint columns a group covers at once
+    int copies;       // lane groups (and sub-grid copies) per wavefront
+    int os_shift;     // log2 OS when OS is a power of two, else -1
+    int tab_bytes;    // size of one phase-major kernel table
 };
 
 struct ChunkIO {
@@ -182,162 +188,307 @@ struct ChunkIO {
     float2 *slabs;             // MODE_GRID / MODE_DEGRID_GRID
 };
 
-// Per visibility: lanes cover the footprint's taps.  With nu <= 64 columns,
-// lane l takes column l % nu and rows l / nu, l / nu + 64 / nu, ... (lanes
-// past (64 / nu) * nu idle); wider footprints loop over column blocks.
-struct TapMap {
-    int col0, row0, row_step, ncol_blocks;
-    bool active;
-};
-
-__device__ __forceinline__ TapMap tap_map(int lane, int nu)
+// Lane groups.  A wavefront splits into C groups of L lanes (L a power of
+// two no smaller than the class's largest footprint extent, C = 64 / L or
+// fewer when LDS runs short).  Group g takes the chunk's visibilities
+// g, g + C, g + 2C, ... in order; within a visibility lane c of the group
+// owns footprint column c and walks the rows, so a group's lanes never touch
+// one cell at the same time.  Each group accumulates into its own copy of
+// the sub-grid; the copies are summed in group order when the slab is
+// written, so the result does not depend on timing.  The kernel-table row
+// offset of footprint row r is computed once per visibility, by lane r of
+// the group, and fetched with a lane shuffle.  Taps of idle lanes (past the
+// footprint, or past the chunk's last visibility) are predicated: they read
+// valid addresses and accumulate into a per-lane spare cell.
+// Phase-major kernel tables.  The reference lays a plane out as
+// [ky][kx] with ky, kx = |offset| * OS in [0, (h+1) OS) (kernel_host_set_up,
+// top.c:252-257), so the taps of one visibility, OS samples apart, fall on
+// different cache lines.  Each call first copies the tables, value for
+// value, into the order [fy][fx][iy][ix] with ky = iy OS + fy, kx = ix OS + fx:
+// a visibility's taps along a row are then consecutive.  Same samples, same
+// products: results are those of the reference layout bit for bit.
+// os_shift >= 0: OS = 1 << os_shift (no division); otherwise divide by OS
+__device__ __forceinline__ int row_part(int ky, int OS, int os_shift, int hp1)
 {
-    TapMap m;
-    if (nu <= kWave) {
-        // (k + 0.5) / nu is at least 0.5 / nu away from an integer, far more
-        // than the error of the hardware reciprocal, so these floors are exact
-        const float rnu = __builtin_amdgcn_rcpf((float)nu);
-        const int rps = (int)(((float)kWave + 0.5f) * rnu);     // floor(64 / nu)
-        const int lr = (int)(((float)lane + 0.5f) * rnu);       // floor(lane / nu)
-        m.col0 = lane - lr * nu;
-        m.row0 = lr;
-        m.row_step = rps;
-        m.active = lr < rps;
-        m.ncol_blocks = 1;
-    } else {
-        m.col0 = lane;
-        m.row0 = 0;
-        m.row_step = 1;
-        m.active = true;
-        m.ncol_blocks = (nu + kWave - 1) / kWave;
-    }
-    return m;
+    const int iy = os_shift >= 0 ? ky >> os_shift : ky / OS;
+    const int fy = ky - iy * OS;
+    return fy * OS * hp1 * hp1 + iy * hp1;
+}
+__device__ __forceinline__ int col_part(int kx, int OS, int os_shift, int hp1)
+{
+    const int ix = os_shift >= 0 ? kx >> os_shift : kx / OS;
+    const int fx = kx - ix * OS;
+    return fx * hp1 * hp1 + ix;
 }
 
-template <int MODE>
+// one workgroup row per plane (blockIdx.y); tab holds (offset, h + 1)
+__global__ void relayout_kernel(const float2 *__restrict__ in, const int2 *__restrict__ tab, int OS,
+                                float2 *__restrict__ out)
+{
+    const int2 t = tab[blockIdx.y];
+    const int n = t.y * OS;
+    const int total = n * n;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int ky = i / n, kx = i - ky * n;
+        out[t.x + row_part(ky, OS, -1, t.y) + col_part(kx, OS, -1, t.y)] = in[t.x + i];
+    }
+}
+
+// kernel-table samples through a buffer resource: an offset past the table
+// reads as zero, which is how idle taps are made to add nothing
+constexpr int kOob = 0x40000000;
+typedef float v2f_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float2 tab_load(__amdgpu_buffer_rsrc_t rs, unsigned off)
+{
+    const v2f_t v = __builtin_bit_cast(v2f_t, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+    return make_float2(v.x, v.y);
+}
+
+struct GroupVis {
+    bool ok;
+    int nu, nv, lu0, lv0;
+    int2 pt;      // (table offset, table row length (h+1)*OS)
+    float conj, gx, gy;
+    uint32_t o;   // caller index
+};
+
+// the record (and caller index) of a group's next visibility, loaded one
+// iteration ahead so its latency overlaps the current visibility's taps
+struct RawVis {
+    VisRec r;
+    uint32_t o;
+};
+
+__device__ __forceinline__ RawVis load_vis(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm, int j,
+                                           bool ok)
+{
+    RawVis x;
+    x.r.gx = x.r.gy = 0.f;
+    x.r.start = 0u;
+    x.r.meta = 0u;
+    x.o = 0u;
+    if (ok) {
+        x.r = recs[j];
+        x.o = perm ? perm[j] : (uint32_t)j;
+    }
+    return x;
+}
+
+__device__ __forceinline__ GroupVis decode_vis(const RawVis &x, const int2 *__restrict__ plane_tab, bool ok,
+                                               int tile_u0, int tile_v0)
+{
+    GroupVis v;
+    v.ok = ok;
+    v.o = x.o;
+    v.lu0 = (int)(x.r.start & 0xffffu) - tile_u0;
+    v.lv0 = (int)(x.r.start >> 16) - tile_v0;
+    v.nu = ok ? (int)(x.r.meta & 0xffu) : 0;
+    v.nv = ok ? (int)((x.r.meta >> 8) & 0xffu) : 0;
+    v.pt = ok ? plane_tab[(x.r.meta >> 16) & 0x7fffu] : make_int2(0, 0);
+    v.conj = (x.r.meta & 0x80000000u) ? -1.0f : 1.0f;
+    v.gx = x.r.gx;
+    v.gy = x.r.gy;
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+template <bool COMP> struct AccT;
+template <> struct AccT<true> { using T = float4; };    // sum and TwoSum error per component
+template <> struct AccT<false> { using T = float2; };
+
+__device__ __forceinline__ void acc_add(float4 &c, float px, float py)
+{
+    two_sum_acc(c.x, c.z, px);
+    two_sum_acc(c.y, c.w, py);
+}
+__device__ __forceinline__ void acc_add(float2 &c, float px, float py)
+{
+    c.x += px;
+    c.y += py;
+}
+__device__ __forceinline__ void acc_zero(float4 &c) { c = make_float4(0.f, 0.f, 0.f, 0.f); }
+__device__ __forceinline__ void acc_zero(float2 &c) { c = make_float2(0.f, 0.f); }
+
+template <int MODE, bool COMP>
 __global__ void __launch_bounds__(kWave)
 chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm, const int2 *__restrict__ plane_tab,
              const int4 *__restrict__ chunks, const long long *__restrict__ slab_off, ChunkArgs a, ChunkIO io)
 {
+    using Acc = typename AccT<COMP>::T;
     extern __shared__ float4 lds4[];
+    constexpr bool kGrid = MODE != MODE_DEGRID;
+    constexpr bool kDegrid = MODE != MODE_GRID;
     const int4 ch = chunks[a.chunk_base + blockIdx.x];   // tile, first record, count, window S
-    const int S = ch.w, stride = S + 1;
+    const int S = ch.w, stride = S + 1, cells = S * stride;
     const int lane = threadIdx.x;
+    const int L = 1 << a.lane_shift, C = a.copies;
+    const int g = lane >> a.lane_shift, c = lane & (L - 1);
+    const bool in_group = g < C;
+    // within a group: footprint column c % W, row slot c / W of R = L / W
+    const int W = 1 << a.col_shift, R = L >> a.col_shift;
+    const int col0 = c & (W - 1), rslot = c >> a.col_shift;
     const int tile_u0 = (ch.x % a.ntx) * a.tile, tile_v0 = (ch.x / a.ntx) * a.tile;
     const int gc = a.G / 2;
     const float os = (float)a.OS;
-    constexpr bool kGrid = MODE != MODE_DEGRID;
-    constexpr bool kDegrid = MODE != MODE_GRID;
+    // raw (stride 0) buffer resources over the tables; word 3 is gfx9's 32-bit data format
+    const __amdgpu_buffer_rsrc_t kg_rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)io.kg, (short)0, a.tab_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t kd_rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)io.kd, (short)0, a.tab_bytes, 0x00020000);
 
-    float4 *acc = lds4;                                                             // S x stride accumulators
-    float2 *win = (float2 *)(kGrid ? (void *)(lds4 + S * stride) : (void *)lds4);  // S x stride model window
-    if (kGrid)
-        for (int i = lane; i < S * stride; i += kWave) acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    // LDS: [model window, S x stride float2][C sub-grid copies][one spare cell per lane]
+    const int win_cells = kDegrid ? (cells + 1) & ~1 : 0;
+    float2 *win = reinterpret_cast<float2 *>(lds4);
+    Acc *acc = reinterpret_cast<Acc *>(win + win_cells);
+    const int cstride = cells + 2;
+    if (kGrid) {
+        for (int i = lane; i < C * cstride + kWave; i += kWave) acc_zero(acc[i]);
+    }
     if (kDegrid) {
         for (int i = lane; i < S * S; i += kWave) {
             const int row = i / S, col = i - row * S;
             const int gy = tile_v0 + row, gx = tile_u0 + col;
-            float2 g = make_float2(0.f, 0.f);
-            if (gy < a.G && gx < a.G) g = io.grid[(size_t)gy * a.G + gx];
-            win[row * stride + col] = g;
+            float2 v = make_float2(0.f, 0.f);
+            if (gy < a.G && gx < a.G) v = io.grid[(size_t)gy * a.G + gx];
+            win[row * stride + col] = v;
         }
     }
     __syncthreads();
 
-    for (int k = 0; k < ch.z; ++k) {
-        const int j = ch.y + k;
-        const VisRec r = recs[j];
-        const uint32_t o = perm ? perm[j] : (uint32_t)j;
-        const int lu0 = (int)(r.start & 0xffffu) - tile_u0, lv0 = (int)(r.start >> 16) - tile_v0;
-        const int nu = r.meta & 0xffu, nv = (r.meta >> 8) & 0xffu;
-        const int2 pt = plane_tab[(r.meta >> 16) & 0x7fffu];
-        const float conj = (r.meta & 0x80000000u) ? -1.0f : 1.0f;
-        const TapMap m = tap_map(lane, nu);
-        const int cell_u0 = tile_u0 + lu0 - gc, cell_v0 = tile_v0 + lv0 - gc;
+    const int iters = (ch.z + C - 1) / C;
+    RawVis nxt = load_vis(recs, perm, ch.y + g, in_group && g < ch.z);
+    float2 nxt_val = make_float2(0.f, 0.f);   // the visibility to grid, or the measured value
+    if (MODE != MODE_DEGRID && in_group && g < ch.z) nxt_val = (MODE == MODE_GRID ? io.vis : io.measured)[nxt.o];
+    for (int it = 0; it < iters; ++it) {
+        const int k = it * C + g;
+        const bool ok = in_group && k < ch.z;
+        const RawVis cur = nxt;
+        const float2 cur_val = nxt_val;
+        {
+            const bool ok_next = in_group && k + C < ch.z;
+            nxt = load_vis(recs, perm, ch.y + k + C, ok_next);
+            if (MODE != MODE_DEGRID && ok_next) nxt_val = (MODE == MODE_GRID ? io.vis : io.measured)[nxt.o];
+        }
+        const GroupVis v = decode_vis(cur, plane_tab, ok, tile_u0, tile_v0);
+        const int cell_u0 = tile_u0 + v.lu0 - gc, cell_v0 = tile_v0 + v.lv0 - gc;
+        // kernel-table row offset of footprint rows c and c + L (std_gridding's
+        // abs((int)ROUND((v - grid_pos) * OS)), degridgrid.c:110, times the row length)
+        // wave-wide row count and column blocks (1 unless the extent exceeds W)
+        const int packed = wave_max(v.nv | (((v.nu + W - 1) >> a.col_shift) << 16));
+        const int nv_max = packed & 0xffff, nb_max = packed >> 16;
+        // byte offset of footprint row c (and c + L) in the phase-major table;
+        // rows past the footprint get an offset past the table, which the
+        // buffer loads below turn into zeros
+        const int kyA = c < v.nv ? 8 * row_part(kernel_tap(cell_v0 + c, v.gy, os), a.OS, a.os_shift, v.pt.y) : kOob;
+        const int kyB = (nv_max > L && c + L < v.nv)
+                            ? 8 * row_part(kernel_tap(cell_v0 + c + L, v.gy, os), a.OS, a.os_shift, v.pt.y) : kOob;
+        const int gbase = g << a.lane_shift;
 
-        float2 v = make_float2(0.f, 0.f);
-        if (MODE == MODE_GRID) v = io.vis[o];
+        float2 res = make_float2(0.f, 0.f);
         if (kDegrid) {
             float sr = 0.f, si = 0.f;
-            if (m.active) {
-                for (int cb = 0; cb < m.ncol_blocks; ++cb) {
-                    const int col = m.col0 + cb * kWave;
-                    if (col >= nu) break;
-                    const int kx = kernel_tap(cell_u0 + col, r.gx, os);
-                    const float2 *kcol = io.kd + pt.x + kx;
-                    const float2 *wcol = win + lv0 * stride + lu0 + col;
-                    for (int row0 = m.row0; row0 < nv; row0 += kBatch * m.row_step) {
-                        float2 kk[kBatch], g[kBatch];
+            for (int b = 0; b < nb_max; ++b) {
+                const int col = col0 + (b << a.col_shift);
+                const bool col_ok = v.ok && col < v.nu;
+                const int nv_lane = col_ok ? v.nv : 0;
+                const int kbase =
+                    col_ok ? 8 * (v.pt.x + col_part(kernel_tap(cell_u0 + col, v.gx, os), a.OS, a.os_shift, v.pt.y))
+                           : kOob;
+                const int wbase = v.lv0 * stride + v.lu0 + col;
+                // rows [0, L) take their table offset from kyA, rows [L, 2L) from kyB;
+                // this lane's rows are rlo + rslot + R t (t < W keeps them below rlo + L)
+                for (int half = 0; half < 2; ++half) {
+                    const int rlo = half ? L : 0, rhi = half ? nv_max : min(nv_max, L);
+                    const int src = half ? kyB : kyA;
+                    const int bp = (gbase + rslot) << 2;
+                    for (int t0 = 0; rlo + t0 * R < rhi; t0 += kRowBatch) {
+                        float2 kk[kRowBatch], w[kRowBatch];
 #pragma unroll
-                        for (int q = 0; q < kBatch; ++q) {
-                            const int row = row0 + q * m.row_step;
-                            if (row < nv) {
-                                kk[q] = kcol[kernel_tap(cell_v0 + row, r.gy, os) * pt.y];
-                                g[q] = wcol[row * stride];
-                            }
+                        for (int q = 0; q < kRowBatch; ++q) {
+                            const int r = rlo + rslot + (t0 + q) * R;
+                            const int ky = __builtin_amdgcn_ds_bpermute(bp + ((t0 + q) * R << 2), src);
+                            kk[q] = tab_load(kd_rsrc, (unsigned)(kbase + ky));
+                            w[q] = win[min(wbase + r * stride, cells - 1)];
+                            if (r >= nv_lane) w[q] = make_float2(0.f, 0.f);
                         }
 #pragma unroll
-                        for (int q = 0; q < kBatch; ++q) {
-                            if (row0 + q * m.row_step < nv) {
-                                const float kyc = conj * kk[q].y;
-                                sr = fmaf(g[q].x, kk[q].x, fmaf(-g[q].y, kyc, sr));
-                                si = fmaf(g[q].y, kk[q].x, fmaf(g[q].x, kyc, si));
-                            }
+                        for (int q = 0; q < kRowBatch; ++q) {
+                            // grid * (k.x, conj * k.y); idle taps add exact zeros
+                            const float kyc = v.conj * kk[q].y;
+                            sr = fmaf(w[q].x, kk[q].x, fmaf(-w[q].y, kyc, sr));
+                            si = fmaf(w[q].y, kk[q].x, fmaf(w[q].x, kyc, si));
                         }
                     }
                 }
             }
-            sr = wave_sum(sr);
-            si = wave_sum(si);
+            // butterfly over the group's lanes: every lane ends with the sum
+            for (int off = L >> 1; off > 0; off >>= 1) {
+                sr += __shfl_xor(sr, off, kWave);
+                si += __shfl_xor(si, off, kWave);
+            }
             if (MODE == MODE_DEGRID) {
-                if (lane == 0) {
-                    float2 res = make_float2(sr, si);
+                if (v.ok && c == 0) {
+                    float2 out = make_float2(sr, si);
                     if (io.measured) {
-                        const float2 mv = io.measured[o];
-                        res = make_float2(mv.x - sr, mv.y - si);
+                        const float2 mv = io.measured[v.o];
+                        out = make_float2(mv.x - sr, mv.y - si);
                     }
-                    io.vis_out[o] = res;
+                    io.vis_out[v.o] = out;
                 }
                 continue;
             }
-            const float2 mv = io.measured[o];
-            v = make_float2(mv.x - sr, mv.y - si);
-            if (io.vis_out && lane == 0) io.vis_out[o] = v;
+            if (v.ok) {
+                res = make_float2(cur_val.x - sr, cur_val.y - si);
+                if (io.vis_out && c == 0) io.vis_out[v.o] = res;
+            }
         }
-        if (kGrid && m.active) {
-            const float vr = v.x, vi = v.y, vrc = conj * v.x, vic = conj * v.y;
-            for (int cb = 0; cb < m.ncol_blocks; ++cb) {
-                const int col = m.col0 + cb * kWave;
-                if (col >= nu) break;
-                const int kx = kernel_tap(cell_u0 + col, r.gx, os);
-                const float2 *kcol = io.kg + pt.x + kx;
-                float4 *acol = acc + lv0 * stride + lu0 + col;
-                for (int row0 = m.row0; row0 < nv; row0 += kBatch * m.row_step) {
-                    float2 kk[kBatch];
+        if (kGrid) {
+            if (MODE == MODE_GRID && v.ok) res = cur_val;
+            const float vr = res.x, vi = res.y, vrc = v.conj * res.x, vic = v.conj * res.y;
+            const int spare_i = C * cstride + lane;
+            for (int b = 0; b < nb_max; ++b) {
+                const int col = col0 + (b << a.col_shift);
+                const bool col_ok = v.ok && col < v.nu;
+                const int nv_lane = col_ok ? v.nv : 0;
+                const int kbase =
+                    col_ok ? 8 * (v.pt.x + col_part(kernel_tap(cell_u0 + col, v.gx, os), a.OS, a.os_shift, v.pt.y))
+                           : kOob;
+                const int cbase = g * cstride + (v.lv0 + rslot) * stride + v.lu0 + col;
+                for (int half = 0; half < 2; ++half) {
+                    const int rlo = half ? L : 0, rhi = half ? nv_max : min(nv_max, L);
+                    const int src = half ? kyB : kyA;
+                    const int bp = (gbase + rslot) << 2;
+                    for (int t0 = 0; rlo + t0 * R < rhi; t0 += kRowBatch) {
+                        float2 kk[kRowBatch];
+                        int ci[kRowBatch];
 #pragma unroll
-                    for (int q = 0; q < kBatch; ++q) {
-                        const int row = row0 + q * m.row_step;
-                        if (row < nv) kk[q] = kcol[kernel_tap(cell_v0 + row, r.gy, os) * pt.y];
-                    }
-                    // distinct rows: the cells of one batch never alias, so all
-                    // reads are issued before the writes
-                    float4 cell[kBatch];
+                        for (int q = 0; q < kRowBatch; ++q) {
+                            const int r = rlo + rslot + (t0 + q) * R;
+                            const int ky = __builtin_amdgcn_ds_bpermute(bp + ((t0 + q) * R << 2), src);
+                            kk[q] = tab_load(kg_rsrc, (unsigned)(kbase + ky));
+                            ci[q] = r < nv_lane ? cbase + (rlo + (t0 + q) * R) * stride : spare_i;
+                        }
+                        // distinct rows per lane (or the lane's spare cell, which
+                        // nothing reads): reads first, then writes
+                        Acc cell[kRowBatch];
 #pragma unroll
-                    for (int q = 0; q < kBatch; ++q)
-                        if (row0 + q * m.row_step < nv) cell[q] = acol[(row0 + q * m.row_step) * stride];
+                        for (int q = 0; q < kRowBatch; ++q) cell[q] = acc[ci[q]];
 #pragma unroll
-                    for (int q = 0; q < kBatch; ++q) {
-                        if (row0 + q * m.row_step < nv) {
-                            // vis * (k.x, conj * k.y)
+                        for (int q = 0; q < kRowBatch; ++q) {
+                            // vis * (k.x, conj * k.y), std_gridding (degridgrid.c:112-118)
                             const float px = fmaf(vr, kk[q].x, -(vic * kk[q].y));
                             const float py = fmaf(vi, kk[q].x, vrc * kk[q].y);
-                            two_sum_acc(cell[q].x, cell[q].z, px);
-                            two_sum_acc(cell[q].y, cell[q].w, py);
+                            acc_add(cell[q], px, py);
                         }
-                    }
 #pragma unroll
-                    for (int q = 0; q < kBatch; ++q)
-                        if (row0 + q * m.row_step < nv) acol[(row0 + q * m.row_step) * stride] = cell[q];
+                        for (int q = kRowBatch - 1; q >= 0; --q) acc[ci[q]] = cell[q];
+                    }
                 }
             }
         }
@@ -347,8 +498,26 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
         float2 *slab = io.slabs + slab_off[a.chunk_base + blockIdx.x];
         for (int i = lane; i < S * S; i += kWave) {
             const int row = i / S, col = i - row * S;
-            const float4 c = acc[row * stride + col];
-            slab[i] = make_float2(c.x + c.z, c.y + c.w);
+            const Acc *p0 = acc + row * stride + col;
+            if (COMP) {
+                float4 t = *reinterpret_cast<const float4 *>(p0);
+                for (int q = 1; q < C; ++q) {
+                    const float4 u = *reinterpret_cast<const float4 *>(p0 + q * cstride);
+                    two_sum_acc(t.x, t.z, u.x);
+                    two_sum_acc(t.y, t.w, u.y);
+                    t.z += u.z;
+                    t.w += u.w;
+                }
+                slab[i] = make_float2(t.x + t.z, t.y + t.w);
+            } else {
+                float2 t = *reinterpret_cast<const float2 *>(p0);
+                for (int q = 1; q < C; ++q) {
+                    const float2 u = *reinterpret_cast<const float2 *>(p0 + q * cstride);
+                    t.x += u.x;
+                    t.y += u.y;
+                }
+                slab[i] = t;
+            }
         }
     }
 }
@@ -512,10 +681,13 @@ struct sdp_grid_plan {
     int n_chunks = 0, n_tiles_used = 0, n_class_launches = 0;
     long long total_taps = 0;
     int class_S[kNumClasses] = {};
+    int class_h[kNumClasses] = {};
     int class_chunk_begin[kNumClasses + 1] = {};
     VisRec *recs = nullptr;        // plan order
     uint32_t *perm = nullptr;      // plan position -> caller index
-    int2 *plane_tab = nullptr;     // (sample offset, (h+1)*OS) per plane
+    int2 *plane_tab = nullptr;     // (sample offset, h+1) per plane
+    long long table_len = 0;       // samples in a kernel table (all planes)
+    float2 *ktab = nullptr;        // phase-major copies: gridding table, then degridding table
     int4 *chunks = nullptr;        // (tile, first, count, S), class-major
     long long *slab_off = nullptr; // first slab cell of each chunk
     float2 *slabs = nullptr;       // one slab per chunk
@@ -539,6 +711,7 @@ void plan_free(sdp_grid_plan *p)
     (void)hipFree(p->recs);
     (void)hipFree(p->perm);
     (void)hipFree(p->plane_tab);
+    (void)hipFree(p->ktab);
     (void)hipFree(p->chunks);
     (void)hipFree(p->slab_off);
     (void)hipFree(p->slabs);
@@ -549,30 +722,98 @@ void plan_free(sdp_grid_plan *p)
     delete p;
 }
 
-size_t lds_bytes(int mode, int S)
+// lanes per visibility for a class: a power of two >= the largest footprint
+// extent (2h half-open, 2h + 1 inclusive), at least 16, at most 64
+int class_lane_shift(int h, int snap_mode)
+{
+    const int extent = snap_mode == SDP_SNAP_STD ? 2 * h : 2 * h + 1;
+    int shift = 4;
+    while (shift < 6 && (1 << shift) < extent) ++shift;
+    return shift;
+}
+
+size_t lds_bytes(int mode, int S, int copies, bool comp)
 {
     const size_t cells = (size_t)S * (S + 1);
-    if (mode == MODE_GRID) return cells * sizeof(float4);
-    if (mode == MODE_DEGRID) return cells * sizeof(float2);
-    return cells * (sizeof(float4) + sizeof(float2));
+    const size_t win = mode == MODE_GRID ? 0 : ((cells + 1) & ~(size_t)1) * sizeof(float2);
+    if (mode == MODE_DEGRID) return win;
+    const size_t acc = comp ? sizeof(float4) : sizeof(float2);
+    return win + ((size_t)copies * (cells + 2) + kWave) * acc;
+}
+
+// sub-grid copies that fit LDS for a class (a power of two)
+int class_copies(int mode, int S, int col_shift, bool comp)
+{
+    int c = kWave >> col_shift;
+    while (c > 1 && lds_bytes(mode, S, c, comp) > kMaxLds) c >>= 1;
+    return c;
+}
+
+// copies per wavefront when gridding: more copies grid more visibilities at
+// once, fewer leave LDS for more resident wavefronts.  SDP_GRID_COPIES
+// (1, 2 or 4) overrides the default for tuning.
+int grid_copies(bool comp)
+{
+    static int env = -1;
+    if (env < 0) {
+        const char *e = getenv("SDP_GRID_COPIES");
+        const int v = e ? atoi(e) : 0;
+        env = (v == 1 || v == 2 || v == 4) ? v : 0;
+    }
+    if (env > 0) return env;
+    return comp ? 1 : 4;   // measured at 16.7M visibilities, 32 w-planes, h 4-16
+}
+
+template <int MODE, bool COMP>
+int launch_class(const sdp_grid_plan *plan, int c, const ChunkIO &io, const uint32_t *perm, hipStream_t st)
+{
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void *)chunk_kernel<MODE, COMP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kMaxLds);
+        attr_set = true;
+    }
+    const int b = plan->class_chunk_begin[c], e = plan->class_chunk_begin[c + 1];
+    const int S = plan->class_S[c];
+    const int col_shift = class_lane_shift(plan->class_h[c], plan->p.snap_mode);
+    int copies = kWave >> col_shift;   // one visibility per group, one row at a time
+    if (MODE != MODE_DEGRID) copies = std::min(copies, std::min(grid_copies(COMP), class_copies(MODE, S, col_shift, COMP)));
+    int lane_shift = 0;
+    while ((kWave >> lane_shift) > copies) ++lane_shift;   // lanes per group = 64 / copies (copies a power of 2)
+    copies = kWave >> lane_shift;
+    const int OS = plan->p.oversampling;
+    int os_shift = -1;
+    for (int k = 0; k < 31; ++k)
+        if ((1 << k) == OS) os_shift = k;
+    ChunkArgs a{plan->p.grid_size, OS, plan->tile, plan->ntx, b, lane_shift, col_shift, copies, os_shift,
+                (int)(8 * plan->table_len)};
+    chunk_kernel<MODE, COMP><<<e - b, kWave, lds_bytes(MODE, S, copies, COMP), st>>>(
+        plan->recs, perm, plan->plane_tab, plan->chunks, plan->slab_off, a, io);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+// phase-major copy of a kernel table into the plan (slot 0 gridding, 1 degridding)
+const float2 *relayout(const sdp_grid_plan *plan, const float *d_kernels, int slot, hipStream_t st)
+{
+    float2 *out = plan->ktab + (size_t)slot * (size_t)plan->table_len;
+    int maxn = 1;
+    for (int c = 0; c < kNumClasses; ++c) maxn = std::max(maxn, (plan->class_h[c] + 1) * plan->p.oversampling);
+    const int bx = (int)std::min<long long>(64, ((long long)maxn * maxn + 255) / 256);
+    relayout_kernel<<<dim3(bx, plan->p.n_kernels), 256, 0, st>>>((const float2 *)d_kernels, plan->plane_tab,
+                                                                  plan->p.oversampling, out);
+    return out;
 }
 
 template <int MODE>
 int launch_chunks(const sdp_grid_plan *plan, const ChunkIO &io, const uint32_t *perm, hipStream_t st)
 {
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)chunk_kernel<MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)kMaxLds);
-        attr_set = true;
-    }
+    const bool comp = plan->p.accumulate == SDP_ACC_COMPENSATED;
     for (int c = 0; c < kNumClasses; ++c) {
-        const int b = plan->class_chunk_begin[c], e = plan->class_chunk_begin[c + 1];
-        if (e <= b) continue;
-        ChunkArgs a{plan->p.grid_size, plan->p.oversampling, plan->tile, plan->ntx, b};
-        chunk_kernel<MODE><<<e - b, kWave, lds_bytes(MODE, plan->class_S[c]), st>>>(
-            plan->recs, perm, plan->plane_tab, plan->chunks, plan->slab_off, a, io);
-        SDP_LAUNCH_CHECK();
+        if (plan->class_chunk_begin[c + 1] <= plan->class_chunk_begin[c]) continue;
+        const int rc = (comp || MODE == MODE_DEGRID) ? launch_class<MODE, true>(plan, c, io, perm, st)
+                                                     : launch_class<MODE, false>(plan, c, io, perm, st);
+        if (rc != SDP_OK) return rc;
     }
     return SDP_OK;
 }
@@ -593,23 +834,42 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     SDP_REQUIRE(n_vis >= 0, "negative visibility count");
     SDP_REQUIRE(p.snap_mode == SDP_SNAP_STD || p.snap_mode == SDP_SNAP_GRIDDING_ACTOR, "bad snap mode");
     SDP_REQUIRE(p.tile >= 0 && p.tile <= 32 && p.chunk >= 0, "tile must be in [1, 32] (0 = default)");
+    SDP_REQUIRE(p.accumulate == SDP_ACC_BLOCKED || p.accumulate == SDP_ACC_COMPENSATED, "bad accumulate mode");
     hipStream_t st = sdp::stream_of(s);
 
     std::vector<int2> tab(p.n_kernels), sup(p.n_kernels);
     std::vector<int> pclass(p.n_kernels);
     int class_h[kNumClasses] = {-1, -1, -1, -1, -1, -1};
     int H = 0;
+    long long table_len = 0;
     for (int k = 0; k < p.n_kernels; ++k) {
         const int h = supports[2 * k], off = supports[2 * k + 1];
-        SDP_REQUIRE(h >= 0, "negative half support for plane %d", k);
+        SDP_REQUIRE(h >= 0 && off >= 0, "negative half support or offset for plane %d", k);
         if (h > kClassMaxH[kNumClasses - 1])
             return sdp::set_error(SDP_E_UNSUPPORTED, "half support %d of plane %d exceeds %d", h, k,
                                   kClassMaxH[kNumClasses - 1]);
         H = std::max(H, h);
         pclass[k] = class_of(h);
         class_h[pclass[k]] = std::max(class_h[pclass[k]], h);
-        tab[k] = make_int2(off, (h + 1) * p.oversampling);
+        tab[k] = make_int2(off, h + 1);
+        table_len = std::max(table_len, (long long)off + (long long)(h + 1) * p.oversampling * (h + 1) * p.oversampling);
         sup[k] = make_int2(h, off);
+    }
+
+    SDP_REQUIRE(table_len < (1LL << 26), "kernel table of %lld samples is too large", table_len);
+    {
+        // the phase-major copy permutes samples inside each plane's region,
+        // so regions must not overlap (planes sharing one region are fine)
+        std::vector<int> order(p.n_kernels);
+        for (int k = 0; k < p.n_kernels; ++k) order[k] = k;
+        std::sort(order.begin(), order.end(), [&](int x, int y) { return tab[x].x < tab[y].x; });
+        for (int i = 1; i < p.n_kernels; ++i) {
+            const int2 a = tab[order[i - 1]], b = tab[order[i]];
+            const long long end = (long long)a.x + (long long)a.y * p.oversampling * a.y * p.oversampling;
+            if (b.x < end && !(b.x == a.x && b.y == a.y))
+                return sdp::set_error(SDP_E_UNSUPPORTED, "kernel planes %d and %d overlap in the table", order[i - 1],
+                                      order[i]);
+        }
     }
 
     sdp_grid_plan *plan = new sdp_grid_plan();
@@ -622,8 +882,10 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     plan->nty = plan->ntx;
     for (int c = 0; c < kNumClasses; ++c) {
         plan->class_S[c] = class_h[c] >= 0 ? plan->tile + 2 * class_h[c] : 0;
+        plan->class_h[c] = std::max(class_h[c], 0);
         plan->smax = std::max(plan->smax, plan->class_S[c]);
-        if (class_h[c] >= 0 && lds_bytes(MODE_DEGRID_GRID, plan->class_S[c]) > kMaxLds) {
+        if (class_h[c] >= 0 &&
+            lds_bytes(MODE_DEGRID_GRID, plan->class_S[c], 1, p.accumulate == SDP_ACC_COMPENSATED) > kMaxLds) {
             const int S = plan->class_S[c], tile = plan->tile;
             delete plan;
             return sdp::set_error(SDP_E_UNSUPPORTED, "window of %d cells (tile %d, half support %d) exceeds LDS", S,
@@ -649,6 +911,8 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
             return fail(sdp::set_error(SDP_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)));  \
     } while (0)
 
+    plan->table_len = table_len;
+    PLAN_TRY(hipMalloc(&plan->ktab, sizeof(float2) * 2 * (size_t)std::max(table_len, 1LL)));
     PLAN_TRY(hipMalloc(&plan->plane_tab, sizeof(int2) * p.n_kernels));
     PLAN_TRY(hipMemcpyAsync(plan->plane_tab, tab.data(), sizeof(int2) * p.n_kernels, hipMemcpyHostToDevice, st));
     int2 *d_supports = nullptr;
@@ -866,12 +1130,13 @@ int sdp_grid_accumulate(const sdp_grid_plan *plan, const float *d_kernels, const
     SDP_REQUIRE(order == SDP_ORDER_ORIGINAL || order == SDP_ORDER_PLAN, "sdp_grid: bad order");
     if (plan->n_chunks == 0) return SDP_OK;
     SDP_REQUIRE(d_kernels && d_vis, "sdp_grid_accumulate: null argument");
+    hipStream_t st = sdp::stream_of(s);
     ChunkIO io{};
-    io.kg = (const float2 *)d_kernels;
+    io.kg = relayout(plan, d_kernels, 0, st);
+    SDP_LAUNCH_CHECK();
     io.vis = (const float2 *)d_vis;
     io.slabs = plan->slabs;
-    return launch_chunks<MODE_GRID>(plan, io, order == SDP_ORDER_ORIGINAL ? plan->perm : nullptr,
-                                    sdp::stream_of(s));
+    return launch_chunks<MODE_GRID>(plan, io, order == SDP_ORDER_ORIGINAL ? plan->perm : nullptr, st);
 }
 
 int sdp_grid_merge(const sdp_grid_plan *plan, float *d_grid, sdp_stream s)
@@ -909,7 +1174,8 @@ int sdp_degrid(const sdp_grid_plan *plan, const float *d_kernels, const float *d
     if (plan->n_chunks > 0) {
         SDP_REQUIRE(d_kernels, "sdp_degrid: null kernels");
         ChunkIO io{};
-        io.kd = (const float2 *)d_kernels;
+        io.kd = relayout(plan, d_kernels, 1, st);
+        SDP_LAUNCH_CHECK();
         io.grid = (const float2 *)d_grid;
         io.measured = (const float2 *)d_vis_measured;
         io.vis_out = (float2 *)d_vis_out;
@@ -934,8 +1200,9 @@ int sdp_degrid_grid_accumulate(const sdp_grid_plan *plan, const float *d_degrid_
     if (plan->n_chunks > 0) {
         SDP_REQUIRE(d_degrid_kernels && d_grid_kernels, "sdp_degrid_grid: null kernels");
         ChunkIO io{};
-        io.kd = (const float2 *)d_degrid_kernels;
-        io.kg = (const float2 *)d_grid_kernels;
+        io.kd = relayout(plan, d_degrid_kernels, 1, st);
+        io.kg = relayout(plan, d_grid_kernels, 0, st);
+        SDP_LAUNCH_CHECK();
         io.grid = (const float2 *)d_model_grid;
         io.measured = (const float2 *)d_vis_measured;
         io.vis_out = (float2 *)d_residual_vis;

@@ -34,6 +34,7 @@ DEGRID_NEAREST, DEGRID_WEIGHTED = 0, 1
 DFT_REFERENCE_FP32, DFT_FP64_PHASE, DFT_MFMA = 0, 1, 2
 SNAP_STD, SNAP_GRIDDING_ACTOR = 0, 1
 ORDER_ORIGINAL, ORDER_PLAN = 0, 1
+ACC_BLOCKED, ACC_COMPENSATED = 0, 1
 FFT_FORWARD, FFT_INVERSE = -1, 1
 FFT_SHIFT_IN, FFT_SHIFT_OUT, FFT_REAL_IN, FFT_REAL_OUT, FFT_CORRECT_OUT, FFT_CORRECT_IN = 1, 2, 4, 8, 16, 32
 HOGBOM_PSF_AS_STORED = 1
@@ -288,14 +289,16 @@ class Event:
 
 class _Params(ctypes.Structure):
     _fields_ = [("grid_size", c_int), ("oversampling", c_int), ("n_kernels", c_int), ("snap_mode", c_int),
-                ("uv_scale", c_double), ("w_scale", c_double), ("tile", c_int), ("chunk", c_int)]
+                ("uv_scale", c_double), ("w_scale", c_double), ("tile", c_int), ("chunk", c_int),
+                ("accumulate", c_int)]
 
 
 def autotune(grid_size, oversampling, supports, uv_scale, w_scale, d_uvw, n_vis, d_grid_kernels, d_degrid_kernels,
-             tiles=(4, 8, 16), chunks=(32768, 65536, 131072, 262144), snap_mode=SNAP_STD, stream=None):
+             tiles=(4, 8, 16), chunks=(32768, 65536, 131072, 262144), snap_mode=SNAP_STD, accumulate=ACC_BLOCKED,
+             stream=None):
     """Fastest (tile, taps per work item) for these coordinates: (tile, chunk, ms)."""
     sup = np.ascontiguousarray(np.asarray(supports, dtype=np.int32).reshape(-1, 2))
-    p = _Params(grid_size, oversampling, sup.shape[0], snap_mode, uv_scale, w_scale, 0, 0)
+    p = _Params(grid_size, oversampling, sup.shape[0], snap_mode, uv_scale, w_scale, 0, 0, accumulate)
     t = np.asarray(tiles, np.int32)
     c = np.asarray(chunks, np.int32)
     bt, bc, ms = c_int(0), c_int(0), c_float(0)
@@ -309,11 +312,11 @@ class GridPlan:
     """Owner of an ``sdp_grid_plan`` (tile-sorted visibility records)."""
 
     def __init__(self, grid_size, oversampling, supports, uv_scale, w_scale, d_uvw, n_vis,
-                 snap_mode=SNAP_STD, tile=0, chunk=0, stream=None):
+                 snap_mode=SNAP_STD, tile=0, chunk=0, accumulate=ACC_BLOCKED, stream=None):
         lib = load()
         check(lib.sdp_init())
         sup = np.ascontiguousarray(np.asarray(supports, dtype=np.int32).reshape(-1, 2))
-        p = _Params(grid_size, oversampling, sup.shape[0], snap_mode, uv_scale, w_scale, tile, chunk)
+        p = _Params(grid_size, oversampling, sup.shape[0], snap_mode, uv_scale, w_scale, tile, chunk, accumulate)
         h = c_void_p()
         check(lib.sdp_grid_plan_create(ctypes.byref(h), ctypes.byref(p), sup.ctypes.data, _ptr(d_uvw),
                                        int(n_vis), stream))
