@@ -250,6 +250,39 @@ int sdp_hogbom_ex(int grid_size, int n_minor, int max_sources, float *d_residual
 int sdp_dft_predict(int n_vis, int n_sources, const float *d_sources, const float *d_uvw,
                     float *d_vis_out, int mode, sdp_stream s);
 
+/* ---- learned minor-cycle proposal ---------------------------------------
+ * No reference counterpart (the reference has no learned component).  An
+ * encoder-decoder CNN over NHWC float tensors: `levels` levels of two 3x3
+ * convolutions + ReLU with `base` << l channels, 2x2 average pooling down,
+ * nearest upsampling + 3x3 convolution + additive skip + ReLU + 3x3
+ * convolution up, and a 3x3 head to out_ch channels.  Parameters are one
+ * float vector, layer by layer (enc0a, enc0b, ..., dec{L-2}u, dec{L-2}d, ...,
+ * dec0u, dec0d, head), each layer its weights [3][3][cin][cout] then its
+ * biases [cout]. */
+enum { SDP_CNN_FP32 = 0,   /* FP32 FMA on the vector ALUs */
+       SDP_CNN_BF16 = 1 }; /* BF16 operands on the matrix cores, FP32 accumulation */
+typedef struct sdp_cnn_params {
+    int levels, base, in_ch, out_ch;
+    int height, width, max_batch;   /* activation buffers; height, width divisible by 2^(levels-1) */
+    int precision;                  /* SDP_CNN_* */
+} sdp_cnn_params;
+typedef struct sdp_cnn sdp_cnn;
+int sdp_cnn_create(sdp_cnn **net, const sdp_cnn_params *params);
+int sdp_cnn_destroy(sdp_cnn *net);
+size_t sdp_cnn_param_count(const sdp_cnn *net);
+/* host copies of the parameter / gradient vectors (set resets Adam) */
+int sdp_cnn_set_params(sdp_cnn *net, const float *h_params);
+int sdp_cnn_get_params(const sdp_cnn *net, float *h_params);
+int sdp_cnn_get_grads(const sdp_cnn *net, float *h_grads);
+/* d_in: batch x height x width x in_ch, d_out: batch x height x width x out_ch */
+int sdp_cnn_forward(sdp_cnn *net, const float *d_in, float *d_out, int batch, sdp_stream s);
+/* gradients of all parameters (overwritten) for the last forward pass, given dL/d(out) */
+int sdp_cnn_backward(sdp_cnn *net, const float *d_grad_out, sdp_stream s);
+/* mean squared error over count values; d_grad = 2 (out - target) / count; synchronous */
+int sdp_cnn_mse(size_t count, const float *d_out, const float *d_target, float *d_grad, double *loss, sdp_stream s);
+/* one Adam step on the current gradients */
+int sdp_cnn_adam(sdp_cnn *net, float lr, float beta1, float beta2, float eps, sdp_stream s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,684 @@
+// sdp_cnn.hip -- the learned minor-cycle proposal: an encoder-decoder CNN
+// that maps (residual patch, PSF patch) to a model-image update, trained
+// and run by the kernels in this file.  The reference has no learned
+// component; the network is defined by its FP64 restatement used in the
+// tests (encoder levels of two 3x3 convolutions + ReLU with 2x2 average
+// pooling, decoder levels of upsampling, a 3x3 convolution, an additive skip
+// from the encoder, ReLU and a second convolution, then a 3x3 head).
+//
+// Every 3x3 convolution, forward and backward, is one implicit GEMM:
+//   forward       M = pixels,      N = C_out, K = 9 C_in
+//   input grad    M = pixels,      N = C_in,  K = 9 C_out  (taps reflected)
+//   weight grad   M = 9 C_in,      N = C_out, K = pixels   (split over K, the
+//                 partial products summed in a fixed order afterwards)
+// Tensors are NHWC in HBM, so a GEMM row's K elements are contiguous within
+// each tap.  Two arithmetic modes: FP32 FMA on the vector ALUs (exact enough
+// for gradient checks), and BF16 operands on the gfx950 matrix cores
+// (v_mfma_f32_16x16x32_bf16) with FP32 accumulation.  Weights are kept in
+// FP32 (master copy for Adam) and rounded to BF16 as tiles are staged.
+//
+// Element-wise epilogues are fused: bias, the decoder's skip addition and
+// ReLU after a forward GEMM; the ReLU mask of the producing layer after an
+// input-gradient GEMM.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "sdp_internal.h"
+
+namespace {
+
+constexpr int kT = 256;        // threads per GEMM workgroup
+constexpr int kBM = 64, kBN = 64, kBK = 32;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+enum { G_FWD = 0, G_BWD_DATA = 1, G_BWD_W = 2 };
+
+struct GemmArgs {
+    int B, H, W, Cin, Cout;    // convolution geometry (input Cin, output Cout channels)
+    int M, N, K;               // GEMM sizes
+    int k_chunk;               // BWD_W: K range of one split (multiple of kBK)
+    const float *a_src;        // FWD/BWD_W: layer input; BWD_DATA: output gradient
+    const float *b_src;        // FWD/BWD_DATA: weights [3][3][Cin][Cout]; BWD_W: output gradient
+    float *out;                // FWD: activation; BWD_DATA: input gradient; BWD_W: split partials
+    const float *bias;         // FWD: bias (or null)
+    const float *skip;         // FWD: added before ReLU (or null)
+    const float *mask;         // BWD_DATA: gradient zeroed where mask <= 0 (or null)
+    int relu;                  // FWD: apply ReLU
+};
+
+// A[m][k] of the GEMM (0 outside the image: zero padding)
+__device__ __forceinline__ float load_a(const GemmArgs &g, int mode, int m, int k)
+{
+    if (m >= g.M || k >= g.K) return 0.f;
+    int pix, t, c, C;
+    if (mode == G_BWD_W) {
+        t = m / g.Cin; c = m - t * g.Cin; pix = k; C = g.Cin;
+    } else if (mode == G_FWD) {
+        t = k / g.Cin; c = k - t * g.Cin; pix = m; C = g.Cin;
+    } else {
+        t = k / g.Cout; c = k - t * g.Cout; pix = m; C = g.Cout;
+    }
+    const int x = pix % g.W, rest = pix / g.W, y = rest % g.H, b = rest / g.H;
+    const int ky = t / 3 - 1, kx = t % 3 - 1;
+    // forward and weight grad read in[y + ky][x + kx]; input grad reads gout[y - ky][x - kx]
+    const int yy = mode == G_BWD_DATA ? y - ky : y + ky;
+    const int xx = mode == G_BWD_DATA ? x - kx : x + kx;
+    if (yy < 0 || yy >= g.H || xx < 0 || xx >= g.W) return 0.f;
+    return g.a_src[((size_t)(b * g.H + yy) * g.W + xx) * C + c];
+}
+
+// B[k][n] of the GEMM
+__device__ __forceinline__ float load_b(const GemmArgs &g, int mode, int k, int n)
+{
+    if (k >= g.K || n >= g.N) return 0.f;
+    if (mode == G_FWD) return g.b_src[(size_t)k * g.Cout + n];                 // w[t][ci][co], k = t Cin + ci
+    if (mode == G_BWD_W) return g.b_src[(size_t)k * g.Cout + n];               // gout[pix][co]
+    const int t = k / g.Cout, co = k - t * g.Cout;                             // w[t][n = ci][co]
+    return g.b_src[((size_t)t * g.Cin + n) * g.Cout + co];
+}
+
+__device__ __forceinline__ void epilogue(const GemmArgs &g, int mode, int m, int n, float v, int split)
+{
+    if (m >= g.M || n >= g.N) return;
+    if (mode == G_FWD) {
+        if (g.bias) v += g.bias[n];
+        if (g.skip) v += g.skip[(size_t)m * g.N + n];
+        if (g.relu) v = fmaxf(v, 0.f);
+        g.out[(size_t)m * g.N + n] = v;
+    } else if (mode == G_BWD_DATA) {
+        if (g.mask && !(g.mask[(size_t)m * g.N + n] > 0.f)) v = 0.f;
+        g.out[(size_t)m * g.N + n] = v;
+    } else {
+        g.out[((size_t)split * g.M + m) * g.N + n] = v;
+    }
+}
+
+// K range of this workgroup
+__device__ __forceinline__ void k_range(const GemmArgs &g, int mode, int &k0, int &k1)
+{
+    if (mode == G_BWD_W) {
+        k0 = blockIdx.z * g.k_chunk;
+        k1 = min(g.K, k0 + g.k_chunk);
+    } else {
+        k0 = 0;
+        k1 = g.K;
+    }
+}
+
+// FP32 vector-ALU GEMM: a 64 x 64 tile per workgroup, 4 x 4 outputs per thread.
+template <int MODE>
+__global__ void __launch_bounds__(kT) gemm_f32_kernel(GemmArgs g)
+{
+    __shared__ float As[kBK][kBM + 4], Bs[kBK][kBN + 4];
+    const int m0 = blockIdx.x * kBM, n0 = blockIdx.y * kBN;
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    int k0, k1;
+    k_range(g, MODE, k0, k1);
+    float acc[4][4] = {};
+    for (int kb = k0; kb < k1; kb += kBK) {
+        for (int i = threadIdx.x; i < kBM * kBK; i += kT) {
+            const int mm = i / kBK, kk = i % kBK;
+            As[kk][mm] = (kb + kk < k1) ? load_a(g, MODE, m0 + mm, kb + kk) : 0.f;
+        }
+        for (int i = threadIdx.x; i < kBN * kBK; i += kT) {
+            const int kk = i / kBN, nn = i % kBN;
+            Bs[kk][nn] = (kb + kk < k1) ? load_b(g, MODE, kb + kk, n0 + nn) : 0.f;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < kBK; ++kk) {
+            float a[4], b[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = As[kk][ty * 4 + r];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) b[c] = Bs[kk][tx * 4 + c];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[r][c] = fmaf(a[r], b[c], acc[r][c]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) epilogue(g, MODE, m0 + ty * 4 + r, n0 + tx * 4 + c, acc[r][c], blockIdx.z);
+}
+
+// BF16 matrix-core GEMM: a 64 x 64 tile per workgroup, four wavefronts of
+// 32 x 32, each a 2 x 2 block of 16 x 16 x 32 MFMAs per K step.  Operand
+// fragments: lane l holds row (A) or column (B) l % 16, K elements
+// 8 (l / 16) .. + 7; the accumulator holds column l % 16, rows 4 (l / 16) .. + 3.
+template <int MODE>
+__global__ void __launch_bounds__(kT) gemm_bf16_kernel(GemmArgs g)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 As[kBM][kBK + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[kBN][kBK + 8];
+    const int m0 = blockIdx.x * kBM, n0 = blockIdx.y * kBN;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    int k0, k1;
+    k_range(g, MODE, k0, k1);
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kb = k0; kb < k1; kb += kBK) {
+        for (int i = threadIdx.x; i < kBM * kBK; i += kT) {
+            const int mm = i / kBK, kk = i % kBK;
+            As[mm][kk] = (__bf16)((kb + kk < k1) ? load_a(g, MODE, m0 + mm, kb + kk) : 0.f);
+        }
+        for (int i = threadIdx.x; i < kBN * kBK; i += kT) {
+            const int nn = i % kBN, kk = i / kBN;
+            Bs[nn][kk] = (__bf16)((kb + kk < k1) ? load_b(g, MODE, kb + kk, n0 + nn) : 0.f);
+        }
+        __syncthreads();
+        const int kq = (lane >> 4) * 8;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&As[wr + i * 16 + (lane & 15)][kq]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&Bs[wc + j * 16 + (lane & 15)][kq]);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                epilogue(g, MODE, m0 + wr + i * 16 + (lane >> 4) * 4 + r, n0 + wc + j * 16 + (lane & 15),
+                         acc[i][j][r], blockIdx.z);
+}
+
+// weight gradient: sum the K-split partials in split order
+__global__ void sum_splits_kernel(const float *part, int splits, size_t n, float *out)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int q = 0; q < splits; ++q) s += part[(size_t)q * n + i];
+        out[i] = s;
+    }
+}
+
+// bias gradient: column sums of the output gradient (one workgroup per column)
+__global__ void bias_grad_kernel(const float *gout, int M, int N, float *gb)
+{
+    __shared__ float red[kT];
+    float s = 0.f;
+    for (int m = threadIdx.x; m < M; m += kT) s += gout[(size_t)m * N + blockIdx.x];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = kT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gb[blockIdx.x] = red[0];
+}
+
+// 2x2 average pool (NHWC), input H x W -> H/2 x W/2
+__global__ void avgpool2_kernel(const float *in, int B, int H, int W, int C, float *out)
+{
+    const size_t n = (size_t)B * (H / 2) * (W / 2) * C;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = i % C;
+        size_t r = i / C;
+        const int x = r % (W / 2);
+        r /= (W / 2);
+        const int y = r % (H / 2);
+        const int b = r / (H / 2);
+        const float *p = in + ((size_t)(b * H + 2 * y) * W + 2 * x) * C + c;
+        out[i] = 0.25f * ((p[0] + p[C]) + (p[(size_t)W * C] + p[(size_t)W * C + C]));
+    }
+}
+
+// ge = (skip_grad + avgpool2 backward of g_low) masked by act > 0, at H x W
+__global__ void pool_back_add_mask_kernel(const float *g_low, const float *skip_grad, const float *act, int B, int H,
+                                          int W, int C, float *out)
+{
+    const size_t n = (size_t)B * H * W * C;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = i % C;
+        size_t r = i / C;
+        const int x = r % W;
+        r /= W;
+        const int y = r % H;
+        const int b = r / H;
+        float v = 0.25f * g_low[((size_t)(b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+        if (skip_grad) v += skip_grad[i];
+        out[i] = act[i] > 0.f ? v : 0.f;
+    }
+}
+
+// nearest 2x upsample (NHWC), input H x W -> 2H x 2W
+__global__ void upsample2_kernel(const float *in, int B, int H, int W, int C, float *out)
+{
+    const size_t n = (size_t)B * (2 * H) * (2 * W) * C;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = i % C;
+        size_t r = i / C;
+        const int x = r % (2 * W);
+        r /= (2 * W);
+        const int y = r % (2 * H);
+        const int b = r / (2 * H);
+        out[i] = in[((size_t)(b * H + y / 2) * W + x / 2) * C + c];
+    }
+}
+
+// upsample2 backward: each low-resolution cell sums its 2x2 block, then the
+// ReLU mask of the low-resolution activation (if any)
+__global__ void upsample2_back_kernel(const float *g, int B, int H, int W, int C, const float *mask, float *out)
+{
+    const size_t n = (size_t)B * H * W * C;   // low resolution
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = i % C;
+        size_t r = i / C;
+        const int x = r % W;
+        r /= W;
+        const int y = r % H;
+        const int b = r / H;
+        const size_t W2 = 2 * (size_t)W;
+        const float *p = g + ((size_t)(b * 2 * H + 2 * y) * W2 + 2 * x) * C + c;
+        float v = (p[0] + p[C]) + (p[W2 * C] + p[W2 * C + C]);
+        if (mask && !(mask[i] > 0.f)) v = 0.f;
+        out[i] = v;
+    }
+}
+
+// mean squared error: grad = 2 (y - t) / n, per-block partial sums of (y - t)^2
+__global__ void mse_kernel(const float *y, const float *t, size_t n, float *grad, double *part)
+{
+    __shared__ double red[kT];
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float d = y[i] - t[i];
+        grad[i] = (float)(2.0 * (double)d / (double)n);
+        s += (double)d * (double)d;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = kT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Adam (Kingma and Ba 2015, Algorithm 1) with bias corrections c1 = 1 - b1^t, c2 = 1 - b2^t
+__global__ void adam_kernel(float *p, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2,
+                            float eps, float c1, float c2)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+unsigned int ew_grid(size_t n)
+{
+    size_t b = (n + kT - 1) / kT;
+    return (unsigned int)std::max<size_t>(1, std::min<size_t>(b, 16384));
+}
+
+}  // namespace
+
+// ---- network object ---------------------------------------------------------
+
+struct sdp_cnn {
+    sdp_cnn_params p;
+    struct Layer {
+        int cin, cout;
+        size_t w_off, b_off;   // into the parameter vector
+    };
+    std::vector<Layer> layers;   // enc0a, enc0b, ..., dec(L-2)u, dec(L-2)d, ..., dec0u, dec0d, head
+    size_t n_params = 0;
+    float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    int adam_t = 0;
+    // activations per level (B_max x H_l x W_l x C)
+    std::vector<float *> x, a, e, up, s, d, genc, g1, g2;
+    float *work = nullptr;        // weight-gradient split partials
+    size_t work_floats = 0;
+    double *loss_part = nullptr;
+    int batch = 0;                // batch of the last forward
+    const float *last_in = nullptr;
+
+    int ch(int l) const { return p.base << l; }
+    int hl(int l) const { return p.height >> l; }
+    int wl(int l) const { return p.width >> l; }
+};
+
+namespace {
+
+int enc_a(int l) { return 2 * l; }
+int enc_b(int l) { return 2 * l + 1; }
+int dec_u(const sdp_cnn *n, int l) { return 2 * n->p.levels + 2 * (n->p.levels - 2 - l); }
+int dec_d(const sdp_cnn *n, int l) { return dec_u(n, l) + 1; }
+int head(const sdp_cnn *n) { return (int)n->layers.size() - 1; }
+
+// K splits of a weight-gradient GEMM: enough workgroups to fill the GPU
+int wgrad_splits(const GemmArgs &g, int *chunk)
+{
+    const int tiles = ((g.M + kBM - 1) / kBM) * ((g.N + kBN - 1) / kBN);
+    int splits = std::max(1, std::min(1024 / std::max(tiles, 1), (g.K + 4 * kBK - 1) / (4 * kBK)));
+    *chunk = ((g.K + splits - 1) / splits + kBK - 1) / kBK * kBK;
+    return (g.K + *chunk - 1) / *chunk;
+}
+
+int run_gemm(const sdp_cnn *n, int mode, GemmArgs g, hipStream_t st)
+{
+    const bool bf16 = n->p.precision == SDP_CNN_BF16;
+    dim3 grid((g.M + kBM - 1) / kBM, (g.N + kBN - 1) / kBN, 1);
+    if (mode == G_BWD_W) {
+        const int splits = wgrad_splits(g, &g.k_chunk);
+        SDP_REQUIRE((size_t)splits * g.M * g.N <= n->work_floats, "weight-gradient workspace too small");
+        grid.z = splits;
+    }
+    switch (mode) {
+    case G_FWD:
+        if (bf16) gemm_bf16_kernel<G_FWD><<<grid, kT, 0, st>>>(g);
+        else gemm_f32_kernel<G_FWD><<<grid, kT, 0, st>>>(g);
+        break;
+    case G_BWD_DATA:
+        if (bf16) gemm_bf16_kernel<G_BWD_DATA><<<grid, kT, 0, st>>>(g);
+        else gemm_f32_kernel<G_BWD_DATA><<<grid, kT, 0, st>>>(g);
+        break;
+    default:
+        if (bf16) gemm_bf16_kernel<G_BWD_W><<<grid, kT, 0, st>>>(g);
+        else gemm_f32_kernel<G_BWD_W><<<grid, kT, 0, st>>>(g);
+        break;
+    }
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+GemmArgs geom(int B, int H, int W, int cin, int cout)
+{
+    GemmArgs g{};
+    g.B = B; g.H = H; g.W = W; g.Cin = cin; g.Cout = cout;
+    return g;
+}
+
+// out = act(conv(in) + bias (+ skip))
+int conv_forward(const sdp_cnn *n, int li, int B, int H, int W, const float *in, float *out, const float *skip,
+                 int relu, hipStream_t st)
+{
+    const auto &L = n->layers[li];
+    GemmArgs g = geom(B, H, W, L.cin, L.cout);
+    g.M = B * H * W; g.N = L.cout; g.K = 9 * L.cin;
+    g.a_src = in; g.b_src = n->params + L.w_off; g.out = out;
+    g.bias = n->params + L.b_off; g.skip = skip; g.relu = relu;
+    return run_gemm(n, G_FWD, g, st);
+}
+
+// gin = conv backward data of gout (masked by mask > 0 when mask is given);
+// weight and bias gradients of the layer from (in, gout)
+int conv_backward(const sdp_cnn *n, int li, int B, int H, int W, const float *in, const float *gout, float *gin,
+                  const float *mask, hipStream_t st)
+{
+    const auto &L = n->layers[li];
+    if (gin) {
+        GemmArgs g = geom(B, H, W, L.cin, L.cout);
+        g.M = B * H * W; g.N = L.cin; g.K = 9 * L.cout;
+        g.a_src = gout; g.b_src = n->params + L.w_off; g.out = gin; g.mask = mask;
+        if (int e = run_gemm(n, G_BWD_DATA, g, st)) return e;
+    }
+    GemmArgs g = geom(B, H, W, L.cin, L.cout);
+    g.M = 9 * L.cin; g.N = L.cout; g.K = B * H * W;
+    g.a_src = in; g.b_src = gout; g.out = n->work;
+    if (int e = run_gemm(n, G_BWD_W, g, st)) return e;
+    int chunk = 0;
+    const int sp = wgrad_splits(g, &chunk);
+    const size_t nw = (size_t)g.M * g.N;
+    sum_splits_kernel<<<ew_grid(nw), kT, 0, st>>>(n->work, sp, nw, n->grads + L.w_off);
+    SDP_LAUNCH_CHECK();
+    bias_grad_kernel<<<L.cout, kT, 0, st>>>(gout, B * H * W, L.cout, n->grads + L.b_off);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sdp_cnn_create(sdp_cnn **out, const sdp_cnn_params *pp)
+{
+    SDP_REQUIRE(out && pp, "sdp_cnn_create: null argument");
+    *out = nullptr;
+    const sdp_cnn_params p = *pp;
+    SDP_REQUIRE(p.levels >= 1 && p.levels <= 6 && p.base >= 1 && p.base <= 512 && p.in_ch >= 1 && p.out_ch >= 1,
+                "bad network shape");
+    SDP_REQUIRE(p.max_batch >= 1 && p.height >= 1 && p.width >= 1, "bad activation size");
+    SDP_REQUIRE(p.height % (1 << (p.levels - 1)) == 0 && p.width % (1 << (p.levels - 1)) == 0,
+                "height and width must be divisible by 2^(levels-1)");
+    SDP_REQUIRE(p.precision == SDP_CNN_FP32 || p.precision == SDP_CNN_BF16, "bad precision");
+    SDP_REQUIRE((long long)p.max_batch * p.height * p.width < (1LL << 31) / std::max(1, p.base << (p.levels - 1)),
+                "activations too large");
+    if (int e = sdp_init()) return e;
+    sdp_cnn *n = new sdp_cnn();
+    n->p = p;
+    auto add = [&](int cin, int cout) {
+        sdp_cnn::Layer L{cin, cout, n->n_params, n->n_params + (size_t)9 * cin * cout};
+        n->n_params += (size_t)9 * cin * cout + cout;
+        n->layers.push_back(L);
+    };
+    int prev = p.in_ch;
+    for (int l = 0; l < p.levels; ++l) {
+        add(prev, n->ch(l));
+        add(n->ch(l), n->ch(l));
+        prev = n->ch(l);
+    }
+    for (int l = p.levels - 2; l >= 0; --l) {
+        add(n->ch(l + 1), n->ch(l));
+        add(n->ch(l), n->ch(l));
+    }
+    add(n->ch(0), p.out_ch);
+
+    bool ok = true;
+    auto alloc = [&](float **ptr, size_t count) {
+        if (!ok) return;
+        if (hipMalloc((void **)ptr, sizeof(float) * std::max<size_t>(count, 1)) != hipSuccess) ok = false;
+    };
+    alloc(&n->params, n->n_params);
+    alloc(&n->grads, n->n_params);
+    alloc(&n->adam_m, n->n_params);
+    alloc(&n->adam_v, n->n_params);
+    const int L = p.levels;
+    n->x.assign(L, nullptr); n->a.assign(L, nullptr); n->e.assign(L, nullptr); n->up.assign(L, nullptr);
+    n->s.assign(L, nullptr); n->d.assign(L, nullptr); n->genc.assign(L, nullptr);
+    n->g1.assign(L, nullptr); n->g2.assign(L, nullptr);
+    size_t wmax = 0;
+    for (int l = 0; l < L; ++l) {
+        const size_t pix = (size_t)p.max_batch * n->hl(l) * n->wl(l);
+        const int c = n->ch(l), cu = l + 1 < L ? n->ch(l + 1) : c;
+        const int cmax = std::max({c, cu, p.in_ch, p.out_ch});
+        if (l > 0) alloc(&n->x[l], pix * c);   // pooled input of level l (channels of level l - 1 <= c)
+        alloc(&n->a[l], pix * c);
+        alloc(&n->e[l], pix * c);
+        if (l < L - 1) {
+            alloc(&n->up[l], pix * cu);
+            alloc(&n->s[l], pix * c);
+            alloc(&n->d[l], pix * c);
+            alloc(&n->genc[l], pix * c);
+        }
+        alloc(&n->g1[l], pix * cmax);
+        alloc(&n->g2[l], pix * cmax);
+        for (const auto &ly : n->layers) wmax = std::max(wmax, (size_t)9 * ly.cin * ly.cout);
+    }
+    n->work_floats = wmax * 1024;
+    alloc(&n->work, n->work_floats);
+    if (ok && hipMalloc((void **)&n->loss_part, sizeof(double) * 1024) != hipSuccess) ok = false;
+    if (ok) {
+        ok = hipMemset(n->grads, 0, sizeof(float) * n->n_params) == hipSuccess &&
+             hipMemset(n->adam_m, 0, sizeof(float) * n->n_params) == hipSuccess &&
+             hipMemset(n->adam_v, 0, sizeof(float) * n->n_params) == hipSuccess &&
+             hipMemset(n->params, 0, sizeof(float) * n->n_params) == hipSuccess;
+    }
+    if (!ok) {
+        sdp_cnn_destroy(n);
+        return sdp::set_error(SDP_E_NOMEM, "sdp_cnn_create: device allocation failed");
+    }
+    *out = n;
+    return SDP_OK;
+}
+
+int sdp_cnn_destroy(sdp_cnn *n)
+{
+    if (!n) return SDP_OK;
+    float *bufs[] = {n->params, n->grads, n->adam_m, n->adam_v, n->work};
+    for (float *b : bufs) (void)hipFree(b);
+    for (auto *v : {&n->x, &n->a, &n->e, &n->up, &n->s, &n->d, &n->genc, &n->g1, &n->g2})
+        for (float *b : *v) (void)hipFree(b);
+    (void)hipFree(n->loss_part);
+    delete n;
+    return SDP_OK;
+}
+
+size_t sdp_cnn_param_count(const sdp_cnn *n) { return n ? n->n_params : 0; }
+
+int sdp_cnn_set_params(sdp_cnn *n, const float *h_params)
+{
+    SDP_REQUIRE(n && h_params, "sdp_cnn_set_params: null argument");
+    SDP_HIP_TRY(hipMemcpy(n->params, h_params, sizeof(float) * n->n_params, hipMemcpyHostToDevice));
+    n->adam_t = 0;
+    SDP_HIP_TRY(hipMemset(n->adam_m, 0, sizeof(float) * n->n_params));
+    SDP_HIP_TRY(hipMemset(n->adam_v, 0, sizeof(float) * n->n_params));
+    return SDP_OK;
+}
+
+int sdp_cnn_get_params(const sdp_cnn *n, float *h_params)
+{
+    SDP_REQUIRE(n && h_params, "sdp_cnn_get_params: null argument");
+    SDP_HIP_TRY(hipMemcpy(h_params, n->params, sizeof(float) * n->n_params, hipMemcpyDeviceToHost));
+    return SDP_OK;
+}
+
+int sdp_cnn_get_grads(const sdp_cnn *n, float *h_grads)
+{
+    SDP_REQUIRE(n && h_grads, "sdp_cnn_get_grads: null argument");
+    SDP_HIP_TRY(hipMemcpy(h_grads, n->grads, sizeof(float) * n->n_params, hipMemcpyDeviceToHost));
+    return SDP_OK;
+}
+
+int sdp_cnn_forward(sdp_cnn *n, const float *d_in, float *d_out, int B, sdp_stream s)
+{
+    SDP_REQUIRE(n && d_in && d_out, "sdp_cnn_forward: null argument");
+    SDP_REQUIRE(B >= 1 && B <= n->p.max_batch, "batch %d outside [1, %d]", B, n->p.max_batch);
+    hipStream_t st = sdp::stream_of(s);
+    const int L = n->p.levels;
+    const float *xin = d_in;
+    for (int l = 0; l < L; ++l) {
+        const int H = n->hl(l), W = n->wl(l);
+        if (int e = conv_forward(n, enc_a(l), B, H, W, xin, n->a[l], nullptr, 1, st)) return e;
+        if (int e = conv_forward(n, enc_b(l), B, H, W, n->a[l], n->e[l], nullptr, 1, st)) return e;
+        if (l < L - 1) {
+            avgpool2_kernel<<<ew_grid((size_t)B * (H / 2) * (W / 2) * n->ch(l)), kT, 0, st>>>(n->e[l], B, H, W,
+                                                                                           n->ch(l), n->x[l + 1]);
+            SDP_LAUNCH_CHECK();
+            xin = n->x[l + 1];
+        }
+    }
+    const float *dcur = n->e[L - 1];
+    for (int l = L - 2; l >= 0; --l) {
+        const int H = n->hl(l), W = n->wl(l);
+        upsample2_kernel<<<ew_grid((size_t)B * H * W * n->ch(l + 1)), kT, 0, st>>>(dcur, B, H / 2, W / 2,
+                                                                                 n->ch(l + 1), n->up[l]);
+        SDP_LAUNCH_CHECK();
+        if (int e = conv_forward(n, dec_u(n, l), B, H, W, n->up[l], n->s[l], n->e[l], 1, st)) return e;
+        if (int e = conv_forward(n, dec_d(n, l), B, H, W, n->s[l], n->d[l], nullptr, 1, st)) return e;
+        dcur = n->d[l];
+    }
+    if (int e = conv_forward(n, head(n), B, n->hl(0), n->wl(0), dcur, d_out, nullptr, 0, st)) return e;
+    n->batch = B;
+    n->last_in = d_in;
+    return SDP_OK;
+}
+
+int sdp_cnn_backward(sdp_cnn *n, const float *d_grad_out, sdp_stream s)
+{
+    SDP_REQUIRE(n && d_grad_out, "sdp_cnn_backward: null argument");
+    SDP_REQUIRE(n->batch >= 1, "sdp_cnn_backward: no forward pass to differentiate");
+    hipStream_t st = sdp::stream_of(s);
+    const int L = n->p.levels, B = n->batch;
+    // head: gradient into d_0 (= e_0 when there is no decoder), masked by its ReLU
+    const float *d0 = L > 1 ? n->d[0] : n->e[0];
+    float *gcur = n->g1[0];
+    if (int e = conv_backward(n, head(n), B, n->hl(0), n->wl(0), d0, d_grad_out, gcur, d0, st)) return e;
+    // decoder, level 0 upwards; gcur holds dL/d(d_l) already masked
+    for (int l = 0; l < L - 1; ++l) {
+        const int H = n->hl(l), W = n->wl(l);
+        // d_l = relu(conv(s_l)): gradient into s_l, masked by s_l > 0 (s_l = relu(u_l + e_l))
+        if (int e = conv_backward(n, dec_d(n, l), B, H, W, n->s[l], gcur, n->genc[l], n->s[l], st)) return e;
+        // u_l = conv(up_l): gradient into up_l
+        if (int e = conv_backward(n, dec_u(n, l), B, H, W, n->up[l], n->genc[l], n->g2[l], nullptr, st)) return e;
+        // up_l = upsample2(d_{l+1}); d_{l+1} is the next decoder output or, at the top, e_top
+        const float *mask = (l + 1 <= L - 2) ? n->d[l + 1] : n->e[L - 1];
+        float *gnext = n->g1[l + 1];
+        upsample2_back_kernel<<<ew_grid((size_t)B * n->hl(l + 1) * n->wl(l + 1) * n->ch(l + 1)), kT, 0, st>>>(
+            n->g2[l], B, n->hl(l + 1), n->wl(l + 1), n->ch(l + 1), mask, gnext);
+        SDP_LAUNCH_CHECK();
+        gcur = gnext;
+    }
+    // encoder, top down.  Per level: ge = dL/d(e_l) masked by e_l > 0 in g1[l]
+    // (at the top it is already there), ga = dL/d(a_l) in g2[l], and the
+    // input gradient dL/d(x_l) back in g1[l] for the level below.
+    for (int l = L - 1; l >= 0; --l) {
+        const int H = n->hl(l), W = n->wl(l);
+        if (l < L - 1) {
+            // e_l feeds the skip of decoder level l and the pool into level l + 1
+            pool_back_add_mask_kernel<<<ew_grid((size_t)B * H * W * n->ch(l)), kT, 0, st>>>(
+                n->g1[l + 1], n->genc[l], n->e[l], B, H, W, n->ch(l), n->g1[l]);
+            SDP_LAUNCH_CHECK();
+        }
+        // e_l = relu(conv(a_l)): gradient into a_l, masked by a_l > 0
+        if (int e = conv_backward(n, enc_b(l), B, H, W, n->a[l], n->g1[l], n->g2[l], n->a[l], st)) return e;
+        // a_l = relu(conv(x_l)): gradient into x_l (not needed at level 0)
+        const float *xl = l == 0 ? n->last_in : n->x[l];
+        if (int e = conv_backward(n, enc_a(l), B, H, W, xl, n->g2[l], l == 0 ? nullptr : n->g1[l], nullptr, st))
+            return e;
+    }
+    return SDP_OK;
+}
+
+int sdp_cnn_mse(size_t count, const float *d_out, const float *d_target, float *d_grad, double *loss, sdp_stream s)
+{
+    SDP_REQUIRE(count > 0 && d_out && d_target && d_grad, "sdp_cnn_mse: bad argument");
+    hipStream_t st = sdp::stream_of(s);
+    double *part = nullptr;
+    SDP_HIP_TRY(hipMalloc(&part, sizeof(double) * 1024));
+    const unsigned int nb = std::min<unsigned int>(1024, ew_grid(count));
+    mse_kernel<<<nb, kT, 0, st>>>(d_out, d_target, count, d_grad, part);
+    std::vector<double> h(nb);
+    hipError_t e = hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(part);
+    if (e != hipSuccess) return sdp::set_error(SDP_E_HIP, "sdp_cnn_mse: %s", hipGetErrorString(e));
+    double sum = 0.0;
+    for (double v : h) sum += v;
+    if (loss) *loss = sum / (double)count;
+    return SDP_OK;
+}
+
+int sdp_cnn_adam(sdp_cnn *n, float lr, float beta1, float beta2, float eps, sdp_stream s)
+{
+    SDP_REQUIRE(n, "sdp_cnn_adam: null network");
+    n->adam_t += 1;
+    const float c1 = (float)(1.0 - std::pow((double)beta1, n->adam_t));
+    const float c2 = (float)(1.0 - std::pow((double)beta2, n->adam_t));
+    adam_kernel<<<ew_grid(n->n_params), kT, 0, sdp::stream_of(s)>>>(n->params, n->grads, n->adam_m, n->adam_v,
+                                                                     n->n_params, lr, beta1, beta2, eps, c1, c2);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+}  // extern "C"

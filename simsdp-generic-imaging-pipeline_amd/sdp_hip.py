@@ -41,6 +41,7 @@ HOGBOM_PSF_AS_STORED = 1
 # major-cycle pipeline (sdp_pipeline.h)
 PIPE_GRAPH, PIPE_REFERENCE_PREDICT, PIPE_PSF_MIRRORED, PIPE_PROFILE = 1, 2, 4, 8
 STAGES = ("predict_fft", "degrid_subtract", "grid", "inverse_fft", "clean")
+CNN_FP32, CNN_BF16 = 0, 1
 IMAGE_RESIDUAL, IMAGE_MODEL, IMAGE_PSF = 0, 1, 2
 
 
@@ -189,6 +190,13 @@ def load() -> ctypes.CDLL:
         "sdp_pipeline_stage_ms": (c_int, [P, P]),
         "sdp_pipeline_sources": (c_int, [P, P, ctypes.POINTER(c_int)]),
         "sdp_pipeline_plan": (c_void_p, [P]),
+        "sdp_cnn_create": (c_int, [ctypes.POINTER(c_void_p), P]), "sdp_cnn_destroy": (c_int, [P]),
+        "sdp_cnn_param_count": (c_size_t, [P]),
+        "sdp_cnn_set_params": (c_int, [P, P]), "sdp_cnn_get_params": (c_int, [P, P]),
+        "sdp_cnn_get_grads": (c_int, [P, P]),
+        "sdp_cnn_forward": (c_int, [P, P, P, c_int, P]), "sdp_cnn_backward": (c_int, [P, P, P]),
+        "sdp_cnn_mse": (c_int, [c_size_t, P, P, P, ctypes.POINTER(c_double), P]),
+        "sdp_cnn_adam": (c_int, [P, c_float, c_float, c_float, c_float, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -534,6 +542,95 @@ class Pipeline:
             self.close()
         except Exception:
             pass
+
+
+class _CnnParams(ctypes.Structure):
+    _fields_ = [("levels", c_int), ("base", c_int), ("in_ch", c_int), ("out_ch", c_int), ("height", c_int),
+                ("width", c_int), ("max_batch", c_int), ("precision", c_int)]
+
+
+def cnn_layer_shapes(levels, base, in_ch=2, out_ch=1):
+    """(name, cin, cout) of every convolution, in parameter order."""
+    ch = [base << l for l in range(levels)]
+    out, prev = [], in_ch
+    for l in range(levels):
+        out += [(f"enc{l}a", prev, ch[l]), (f"enc{l}b", ch[l], ch[l])]
+        prev = ch[l]
+    for l in range(levels - 2, -1, -1):
+        out += [(f"dec{l}u", ch[l + 1], ch[l]), (f"dec{l}d", ch[l], ch[l])]
+    out.append(("head", ch[0], out_ch))
+    return out
+
+
+class Cnn:
+    """The learned minor-cycle proposal network (sdp_cnn_* in sdp_device.h)."""
+
+    def __init__(self, levels, base, height, width, max_batch, in_ch=2, out_ch=1, precision=CNN_BF16):
+        self.levels, self.base, self.in_ch, self.out_ch = levels, base, in_ch, out_ch
+        self.p = _CnnParams(levels, base, in_ch, out_ch, height, width, max_batch, precision)
+        h = c_void_p()
+        check(load().sdp_cnn_create(ctypes.byref(h), ctypes.byref(self.p)))
+        self.handle = h.value
+        self.n_params = int(load().sdp_cnn_param_count(self.handle))
+
+    def flatten(self, params: dict) -> np.ndarray:
+        parts = []
+        for name, ci, co in cnn_layer_shapes(self.levels, self.base, self.in_ch, self.out_ch):
+            parts += [np.asarray(params[name + ".w"], np.float32).reshape(-1),
+                      np.asarray(params[name + ".b"], np.float32).reshape(-1)]
+        flat = np.concatenate(parts)
+        assert flat.size == self.n_params
+        return flat
+
+    def unflatten(self, flat: np.ndarray) -> dict:
+        out, off = {}, 0
+        for name, ci, co in cnn_layer_shapes(self.levels, self.base, self.in_ch, self.out_ch):
+            n = 9 * ci * co
+            out[name + ".w"] = flat[off:off + n].reshape(3, 3, ci, co)
+            out[name + ".b"] = flat[off + n:off + n + co]
+            off += n + co
+        return out
+
+    def set_params(self, params):
+        flat = np.ascontiguousarray(params if isinstance(params, np.ndarray) else self.flatten(params),
+                                    dtype=np.float32)
+        check(load().sdp_cnn_set_params(self.handle, flat.ctypes.data))
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.n_params, np.float32)
+        check(load().sdp_cnn_get_params(self.handle, out.ctypes.data))
+        return out
+
+    def get_grads(self) -> np.ndarray:
+        out = np.empty(self.n_params, np.float32)
+        check(load().sdp_cnn_get_grads(self.handle, out.ctypes.data))
+        return out
+
+    def forward(self, d_in, d_out, batch, stream=None):
+        check(load().sdp_cnn_forward(self.handle, _ptr(d_in), _ptr(d_out), int(batch), stream))
+
+    def backward(self, d_grad_out, stream=None):
+        check(load().sdp_cnn_backward(self.handle, _ptr(d_grad_out), stream))
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, stream=None):
+        check(load().sdp_cnn_adam(self.handle, lr, beta1, beta2, eps, stream))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            check(load().sdp_cnn_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cnn_mse(count, d_out, d_target, d_grad, stream=None) -> float:
+    loss = c_double()
+    check(load().sdp_cnn_mse(count, _ptr(d_out), _ptr(d_target), _ptr(d_grad), ctypes.byref(loss), stream))
+    return loss.value
 
 
 def dft_predict(n_vis, n_sources, d_sources, d_uvw, d_out, mode=DFT_FP64_PHASE, stream=None):
