@@ -220,6 +220,10 @@ int sdp_image_stats(size_t n, const float *d_img, double *sum_sq, float *max_val
 size_t sdp_image_stats_work_bytes(void);
 int sdp_image_stats_async(size_t n, const float *d_img, double *d_stats, void *d_work, sdp_stream s);
 int sdp_fill_complex(size_t n, float *d, float re, float im, sdp_stream s);
+/* out[i] = a x[i] + b y[i] for n floats (out may alias x or y) */
+int sdp_axpby(size_t n, float a, const float *d_x, float b, const float *d_y, float *d_out, sdp_stream s);
+/* out[i] = a[i] * b[i % nb] for n complex values (b repeats: one filter for a batch) */
+int sdp_complex_multiply(size_t n, const float *d_a, const float *d_b, size_t nb, float *d_out, sdp_stream s);
 /* d[i] = (float)((double)d[i] / by): psf_host_set_up's normalisation (top.c:505-507) */
 int sdp_divide(size_t n, float *d, double by, sdp_stream s);
 
@@ -270,6 +274,7 @@ typedef struct sdp_cnn sdp_cnn;
 int sdp_cnn_create(sdp_cnn **net, const sdp_cnn_params *params);
 int sdp_cnn_destroy(sdp_cnn *net);
 size_t sdp_cnn_param_count(const sdp_cnn *net);
+int sdp_cnn_describe(const sdp_cnn *net, sdp_cnn_params *params);
 /* host copies of the parameter / gradient vectors (set resets Adam) */
 int sdp_cnn_set_params(sdp_cnn *net, const float *h_params);
 int sdp_cnn_get_params(const sdp_cnn *net, float *h_params);
@@ -282,6 +287,23 @@ int sdp_cnn_backward(sdp_cnn *net, const float *d_grad_out, sdp_stream s);
 int sdp_cnn_mse(size_t count, const float *d_out, const float *d_target, float *d_grad, double *loss, sdp_stream s);
 /* one Adam step on the current gradients */
 int sdp_cnn_adam(sdp_cnn *net, float lr, float beta1, float beta2, float eps, sdp_stream s);
+/* Patches of a G x G image for the network.  Candidate patch p of the grid
+ * of stride T - 2 margin has origin (x0, y0) = ((p % npx) stride - margin,
+ * (p / npx) stride - margin), npx = ceil(G / stride); the centres
+ * [margin, T - margin)^2 of the candidates tile the image.
+ * sdp_cnn_patch_peaks: largest |residual| over each candidate's T x T tile
+ * (d_peaks may be NULL to only count the candidates).
+ * sdp_cnn_gather_patches: for n chosen patches (origins: int pairs x0, y0;
+ * peaks: their tiles' largest |residual|), batch[p][i][j] =
+ * {residual[y0+i][x0+j] / peak[p], psf[G/2 - T/2 + i][G/2 - T/2 + j]}.
+ * sdp_cnn_scatter_add: model += scale * peak[p] * out[p] over each centre. */
+int sdp_cnn_patch_peaks(int grid_size, int patch, int margin, const float *d_residual, float *d_peaks,
+                        int *n_patches, sdp_stream s);
+int sdp_cnn_gather_patches(int grid_size, int patch, int n, const int *d_origins, const float *d_peaks,
+                           const float *d_residual, const float *d_psf, float *d_batch, sdp_stream s);
+int sdp_cnn_scatter_add(int grid_size, int patch, int margin, int n, const int *d_origins, const float *d_peaks,
+                        const float *d_out, float scale, float floor, const float *d_batch_in, float in_floor,
+                        float *d_model, sdp_stream s);
 
 #ifdef __cplusplus
 }

@@ -87,6 +87,28 @@ enum { SDP_STAGE_PREDICT_FFT = 0, SDP_STAGE_DEGRID_SUBTRACT, SDP_STAGE_GRID, SDP
        SDP_STAGE_CLEAN, SDP_STAGE_COUNT };
 int sdp_pipeline_stage_ms(const sdp_pipeline *p, float *ms);
 
+/* Learned minor cycle.  With a network set, each major cycle, after forming
+ * the residual, proposes a model update with it instead of running Hogbom:
+ * the residual is cut into patch x patch tiles (their centres, patch -
+ * 2 margin cells wide, tile the image); tiles whose peak |residual| is at
+ * least `threshold` times the image's are fed to the network, each divided
+ * by its own peak, with the central PSF patch; the network returns the sky
+ * in those units, and gain times its values above `floor` (at pixels where
+ * the tile's residual / peak exceeds input_floor), rescaled by the
+ * tile's peak over psf_max, is added to the model over each tile's centre;
+ * with Hogbom iterations set (n_minor > 0) the residual is then updated by
+ * the PSF-convolved proposal and cleaned by Hogbom (the convention of
+ * the training pairs made by sdp_learned.py).  Hogbom is the fallback: when
+ * a cycle's residual RMS comes out above the previous cycle's, the last
+ * proposal is undone and the pipeline runs Hogbom cycles from then on; it
+ * also does so after max_cycles learned cycles (0: no limit).  The
+ * network (height = width = patch, in_ch 2, out_ch 1) must outlive the
+ * pipeline; NULL switches back to Hogbom.  Learned cycles run eagerly. */
+int sdp_pipeline_set_learned(sdp_pipeline *p, sdp_cnn *net, int patch, int margin, double gain, double threshold,
+                             double floor, double input_floor, int max_cycles);
+/* 1 while learned cycles are in use, 0 after a fallback to Hogbom or without a network */
+int sdp_pipeline_learned_active(const sdp_pipeline *p);
+
 /* Restarts from an empty model (PSF and plan kept). */
 int sdp_pipeline_reset(sdp_pipeline *p);
 /* Replaces the model image with G x G floats from device memory d_model

@@ -325,6 +325,66 @@ __global__ void adam_kernel(float *p, const float *g, float *m, float *v, size_t
     }
 }
 
+// network input from G x G images: per patch p with origin (x0, y0),
+// channel 0 = residual[y0 + i][x0 + j] / peak[p] (0 outside the image),
+// channel 1 = psf[G/2 - T/2 + i][G/2 - T/2 + j]
+__global__ void gather_patches_kernel(int G, int T, int n, const int2 *origins, const float *res, const float *peak,
+                                      const float *psf, float *batch)
+{
+    const size_t total = (size_t)n * T * T;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int x = i % T, y = (i / T) % T, p = (int)(i / ((size_t)T * T));
+        const int2 o = origins[p];
+        const int gy = o.y + y, gx = o.x + x;
+        const float r = (gy >= 0 && gy < G && gx >= 0 && gx < G) ? res[(size_t)gy * G + gx] / peak[p] : 0.f;
+        const int py = G / 2 - T / 2 + y, px = G / 2 - T / 2 + x;
+        const float q = (py >= 0 && py < G && px >= 0 && px < G) ? psf[(size_t)py * G + px] : 0.f;
+        batch[2 * i] = r;
+        batch[2 * i + 1] = q;
+    }
+}
+
+// model[y0 + i][x0 + j] += scale * peak[p] * out[p][i][j] over each patch's
+// centre [margin, T - margin)^2 (centres of distinct patches do not overlap),
+// for outputs above floor only
+__global__ void scatter_add_kernel(int G, int T, int margin, int n, const int2 *origins, const float *out,
+                                   const float *peak, float scale, float floor, const float *batch_in, float in_floor,
+                                   float *model)
+{
+    const int C = T - 2 * margin;
+    const size_t total = (size_t)n * C * C;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int x = margin + (int)(i % C), y = margin + (int)((i / C) % C), p = (int)(i / ((size_t)C * C));
+        const int2 o = origins[p];
+        const int gy = o.y + y, gx = o.x + x;
+        const size_t k = ((size_t)p * T + y) * T + x;
+        const float v = out[k];
+        const bool near = !batch_in || batch_in[2 * k] > in_floor;   // input residual / peak above in_floor
+        if (gy >= 0 && gy < G && gx >= 0 && gx < G && v > floor && near)
+            model[(size_t)gy * G + gx] += scale * peak[p] * v;
+    }
+}
+
+// largest |residual| over each candidate patch (the whole T x T tile)
+__global__ void patch_peak_kernel(int G, int T, int margin, int stride, int npx, const float *res, float *peak)
+{
+    __shared__ float red[kT];
+    const int p = blockIdx.x;
+    const int x0 = (p % npx) * stride - margin, y0 = (p / npx) * stride - margin;
+    float m = 0.f;
+    for (int i = threadIdx.x; i < T * T; i += kT) {
+        const int gy = y0 + i / T, gx = x0 + i % T;
+        if (gy >= 0 && gy < G && gx >= 0 && gx < G) m = fmaxf(m, fabsf(res[(size_t)gy * G + gx]));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = kT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) peak[p] = red[0];
+}
+
 unsigned int ew_grid(size_t n)
 {
     size_t b = (n + kT - 1) / kT;
@@ -546,6 +606,13 @@ int sdp_cnn_destroy(sdp_cnn *n)
 
 size_t sdp_cnn_param_count(const sdp_cnn *n) { return n ? n->n_params : 0; }
 
+int sdp_cnn_describe(const sdp_cnn *n, sdp_cnn_params *out)
+{
+    SDP_REQUIRE(n && out, "sdp_cnn_describe: null argument");
+    *out = n->p;
+    return SDP_OK;
+}
+
 int sdp_cnn_set_params(sdp_cnn *n, const float *h_params)
 {
     SDP_REQUIRE(n && h_params, "sdp_cnn_set_params: null argument");
@@ -666,6 +733,46 @@ int sdp_cnn_mse(size_t count, const float *d_out, const float *d_target, float *
     double sum = 0.0;
     for (double v : h) sum += v;
     if (loss) *loss = sum / (double)count;
+    return SDP_OK;
+}
+
+int sdp_cnn_gather_patches(int G, int T, int n, const int *d_origins, const float *d_peaks, const float *d_residual,
+                           const float *d_psf, float *d_batch, sdp_stream s)
+{
+    SDP_REQUIRE(G >= 1 && T >= 1 && n >= 0 && d_origins && d_peaks && d_residual && d_psf && d_batch,
+                "sdp_cnn_gather_patches: bad argument");
+    if (n == 0) return SDP_OK;
+    gather_patches_kernel<<<ew_grid((size_t)n * T * T), kT, 0, sdp::stream_of(s)>>>(
+        G, T, n, (const int2 *)d_origins, d_residual, d_peaks, d_psf, d_batch);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_cnn_scatter_add(int G, int T, int margin, int n, const int *d_origins, const float *d_peaks,
+                        const float *d_out, float scale, float floor, const float *d_batch_in, float in_floor,
+                        float *d_model, sdp_stream s)
+{
+    SDP_REQUIRE(G >= 1 && T >= 1 && margin >= 0 && 2 * margin < T && n >= 0 && d_origins && d_peaks && d_out &&
+                    d_model, "sdp_cnn_scatter_add: bad argument");
+    if (n == 0) return SDP_OK;
+    const int C = T - 2 * margin;
+    scatter_add_kernel<<<ew_grid((size_t)n * C * C), kT, 0, sdp::stream_of(s)>>>(
+        G, T, margin, n, (const int2 *)d_origins, d_out, d_peaks, scale, floor, d_batch_in, in_floor, d_model);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_cnn_patch_peaks(int G, int T, int margin, const float *d_residual, float *d_peaks, int *n_patches,
+                        sdp_stream s)
+{
+    SDP_REQUIRE(G >= 1 && T >= 1 && margin >= 0 && 2 * margin < T && d_residual && n_patches,
+                "sdp_cnn_patch_peaks: bad argument");
+    const int stride = T - 2 * margin;
+    const int npx = (G + stride - 1) / stride;
+    *n_patches = npx * npx;
+    if (!d_peaks) return SDP_OK;
+    patch_peak_kernel<<<npx * npx, kT, 0, sdp::stream_of(s)>>>(G, T, margin, stride, npx, d_residual, d_peaks);
+    SDP_LAUNCH_CHECK();
     return SDP_OK;
 }
 

@@ -200,6 +200,22 @@ __global__ void divide_kernel(float *d, size_t n, double by)
         d[i] = (float)((double)d[i] / by);
 }
 
+// out[i] = a[i] * b[i % nb] (complex)
+__global__ void complex_multiply_kernel(const float2 *a, const float2 *b, size_t n, size_t nb, float2 *out)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float2 x = a[i], y = b[i % nb];
+        out[i] = make_float2(x.x * y.x - x.y * y.y, x.x * y.y + x.y * y.x);
+    }
+}
+
+// out[i] = a x[i] + b y[i]
+__global__ void axpby_kernel(size_t n, float a, const float *x, float b, const float *y, float *out)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = a * x[i] + b * y[i];
+}
+
 int check_grid(int G)
 {
     if (G < 2 || (G & 1)) return sdp::set_error(SDP_E_ARG, "grid size %d must be even and >= 2", G);
@@ -322,6 +338,25 @@ int sdp_image_stats(size_t n, const float *d_img, double *sum_sq, float *max_val
     if (sum_sq) *sum_sq = h[0];
     if (max_value) *max_value = (float)h[1];
     if (max_abs) *max_abs = (float)h[2];
+    return SDP_OK;
+}
+
+int sdp_complex_multiply(size_t n, const float *d_a, const float *d_b, size_t nb, float *d_out, sdp_stream s)
+{
+    if (!n) return SDP_OK;
+    SDP_REQUIRE(d_a && d_b && d_out && nb > 0, "sdp_complex_multiply: bad argument");
+    complex_multiply_kernel<<<grid_for(n), kBlock, 0, sdp::stream_of(s)>>>((const float2 *)d_a, (const float2 *)d_b,
+                                                                           n, nb, (float2 *)d_out);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_axpby(size_t n, float a, const float *d_x, float b, const float *d_y, float *d_out, sdp_stream s)
+{
+    if (!n) return SDP_OK;
+    SDP_REQUIRE(d_x && d_y && d_out, "sdp_axpby: null argument");
+    axpby_kernel<<<grid_for(n), kBlock, 0, sdp::stream_of(s)>>>(n, a, d_x, b, d_y, d_out);
+    SDP_LAUNCH_CHECK();
     return SDP_OK;
 }
 

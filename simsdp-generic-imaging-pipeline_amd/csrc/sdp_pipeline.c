@@ -51,6 +51,15 @@ struct sdp_pipeline {
     sdp_event ev0, ev1;
     sdp_event stage_ev[SDP_STAGE_COUNT + 1];
     float stage_ms[SDP_STAGE_COUNT];
+    /* learned minor cycle */
+    sdp_cnn *net;
+    int patch, margin, batch_max, n_cand, learned, max_learned, learned_done;
+    double gain, threshold, floor, in_floor, last_rms;
+    int have_backup;
+    int *origins, *h_origins;   /* device / host int pairs, n_cand of them */
+    float *peaks, *h_peaks, *sel_peaks, *h_sel_peaks, *batch, *out, *model_backup;
+    float *delta;     /* G*G: model update, then its dirty image */
+    float *psf_ft;    /* G*G complex: psf_max / G^2 * FFT(PSF moved to the origin) */
 };
 
 #define TRY(x) do { int rc_ = (x); if (rc_ != SDP_OK) return rc_; } while (0)
@@ -65,8 +74,8 @@ static int mark(sdp_pipeline *P, int stage)
     return sdp_event_record(P->stage_ev[stage], P->s);
 }
 
-/* Stages 1-7 above: launches only, no synchronisation. */
-static int enqueue_cycle(sdp_pipeline *P)
+/* Stages 1-5 above: launches only, no synchronisation. */
+static int enqueue_residual(sdp_pipeline *P)
 {
     const sdp_pipeline_params *p = &P->p;
     const int G = p->grid_size;
@@ -84,12 +93,134 @@ static int enqueue_cycle(sdp_pipeline *P)
                   P->fft_work, P->s));
     TRY(mark(P, SDP_STAGE_CLEAN));
     TRY(sdp_image_stats_async(pixels(P), P->residual, P->stats, P->stats_work, P->s));
+    return SDP_OK;
+}
+
+/* Stages 6-7: Hogbom on the residual, statistics after it. */
+static int enqueue_clean(sdp_pipeline *P)
+{
+    const sdp_pipeline_params *p = &P->p;
+    const int G = p->grid_size;
     TRY(sdp_hogbom_ex(G, p->n_minor, p->max_sources, P->residual, P->psf, P->psf_max, p->loop_gain,
                       p->noise_factor, p->weak_source_percent, p->cell_size, P->model, P->model, P->sources,
                       P->num_sources, P->hog_work,
                       (p->flags & SDP_PIPE_PSF_MIRRORED) ? 0 : SDP_HOGBOM_PSF_AS_STORED, P->s));
     TRY(sdp_image_stats_async(pixels(P), P->residual, P->stats + 3, P->stats_work, P->s));
     TRY(mark(P, SDP_STAGE_COUNT));
+    return SDP_OK;
+}
+
+static int enqueue_cycle(sdp_pipeline *P)
+{
+    TRY(enqueue_residual(P));
+    return enqueue_clean(P);
+}
+
+/* The network's proposal from the current residual (stats[0..2] already on
+ * the host in h): tiles with a large enough peak, in batches. */
+static int propose(sdp_pipeline *P, const double *h)
+{
+    const int G = P->p.grid_size;
+    const double maxabs = h[2];
+    if (!(maxabs > 0.0)) return SDP_OK;
+    int n = 0;
+    TRY(sdp_cnn_patch_peaks(G, P->patch, P->margin, P->residual, P->peaks, &n, P->s));
+    TRY(sdp_memcpy_dtoh(P->h_peaks, P->peaks, sizeof(float) * (size_t)n, P->s));
+    const int stride = P->patch - 2 * P->margin, npx = (G + stride - 1) / stride;
+    int sel = 0;
+    for (int q = 0; q < n; ++q) {
+        if (!((double)P->h_peaks[q] >= P->threshold * maxabs) || !(P->h_peaks[q] > 0.f)) continue;
+        /* only tiles whose centre lies in the inner 80% of the field, where
+         * the gridding taper that the predict divides by stays large */
+        const int cx = (q % npx) * stride + stride / 2 - G / 2, cy = (q / npx) * stride + stride / 2 - G / 2;
+        if (10 * (cx < 0 ? -cx : cx) > 4 * G || 10 * (cy < 0 ? -cy : cy) > 4 * G) continue;
+        P->h_origins[2 * sel] = (q % npx) * stride - P->margin;
+        P->h_origins[2 * sel + 1] = (q / npx) * stride - P->margin;
+        P->h_sel_peaks[sel] = P->h_peaks[q];
+        ++sel;
+    }
+    if (sel == 0) return SDP_OK;
+    TRY(sdp_memcpy_htod(P->origins, P->h_origins, sizeof(int) * 2 * (size_t)sel, P->s));
+    TRY(sdp_memcpy_htod(P->sel_peaks, P->h_sel_peaks, sizeof(float) * (size_t)sel, P->s));
+    /* each tile enters divided by its peak; the network returns the sky in
+     * those units: model += gain * out * peak / psf_max */
+    const float out_scale = (float)(P->gain / P->psf_max);
+    for (int b0 = 0; b0 < sel; b0 += P->batch_max) {
+        const int nb = sel - b0 < P->batch_max ? sel - b0 : P->batch_max;
+        TRY(sdp_cnn_gather_patches(G, P->patch, nb, P->origins + 2 * b0, P->sel_peaks + b0, P->residual, P->psf,
+                                   P->batch, P->s));
+        TRY(sdp_cnn_forward(P->net, P->batch, P->out, nb, P->s));
+        TRY(sdp_cnn_scatter_add(G, P->patch, P->margin, nb, P->origins + 2 * b0, P->sel_peaks + b0, P->out,
+                                out_scale, (float)P->floor, P->batch, (float)P->in_floor, P->model, P->s));
+    }
+    return SDP_OK;
+}
+
+/* A cycle that follows a proposal: form the residual; when the proposal
+ * raised its RMS above the previous cycle's, undo it (the fallback) and
+ * form the residual again.  Returns 1 when the proposal was undone. */
+static int check_proposal(sdp_pipeline *P, double *h, int *undone)
+{
+    TRY(enqueue_residual(P));
+    TRY(sdp_memcpy_dtoh(h, P->stats, sizeof(double) * 3, P->s));
+    const double rms = sqrt(h[0] / (double)pixels(P));
+    *undone = 0;
+    if (P->have_backup && rms > P->last_rms) {
+        P->learned = 0;
+        *undone = 1;
+        TRY(sdp_memcpy_dtod(P->model, P->model_backup, sizeof(float) * pixels(P), P->s));
+        TRY(enqueue_residual(P));
+        TRY(sdp_memcpy_dtoh(h, P->stats, sizeof(double) * 3, P->s));
+    }
+    P->have_backup = 0;
+    P->last_rms = sqrt(h[0] / (double)pixels(P));
+    return SDP_OK;
+}
+
+/* A Hogbom cycle after a proposal (the learned cycles are over or were
+ * undone): checked residual, then Hogbom. */
+static int checked_clean_cycle(sdp_pipeline *P, double *h)
+{
+    int undone = 0;
+    TRY(check_proposal(P, h, &undone));
+    TRY(enqueue_clean(P));
+    TRY(sdp_memcpy_dtoh(h + 3, P->stats + 3, sizeof(double) * 3, P->s));
+    return SDP_OK;
+}
+
+/* One learned cycle: checked residual, then the proposal (or, when the
+ * previous proposal was undone, Hogbom). */
+static int learned_cycle(sdp_pipeline *P, double *h)
+{
+    int undone = 0;
+    TRY(check_proposal(P, h, &undone));
+    if (undone) {
+        TRY(enqueue_clean(P));
+        TRY(sdp_memcpy_dtoh(h + 3, P->stats + 3, sizeof(double) * 3, P->s));
+        return SDP_OK;
+    }
+    const double rms = P->last_rms;
+    TRY(sdp_memcpy_dtod(P->model_backup, P->model, sizeof(float) * pixels(P), P->s));
+    P->have_backup = 1;
+    P->last_rms = rms;
+    TRY(propose(P, h));
+    if (P->p.n_minor > 0) {
+        /* residual -= PSF (*) update (circular convolution, in the residual's
+         * units), then Hogbom cleans what the proposal left */
+        const int G = P->p.grid_size;
+        const size_t n = pixels(P);
+        TRY(sdp_axpby(n, 1.f, P->model, -1.f, P->model_backup, P->delta, P->s));
+        TRY(sdp_fft2d(P->delta, P->grid, G, SDP_FFT_FORWARD, SDP_FFT_REAL_IN, NULL, P->fft_work, P->s));
+        TRY(sdp_complex_multiply(n, P->grid, P->psf_ft, n, P->grid, P->s));
+        TRY(sdp_fft2d(P->grid, P->delta, G, SDP_FFT_INVERSE, SDP_FFT_REAL_OUT, NULL, P->fft_work, P->s));
+        TRY(sdp_axpby(n, 1.f, P->residual, -1.f, P->delta, P->residual, P->s));
+        TRY(enqueue_clean(P));
+        TRY(sdp_memcpy_dtoh(h + 3, P->stats + 3, sizeof(double) * 3, P->s));
+    } else {
+        h[3] = h[0];   /* no image-domain update: the next cycle measures the result */
+        h[4] = h[1];
+        h[5] = h[2];
+    }
     return SDP_OK;
 }
 
@@ -114,8 +245,11 @@ static int make_psf(sdp_pipeline *P)
     return SDP_OK;
 }
 
+static void free_learned(sdp_pipeline *P);
+
 static void release(sdp_pipeline *P)
 {
+    free_learned(P);
     if (P->graph) (void)sdp_graph_destroy(P->graph);
     if (P->ev0) (void)sdp_event_destroy(P->ev0);
     if (P->ev1) (void)sdp_event_destroy(P->ev1);
@@ -207,6 +341,9 @@ int sdp_pipeline_reset(sdp_pipeline *P)
     TRY(sdp_memset_zero(P->sources, sizeof(float) * 3 * (size_t)P->p.max_sources, P->s));
     TRY(sdp_memset_zero(P->num_sources, 16, P->s));
     P->cycles_done = 0;
+    P->have_backup = 0;
+    P->learned_done = 0;
+    if (P->net) P->learned = 1;
     return sdp_stream_sync(P->s);
 }
 
@@ -221,8 +358,18 @@ int sdp_pipeline_cycle(sdp_pipeline *P, double *rms_residual, double *rms_after,
 {
     if (!P) return FAIL(SDP_E_ARG, "null pipeline");
     TRY(sdp_event_record(P->ev0, P->s));
+    double h[6];
     const int use_graph = (P->p.flags & SDP_PIPE_GRAPH) && !(P->p.flags & SDP_PIPE_PROFILE);
-    if (use_graph && P->cycles_done > 0) {
+    if (P->net && P->learned && P->max_learned > 0 && P->learned_done >= P->max_learned) P->learned = 0;
+    const int learned = P->net && P->learned;
+    const int checked = !learned && P->net && P->have_backup;   /* first Hogbom cycle after a proposal */
+    const int host_h = learned || checked;                        /* h filled on the host by the cycle */
+    if (learned) P->learned_done++;
+    if (learned) {
+        TRY(learned_cycle(P, h));
+    } else if (checked) {
+        TRY(checked_clean_cycle(P, h));
+    } else if (use_graph && P->cycles_done > 0) {
         if (!P->graph) {
             /* the first cycle ran eagerly, so every lazily created resource
              * (twiddle tables, kernel attributes) exists before capture */
@@ -243,8 +390,8 @@ int sdp_pipeline_cycle(sdp_pipeline *P, double *rms_residual, double *rms_after,
         TRY(enqueue_cycle(P));
     }
     TRY(sdp_event_record(P->ev1, P->s));
-    double h[6];
-    TRY(sdp_memcpy_dtoh(h, P->stats, sizeof h, P->s));
+    if (!host_h) TRY(sdp_memcpy_dtoh(h, P->stats, sizeof h, P->s));
+    else TRY(sdp_event_sync(P->ev1));
     float t = 0.f;
     TRY(sdp_event_elapsed_ms(P->ev0, P->ev1, &t));
     if (P->p.flags & SDP_PIPE_PROFILE)
@@ -257,6 +404,80 @@ int sdp_pipeline_cycle(sdp_pipeline *P, double *rms_residual, double *rms_after,
     if (ms) *ms = t;
     return SDP_OK;
 }
+
+static void free_learned(sdp_pipeline *P)
+{
+    void *bufs[] = {P->origins, P->peaks, P->sel_peaks, P->batch, P->out, P->model_backup, P->delta, P->psf_ft};
+    for (size_t i = 0; i < sizeof bufs / sizeof bufs[0]; ++i) (void)sdp_device_free(bufs[i]);
+    free(P->h_origins);
+    free(P->h_peaks);
+    free(P->h_sel_peaks);
+    P->origins = NULL;
+    P->peaks = P->sel_peaks = P->batch = P->out = P->model_backup = P->delta = P->psf_ft = NULL;
+    P->h_origins = NULL;
+    P->h_peaks = P->h_sel_peaks = NULL;
+    P->net = NULL;
+    P->learned = 0;
+    P->have_backup = 0;
+}
+
+int sdp_pipeline_set_learned(sdp_pipeline *P, sdp_cnn *net, int patch, int margin, double gain, double threshold,
+                             double floor, double input_floor, int max_cycles)
+{
+    if (!P) return FAIL(SDP_E_ARG, "null pipeline");
+    TRY(sdp_stream_sync(P->s));
+    free_learned(P);
+    if (!net) return SDP_OK;
+    sdp_cnn_params cp;
+    TRY(sdp_cnn_describe(net, &cp));
+    if (cp.height != patch || cp.width != patch || cp.in_ch != 2 || cp.out_ch != 1)
+        return FAIL(SDP_E_ARG, "network shape does not match the patch (height = width = patch, 2 in, 1 out)");
+    if (margin < 0 || 2 * margin >= patch || !(gain > 0.0) || !(threshold >= 0.0))
+        return FAIL(SDP_E_ARG, "bad margin, gain or threshold");
+    const int G = P->p.grid_size;
+    const int stride = patch - 2 * margin;
+    const int npx = (G + stride - 1) / stride;
+    P->n_cand = npx * npx;
+    P->batch_max = cp.max_batch;
+    const size_t tile = (size_t)patch * patch;
+    int rc = sdp_device_malloc((void **)&P->origins, sizeof(int) * 2 * (size_t)P->n_cand);
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->peaks, sizeof(float) * (size_t)P->n_cand);
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->sel_peaks, sizeof(float) * (size_t)P->n_cand);
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->batch, sizeof(float) * 2 * tile * (size_t)P->batch_max);
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->out, sizeof(float) * tile * (size_t)P->batch_max);
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->model_backup, sizeof(float) * pixels(P));
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->delta, sizeof(float) * pixels(P));
+    if (rc == SDP_OK) rc = sdp_device_malloc((void **)&P->psf_ft, sizeof(float) * 2 * pixels(P));
+    if (rc == SDP_OK) {
+        /* FFT of the PSF with its peak moved from (G/2, G/2) to the origin:
+         * the output checkerboard is that shift, for even G */
+        rc = sdp_fft2d(P->psf, P->psf_ft, G, SDP_FFT_FORWARD, SDP_FFT_REAL_IN | SDP_FFT_SHIFT_OUT, NULL, P->fft_work,
+                       P->s);
+        if (rc == SDP_OK) rc = sdp_divide(2 * pixels(P), P->psf_ft, (double)pixels(P) / P->psf_max, P->s);
+        if (rc == SDP_OK) rc = sdp_stream_sync(P->s);
+    }
+    P->h_origins = (int *)malloc(sizeof(int) * 2 * (size_t)P->n_cand);
+    P->h_peaks = (float *)malloc(sizeof(float) * (size_t)P->n_cand);
+    P->h_sel_peaks = (float *)malloc(sizeof(float) * (size_t)P->n_cand);
+    if (rc == SDP_OK && (!P->h_origins || !P->h_peaks || !P->h_sel_peaks)) rc = FAIL(SDP_E_NOMEM, "out of host memory");
+    if (rc != SDP_OK) {
+        free_learned(P);
+        return rc;
+    }
+    P->net = net;
+    P->patch = patch;
+    P->margin = margin;
+    P->gain = gain;
+    P->threshold = threshold;
+    P->floor = floor;
+    P->in_floor = input_floor;
+    P->max_learned = max_cycles;
+    P->learned_done = 0;
+    P->learned = 1;
+    return SDP_OK;
+}
+
+int sdp_pipeline_learned_active(const sdp_pipeline *P) { return P && P->net && P->learned; }
 
 const float *sdp_pipeline_image(const sdp_pipeline *P, int which)
 {

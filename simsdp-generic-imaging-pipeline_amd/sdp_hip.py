@@ -197,6 +197,14 @@ def load() -> ctypes.CDLL:
         "sdp_cnn_forward": (c_int, [P, P, P, c_int, P]), "sdp_cnn_backward": (c_int, [P, P, P]),
         "sdp_cnn_mse": (c_int, [c_size_t, P, P, P, ctypes.POINTER(c_double), P]),
         "sdp_cnn_adam": (c_int, [P, c_float, c_float, c_float, c_float, P]),
+        "sdp_cnn_describe": (c_int, [P, P]),
+        "sdp_cnn_patch_peaks": (c_int, [c_int, c_int, c_int, P, P, ctypes.POINTER(c_int), P]),
+        "sdp_cnn_gather_patches": (c_int, [c_int, c_int, c_int, P, P, P, P, P, P]),
+        "sdp_cnn_scatter_add": (c_int, [c_int, c_int, c_int, c_int, P, P, P, c_float, c_float, P, c_float, P, P]),
+        "sdp_complex_multiply": (c_int, [c_size_t, P, P, c_size_t, P, P]),
+        "sdp_axpby": (c_int, [c_size_t, c_float, P, c_float, P, P, P]),
+        "sdp_pipeline_set_learned": (c_int, [P, P, c_int, c_int, c_double, c_double, c_double, c_double, c_int]),
+        "sdp_pipeline_learned_active": (c_int, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -499,6 +507,16 @@ class Pipeline:
     def reset(self):
         check(load().sdp_pipeline_reset(self.handle))
 
+    def set_learned(self, net, patch, margin, gain=0.5, threshold=0.02, floor=0.01, input_floor=-1.0, max_cycles=0):
+        """Learned minor cycle with network `net` (a Cnn), or Hogbom when None."""
+        self._net = net
+        check(load().sdp_pipeline_set_learned(self.handle, net.handle if net is not None else None, patch, margin,
+                                              gain, threshold, floor, input_floor, max_cycles))
+
+    @property
+    def learned_active(self) -> bool:
+        return bool(load().sdp_pipeline_learned_active(self.handle))
+
     def load_model(self, d_model):
         check(load().sdp_pipeline_load_model(self.handle, _ptr(d_model)))
 
@@ -625,6 +643,14 @@ class Cnn:
             self.close()
         except Exception:
             pass
+
+
+def axpby(n, a, d_x, b, d_y, d_out, stream=None):
+    check(load().sdp_axpby(n, a, _ptr(d_x), b, _ptr(d_y), _ptr(d_out), stream))
+
+
+def complex_multiply(n, d_a, d_b, nb, d_out, stream=None):
+    check(load().sdp_complex_multiply(n, _ptr(d_a), _ptr(d_b), nb, _ptr(d_out), stream))
 
 
 def cnn_mse(count, d_out, d_target, d_grad, stream=None) -> float:

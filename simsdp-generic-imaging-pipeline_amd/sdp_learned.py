@@ -1,0 +1,243 @@
+"""Training of the learned minor-cycle proposal (sdp_cnn_* kernels).
+
+A training pair is made from a seeded random sky patch S (Jy per pixel):
+power-law point sources plus elliptical Gaussians.  Its dirty patch is
+R = P (*) S, the circular convolution with the T x T centre of the imaging
+pipeline's own PSF P (peak 1), computed with the library's FFT on the GPU.
+The network input is (R / s, P) with s = max |R|, the target S / s: the same
+normalisation the pipeline applies to residual tiles (sdp_pipeline.h), so
+the network returns the sky in the units of its input.
+
+The network, its gradients and the optimiser all run in the HIP kernels;
+this module only draws the random skies and drives the training loop.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+import sdp_hip as hip
+
+
+def psf_patch(psf_image: np.ndarray, T: int) -> np.ndarray:
+    """Centre T x T of a G x G PSF image whose peak is at (G/2, G/2)."""
+    G = psf_image.shape[0]
+    c = G // 2 - T // 2
+    return np.ascontiguousarray(psf_image[c:c + T, c:c + T], dtype=np.float32)
+
+
+def random_sky(T: int, rng: np.random.Generator, n_points=(1, 30), n_gauss=(0, 20), fwhm=(2.0, 30.0),
+               flux=(1e-3, 1.0), index=-1.5, margin=8) -> np.ndarray:
+    """Point sources with dN/dS ~ S^index on [flux] plus elliptical Gaussians
+    (FWHM per axis uniform in [fwhm], random orientation, peak 0.2 times a
+    flux drawn the same way), kept `margin` pixels away from the edges of
+    the T x T sky."""
+    sky = np.zeros((T, T), np.float64)
+    a = index + 1.0
+
+    def draw(n):
+        u = rng.random(n)
+        return (flux[0] ** a + u * (flux[1] ** a - flux[0] ** a)) ** (1.0 / a)
+
+    n = int(rng.integers(n_points[0], n_points[1] + 1))
+    ys = rng.integers(margin, T - margin, n)
+    xs = rng.integers(margin, T - margin, n)
+    np.add.at(sky, (ys, xs), draw(n))
+    ng = int(rng.integers(n_gauss[0], n_gauss[1] + 1))
+    k = 1.0 / (8.0 * math.log(2.0))   # sigma^2 = FWHM^2 k
+    for f, cy, cx, fa, fb, th in zip(draw(ng), rng.uniform(margin, T - margin, ng),
+                                     rng.uniform(margin, T - margin, ng), rng.uniform(*fwhm, ng),
+                                     rng.uniform(*fwhm, ng), rng.uniform(0, np.pi, ng)):
+        r = int(math.ceil(3.0 * max(fa, fb)))   # stamp to 3 FWHM
+        y0, y1 = max(0, int(cy) - r), min(T, int(cy) + r + 1)
+        x0, x1 = max(0, int(cx) - r), min(T, int(cx) + r + 1)
+        yy, xx = np.mgrid[y0:y1, x0:x1].astype(np.float64)
+        dy, dx = yy - cy, xx - cx
+        u = dx * math.cos(th) + dy * math.sin(th)
+        v = -dx * math.sin(th) + dy * math.cos(th)
+        sky[y0:y1, x0:x1] += 0.2 * f * np.exp(-0.5 * (u * u / (fa * fa * k) + v * v / (fb * fb * k)))
+    return sky.astype(np.float32)
+
+
+def init_params(net: "hip.Cnn", seed: int) -> np.ndarray:
+    """He-normal weights (standard deviation sqrt(2 / (9 cin))), zero biases,
+    as one parameter vector in the network's layer order."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for name, ci, co in hip.cnn_layer_shapes(net.levels, net.base, net.in_ch, net.out_ch):
+        parts.append((rng.standard_normal(9 * ci * co) * math.sqrt(2.0 / (9 * ci))).astype(np.float32))
+        parts.append(np.zeros(co, np.float32))
+    flat = np.concatenate(parts)
+    net.set_params(flat)
+    return flat
+
+
+class Convolver:
+    """Circular convolution of T x T real patches with the PSF patch, on the
+    GPU: FFT (library kernels), multiplication by the PSF's transform,
+    inverse FFT, real part."""
+
+    def __init__(self, psf: np.ndarray):
+        self.T = T = psf.shape[0]
+        # transform of the PSF with its peak moved to (0, 0), divided by T^2
+        # (the library's inverse FFT is unnormalised)
+        ft = np.fft.fft2(np.fft.ifftshift(psf.astype(np.float64))) / (T * T)
+        self.d_ft = hip.DeviceBuffer.from_array(np.stack([ft.real, ft.imag], -1).astype(np.float32))
+        self.d_in = hip.DeviceBuffer(4 * T * T)
+        self.d_c = hip.DeviceBuffer(8 * T * T)
+        self.d_out = hip.DeviceBuffer(4 * T * T)
+        self.d_work = hip.DeviceBuffer(hip.fft_work_bytes(T))
+
+    def __call__(self, sky: np.ndarray) -> np.ndarray:
+        T = self.T
+        self.d_in.upload(np.ascontiguousarray(sky, np.float32))
+        hip.fft2d(self.d_in, self.d_c, T, hip.FFT_FORWARD, hip.FFT_REAL_IN, None, self.d_work)
+        hip.complex_multiply(T * T, self.d_c, self.d_ft, T * T, self.d_c)
+        hip.fft2d(self.d_c, self.d_out, T, hip.FFT_INVERSE, hip.FFT_REAL_OUT, None, self.d_work)
+        return self.d_out.download((T, T), np.float32)
+
+
+class DataTerm:
+    """Residual of a proposal in the data domain: r = P (*) out - dirty for a
+    batch of T x T patches (circular convolution by FFT), its mean square,
+    and its gradient 2 P^T (*) r / n added to a gradient buffer: the part of
+    the loss that scores a proposal the way the next major cycle will."""
+
+    def __init__(self, psf: np.ndarray, batch: int):
+        self.T = T = psf.shape[0]
+        self.batch = batch
+        ft = np.fft.fft2(np.fft.ifftshift(psf.astype(np.float64))) / (T * T)
+        self.d_ft = hip.DeviceBuffer.from_array(np.stack([ft.real, ft.imag], -1).astype(np.float32))
+        self.d_ftc = hip.DeviceBuffer.from_array(np.stack([ft.real, -ft.imag], -1).astype(np.float32))
+        self.d_c = hip.DeviceBuffer(8 * T * T * batch)
+        self.d_r = hip.DeviceBuffer(4 * T * T * batch)
+        self.d_work = hip.DeviceBuffer(hip.fft_work_bytes(T))
+
+    def _conv(self, d_src, d_dst, d_filter, n):
+        T = self.T
+        for b in range(n):
+            hip.fft2d(d_src.ptr + 4 * b * T * T, self.d_c.ptr + 8 * b * T * T, T, hip.FFT_FORWARD, hip.FFT_REAL_IN,
+                      None, self.d_work)
+        hip.complex_multiply(n * T * T, self.d_c, d_filter, T * T, self.d_c)
+        for b in range(n):
+            hip.fft2d(self.d_c.ptr + 8 * b * T * T, d_dst.ptr + 4 * b * T * T, T, hip.FFT_INVERSE, hip.FFT_REAL_OUT,
+                      None, self.d_work)
+
+    def add_grad(self, d_out, d_dirty, d_grad, n, weight) -> float:
+        """d_grad += weight * 2 P^T (*) r / count; returns mean(r^2)."""
+        count = n * self.T * self.T
+        self._conv(d_out, self.d_r, self.d_ft, n)
+        hip.axpby(count, 1.0, self.d_r, -1.0, d_dirty, self.d_r)
+        sq, _, _ = hip.image_stats(count, self.d_r)
+        self._conv(self.d_r, self.d_r, self.d_ftc, n)
+        hip.axpby(count, 1.0, d_grad, 2.0 * weight / count, self.d_r, d_grad)
+        return sq / count
+
+
+@dataclass
+class Pool:
+    inputs: np.ndarray    # (n, T, T, 2)
+    targets: np.ndarray   # (n, T, T, 1)
+
+
+def smooth(image: np.ndarray, sigma: float) -> np.ndarray:
+    """Circular convolution with a unit-sum Gaussian of standard deviation
+    sigma pixels (sigma 0: unchanged)."""
+    if sigma <= 0:
+        return image
+    G = image.shape[0]
+    f = np.fft.fftfreq(G)
+    g = np.exp(-2.0 * (math.pi * sigma) ** 2 * (f[:, None] ** 2 + f[None, :] ** 2))
+    return np.fft.ifft2(np.fft.fft2(image) * g).real.astype(np.float32)
+
+
+def make_pool(psf: np.ndarray, n: int, seed: int, T: int | None = None, crops_per_sky: int = 4,
+              target_sigma: float = 0.0, **sky_kw) -> Pool:
+    """n training pairs of T x T patches.  psf is the pipeline's PSF image
+    (G x G, peak 1 at G/2; pass a T x T PSF patch to draw skies of the patch
+    size).  Each random sky covers the PSF's full G x G field; its dirty
+    image is the circular convolution with the PSF; crops_per_sky patches
+    are cut from it at random positions, so patches carry the sidelobes of
+    sources outside them, as residual tiles in the major cycle do.  Input
+    (dirty / s, central PSF patch), target sky / s, s = max |dirty patch|;
+    with target_sigma > 0 the target sky is smoothed by a Gaussian of that
+    many pixels (a compact model the network can localise)."""
+    G = psf.shape[0]
+    T = T or G
+    conv = Convolver(psf)
+    rng = np.random.default_rng(seed)
+    pp = psf_patch(psf, T)
+    x = np.empty((n, T, T, 2), np.float32)
+    y = np.empty((n, T, T, 1), np.float32)
+    i = 0
+    while i < n:
+        sky = random_sky(G, rng, **sky_kw)
+        dirty = conv(sky)
+        target = smooth(sky, target_sigma)
+        for _ in range(crops_per_sky):
+            if i >= n:
+                break
+            oy, ox = (int(v) for v in rng.integers(0, G - T + 1, 2))
+            d = dirty[oy:oy + T, ox:ox + T]
+            s = float(np.abs(d).max()) or 1.0
+            x[i, :, :, 0] = d / s
+            x[i, :, :, 1] = pp
+            y[i, :, :, 0] = target[oy:oy + T, ox:ox + T] / s
+            i += 1
+    return Pool(x, y)
+
+
+def train(net: "hip.Cnn", pool: Pool, steps: int, batch: int, lr: float = 1e-3, seed: int = 0,
+          lr_final: float = 1e-4, log_every: int = 0, sky_weight: float = 1.0, data_weight: float = 0.0,
+          psf: np.ndarray | None = None) -> list[float]:
+    """Adam on batches drawn from the pool; learning rate decays
+    geometrically from lr to lr_final.  The loss is sky_weight times the
+    mean square error against the true sky plus data_weight times the mean
+    square of the proposal's data-domain residual (DataTerm, needs the PSF
+    patch).  Returns the loss per step."""
+    n, T = pool.inputs.shape[0], pool.inputs.shape[1]
+    rng = np.random.default_rng(seed)
+    d_x = hip.DeviceBuffer(4 * batch * T * T * 2)
+    d_t = hip.DeviceBuffer(4 * batch * T * T)
+    d_y = hip.DeviceBuffer(4 * batch * T * T)
+    d_g = hip.DeviceBuffer(4 * batch * T * T)
+    d_dirty = hip.DeviceBuffer(4 * batch * T * T)
+    data = DataTerm(psf, batch) if data_weight > 0 else None
+    losses = []
+    for step in range(steps):
+        idx = rng.choice(n, batch, replace=False)
+        d_x.upload(np.ascontiguousarray(pool.inputs[idx]))
+        d_t.upload(np.ascontiguousarray(pool.targets[idx]))
+        net.forward(d_x, d_y, batch)
+        loss = sky_weight * hip.cnn_mse(batch * T * T, d_y, d_t, d_g)
+        if sky_weight != 1.0:
+            hip.axpby(batch * T * T, sky_weight, d_g, 0.0, d_g, d_g)
+        if data is not None:
+            d_dirty.upload(np.ascontiguousarray(pool.inputs[idx, :, :, 0]))
+            loss += data_weight * data.add_grad(d_y, d_dirty, d_g, batch, data_weight)
+        losses.append(loss)
+        net.backward(d_g)
+        frac = step / max(steps - 1, 1)
+        net.adam(lr * (lr_final / lr) ** frac)
+        if log_every and step % log_every == 0:
+            print(f"step {step}: loss {losses[-1]:.4g}", flush=True)
+    return losses
+
+
+def evaluate(net: "hip.Cnn", pool: Pool, batch: int) -> float:
+    """Mean squared error of the network on the pool."""
+    n, T = pool.inputs.shape[0], pool.inputs.shape[1]
+    d_x = hip.DeviceBuffer(4 * batch * T * T * 2)
+    d_t = hip.DeviceBuffer(4 * batch * T * T)
+    d_y = hip.DeviceBuffer(4 * batch * T * T)
+    d_g = hip.DeviceBuffer(4 * batch * T * T)
+    tot, cnt = 0.0, 0
+    for b0 in range(0, n - batch + 1, batch):
+        d_x.upload(np.ascontiguousarray(pool.inputs[b0:b0 + batch]))
+        d_t.upload(np.ascontiguousarray(pool.targets[b0:b0 + batch]))
+        net.forward(d_x, d_y, batch)
+        tot += hip.cnn_mse(batch * T * T, d_y, d_t, d_g) * batch
+        cnt += batch
+    return tot / max(cnt, 1)
