@@ -109,7 +109,86 @@ def major_cycle(hip, sdp_synth, d, d_uvw, d_meas, d_gk, d_dk, tile, chunk, targe
             "cycles": cycles, "s_total": round(elapsed_ms * 1e-3, 4),
             "target_rms": float(f"{target:.6g}"), "target": f"{target_frac} x dirty-image RMS",
             "s_to_target": None if s_to_target is None else round(s_to_target, 4),
-            "components_last_cycle": n_src, "learned": None}
+            "components_last_cycle": n_src}
+
+
+def learned_vs_hogbom(hip, steps, seed=5):
+    """Major-cycle time to a fixed residual RMS, learned proposal + Hogbom
+    against plain Hogbom, on the learned-proposal workload: config-1 array
+    (516,096 visibilities, 1024^2 grid, OS 16, no w-term) observing 50
+    power-law point sources and 20 elliptical Gaussians (FWHM 2-30 px) in a
+    2 x 2 degree box, noise 0.01 Jy.  The plain path runs 5 major cycles x
+    1000 Hogbom iterations (gain 0.1); its final residual RMS is the target.
+    The network (3 levels, 16 base channels, BF16 MFMA) is trained here by
+    the library's kernels on pairs made from the pipeline's own PSF; the
+    training time is reported, not counted in the major-cycle time."""
+    import numpy as np
+    import sdp_learned as L
+
+    w = L.learned_workload(seed=seed)
+    G, T, margin = w.ds.grid_size, 256, 32
+
+    def run(pipe, target, max_cycles):
+        hip.synchronize()
+        t0 = time.perf_counter()
+        ms, rows = L.cycles_to_target(pipe, target, max_cycles)
+        hip.synchronize()
+        return ms, rows, (time.perf_counter() - t0) * 1e3
+
+    plain = w.pipeline(1000, flags=hip.PIPE_GRAPH)
+    psf = plain.image(hip.IMAGE_PSF)
+    _, hog_rows, hog_wall = run(plain, 0.0, 5)
+    plain.close()
+    target = hog_rows[-1][1]
+    hog_ms = sum(r[2] for r in hog_rows)
+    log(f"learned workload: plain Hogbom 5 x 1000 -> RMS {target:.4g} Jy in {hog_ms:.1f} ms")
+    p300 = w.pipeline(300, flags=hip.PIPE_GRAPH)
+    h300_ms, h300_rows, _ = run(p300, target, 40)
+    p300.close()
+    t = time.perf_counter()
+    pool = L.make_pool(psf, 3072, seed=1, T=T, crops_per_sky=8, global_scale=True,
+                       crop_range=(G // 2 - T - 64, G // 2 + 64), n_points=(20, 80), n_gauss=(0, 20), box=w.box)
+    pool_s = time.perf_counter() - t
+    net = hip.Cnn(3, 16, T, T, 16, precision=hip.CNN_BF16)
+    L.init_params(net, 0)
+    t = time.perf_counter()
+    losses = L.train(net, pool, steps, 16, lr=3e-4, lr_final=3e-5, data_weight=1.0, psf=L.psf_patch(psf, T))
+    hip.synchronize()
+    train_s = time.perf_counter() - t
+    log(f"network trained: {steps} steps in {train_s:.1f} s, loss {losses[0]:.3g} -> {np.mean(losses[-50:]):.3g}")
+    lp = w.pipeline(300)
+    lp.set_learned(net, T, margin, gain=1.0, threshold=0.05, floor=0.0, max_cycles=3,
+                   scale=hip.LEARNED_GLOBAL_SCALE)
+    l_ms, l_rows, l_wall = run(lp, target, 14)
+    lp.close()
+    net.close()
+    log(f"learned + Hogbom: {'not reached' if l_ms is None else f'{l_ms:.1f} ms'} in {len(l_rows)} cycles")
+
+    def fmt(rows):
+        return [[float(f"{a:.5g}"), float(f"{b:.5g}"), round(ms, 3)] for a, b, ms in rows]
+
+    return {
+        "workload": f"config-1 array ({w.ds.n_vis} vis), {G}^2 grid, OS 16, no w-term; sky of 50 power-law points "
+                    f"+ 20 elliptical Gaussians FWHM 2-30 px in a 2 deg box (seed {seed}), noise 0.01 Jy",
+        "target_rms": float(f"{target:.6g}"), "target": "residual RMS after 5 x 1000 Hogbom iterations",
+        "hogbom": {"n_minor": 1000, "gain": 0.1, "ms": round(hog_ms, 3), "wall_ms": round(hog_wall, 3),
+                   "cycles": fmt(hog_rows)},
+        "hogbom_n300": {"n_minor": 300, "ms": None if h300_ms is None else round(h300_ms, 3),
+                        "n_cycles": len(h300_rows)},
+        "learned": {"n_minor": 300, "learned_cycles": 3, "gain": 1.0, "threshold": 0.05, "patch": T,
+                    "margin": margin, "scale": "global",
+                    "ms": None if l_ms is None else round(l_ms, 3), "wall_ms": round(l_wall, 3),
+                    "cycles": fmt(l_rows)},
+        "ratio": None if l_ms is None else round(l_ms / hog_ms, 4),
+        "net": {"levels": 3, "base": 16, "params": int(net_params(hip, 3, 16)), "precision": "bf16 MFMA, fp32 acc"},
+        "train": {"pairs": int(pool.inputs.shape[0]), "steps": steps, "batch": 16, "s": round(train_s, 2),
+                  "pool_s": round(pool_s, 2), "loss_first": float(f"{losses[0]:.4g}"),
+                  "loss_last50": float(f"{np.mean(losses[-50:]):.4g}")},
+    }
+
+
+def net_params(hip, levels, base):
+    return sum(9 * ci * co + co for _, ci, co in hip.cnn_layer_shapes(levels, base, 2, 1))
 
 
 def main():
@@ -121,6 +200,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-Hogbom major-cycle comparison")
+    ap.add_argument("--learned-steps", type=int, default=2000, help="training steps of the proposal network")
     ap.add_argument("--accumulate", choices=["blocked", "compensated"], default="blocked",
                     help="gridder accumulation inside a work item: float (blocked) or TwoSum; "
                          "work items are merged TwoSum-compensated either way")
@@ -274,6 +355,9 @@ def main():
         extras["gtaps_per_s"] = round(2 * taps / ((k_ms["degrid_subtract"] + k_ms["grid_accumulate"]) * 1e-3) / 1e9, 1)
         extras["plan"] = info
         extras["major_cycle"] = major_cycle(hip, sdp_synth, d, d_uvw, d_meas, d_gk, d_dk, tile, chunk)
+        if not args.no_learned:
+            log("learned vs Hogbom ...")
+            extras["learned_vs_hogbom"] = learned_vs_hogbom(hip, args.learned_steps)
 
     out = {
         "metric": METRIC,

@@ -294,9 +294,14 @@ int sdp_cnn_adam(sdp_cnn *net, float lr, float beta1, float beta2, float eps, sd
  * sdp_cnn_patch_peaks: largest |residual| over each candidate's T x T tile
  * (d_peaks may be NULL to only count the candidates).
  * sdp_cnn_gather_patches: for n chosen patches (origins: int pairs x0, y0;
- * peaks: their tiles' largest |residual|), batch[p][i][j] =
- * {residual[y0+i][x0+j] / peak[p], psf[G/2 - T/2 + i][G/2 - T/2 + j]}.
- * sdp_cnn_scatter_add: model += scale * peak[p] * out[p] over each centre. */
+ * peaks: each patch's scale, e.g. its tile's largest |residual|),
+ * batch[p][i][j] = {residual[y0+i][x0+j] / peak[p],
+ * psf[G/2 - T/2 + i][G/2 - T/2 + j]}, zero outside the image.
+ * sdp_cnn_scatter_add: model[y0+i][x0+j] += scale * peak[p] * out[p][i][j]
+ * over each centre [margin, T - margin)^2 inside the image, where
+ * out > floor and, when d_batch_in (the gathered batch) is given, where
+ * batch_in[p][i][j][0] > in_floor.  The centres of the n patches must not
+ * overlap (no atomics). */
 int sdp_cnn_patch_peaks(int grid_size, int patch, int margin, const float *d_residual, float *d_peaks,
                         int *n_patches, sdp_stream s);
 int sdp_cnn_gather_patches(int grid_size, int patch, int n, const int *d_origins, const float *d_peaks,

@@ -90,22 +90,31 @@ int sdp_pipeline_stage_ms(const sdp_pipeline *p, float *ms);
 /* Learned minor cycle.  With a network set, each major cycle, after forming
  * the residual, proposes a model update with it instead of running Hogbom:
  * the residual is cut into patch x patch tiles (their centres, patch -
- * 2 margin cells wide, tile the image); tiles whose peak |residual| is at
- * least `threshold` times the image's are fed to the network, each divided
- * by its own peak, with the central PSF patch; the network returns the sky
- * in those units, and gain times its values above `floor` (at pixels where
- * the tile's residual / peak exceeds input_floor), rescaled by the
- * tile's peak over psf_max, is added to the model over each tile's centre;
- * with Hogbom iterations set (n_minor > 0) the residual is then updated by
- * the PSF-convolved proposal and cleaned by Hogbom (the convention of
- * the training pairs made by sdp_learned.py).  Hogbom is the fallback: when
- * a cycle's residual RMS comes out above the previous cycle's, the last
- * proposal is undone and the pipeline runs Hogbom cycles from then on; it
- * also does so after max_cycles learned cycles (0: no limit).  The
- * network (height = width = patch, in_ch 2, out_ch 1) must outlive the
- * pipeline; NULL switches back to Hogbom.  Learned cycles run eagerly. */
-int sdp_pipeline_set_learned(sdp_pipeline *p, sdp_cnn *net, int patch, int margin, double gain, double threshold,
-                             double floor, double input_floor, int max_cycles);
+ * 2 margin cells wide, tile the inner 80% of the image); tiles whose peak
+ * |residual| is at least `threshold` times the image's are fed to the
+ * network with the central PSF patch, each divided by a scale: its own
+ * peak (SDP_LEARNED_TILE_SCALE) or the image's peak |residual|
+ * (SDP_LEARNED_GLOBAL_SCALE).  The network returns the sky in those units;
+ * gain times its values above `floor` (at pixels where the tile's residual
+ * / scale exceeds input_floor; a negative input_floor keeps every pixel),
+ * rescaled by scale / psf_max, is added to the model over each tile's
+ * centre.  With Hogbom iterations set (n_minor > 0) the residual is then
+ * updated by the PSF-convolved proposal and cleaned by Hogbom (the
+ * convention of the training pairs made by sdp_learned.py).  Hogbom is the
+ * fallback: when a cycle's residual RMS comes out above the previous
+ * cycle's, the last proposal is undone and the pipeline runs Hogbom cycles
+ * from then on; it also does so after max_cycles learned cycles (0: no
+ * limit).  The network (height = width = patch, in_ch 2, out_ch 1) must
+ * outlive the pipeline; NULL switches back to Hogbom.  Learned cycles run
+ * eagerly. */
+enum { SDP_LEARNED_TILE_SCALE = 0, SDP_LEARNED_GLOBAL_SCALE = 1 };
+typedef struct sdp_learned_params {
+    int patch, margin;
+    double gain, threshold, floor, input_floor;
+    int max_cycles;
+    int scale;
+} sdp_learned_params;
+int sdp_pipeline_set_learned(sdp_pipeline *p, sdp_cnn *net, const sdp_learned_params *params);
 /* 1 while learned cycles are in use, 0 after a fallback to Hogbom or without a network */
 int sdp_pipeline_learned_active(const sdp_pipeline *p);
 

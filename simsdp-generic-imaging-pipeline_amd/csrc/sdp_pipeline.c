@@ -55,6 +55,7 @@ struct sdp_pipeline {
     sdp_cnn *net;
     int patch, margin, batch_max, n_cand, learned, max_learned, learned_done;
     double gain, threshold, floor, in_floor, last_rms;
+    int global_scale;
     int have_backup;
     int *origins, *h_origins;   /* device / host int pairs, n_cand of them */
     float *peaks, *h_peaks, *sel_peaks, *h_sel_peaks, *batch, *out, *model_backup;
@@ -136,14 +137,15 @@ static int propose(sdp_pipeline *P, const double *h)
         if (10 * (cx < 0 ? -cx : cx) > 4 * G || 10 * (cy < 0 ? -cy : cy) > 4 * G) continue;
         P->h_origins[2 * sel] = (q % npx) * stride - P->margin;
         P->h_origins[2 * sel + 1] = (q / npx) * stride - P->margin;
-        P->h_sel_peaks[sel] = P->h_peaks[q];
+        P->h_sel_peaks[sel] = P->global_scale ? (float)maxabs : P->h_peaks[q];
         ++sel;
     }
     if (sel == 0) return SDP_OK;
     TRY(sdp_memcpy_htod(P->origins, P->h_origins, sizeof(int) * 2 * (size_t)sel, P->s));
     TRY(sdp_memcpy_htod(P->sel_peaks, P->h_sel_peaks, sizeof(float) * (size_t)sel, P->s));
-    /* each tile enters divided by its peak; the network returns the sky in
-     * those units: model += gain * out * peak / psf_max */
+    /* each tile enters divided by its scale (its peak or the image's); the
+     * network returns the sky in those units: model += gain * out * scale /
+     * psf_max */
     const float out_scale = (float)(P->gain / P->psf_max);
     for (int b0 = 0; b0 < sel; b0 += P->batch_max) {
         const int nb = sel - b0 < P->batch_max ? sel - b0 : P->batch_max;
@@ -421,13 +423,18 @@ static void free_learned(sdp_pipeline *P)
     P->have_backup = 0;
 }
 
-int sdp_pipeline_set_learned(sdp_pipeline *P, sdp_cnn *net, int patch, int margin, double gain, double threshold,
-                             double floor, double input_floor, int max_cycles)
+int sdp_pipeline_set_learned(sdp_pipeline *P, sdp_cnn *net, const sdp_learned_params *lp)
 {
     if (!P) return FAIL(SDP_E_ARG, "null pipeline");
     TRY(sdp_stream_sync(P->s));
     free_learned(P);
     if (!net) return SDP_OK;
+    if (!lp) return FAIL(SDP_E_ARG, "null learned parameters");
+    const int patch = lp->patch, margin = lp->margin;
+    const double gain = lp->gain, threshold = lp->threshold;
+    if (lp->scale != SDP_LEARNED_TILE_SCALE && lp->scale != SDP_LEARNED_GLOBAL_SCALE)
+        return FAIL(SDP_E_ARG, "bad learned scale mode");
+    if (lp->max_cycles < 0) return FAIL(SDP_E_ARG, "negative max_cycles");
     sdp_cnn_params cp;
     TRY(sdp_cnn_describe(net, &cp));
     if (cp.height != patch || cp.width != patch || cp.in_ch != 2 || cp.out_ch != 1)
@@ -469,9 +476,10 @@ int sdp_pipeline_set_learned(sdp_pipeline *P, sdp_cnn *net, int patch, int margi
     P->margin = margin;
     P->gain = gain;
     P->threshold = threshold;
-    P->floor = floor;
-    P->in_floor = input_floor;
-    P->max_learned = max_cycles;
+    P->floor = lp->floor;
+    P->in_floor = lp->input_floor;
+    P->max_learned = lp->max_cycles;
+    P->global_scale = lp->scale == SDP_LEARNED_GLOBAL_SCALE;
     P->learned_done = 0;
     P->learned = 1;
     return SDP_OK;

@@ -42,6 +42,7 @@ HOGBOM_PSF_AS_STORED = 1
 PIPE_GRAPH, PIPE_REFERENCE_PREDICT, PIPE_PSF_MIRRORED, PIPE_PROFILE = 1, 2, 4, 8
 STAGES = ("predict_fft", "degrid_subtract", "grid", "inverse_fft", "clean")
 CNN_FP32, CNN_BF16 = 0, 1
+LEARNED_TILE_SCALE, LEARNED_GLOBAL_SCALE = 0, 1
 IMAGE_RESIDUAL, IMAGE_MODEL, IMAGE_PSF = 0, 1, 2
 
 
@@ -203,7 +204,7 @@ def load() -> ctypes.CDLL:
         "sdp_cnn_scatter_add": (c_int, [c_int, c_int, c_int, c_int, P, P, P, c_float, c_float, P, c_float, P, P]),
         "sdp_complex_multiply": (c_int, [c_size_t, P, P, c_size_t, P, P]),
         "sdp_axpby": (c_int, [c_size_t, c_float, P, c_float, P, P, P]),
-        "sdp_pipeline_set_learned": (c_int, [P, P, c_int, c_int, c_double, c_double, c_double, c_double, c_int]),
+        "sdp_pipeline_set_learned": (c_int, [P, P, P]),
         "sdp_pipeline_learned_active": (c_int, [P]),
     }
     for name, (res, args) in sig.items():
@@ -479,6 +480,11 @@ class _PipelineParams(ctypes.Structure):
                 ("weak_source_percent", c_double), ("flags", c_int), ("tile", c_int), ("chunk", c_int)]
 
 
+class _LearnedParams(ctypes.Structure):
+    _fields_ = [("patch", c_int), ("margin", c_int), ("gain", c_double), ("threshold", c_double),
+                ("floor", c_double), ("input_floor", c_double), ("max_cycles", c_int), ("scale", c_int)]
+
+
 class Pipeline:
     """The device-resident major cycle (sdp_pipeline.h)."""
 
@@ -507,11 +513,13 @@ class Pipeline:
     def reset(self):
         check(load().sdp_pipeline_reset(self.handle))
 
-    def set_learned(self, net, patch, margin, gain=0.5, threshold=0.02, floor=0.01, input_floor=-1.0, max_cycles=0):
+    def set_learned(self, net, patch=0, margin=0, gain=0.5, threshold=0.02, floor=0.01, input_floor=-1.0,
+                    max_cycles=0, scale=LEARNED_TILE_SCALE):
         """Learned minor cycle with network `net` (a Cnn), or Hogbom when None."""
         self._net = net
-        check(load().sdp_pipeline_set_learned(self.handle, net.handle if net is not None else None, patch, margin,
-                                              gain, threshold, floor, input_floor, max_cycles))
+        lp = _LearnedParams(patch, margin, gain, threshold, floor, input_floor, max_cycles, scale)
+        check(load().sdp_pipeline_set_learned(self.handle, net.handle if net is not None else None,
+                                              ctypes.byref(lp)))
 
     @property
     def learned_active(self) -> bool:

@@ -29,27 +29,28 @@ def psf_patch(psf_image: np.ndarray, T: int) -> np.ndarray:
 
 
 def random_sky(T: int, rng: np.random.Generator, n_points=(1, 30), n_gauss=(0, 20), fwhm=(2.0, 30.0),
-               flux=(1e-3, 1.0), index=-1.5, margin=8) -> np.ndarray:
+               flux=(1e-3, 1.0), index=-1.5, margin=8, box: int | None = None) -> np.ndarray:
     """Point sources with dN/dS ~ S^index on [flux] plus elliptical Gaussians
     (FWHM per axis uniform in [fwhm], random orientation, peak 0.2 times a
-    flux drawn the same way), kept `margin` pixels away from the edges of
-    the T x T sky."""
+    flux drawn the same way), centred `margin` pixels away from the edges of
+    the T x T sky, or, with `box`, within `box` pixels of its centre pixel
+    (T/2, T/2) on either axis."""
     sky = np.zeros((T, T), np.float64)
     a = index + 1.0
+    lo, hi = (margin, T - margin) if box is None else (T // 2 - box, T // 2 + box)
 
     def draw(n):
         u = rng.random(n)
         return (flux[0] ** a + u * (flux[1] ** a - flux[0] ** a)) ** (1.0 / a)
 
     n = int(rng.integers(n_points[0], n_points[1] + 1))
-    ys = rng.integers(margin, T - margin, n)
-    xs = rng.integers(margin, T - margin, n)
+    ys = rng.integers(lo, hi, n)
+    xs = rng.integers(lo, hi, n)
     np.add.at(sky, (ys, xs), draw(n))
     ng = int(rng.integers(n_gauss[0], n_gauss[1] + 1))
     k = 1.0 / (8.0 * math.log(2.0))   # sigma^2 = FWHM^2 k
-    for f, cy, cx, fa, fb, th in zip(draw(ng), rng.uniform(margin, T - margin, ng),
-                                     rng.uniform(margin, T - margin, ng), rng.uniform(*fwhm, ng),
-                                     rng.uniform(*fwhm, ng), rng.uniform(0, np.pi, ng)):
+    for f, cy, cx, fa, fb, th in zip(draw(ng), rng.uniform(lo, hi, ng), rng.uniform(lo, hi, ng),
+                                     rng.uniform(*fwhm, ng), rng.uniform(*fwhm, ng), rng.uniform(0, np.pi, ng)):
         r = int(math.ceil(3.0 * max(fa, fb)))   # stamp to 3 FWHM
         y0, y1 = max(0, int(cy) - r), min(T, int(cy) + r + 1)
         x0, x1 = max(0, int(cx) - r), min(T, int(cx) + r + 1)
@@ -154,18 +155,24 @@ def smooth(image: np.ndarray, sigma: float) -> np.ndarray:
 
 
 def make_pool(psf: np.ndarray, n: int, seed: int, T: int | None = None, crops_per_sky: int = 4,
-              target_sigma: float = 0.0, **sky_kw) -> Pool:
+              target_sigma: float = 0.0, global_scale: bool = False, crop_range: tuple[int, int] | None = None,
+              **sky_kw) -> Pool:
     """n training pairs of T x T patches.  psf is the pipeline's PSF image
     (G x G, peak 1 at G/2; pass a T x T PSF patch to draw skies of the patch
     size).  Each random sky covers the PSF's full G x G field; its dirty
     image is the circular convolution with the PSF; crops_per_sky patches
-    are cut from it at random positions, so patches carry the sidelobes of
-    sources outside them, as residual tiles in the major cycle do.  Input
-    (dirty / s, central PSF patch), target sky / s, s = max |dirty patch|;
-    with target_sigma > 0 the target sky is smoothed by a Gaussian of that
-    many pixels (a compact model the network can localise)."""
+    are cut from it at random origins (uniform on [crop_range) per axis,
+    default the whole field), so patches carry the sidelobes of sources
+    outside them, as residual tiles in the major cycle do.  Input
+    (dirty / s, central PSF patch), target sky / s, with s = max |dirty| over
+    the patch, or over the whole field with global_scale (the pipeline's
+    SDP_LEARNED_TILE_SCALE and SDP_LEARNED_GLOBAL_SCALE); with target_sigma
+    > 0 the target sky is smoothed by a Gaussian of that many pixels."""
     G = psf.shape[0]
     T = T or G
+    lo, hi = crop_range if crop_range is not None else (0, G - T + 1)
+    if lo < 0 or hi > G - T + 1 or lo >= hi:
+        raise ValueError("crop_range outside the field")
     conv = Convolver(psf)
     rng = np.random.default_rng(seed)
     pp = psf_patch(psf, T)
@@ -176,12 +183,13 @@ def make_pool(psf: np.ndarray, n: int, seed: int, T: int | None = None, crops_pe
         sky = random_sky(G, rng, **sky_kw)
         dirty = conv(sky)
         target = smooth(sky, target_sigma)
+        s_field = float(np.abs(dirty).max()) or 1.0
         for _ in range(crops_per_sky):
             if i >= n:
                 break
-            oy, ox = (int(v) for v in rng.integers(0, G - T + 1, 2))
+            oy, ox = (int(v) for v in rng.integers(lo, hi, 2))
             d = dirty[oy:oy + T, ox:ox + T]
-            s = float(np.abs(d).max()) or 1.0
+            s = s_field if global_scale else (float(np.abs(d).max()) or 1.0)
             x[i, :, :, 0] = d / s
             x[i, :, :, 1] = pp
             y[i, :, :, 0] = target[oy:oy + T, ox:ox + T] / s
@@ -199,23 +207,31 @@ def train(net: "hip.Cnn", pool: Pool, steps: int, batch: int, lr: float = 1e-3, 
     patch).  Returns the loss per step."""
     n, T = pool.inputs.shape[0], pool.inputs.shape[1]
     rng = np.random.default_rng(seed)
+    # the pool stays in device memory; batches are gathered there
+    d_pool_x = hip.DeviceBuffer.from_array(pool.inputs)
+    d_pool_t = hip.DeviceBuffer.from_array(pool.targets)
+    d_pool_d = hip.DeviceBuffer.from_array(np.ascontiguousarray(pool.inputs[..., 0])) if data_weight > 0 else None
     d_x = hip.DeviceBuffer(4 * batch * T * T * 2)
     d_t = hip.DeviceBuffer(4 * batch * T * T)
     d_y = hip.DeviceBuffer(4 * batch * T * T)
     d_g = hip.DeviceBuffer(4 * batch * T * T)
     d_dirty = hip.DeviceBuffer(4 * batch * T * T)
     data = DataTerm(psf, batch) if data_weight > 0 else None
+    lib = hip.load()
+    px, pt = 4 * T * T * 2, 4 * T * T
     losses = []
     for step in range(steps):
         idx = rng.choice(n, batch, replace=False)
-        d_x.upload(np.ascontiguousarray(pool.inputs[idx]))
-        d_t.upload(np.ascontiguousarray(pool.targets[idx]))
+        for b, k in enumerate(idx):
+            hip.check(lib.sdp_memcpy_dtod(d_x.ptr + b * px, d_pool_x.ptr + int(k) * px, px, None))
+            hip.check(lib.sdp_memcpy_dtod(d_t.ptr + b * pt, d_pool_t.ptr + int(k) * pt, pt, None))
+            if d_pool_d is not None:
+                hip.check(lib.sdp_memcpy_dtod(d_dirty.ptr + b * pt, d_pool_d.ptr + int(k) * pt, pt, None))
         net.forward(d_x, d_y, batch)
         loss = sky_weight * hip.cnn_mse(batch * T * T, d_y, d_t, d_g)
         if sky_weight != 1.0:
             hip.axpby(batch * T * T, sky_weight, d_g, 0.0, d_g, d_g)
         if data is not None:
-            d_dirty.upload(np.ascontiguousarray(pool.inputs[idx, :, :, 0]))
             loss += data_weight * data.add_grad(d_y, d_dirty, d_g, batch, data_weight)
         losses.append(loss)
         net.backward(d_g)
@@ -241,3 +257,79 @@ def evaluate(net: "hip.Cnn", pool: Pool, batch: int) -> float:
         tot += hip.cnn_mse(batch * T * T, d_y, d_t, d_g) * batch
         cnt += batch
     return tot / max(cnt, 1)
+
+
+# ------------------------------------------------------- learned workload
+
+def sky_visibilities(sky: np.ndarray, cell: float, d_uvw: "hip.DeviceBuffer", n_vis: int, noise: float,
+                     seed: int) -> "hip.DeviceBuffer":
+    """Visibilities of a pixel sky (Jy per pixel; pixel (y, x) sits at
+    l = (x - G/2) cell, m = (y - G/2) cell) from the library's FP64-phase DFT
+    over its non-zero pixels, plus Gaussian noise of standard deviation
+    `noise` per component (seeded)."""
+    G = sky.shape[0]
+    yy, xx = np.nonzero(sky)
+    src = np.stack([(xx - G // 2) * cell, (yy - G // 2) * cell, sky[yy, xx]], 1).astype(np.float32)
+    d_src = hip.DeviceBuffer.from_array(src)
+    d_vis = hip.DeviceBuffer(8 * n_vis)
+    hip.dft_predict(n_vis, src.shape[0], d_src, d_uvw, d_vis)
+    if noise > 0:
+        vis = d_vis.download((n_vis, 2), np.float32)
+        vis += np.random.default_rng(seed).normal(0.0, noise, vis.shape).astype(np.float32)
+        d_vis.upload(vis)
+    d_src.close()
+    return d_vis
+
+
+@dataclass
+class Workload:
+    """A sky, its measured visibilities and the imaging set-up, in device memory."""
+    ds: object
+    sky: np.ndarray
+    box: int
+    d_uvw: "hip.DeviceBuffer"
+    d_vis: "hip.DeviceBuffer"
+    d_gk: "hip.DeviceBuffer"
+    d_dk: "hip.DeviceBuffer"
+    d_taper: "hip.DeviceBuffer"
+
+    def pipeline(self, n_minor: int, **kw) -> "hip.Pipeline":
+        d = self.ds
+        return hip.Pipeline(d.grid_size, d.oversampling, d.supports, d.uv_scale, d.w_scale, d.cell_size, self.d_uvw,
+                            self.d_vis, d.n_vis, self.d_gk, self.d_dk, self.d_taper, n_minor, **kw)
+
+
+def learned_workload(n_time: int = 256, grid_size: int = 1024, seed: int = 5, noise: float = 0.01,
+                     n_points: int = 50, n_gauss: int = 20, cell_size: float = 4.0e-4) -> Workload:
+    """The learned-proposal workload: the single-channel array and imaging
+    set-up of sdp_synth.config1 (64 antennas, 2 km disk, 2016 n_time
+    visibilities, PSWF support 7, OS 16, no w-term), observing a random sky
+    (random_sky, seed) of n_points power-law point sources and n_gauss
+    elliptical Gaussians (FWHM 2-30 pixels) centred in the 2 x 2 degree box
+    of config1's sources, with noise sigma 0.01 Jy per component.  The
+    cell stays 4e-4 rad at every grid size, so that the uv grid (1250
+    wavelengths across half of it) holds the longest baselines (1000)."""
+    import sdp_synth
+    d = sdp_synth.config1(n_time=n_time, grid_size=grid_size, with_vis=False, cell_size=cell_size)
+    box = int(round(math.radians(1.0) / d.cell_size))
+    sky = random_sky(grid_size, np.random.default_rng(seed), n_points=(n_points, n_points),
+                     n_gauss=(n_gauss, n_gauss), box=box)
+    up = hip.DeviceBuffer.from_array
+    d_uvw = up(d.uvw)
+    d_vis = sky_visibilities(sky, d.cell_size, d_uvw, d.n_vis, noise, seed + 1000)
+    return Workload(d, sky, box, d_uvw, d_vis, up(d.kernels), up(d.degrid_kernels),
+                    up(sdp_synth.taper_padded(grid_size)))
+
+
+def cycles_to_target(pipe: "hip.Pipeline", target: float, max_cycles: int) -> tuple[float | None, list]:
+    """Runs major cycles until the residual RMS after the minor cycle is at
+    most `target`; returns (summed cycle ms when reached or None, [(rms
+    before, rms after, ms) per cycle])."""
+    rows, total = [], 0.0
+    for _ in range(max_cycles):
+        a, b, ms = pipe.cycle()
+        total += ms
+        rows.append((a, b, ms))
+        if b <= target:
+            return total, rows
+    return None, rows

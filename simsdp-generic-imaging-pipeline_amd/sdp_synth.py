@@ -195,12 +195,15 @@ class Dataset:
         return self.uvw.shape[0]
 
 
-def config1(n_time: int = 256, grid_size: int = 1024, with_vis: bool = True) -> Dataset:
+def config1(n_time: int = 256, grid_size: int = 1024, with_vis: bool = True,
+            cell_size: float | None = None) -> Dataset:
     """Plain FFT imaging, single channel: 64 antennas uniform in a 2 km
     diameter disk (seed 1), latitude -30 deg, dec -30 deg, HA -3..+3 h in
     n_time steps at 150 MHz -> 2016 * n_time visibilities; 50 sources with
     flux U[0.1, 1] Jy in a 2 x 2 deg box (seed 2), noise sigma 0.01 Jy per
-    component (seed 3); PSWF support 7 (h = 3), OS 16, one plane."""
+    component (seed 3); PSWF support 7 (h = 3), OS 16, one plane.  The cell
+    defaults to 4e-4 rad at 1024^2 (field of view 23.5 deg whatever the
+    grid size); pass cell_size to keep the uv extent instead."""
     enu = disk_layout(64, 1000.0, seed=1)
     ha = np.linspace(-3.0, 3.0, n_time)
     uvw = uvw_tracks(enu, -30.0, -30.0, ha, 150e6)
@@ -208,7 +211,7 @@ def config1(n_time: int = 256, grid_size: int = 1024, with_vis: bool = True) -> 
     half_box = np.radians(1.0)
     src = np.stack([rng.uniform(-half_box, half_box, 50), rng.uniform(-half_box, half_box, 50),
                     rng.uniform(0.1, 1.0, 50)], axis=1).astype(np.float32)
-    cell = 4.0e-4 * 1024 / grid_size
+    cell = cell_size if cell_size is not None else 4.0e-4 * 1024 / grid_size
     gk, dk, sup = kernel_tables([3], 16, fov=cell * grid_size, w_scale=0.0)
     uvw32 = uvw.astype(np.float32)
     if with_vis:
