@@ -20,6 +20,10 @@
 // into registers, updated there by the subtraction and scanned from there:
 // per iteration one read of the residual, one read of the PSF window and one
 // write of the window (the window is nearly the whole image, :259-264).
+// The PSF is read from a copy in the work buffer with 4 zero floats on each
+// side, through a buffer resource: a group's 4 PSF values are one 16-byte
+// load at any dword offset, issued for every group of a chunk before any is
+// used (no per-group branches); lanes outside the window are masked.
 //
 // The subtraction is evaluated in double exactly as the reference writes it,
 // ((gain * max_val) * psf) * psf_max, then subtracted from the float residual
@@ -52,20 +56,34 @@ struct State {
 
 struct HogbomArgs {
     int G, n_minor, max_sources, nblocks, mirror, vec;
+    int psf_pad_bytes;
     double psf_max, gain, noise, weak;
 };
+
+typedef float v4f_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ bool better(float av, int ai, float bv, int bi)
 {
     return av > bv || (av == bv && ai < bi);
 }
 
-__global__ void hogbom_init_kernel(State *st, const float *model_in, float *model_out, size_t n)
+constexpr int kPsfPad = 4;   // zero floats before and after the PSF copy
+
+// state slots, the model copy (n_model floats, 0 when in place) and the
+// padded PSF copy: kPsfPad zeros, the npix + 1 PSF floats, kPsfPad - 1 zeros
+__global__ void hogbom_init_kernel(State *st, const float *model_in, float *model_out, size_t n_model,
+                                   const float *psf, float *psf_pad, size_t npix)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
     if (i < 2) st[i] = State{0, 0, 0.f, 0};
-    for (size_t k = i; k < n; k += (size_t)gridDim.x * blockDim.x) model_out[k] = model_in[k];
+    for (size_t k = i; k < n_model; k += stride) model_out[k] = model_in[k];
+    const size_t total = npix + 2 * kPsfPad;
+    for (size_t k = i; k < total; k += stride)
+        psf_pad[k] = (k >= (size_t)kPsfPad && k <= npix + kPsfPad) ? psf[k - kPsfPad] : 0.f;
 }
+
+constexpr unsigned kOob = 0x80000000u;   // a buffer offset past any PSF copy: reads as zero
 
 // residual - ((gain * max_val) * psf) * psf_max, in double, rounded to float
 // (deconvolution_run.c:266-273)
@@ -132,8 +150,8 @@ __device__ void reduce_partials(const Partial *__restrict__ parts, int nparts, f
 template <int kVecPerThread>
 __global__ void __launch_bounds__(kThreads)
 hogbom_step_kernel(int iter, HogbomArgs a, float *__restrict__ r, const float *__restrict__ psf,
-                   float *__restrict__ model, float *__restrict__ sources, int *__restrict__ num_sources,
-                   State *__restrict__ st, Partial *__restrict__ parts)
+                   const float *__restrict__ psf_pad, float *__restrict__ model, float *__restrict__ sources,
+                   int *__restrict__ num_sources, State *__restrict__ st, Partial *__restrict__ parts)
 {
     __shared__ float red_v[kThreads], red_val[kThreads];
     __shared__ int red_i[kThreads];
@@ -149,7 +167,7 @@ hogbom_step_kernel(int iter, HogbomArgs a, float *__restrict__ r, const float *_
         return;
     }
     const int G = a.G;
-    const int npix = G * G;                  // G <= 32768
+    const int npix = G * G;                  // G <= 23168
     Partial *prev = parts + (size_t)((iter + 1) & 1) * a.nblocks;   // written by launch it - 1
     Partial *next = parts + (size_t)(iter & 1) * a.nblocks;         // read by launch it + 1
 
@@ -226,63 +244,44 @@ hogbom_step_kernel(int iter, HogbomArgs a, float *__restrict__ r, const float *_
 
         if (subtract) {
             if (a.vec) {
-                // G % 4 == 0: a group never crosses a row.  Groups wholly
-                // inside the window read their 4 PSF values with aligned
-                // 16-byte loads, all issued before any is used; groups on
-                // the window's left or right edge go lane by lane.
-                float4 pa[kVecPerThread], pb[kVecPerThread];
-                unsigned full = 0, part = 0;
+                // G % 4 == 0: a group never crosses a row.  Its 4 PSF values
+                // psf[lo .. lo+3] are one 16-byte load from the padded copy
+                // (lo >= -4 and lo + 3 <= npix + 3 whenever a lane of the
+                // group is in the window); groups with no lane in the
+                // window load from past the copy and read zeros.
+                const __amdgpu_buffer_rsrc_t rs =
+                    __builtin_amdgcn_make_buffer_rsrc((void *)psf_pad, (short)0, a.psf_pad_bytes, 0x00020000);
+                v4f_t w[kVecPerThread];
+                unsigned lanes[kVecPerThread];
 #pragma unroll
                 for (int k = 0; k < kVecPerThread; ++k) {
                     const int q = q0 + k * kThreads + threadIdx.x;
-                    pa[k] = pb[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (q >= n4) continue;
                     const int y = q / gpr, x0 = (q - y * gpr) * 4;
-                    if (y < ylo || y >= yhi || x0 + 3 < xlo || x0 >= xhi) continue;
+                    const bool row = q < n4 && y >= ylo && y < yhi;
+                    unsigned m = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) m |= (row && x0 + j >= xlo && x0 + j < xhi) ? 1u << j : 0u;
+                    lanes[k] = m;
                     const int prow = (y - my + half) * G;
                     const int lo = a.mirror ? prow + (G - (x0 + 3 - mx + half)) : prow + (x0 - mx + half);
-                    const int d = lo & 3;
-                    if (x0 >= xlo && x0 + 3 < xhi && (lo - d) + (d ? 8 : 4) <= npix + 1) {
-                        const float4 *g = reinterpret_cast<const float4 *>(psf + (lo - d));
-                        pa[k] = g[0];
-                        if (d) pb[k] = g[1];
-                        full |= 1u << k;
-                    } else {
-                        part |= 1u << k;
-                    }
+                    const unsigned off = m ? (unsigned)(lo + kPsfPad) * 4u : kOob;
+                    w[k] = __builtin_bit_cast(v4f_t, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
                 }
 #pragma unroll
                 for (int k = 0; k < kVecPerThread; ++k) {
+                    const unsigned m = lanes[k];
+                    if (!m) continue;
                     const int q = q0 + k * kThreads + threadIdx.x;
-                    if (full & (1u << k)) {
-                        const int y = q / gpr, x0 = (q - y * gpr) * 4;
-                        const int prow = (y - my + half) * G;
-                        const int lo = a.mirror ? prow + (G - (x0 + 3 - mx + half)) : prow + (x0 - mx + half);
-                        const int d = lo & 3;
-                        const float4 A = pa[k], B = pb[k];
-                        const float4 w = d == 0 ? A
-                                       : d == 1 ? make_float4(A.y, A.z, A.w, B.x)
-                                       : d == 2 ? make_float4(A.z, A.w, B.x, B.y)
-                                                : make_float4(A.w, B.x, B.y, B.z);    // psf[lo .. lo+3]
-                        float4 e = v[k];
-                        if (a.mirror) {   // index decreases with x: lane j reads psf[lo + 3 - j]
-                            e.x = subtract_one(e.x, scale, w.w, pm);
-                            e.y = subtract_one(e.y, scale, w.z, pm);
-                            e.z = subtract_one(e.z, scale, w.y, pm);
-                            e.w = subtract_one(e.w, scale, w.x, pm);
-                        } else {
-                            e.x = subtract_one(e.x, scale, w.x, pm);
-                            e.y = subtract_one(e.y, scale, w.y, pm);
-                            e.z = subtract_one(e.z, scale, w.z, pm);
-                            e.w = subtract_one(e.w, scale, w.w, pm);
-                        }
-                        v[k] = e;
-                        r4[q] = e;
-                    } else if (part & (1u << k)) {
-                        v[k] = subtract_lanes(v[k], q * 4, G, mx, my, half, xlo, xhi, ylo, yhi, a.mirror, psf,
-                                              scale, pm);
-                        r4[q] = v[k];
-                    }
+                    // lane j reads psf[lo + j], or psf[lo + 3 - j] mirrored
+                    const float p0 = a.mirror ? w[k].w : w[k].x, p1 = a.mirror ? w[k].z : w[k].y;
+                    const float p2 = a.mirror ? w[k].y : w[k].z, p3 = a.mirror ? w[k].x : w[k].w;
+                    float4 e = v[k];
+                    if (m & 1u) e.x = subtract_one(e.x, scale, p0, pm);
+                    if (m & 2u) e.y = subtract_one(e.y, scale, p1, pm);
+                    if (m & 4u) e.z = subtract_one(e.z, scale, p2, pm);
+                    if (m & 8u) e.w = subtract_one(e.w, scale, p3, pm);
+                    v[k] = e;
+                    r4[q] = e;
                 }
             } else {
                 // G % 4 == 2: groups may cross a row end; every lane on its own
@@ -363,6 +362,14 @@ __global__ void hogbom_finish_kernel(const State *st, int G, double cell_size, f
 
 size_t partial_offset() { return 256; }
 
+size_t partial_count(size_t npix) { return (npix + kMinPixelsPerChunk - 1) / kMinPixelsPerChunk; }
+
+// the padded PSF copy follows the two launches' partials, 256-byte aligned
+size_t psf_pad_offset(size_t npix)
+{
+    return (partial_offset() + 2 * partial_count(npix) * sizeof(Partial) + 255) & ~(size_t)255;
+}
+
 // groups per thread (4 or 8); SDP_HOGBOM_GROUPS overrides the default for tuning
 int hogbom_groups()
 {
@@ -396,8 +403,8 @@ size_t sdp_hogbom_work_bytes(int G)
 {
     if (G < 2) return 0;
     const size_t npix = (size_t)G * G;
-    const size_t nb = (npix + kMinPixelsPerChunk - 1) / kMinPixelsPerChunk;
-    return partial_offset() + 2 * nb * sizeof(Partial);   // partials of two consecutive launches
+    // partials of two consecutive launches, then the padded PSF copy
+    return psf_pad_offset(npix) + (npix + 2 * kPsfPad) * sizeof(float);
 }
 
 int sdp_hogbom(int G, int n_minor, int max_sources, float *d_residual, const float *d_psf, double psf_max,
@@ -415,7 +422,7 @@ int sdp_hogbom_ex(int G, int n_minor, int max_sources, float *d_residual, const 
                   const float *d_model_in, float *d_model_out, float *d_sources, int *d_num_sources, void *d_work,
                   int flags, sdp_stream s)
 {
-    SDP_REQUIRE(G >= 2 && G % 2 == 0 && G <= 32768, "bad grid size %d", G);
+    SDP_REQUIRE(G >= 2 && G % 2 == 0 && G <= 23168, "bad grid size %d", G);   // padded PSF bytes fit an int
     SDP_REQUIRE(n_minor >= 0 && max_sources >= 0, "bad iteration / source counts");
     SDP_REQUIRE(d_residual && d_psf && d_model_in && d_model_out && d_sources && d_num_sources && d_work,
                 "sdp_hogbom: null pointer");
@@ -446,13 +453,15 @@ int sdp_hogbom_ex(int G, int n_minor, int max_sources, float *d_residual, const 
     a.weak = weak_source_percent;
     State *state = (State *)d_work;
     Partial *parts = (Partial *)((char *)d_work + partial_offset());
+    float *psf_pad = (float *)((char *)d_work + psf_pad_offset(npix));
+    a.psf_pad_bytes = (int)((npix + 2 * kPsfPad) * sizeof(float));
 
     hogbom_init_kernel<<<sdp_div_up(npix, 256) > 4096 ? 4096 : sdp_div_up(npix, 256), 256, 0, st>>>(
-        state, d_model_in, d_model_out, d_model_in == d_model_out ? 0 : npix);
+        state, d_model_in, d_model_out, d_model_in == d_model_out ? 0 : npix, d_psf, psf_pad, npix);
     SDP_LAUNCH_CHECK();
     for (int it = 0; it <= n_minor; ++it) {
-        kern<<<a.nblocks, kThreads, 0, st>>>(it, a, d_residual, d_psf, d_model_out, d_sources, d_num_sources,
-                                             state, parts);
+        kern<<<a.nblocks, kThreads, 0, st>>>(it, a, d_residual, d_psf, psf_pad, d_model_out, d_sources,
+                                             d_num_sources, state, parts);
         SDP_LAUNCH_CHECK();
     }
     hogbom_finish_kernel<<<4, 256, 0, st>>>(state + ((n_minor + 1) & 1), G, cell_size, d_sources, d_num_sources);
