@@ -187,6 +187,63 @@ def learned_vs_hogbom(hip, steps, seed=5):
     }
 
 
+def dft_precision_variant(hip, d, n_src=4096):
+    """DFT predict of n_src power-law sources onto the array's 16.7M
+    visibilities: the array-factorised matrix-core form (BF16 and FP16
+    operands with the hi + lo split, and without it) against the
+    per-visibility FP64-phase kernel; errors against the FP64 restatement
+    over the first instant's 32,640 baselines, relative to sum |g|."""
+    import numpy as np
+    import sdp_synth
+
+    enu, ha = d.extra["enu"], d.extra["ha_hours"]
+    ant = sdp_synth.antenna_uvw(enu, -30.0, -30.0, ha, 150e6)
+    src = sdp_synth.config3_sources(n_src)
+    n_time, n_ant = ant.shape[0], ant.shape[1]
+    n_pairs = n_ant * (n_ant - 1) // 2
+    N = n_time * n_pairs
+    pairs = sdp_synth.receiver_pairs(n_ant)
+    ref = sdp_synth.dft_f64((ant[:1, pairs[:, 1]] - ant[:1, pairs[:, 0]]).reshape(-1, 3), src)
+    l, m, f = (src[:, k].astype(np.float64) for k in range(3))
+    gsum = float(np.abs(f / (1 - 0.5 * (l * l + m * m))).sum())
+    d_ant, d_src = hip.DeviceBuffer.from_array(ant), hip.DeviceBuffer.from_array(src)
+    d_out = hip.DeviceBuffer(8 * N)
+    work = hip.DeviceBuffer(hip.dft_array_work_bytes(n_src))
+    ev = [hip.Event(), hip.Event()]
+    nt = (n_ant + 127) // 128
+    out = {"workload": f"{n_src} sources x {N} vis ({n_ant} antennas x {n_time} instants)",
+           "error_ref": "FP64 restatement, first instant, max |V - V64| / sum|g|"}
+    for name, flags in (("bf16_split", hip.DFT_ARRAY_BF16), ("fp16_split", hip.DFT_ARRAY_FP16),
+                        ("bf16", hip.DFT_ARRAY_BF16 | hip.DFT_ARRAY_NO_SPLIT),
+                        ("fp16", hip.DFT_ARRAY_FP16 | hip.DFT_ARRAY_NO_SPLIT)):
+        hip.dft_predict_array(n_time, n_ant, d_ant, n_src, d_src, d_out, flags, work)
+        ev[0].record()
+        for _ in range(3):
+            hip.dft_predict_array(n_time, n_ant, d_ant, n_src, d_src, d_out, flags, work)
+        ev[1].record()
+        ev[1].sync()
+        ms = ev[0].elapsed_ms(ev[1]) / 3
+        got = d_out.download((n_pairs, 2), np.float32)
+        products = 1 if flags & hip.DFT_ARRAY_NO_SPLIT else 3
+        mfma_flops = n_time * nt * (nt + 1) // 2 * 128 * 128 * ((n_src + 15) // 16 * 32) * 2 * 2 * products
+        out[name] = {"ms": round(ms, 3), "gpairs_per_s": round(N * n_src / ms / 1e6, 1),
+                     "max_err": float(f"{np.abs(got - ref).max() / gsum:.3g}"),
+                     "mfma_tflops": round(mfma_flops / ms / 1e9, 1),
+                     "mfma_frac": round(mfma_flops / ms / 1e9 / 2500.0, 4)}
+        log(f"array DFT {name}: {ms:.3f} ms, error {out[name]['max_err']}")
+    uvw = (ant[:, pairs[:, 1]] - ant[:, pairs[:, 0]]).reshape(-1, 3).astype(np.float32)
+    d_uvw = hip.DeviceBuffer.from_array(uvw)
+    ev[0].record()
+    hip.dft_predict(N, n_src, d_src, d_uvw, d_out, hip.DFT_FP64_PHASE)
+    ev[1].record()
+    ev[1].sync()
+    ms = ev[0].elapsed_ms(ev[1])
+    got = d_out.download((n_pairs, 2), np.float32)
+    out["per_vis_fp64_phase"] = {"ms": round(ms, 3), "gpairs_per_s": round(N * n_src / ms / 1e6, 1),
+                                 "max_err": float(f"{np.abs(got - ref).max() / gsum:.3g}")}
+    return out
+
+
 def net_params(hip, levels, base):
     return sum(9 * ci * co + co for _, ci, co in hip.cnn_layer_shapes(levels, base, 2, 1))
 
@@ -355,6 +412,9 @@ def main():
         extras["gtaps_per_s"] = round(2 * taps / ((k_ms["degrid_subtract"] + k_ms["grid_accumulate"]) * 1e-3) / 1e9, 1)
         extras["plan"] = info
         extras["major_cycle"] = major_cycle(hip, sdp_synth, d, d_uvw, d_meas, d_gk, d_dk, tile, chunk)
+        if "enu" in d.extra:
+            log("DFT precision variant ...")
+            extras["dft_precision_variant"] = dft_precision_variant(hip, d)
         if not args.no_learned:
             log("learned vs Hogbom ...")
             extras["learned_vs_hogbom"] = learned_vs_hogbom(hip, args.learned_steps)

@@ -78,10 +78,10 @@ enum { SDP_DEGRID_NEAREST = 0, SDP_DEGRID_WEIGHTED = 1 };
 void sdp_set_degrid_mode(int mode);
 int sdp_get_degrid_mode(void);
 
-enum { SDP_DFT_REFERENCE_FP32 = 0, SDP_DFT_FP64_PHASE = 1, SDP_DFT_MFMA = 2 };
-/* dft_actor arithmetic: the reference's FP32 phase (dft_run.c:21-32), the
- * same formula with the phase formed and range-reduced in FP64, or the
- * blocked MFMA contraction (see sdp_dft_predict). */
+enum { SDP_DFT_REFERENCE_FP32 = 0, SDP_DFT_FP64_PHASE = 1 };
+/* dft_actor arithmetic: the reference's FP32 phase (dft_run.c:21-32), or the
+ * same formula with the phase formed and range-reduced in FP64.  (The
+ * matrix-core form needs per-antenna tracks: sdp_dft_predict_array.) */
 void sdp_set_dft_mode(int mode);
 int sdp_get_dft_mode(void);
 
@@ -253,6 +253,25 @@ int sdp_hogbom_ex(int grid_size, int n_minor, int max_sources, float *d_residual
  * sources (l, m, flux) in radians.  mode: SDP_DFT_*. */
 int sdp_dft_predict(int n_vis, int n_sources, const float *d_sources, const float *d_uvw,
                     float *d_vis_out, int mode, sdp_stream s);
+
+/* The same predict for a whole array, factorised per antenna and contracted
+ * on the matrix cores.  For n_ant antennas at n_time instants and every
+ * baseline (a, b), a < b, in calculate_receiver_pairs order
+ * (top.c:119-137), visibility index t * n_ant (n_ant - 1) / 2 + k:
+ *   V_ab(t) = sum_s g_s E_bs(t) conj(E_as(t)),  g_s = f_s / (1 - r_s^2/2),
+ *   E_xs(t) = exp(-i 2 pi (u_x l_s + v_x m_s - w_x r_s^2/2)),
+ * which is dft_actor's sum (dft_run.c:21-41) for uvw_ab = uvw_b - uvw_a.
+ * d_ant_uvw: n_time * n_ant * 3 doubles, antenna uvw in wavelengths about
+ * any common origin.  Each phase is formed and range-reduced in FP64 per
+ * antenna and source; the contraction over sources is a complex matrix
+ * product per instant on the matrix cores (BF16 or FP16 operands, FP32
+ * accumulation), each operand split into hi + lo parts with the three
+ * products hi*hi + hi*lo + lo*hi, or hi only with SDP_DFT_ARRAY_NO_SPLIT.
+ * d_work: sdp_dft_array_work_bytes(n_sources) bytes. */
+enum { SDP_DFT_ARRAY_BF16 = 0, SDP_DFT_ARRAY_FP16 = 1, SDP_DFT_ARRAY_NO_SPLIT = 4 };
+size_t sdp_dft_array_work_bytes(int n_sources);
+int sdp_dft_predict_array(int n_time, int n_ant, const double *d_ant_uvw, int n_sources, const float *d_sources,
+                          float *d_vis_out, int flags, void *d_work, sdp_stream s);
 
 /* ---- learned minor-cycle proposal ---------------------------------------
  * No reference counterpart (the reference has no learned component).  An

@@ -233,7 +233,7 @@ void sdp_set_degrid_mode(int mode) { g_degrid_mode = mode == SDP_DEGRID_WEIGHTED
 int sdp_get_degrid_mode(void) { return g_degrid_mode; }
 void sdp_set_dft_mode(int mode)
 {
-    if (mode == SDP_DFT_REFERENCE_FP32 || mode == SDP_DFT_FP64_PHASE || mode == SDP_DFT_MFMA) g_dft_mode = mode;
+    if (mode == SDP_DFT_REFERENCE_FP32 || mode == SDP_DFT_FP64_PHASE) g_dft_mode = mode;
 }
 int sdp_get_dft_mode(void) { return g_dft_mode; }
 

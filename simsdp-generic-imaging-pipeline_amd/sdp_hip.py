@@ -31,7 +31,8 @@ c_void_p, c_char_p = ctypes.c_void_p, ctypes.c_char_p
 
 # device-layer enums (sdp_device.h)
 DEGRID_NEAREST, DEGRID_WEIGHTED = 0, 1
-DFT_REFERENCE_FP32, DFT_FP64_PHASE, DFT_MFMA = 0, 1, 2
+DFT_REFERENCE_FP32, DFT_FP64_PHASE = 0, 1
+DFT_ARRAY_BF16, DFT_ARRAY_FP16, DFT_ARRAY_NO_SPLIT = 0, 1, 4
 SNAP_STD, SNAP_GRIDDING_ACTOR = 0, 1
 ORDER_ORIGINAL, ORDER_PLAN = 0, 1
 ACC_BLOCKED, ACC_COMPENSATED = 0, 1
@@ -199,6 +200,8 @@ def load() -> ctypes.CDLL:
         "sdp_cnn_mse": (c_int, [c_size_t, P, P, P, ctypes.POINTER(c_double), P]),
         "sdp_cnn_adam": (c_int, [P, c_float, c_float, c_float, c_float, P]),
         "sdp_cnn_describe": (c_int, [P, P]),
+        "sdp_dft_array_work_bytes": (c_size_t, [c_int]),
+        "sdp_dft_predict_array": (c_int, [c_int, c_int, P, c_int, P, P, c_int, P, P]),
         "sdp_cnn_patch_peaks": (c_int, [c_int, c_int, c_int, P, P, ctypes.POINTER(c_int), P]),
         "sdp_cnn_gather_patches": (c_int, [c_int, c_int, c_int, P, P, P, P, P, P]),
         "sdp_cnn_scatter_add": (c_int, [c_int, c_int, c_int, c_int, P, P, P, c_float, c_float, P, c_float, P, P]),
@@ -669,6 +672,20 @@ def cnn_mse(count, d_out, d_target, d_grad, stream=None) -> float:
 
 def dft_predict(n_vis, n_sources, d_sources, d_uvw, d_out, mode=DFT_FP64_PHASE, stream=None):
     check(load().sdp_dft_predict(n_vis, n_sources, _ptr(d_sources), _ptr(d_uvw), _ptr(d_out), mode, stream))
+
+
+def dft_array_work_bytes(n_sources) -> int:
+    return load().sdp_dft_array_work_bytes(n_sources)
+
+
+def dft_predict_array(n_time, n_ant, d_ant_uvw, n_sources, d_sources, d_out, flags=DFT_ARRAY_BF16, d_work=None,
+                      stream=None):
+    """Array-factorised predict on the matrix cores (sdp_dft_predict_array);
+    d_ant_uvw holds n_time * n_ant * 3 doubles."""
+    work = d_work if d_work is not None else DeviceBuffer(dft_array_work_bytes(n_sources))
+    check(load().sdp_dft_predict_array(n_time, n_ant, _ptr(d_ant_uvw), n_sources, _ptr(d_sources), _ptr(d_out),
+                                       flags, _ptr(work), stream))
+    return work
 
 
 def degrid_nearest(n_vis, n_out, grid_size, uv_scale, d_grid, d_uvw, d_out, stream=None):

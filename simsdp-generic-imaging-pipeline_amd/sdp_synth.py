@@ -99,6 +99,21 @@ def uvw_tracks(enu: np.ndarray, lat_deg: float, dec_deg: float, ha_hours: np.nda
     return np.stack([u, v, w], axis=-1).reshape(-1, 3) / lam
 
 
+def antenna_uvw(enu: np.ndarray, lat_deg: float, dec_deg: float, ha_hours: np.ndarray,
+                freq_hz: float) -> np.ndarray:
+    """Per-antenna uvw (wavelengths, float64), shape (n_time, n_ant, 3), about
+    the array origin: baseline (a, b) of uvw_tracks is [:, b] - [:, a]."""
+    lat, dec = np.radians(lat_deg), np.radians(dec_deg)
+    X = -np.sin(lat) * enu[:, 1] + np.cos(lat) * enu[:, 2]
+    Y = enu[:, 0]
+    Z = np.cos(lat) * enu[:, 1] + np.sin(lat) * enu[:, 2]
+    H = np.radians(15.0 * np.asarray(ha_hours))[:, None]
+    u = np.sin(H) * X + np.cos(H) * Y
+    v = -np.sin(dec) * np.cos(H) * X + np.sin(dec) * np.sin(H) * Y + np.cos(dec) * Z
+    w = np.cos(dec) * np.cos(H) * X - np.cos(dec) * np.sin(H) * Y + np.sin(dec) * Z
+    return np.stack([u, v, w], axis=-1) / (C_LIGHT / freq_hz)
+
+
 def dft_f64(uvw: np.ndarray, sources: np.ndarray, block: int = 65536) -> np.ndarray:
     """Exact FP64 predict with the reference's formula (dft_run.c:21-32)."""
     out = np.zeros((uvw.shape[0], 2), np.float64)
@@ -253,8 +268,23 @@ def config2(n_ant: int = 256, n_time: int = 512, grid_size: int = 4096, n_planes
     lm = rng.uniform(-fov / 4, fov / 4, (n_src, 2))
     src = np.concatenate([lm, flux[:, None]], axis=1).astype(np.float32)
     vis = np.zeros((uvw.shape[0], 2), np.float32)
-    return Dataset("config2", grid_size, cell, 8, uvw, vis, gk, dk, sup, src, max_w=max_w,
-                   n_minor=1000, n_major=5)
+    ds = Dataset("config2", grid_size, cell, 8, uvw, vis, gk, dk, sup, src, max_w=max_w,
+                 n_minor=1000, n_major=5)
+    ds.extra["enu"] = enu
+    ds.extra["ha_hours"] = ha
+    return ds
+
+
+def config3_sources(n_src: int = 4096, grid_size: int = 4096, seed: int = 13) -> np.ndarray:
+    """Precision-variant sky: n_src sources with dN/dS ~ S^-1.5 on
+    [1 mJy, 1 Jy] (seed) in the central half of config2's field."""
+    rng = np.random.default_rng(seed)
+    fov = 1.5e-5 * grid_size
+    a = -1.5
+    u01 = rng.random(n_src)
+    flux = (1e-3 ** (a + 1) + u01 * (1.0 - 1e-3 ** (a + 1))) ** (1.0 / (a + 1))
+    lm = rng.uniform(-fov / 4, fov / 4, (n_src, 2))
+    return np.concatenate([lm, flux[:, None]], axis=1).astype(np.float32)
 
 
 def small_case(n_vis: int = 4000, grid_size: int = 128, n_planes: int = 4, oversampling: int = 16,
