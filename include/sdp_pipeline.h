@@ -134,6 +134,42 @@ int sdp_pipeline_sources(const sdp_pipeline *p, float *h_sources, int *n);
 /* Gridding plan used by the pipeline (for its info). */
 const sdp_grid_plan *sdp_pipeline_plan(const sdp_pipeline *p);
 
+/* ---- batched multi-channel residual imaging -----------------------------
+ * No reference counterpart: the reference images one channel per MPI rank
+ * (PN/generated/main.c:44-51), each running the cycle above.  Here
+ * n_channels channels share one GPU: each has its own visibilities (any
+ * count, e.g. after per-channel flagging), gridding plan, model and
+ * residual images; all share the kernel tables, one complex grid and the
+ * FFT work buffer, and are processed one after another on one stream.
+ * sdp_batch_residuals runs, for every channel, the first four stages of the
+ * major cycle (model -> predict FFT -> degrid and subtract -> grid ->
+ * inverse FFT and correction -> residual image); with SDP_BATCH_GRAPH the
+ * whole multi-channel pass is captured once (after one eager pass) and
+ * replayed as one graph per cycle.  tile and chunk (0: defaults) apply to
+ * every channel's plan (sdp_grid_autotune picks them). */
+enum { SDP_BATCH_GRAPH = 1, SDP_BATCH_PROFILE = 2 };
+typedef struct sdp_batch_params {
+    int n_channels, grid_size, oversampling, n_kernels;
+    double uv_scale, w_scale;
+    int tile, chunk, accumulate, flags;
+} sdp_batch_params;
+typedef struct sdp_batch sdp_batch;
+/* supports: host (half_support, offset) pairs; n_vis: host counts per
+ * channel; d_uvw, d_vis: host arrays of n_channels device pointers (uvw in
+ * wavelengths for that channel's frequency, float3; measured float2). */
+int sdp_batch_create(sdp_batch **batch, const sdp_batch_params *params, const int *supports, const int *n_vis,
+                     const float *const *d_uvw, const float *const *d_vis, const float *d_grid_kernels,
+                     const float *d_degrid_kernels, const float *d_taper, sdp_stream s);
+int sdp_batch_destroy(sdp_batch *batch);
+/* one multi-channel pass; *ms: its device time.  With SDP_BATCH_PROFILE
+ * (eager, events between stages) stage_ms[SDP_STAGE_COUNT] receives the
+ * per-stage sums over channels (SDP_STAGE_CLEAN stays 0). */
+int sdp_batch_residuals(sdp_batch *batch, float *ms, float *stage_ms);
+/* device G x G float images of one channel (the model may be written) */
+float *sdp_batch_model(sdp_batch *batch, int channel);
+const float *sdp_batch_residual(const sdp_batch *batch, int channel);
+const sdp_grid_plan *sdp_batch_plan(const sdp_batch *batch, int channel);
+
 #ifdef __cplusplus
 }
 #endif

@@ -192,6 +192,12 @@ def load() -> ctypes.CDLL:
         "sdp_pipeline_stage_ms": (c_int, [P, P]),
         "sdp_pipeline_sources": (c_int, [P, P, ctypes.POINTER(c_int)]),
         "sdp_pipeline_plan": (c_void_p, [P]),
+        "sdp_batch_create": (c_int, [ctypes.POINTER(c_void_p), P, P, P, P, P, P, P, P, P]),
+        "sdp_batch_destroy": (c_int, [P]),
+        "sdp_batch_residuals": (c_int, [P, P, P]),
+        "sdp_batch_model": (c_void_p, [P, c_int]),
+        "sdp_batch_residual": (c_void_p, [P, c_int]),
+        "sdp_batch_plan": (c_void_p, [P, c_int]),
         "sdp_cnn_create": (c_int, [ctypes.POINTER(c_void_p), P]), "sdp_cnn_destroy": (c_int, [P]),
         "sdp_cnn_param_count": (c_size_t, [P]),
         "sdp_cnn_set_params": (c_int, [P, P]), "sdp_cnn_get_params": (c_int, [P, P]),
@@ -481,6 +487,70 @@ class _PipelineParams(ctypes.Structure):
                 ("uv_scale", c_double), ("w_scale", c_double), ("cell_size", c_double),
                 ("n_minor", c_int), ("max_sources", c_int), ("loop_gain", c_double), ("noise_factor", c_double),
                 ("weak_source_percent", c_double), ("flags", c_int), ("tile", c_int), ("chunk", c_int)]
+
+
+BATCH_GRAPH, BATCH_PROFILE = 1, 2
+
+
+class _BatchParams(ctypes.Structure):
+    _fields_ = [("n_channels", c_int), ("grid_size", c_int), ("oversampling", c_int), ("n_kernels", c_int),
+                ("uv_scale", c_double), ("w_scale", c_double), ("tile", c_int), ("chunk", c_int),
+                ("accumulate", c_int), ("flags", c_int)]
+
+
+class Batch:
+    """Batched multi-channel residual imaging (sdp_batch_* in sdp_pipeline.h)."""
+
+    def __init__(self, grid_size, oversampling, supports, uv_scale, w_scale, d_uvws, d_viss, n_vis, d_grid_kernels,
+                 d_degrid_kernels, d_taper, tile=0, chunk=0, accumulate=0, flags=0, stream=None):
+        self._sup = np.ascontiguousarray(supports, dtype=np.int32)
+        self.n_channels = len(n_vis)
+        self.grid_size = grid_size
+        self.params = _BatchParams(self.n_channels, grid_size, oversampling, self._sup.shape[0], uv_scale, w_scale,
+                                   tile, chunk, accumulate, flags)
+        self._n = np.ascontiguousarray(n_vis, dtype=np.int32)
+        self._uvw = (c_void_p * self.n_channels)(*[_ptr(b) for b in d_uvws])
+        self._vis = (c_void_p * self.n_channels)(*[_ptr(b) for b in d_viss])
+        self._keep = (d_grid_kernels, d_degrid_kernels, d_taper)
+        h = c_void_p()
+        check(load().sdp_batch_create(ctypes.byref(h), ctypes.byref(self.params), _ptr(self._sup), _ptr(self._n),
+                                      self._uvw, self._vis, _ptr(d_grid_kernels), _ptr(d_degrid_kernels),
+                                      _ptr(d_taper), stream))
+        self.handle = h.value
+
+    def residuals(self) -> tuple[float, list]:
+        """One multi-channel pass: (ms, per-stage ms summed over channels)."""
+        ms = c_float()
+        st = (c_float * len(STAGES))()
+        check(load().sdp_batch_residuals(self.handle, ctypes.byref(ms), st))
+        return ms.value, list(st)
+
+    def model_ptr(self, k) -> int:
+        return load().sdp_batch_model(self.handle, k)
+
+    def residual(self, k) -> np.ndarray:
+        G = self.grid_size
+        out = np.empty((G, G), np.float32)
+        check(load().sdp_memcpy_dtoh(out.ctypes.data, load().sdp_batch_residual(self.handle, k), out.nbytes, None))
+        return out
+
+    def plan_info(self, k) -> dict:
+        info = np.zeros(10, np.int64)
+        check(load().sdp_grid_plan_info(load().sdp_batch_plan(self.handle, k), info.ctypes.data))
+        keys = ["kept", "dropped", "tiles", "chunks", "tile", "chunk_taps", "max_half_support", "scratch_bytes",
+                "class_launches", "taps"]
+        return dict(zip(keys, (int(v) for v in info)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().sdp_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class _LearnedParams(ctypes.Structure):

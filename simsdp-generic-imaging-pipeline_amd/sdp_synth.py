@@ -287,6 +287,77 @@ def config3_sources(n_src: int = 4096, grid_size: int = 4096, seed: int = 13) ->
     return np.concatenate([lm, flux[:, None]], axis=1).astype(np.float32)
 
 
+@dataclass
+class MultiChannel:
+    """Batched multi-channel workload: one array observation at n_channels
+    frequencies.  base_uvw holds every (time, baseline) uvw in metres, in a
+    seeded random order; channel c keeps the first n_vis[c] rows (its
+    unflagged visibilities: a uniformly random subset of the fraction
+    1 - flag[c]) and scales them by freqs[c] / c."""
+    grid_size: int
+    cell_size: float
+    oversampling: int
+    base_uvw: np.ndarray       # (N, 3) float32 metres
+    freqs: np.ndarray          # (n_channels,) Hz
+    n_vis: np.ndarray          # (n_channels,) int
+    flag: np.ndarray           # (n_channels,) flagged fraction
+    kernels: np.ndarray
+    degrid_kernels: np.ndarray
+    supports: np.ndarray
+    sources: np.ndarray        # (n_src, 3) l, m, flux at ref_freq
+    spectral_index: float
+    ref_freq: float
+    max_w: float
+
+    @property
+    def uv_scale(self) -> float:
+        return self.cell_size * self.grid_size
+
+    @property
+    def w_scale(self) -> float:
+        return float((self.supports.shape[0] - 1) ** 2) / self.max_w
+
+    def channel_uvw(self, c: int) -> np.ndarray:
+        return (self.base_uvw[:self.n_vis[c]] * np.float32(self.freqs[c] / C_LIGHT)).astype(np.float32)
+
+    def channel_sources(self, c: int) -> np.ndarray:
+        s = self.sources.copy()
+        s[:, 2] *= (self.freqs[c] / self.ref_freq) ** self.spectral_index
+        return s
+
+
+def config4(n_channels: int = 64, n_time: int = 8192, grid_size: int = 2048, n_planes: int = 16,
+            f_lo: float = 100e6, f_hi: float = 200e6) -> MultiChannel:
+    """Batched multi-channel: config1's array (64 antennas, 2 km disk, seed 1)
+    and sky (50 sources, U[0.1, 1] Jy at 150 MHz, seed 2) with spectral
+    index -0.7, n_time steps over HA -3..+3 h (8192: 2016 * 8192 =
+    16,515,072 rows per channel), n_channels frequencies on [f_lo, f_hi],
+    per-channel flagged fraction U[0, 0.6] (seed 3).  grid_size^2 grids with
+    cell 3e-5 rad (the 3.5 degree field of config2, so that w-projection
+    supports stay small at the longest w), n_planes w-planes, half support
+    4 -> 7 with |w|, OS 16."""
+    enu = disk_layout(64, 1000.0, seed=1)
+    ha = np.linspace(-3.0, 3.0, n_time)
+    base = uvw_tracks(enu, -30.0, -30.0, ha, C_LIGHT)           # metres (lambda = 1 m)
+    perm = np.random.default_rng(3).permutation(base.shape[0])
+    base = base[perm].astype(np.float32)
+    freqs = np.linspace(f_lo, f_hi, n_channels)
+    flag = np.random.default_rng(3).uniform(0.0, 0.6, n_channels)
+    n_vis = np.rint(base.shape[0] * (1.0 - flag)).astype(np.int64)
+    cell = 3.0e-5
+    fov = cell * grid_size
+    max_w = float(np.abs(base[:, 2]).max()) * f_hi / C_LIGHT * 1.0001
+    w_scale = (n_planes - 1) ** 2 / max_w
+    sup_h = [4 + int(round(3 * p / (n_planes - 1))) for p in range(n_planes)]
+    assert all(3 + (p * p / w_scale) * fov * fov / 2 <= h for p, h in enumerate(sup_h))
+    gk, dk, sup = kernel_tables(sup_h, 16, fov=fov, w_scale=w_scale)
+    rng = np.random.default_rng(2)
+    half_box = np.radians(1.0)
+    src = np.stack([rng.uniform(-half_box, half_box, 50), rng.uniform(-half_box, half_box, 50),
+                    rng.uniform(0.1, 1.0, 50)], axis=1).astype(np.float32)
+    return MultiChannel(grid_size, cell, 16, base, freqs, n_vis, flag, gk, dk, sup, src, -0.7, 150e6, max_w)
+
+
 def small_case(n_vis: int = 4000, grid_size: int = 128, n_planes: int = 4, oversampling: int = 16,
                seed: int = 7, h_list=None, fill: float = 0.98, w_max: float = 200.0) -> Dataset:
     """A small random case for parity tests: uvw uniform so that footprints
