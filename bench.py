@@ -29,6 +29,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(phase):
+    """HBM bytes per bench step of `phase` (its launches together), from the
+    FETCH_SIZE / WRITE_SIZE counter summary committed under profiles/ for
+    this workload (calibrated against a plain copy); None when absent."""
+    path = os.path.join(REPO, "profiles", "bench_config2_hbm_traffic.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return int(json.loads(block)[phase])
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(d, seconds):
     """Oracle std_gridding + std_degridding (the reference's CPU actors,
     restated) on a random sample of the same visibilities, one core."""
@@ -444,6 +457,7 @@ def main():
         "grid_merge": 8 * G * G,                   # residual grid written once
     }
     dom = max(k_ms, key=k_ms.get)
+    traffic = measured_traffic(dom) if args.config == "config2" else None
     launches = int(info["classes"]) if dom != "grid_merge" else 1 + (1 if info["chunks"] > info["tiles"] else 0)
     achieved = alg[dom] / (k_ms[dom] * 1e-3) / 1e9
     b_gd = N * 40 + 16 * G * G
@@ -509,7 +523,7 @@ def main():
                    "n_vis": N, "grid": G, "w_planes": int(d.supports.shape[0]), "oversampling": d.oversampling,
                    "accumulate": args.accumulate},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_per": "step",
                      "alg_bytes_per_step": alg[dom], "launches_per_step": launches,
                      "ms_per_step": round(k_ms[dom], 4)},
     }
