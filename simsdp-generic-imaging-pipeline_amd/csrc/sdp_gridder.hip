@@ -17,16 +17,17 @@
 //   list is cut into work items ("chunks") of about `chunk_taps` taps.
 //
 // Kernels: one wavefront per chunk.
-//   grid    The wavefront owns a private S x S LDS sub-grid of compensated
-//           accumulators (sum and TwoSum error per component).  It takes the
-//           chunk's visibilities in order; for each one its 64 lanes cover
-//           the footprint's taps (lane = one column, consecutive lanes
-//           consecutive columns, rows strided), so no two lanes touch one
-//           cell within a visibility and LDS order serialises visibilities:
-//           no atomics, deterministic, every lane busy.  The sub-grid is
-//           written to the chunk's slab; a merge pass sums, per grid cell,
-//           the slabs covering it in a fixed order (compensated) and writes
-//           the whole grid once.
+//   grid    The wavefront is split into lane groups, each with a private
+//           S x S LDS sub-grid copy.  A group takes its share of the chunk's
+//           visibilities in order; its lanes cover a visibility's taps (lane
+//           = one column of a row slot, consecutive lanes consecutive
+//           columns), so no two lanes touch one cell within a visibility and
+//           LDS order serialises visibilities: no atomics, deterministic.
+//           The copies are summed in group order (TwoSum) into the chunk's
+//           slab.  The merge then sums a (class, tile)'s pieces into one
+//           slab (reduce_pieces_kernel) and, per 32 x 32 block of grid cells,
+//           the slabs covering each cell in a fixed order (compensated),
+//           writing the whole grid once (merge_region_kernel).
 //   degrid  The wavefront stages the grid's S x S window in LDS; lanes sweep
 //           the taps with the same mapping and a shuffle tree reduces them.
 //   degrid+subtract+grid (major cycle): both of the above per visibility
@@ -522,86 +523,98 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
     }
 }
 
-struct MergeArgs {
-    int G, tile, ntx, nty, smax;
-};
-
 constexpr int kMergeThreads = 256;
-constexpr int kMergeList = 512;    // key-slab descriptors staged per pass
-constexpr int kMergeTiles = 256;   // covering tiles per output tile (checked by the plan)
 
 // Sum the slabs of a (class, tile) key's pieces, in piece order, into the
 // key's slab.  One workgroup per key with more than one piece; each thread
-// owns cells and issues the pieces' loads in groups of 8.
+// owns 4 cells per pass and issues the loads of 8 pieces x 4 cells together.
 __global__ void __launch_bounds__(kMergeThreads)
 reduce_pieces_kernel(const float2 *__restrict__ slabs, const long long *__restrict__ slab_off,
                      const int4 *__restrict__ multi, float2 *__restrict__ key_slabs)
 {
+    constexpr int kC = 4, kP = 8;
     const int4 k = multi[blockIdx.x];   // first chunk, piece count, S*S, key slab start (cells, / 1)
     const long long out0 = ((const long long *)(multi + gridDim.x))[blockIdx.x];
-    for (int i = threadIdx.x; i < k.z; i += kMergeThreads) {
-        float sr = 0.f, si = 0.f, cr = 0.f, ci = 0.f;
-        for (int p0 = 0; p0 < k.y; p0 += 8) {
-            float2 v[8];
+    for (int i0 = 0; i0 < k.z; i0 += kMergeThreads * kC) {
+        float sr[kC], si[kC], cr[kC], ci[kC];
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-                if (p0 + q < k.y) v[q] = slabs[slab_off[k.x + p0 + q] + i];
+        for (int c = 0; c < kC; ++c) sr[c] = si[c] = cr[c] = ci[c] = 0.f;
+        for (int p0 = 0; p0 < k.y; p0 += kP) {
+            float2 v[kP][kC];
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-                if (p0 + q < k.y) {
-                    two_sum_acc(sr, cr, v[q].x);
-                    two_sum_acc(si, ci, v[q].y);
+            for (int q = 0; q < kP; ++q) {
+                const bool pq = p0 + q < k.y;
+                const long long off = pq ? slab_off[k.x + p0 + q] : 0;
+#pragma unroll
+                for (int c = 0; c < kC; ++c) {
+                    const int i = i0 + c * kMergeThreads + threadIdx.x;
+                    v[q][c] = (pq && i < k.z) ? slabs[off + i] : make_float2(0.f, 0.f);
+                }
+            }
+            // absent values are zeros: adding them leaves sum and compensation unchanged
+#pragma unroll
+            for (int q = 0; q < kP; ++q)
+#pragma unroll
+                for (int c = 0; c < kC; ++c) {
+                    two_sum_acc(sr[c], cr[c], v[q][c].x);
+                    two_sum_acc(si[c], ci[c], v[q][c].y);
                 }
         }
-        key_slabs[out0 + i] = make_float2(sr + cr, si + ci);
+#pragma unroll
+        for (int c = 0; c < kC; ++c) {
+            const int i = i0 + c * kMergeThreads + threadIdx.x;
+            if (i < k.z) key_slabs[out0 + i] = make_float2(sr[c] + cr[c], si[c] + ci[c]);
+        }
     }
 }
 
-// One workgroup per output tile: grid cell = compensated sum of the key
-// slabs covering it, in (tile row, tile column, class) order.  The
-// workgroup loads the covering keys' descriptors into LDS in parallel; with
-// tiles of at most 128 cells the threads split the descriptors into
-// interleaved groups whose partial sums are combined in group order.
-// Writes every cell.
+// Merge by output region: one workgroup per R x R block of grid cells
+// (R = 32, or G when smaller).  Its cells are covered by the key slabs (one
+// per (class, tile), pieces already summed) of the tiles whose windows
+// [tile origin, tile origin + S) reach into the block; every cell is the
+// compensated sum of those slabs' values in (tile row, tile column, class)
+// order, and every cell is written.  Consecutive lanes read consecutive
+// cells of a slab row, and the loads of 8 slabs are in flight together.
+constexpr int kRegion = 32;
+constexpr int kRegionList = 512;    // slab descriptors staged per pass
+constexpr int kRegionCover = 1024;  // covering tiles per region (checked by the plan)
+constexpr int kRegionBatch = 8;     // slabs whose loads are issued together
+
+struct RegionArgs {
+    int G, tile, ntx, nty, smax, R;
+};
+
 __global__ void __launch_bounds__(kMergeThreads)
-merge_slabs_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ key_S,
-                   const long long *__restrict__ key_off, const int *__restrict__ tile_key_begin,
-                   MergeArgs a, float2 *__restrict__ grid)
+merge_region_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ key_S,
+                    const long long *__restrict__ key_off, const int *__restrict__ tile_key_begin, RegionArgs a,
+                    float2 *__restrict__ grid)
 {
-    constexpr int kCells = 4;   // output cells per thread when a tile has more than 256 cells
-    __shared__ long long s_off[kMergeList];
-    __shared__ int s_geo[kMergeList];          // S | by << 10 | bx << 20
-    __shared__ int s_base[kMergeTiles + 1];    // first list position of each covering tile
-    __shared__ int s_k0[kMergeTiles];          // its first key
-    __shared__ int s_b[kMergeTiles];           // by | bx << 10 for that tile
-    __shared__ float4 s_part[kMergeThreads];
-    const int oty = blockIdx.y, otx = blockIdx.x;
-    const int oy0 = oty * a.tile, ox0 = otx * a.tile;
-    const int ncell = a.tile * a.tile;
-    const int groups = ncell <= 128 ? kMergeThreads / ncell : 1;   // ncell divides 256 when <= 128? not needed
-    const int grp = ncell <= 128 ? threadIdx.x / ncell : 0;
-    const bool in_group = ncell <= 128 ? grp < groups && threadIdx.x < groups * ncell : true;
+    constexpr int kCells = kRegion * kRegion / kMergeThreads;   // cells per thread at R = 32
+    __shared__ long long s_off[kRegionList];
+    __shared__ int s_geo[kRegionList];          // S | (by + 512) << 10 | (bx + 512) << 20
+    __shared__ int s_base[kRegionCover + 1];    // first list position of each covering tile
+    __shared__ int s_k0[kRegionCover];          // its first key
+    __shared__ int s_b[kRegionCover];           // (by + 512) | (bx + 512) << 10
+    const int R = a.R;
+    const int oy0 = blockIdx.y * R, ox0 = blockIdx.x * R;
+    const int rows_per_pass = kMergeThreads / R;
+    const int col = threadIdx.x % R, row0 = threadIdx.x / R;
     float sr[kCells], si[kCells], cr[kCells], ci[kCells];
-    int dy[kCells], dx[kCells];
-    bool live[kCells];
 #pragma unroll
-    for (int q = 0; q < kCells; ++q) {
-        const int i = ncell <= 128 ? (q == 0 ? threadIdx.x - grp * ncell : ncell) : threadIdx.x + q * kMergeThreads;
-        dy[q] = i / a.tile;
-        dx[q] = i - dy[q] * a.tile;
-        live[q] = in_group && i < ncell && oy0 + dy[q] < a.G && ox0 + dx[q] < a.G;
-        sr[q] = si[q] = cr[q] = ci[q] = 0.f;
-    }
-    const int ty_lo = max(0, (oy0 - a.smax + a.tile) / a.tile);
-    const int tx_lo = max(0, (ox0 - a.smax + a.tile) / a.tile);
-    const int ntx_c = otx - tx_lo + 1;
-    const int nt = (oty - ty_lo + 1) * ntx_c;
+    for (int q = 0; q < kCells; ++q) sr[q] = si[q] = cr[q] = ci[q] = 0.f;
+
+    // covering tiles: ty * tile < oy0 + R and ty * tile + smax > oy0
+    const int ty_lo = oy0 - a.smax < 0 ? 0 : (oy0 - a.smax) / a.tile + 1;
+    const int tx_lo = ox0 - a.smax < 0 ? 0 : (ox0 - a.smax) / a.tile + 1;
+    const int ty_hi = min(a.nty - 1, (oy0 + R - 1) / a.tile), tx_hi = min(a.ntx - 1, (ox0 + R - 1) / a.tile);
+    const int ntx_c = tx_hi - tx_lo + 1;
+    const int nt = (ty_hi - ty_lo + 1) * ntx_c;
     for (int k = threadIdx.x; k < nt; k += kMergeThreads) {
         const int ty = ty_lo + k / ntx_c, tx = tx_lo + k % ntx_c;
         const int t = ty * a.ntx + tx;
         s_k0[k] = tile_key_begin[t];
         s_base[k + 1] = tile_key_begin[t + 1] - tile_key_begin[t];
-        s_b[k] = (oy0 - ty * a.tile) | ((ox0 - tx * a.tile) << 10);
+        s_b[k] = (oy0 - ty * a.tile + 512) | ((ox0 - tx * a.tile + 512) << 10);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -610,8 +623,8 @@ merge_slabs_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ ke
     }
     __syncthreads();
     const int total = s_base[nt];
-    for (int base = 0; base < total; base += kMergeList) {
-        const int n = min(kMergeList, total - base);
+    for (int base = 0; base < total; base += kRegionList) {
+        const int n = min(kRegionList, total - base);
         for (int i = threadIdx.x; i < n; i += kMergeThreads) {
             const int g = base + i;
             int lo = 0, hi = nt - 1;   // covering tile k with s_base[k] <= g < s_base[k + 1]
@@ -624,42 +637,39 @@ merge_slabs_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ ke
             s_geo[i] = key_S[key] | (s_b[lo] << 10);
         }
         __syncthreads();
-        for (int e = grp; e < n; e += groups) {
-            const int geo = s_geo[e];
-            const int S = geo & 1023, by = (geo >> 10) & 1023, bx = (geo >> 20) & 1023;
-            const float2 *slab = kslabs + s_off[e];
+        for (int e0 = 0; e0 < n; e0 += kRegionBatch) {
+            float2 v[kRegionBatch][kCells];
 #pragma unroll
-            for (int q = 0; q < kCells; ++q) {
-                const int ly = by + dy[q], lx = bx + dx[q];
-                if (live[q] && ly < S && lx < S) {
-                    const float2 v = slab[ly * S + lx];
-                    two_sum_acc(sr[q], cr[q], v.x);
-                    two_sum_acc(si[q], ci[q], v.y);
+            for (int u = 0; u < kRegionBatch; ++u) {
+                const int e = e0 + u;
+                const int geo = e < n ? s_geo[e] : 0;
+                const int S = geo & 1023, by = ((geo >> 10) & 1023) - 512, bx = ((geo >> 20) & 1023) - 512;
+                const int lx = bx + col;
+                const bool colok = e < n && lx >= 0 && lx < S;
+                const float2 *slab = kslabs + (e < n ? s_off[e] : 0) + lx;
+#pragma unroll
+                for (int q = 0; q < kCells; ++q) {
+                    const int r = row0 + q * rows_per_pass, ly = by + r;
+                    v[u][q] = (colok && r < R && ly >= 0 && ly < S) ? slab[ly * S] : make_float2(0.f, 0.f);
                 }
             }
+            // absent values are zeros: adding them leaves sum and compensation unchanged
+#pragma unroll
+            for (int u = 0; u < kRegionBatch; ++u)
+#pragma unroll
+                for (int q = 0; q < kCells; ++q) {
+                    two_sum_acc(sr[q], cr[q], v[u][q].x);
+                    two_sum_acc(si[q], ci[q], v[u][q].y);
+                }
         }
         __syncthreads();
-    }
-    if (ncell <= 128) {
-        // combine the groups' partial sums in group order
-        s_part[threadIdx.x] = make_float4(sr[0], si[0], cr[0], ci[0]);
-        __syncthreads();
-        if (grp == 0 && live[0]) {
-            float tr = 0.f, ti = 0.f, er = 0.f, ei = 0.f;
-            for (int g = 0; g < groups; ++g) {
-                const float4 p = s_part[g * ncell + threadIdx.x];
-                two_sum_acc(tr, er, p.x);
-                two_sum_acc(ti, ei, p.y);
-                two_sum_acc(tr, er, p.z);
-                two_sum_acc(ti, ei, p.w);
-            }
-            grid[(size_t)(oy0 + dy[0]) * a.G + ox0 + dx[0]] = make_float2(tr + er, ti + ei);
-        }
-        return;
     }
 #pragma unroll
-    for (int q = 0; q < kCells; ++q)
-        if (live[q]) grid[(size_t)(oy0 + dy[q]) * a.G + ox0 + dx[q]] = make_float2(sr[q] + cr[q], si[q] + ci[q]);
+    for (int q = 0; q < kCells; ++q) {
+        const int r = row0 + q * rows_per_pass;
+        if (r < R && oy0 + r < a.G && ox0 + col < a.G)
+            grid[(size_t)(oy0 + r) * a.G + ox0 + col] = make_float2(sr[q] + cr[q], si[q] + ci[q]);
+    }
 }
 
 // visibilities the plan dropped (outside the grid): prediction 0
@@ -699,6 +709,7 @@ struct sdp_grid_plan {
     int *key_S = nullptr;
     long long *key_off = nullptr;  // into key_base (see key_base_of)
     int *tile_key_begin = nullptr;
+    int region = 0;                    // merge region edge
     int4 *multi = nullptr;         // multi-piece keys: first chunk, pieces, S*S, -, then their output starts
     float2 *key_slabs = nullptr;   // contiguous with slabs: slabs followed by the reduced key slabs
 };
@@ -895,8 +906,9 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     const int ntiles = plan->ntx * plan->nty;
     const int nkeys = kNumClasses * ntiles;
     {
-        const int span = (plan->smax + 2 * plan->tile - 1) / plan->tile;   // covering tiles per axis, upper bound
-        if (span * span > kMergeTiles) {
+        plan->region = std::min(kRegion, p.grid_size);
+        const int rspan = (plan->smax + plan->region + 2 * plan->tile - 1) / plan->tile;
+        if (rspan * rspan > kRegionCover) {
             const int tile = plan->tile;
             delete plan;
             return sdp::set_error(SDP_E_UNSUPPORTED, "tile %d too small for half support %d", tile, H);
@@ -1149,10 +1161,11 @@ int sdp_grid_merge(const sdp_grid_plan *plan, float *d_grid, sdp_stream s)
                                                                        plan->key_slabs);
         SDP_LAUNCH_CHECK();
     }
-    MergeArgs m{G, plan->tile, plan->ntx, plan->nty, std::max(plan->smax, 1)};
-    dim3 mg(plan->ntx, plan->nty);
-    merge_slabs_kernel<<<mg, kMergeThreads, 0, st>>>(plan->key_slabs, plan->key_S, plan->key_off,
-                                                     plan->tile_key_begin, m, (float2 *)d_grid);
+    const int R = plan->region;
+    RegionArgs ra{G, plan->tile, plan->ntx, plan->nty, std::max(plan->smax, 1), R};
+    dim3 rg((G + R - 1) / R, (G + R - 1) / R);
+    merge_region_kernel<<<rg, kMergeThreads, 0, st>>>(plan->key_slabs, plan->key_S, plan->key_off,
+                                                     plan->tile_key_begin, ra, (float2 *)d_grid);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }
