@@ -192,15 +192,16 @@ struct ChunkIO {
 // Lane groups.  A wavefront splits into C groups of L lanes (L a power of
 // two no smaller than the class's largest footprint extent, C = 64 / L or
 // fewer when LDS runs short).  Group g takes the chunk's visibilities
-// g, g + C, g + 2C, ... in order; within a visibility lane c of the group
-// owns footprint column c and walks the rows, so a group's lanes never touch
-// one cell at the same time.  Each group accumulates into its own copy of
-// the sub-grid; the copies are summed in group order when the slab is
-// written, so the result does not depend on timing.  The kernel-table row
-// offset of footprint row r is computed once per visibility, by lane r of
-// the group, and fetched with a lane shuffle.  Taps of idle lanes (past the
-// footprint, or past the chunk's last visibility) are predicated: they read
-// valid addresses and accumulate into a per-lane spare cell.
+// g, g + C, g + 2C, ... in order; within a visibility the group's lanes walk
+// the footprint's taps in row-major order, L at a time, so a group's lanes
+// never touch one cell at the same time and no lane idles while taps remain.
+// Each group accumulates into its own copy of the sub-grid; the copies are
+// summed in group order when the slab is written, so the result does not
+// depend on timing.  The kernel-table offsets of footprint row r and column
+// u are computed once per visibility, by lanes r and u of the group, and
+// fetched with lane shuffles.  Taps of idle lanes (past the footprint, or
+// past the chunk's last visibility) are predicated: they read zeros and
+// accumulate into a per-lane spare cell.
 // Phase-major kernel tables.  The reference lays a plane out as
 // [ky][kx] with ky, kx = |offset| * OS in [0, (h+1) OS) (kernel_host_set_up,
 // top.c:252-257), so the taps of one visibility, OS samples apart, fall on
@@ -331,9 +332,6 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
     const int L = 1 << a.lane_shift, C = a.copies;
     const int g = lane >> a.lane_shift, c = lane & (L - 1);
     const bool in_group = g < C;
-    // within a group: footprint column c % W, row slot c / W of R = L / W
-    const int W = 1 << a.col_shift, R = L >> a.col_shift;
-    const int col0 = c & (W - 1), rslot = c >> a.col_shift;
     const int tile_u0 = (ch.x % a.ntx) * a.tile, tile_v0 = (ch.x / a.ntx) * a.tile;
     const int gc = a.G / 2;
     const float os = (float)a.OS;
@@ -378,54 +376,53 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
         }
         const GroupVis v = decode_vis(cur, plane_tab, ok, tile_u0, tile_v0);
         const int cell_u0 = tile_u0 + v.lu0 - gc, cell_v0 = tile_v0 + v.lv0 - gc;
-        // kernel-table row offset of footprint rows c and c + L (std_gridding's
-        // abs((int)ROUND((v - grid_pos) * OS)), degridgrid.c:110, times the row length)
-        // wave-wide row count and column blocks (1 unless the extent exceeds W)
-        const int packed = wave_max(v.nv | (((v.nu + W - 1) >> a.col_shift) << 16));
-        const int nv_max = packed & 0xffff, nb_max = packed >> 16;
-        // byte offset of footprint row c (and c + L) in the phase-major table;
-        // rows past the footprint get an offset past the table, which the
-        // buffer loads below turn into zeros
-        const int kyA = c < v.nv ? 8 * row_part(kernel_tap(cell_v0 + c, v.gy, os), a.OS, a.os_shift, v.pt.y) : kOob;
-        const int kyB = (nv_max > L && c + L < v.nv)
-                            ? 8 * row_part(kernel_tap(cell_v0 + c + L, v.gy, os), a.OS, a.os_shift, v.pt.y) : kOob;
+        // The group's L lanes walk the footprint's nu x nv taps in row-major
+        // order, L at a time: lane c takes taps c, c + L, c + 2L, ...  Lane
+        // r (r < nv) holds the kernel-table byte offset of footprint row r
+        // (std_gridding's abs((int)ROUND((v - grid_pos) * OS)), degridgrid.c:110,
+        // in the phase-major layout) and lane u (u < nu) that of column u;
+        // each tap fetches both with a lane shuffle.  Offsets of absent rows
+        // and columns lie past the table, so their loads read zeros.
+        const int ntap = v.nu * v.nv;
+        const int iters = wave_max((ntap + L - 1) >> a.lane_shift);   // uniform trip count
+        const int ky_own = c < v.nv ? 8 * row_part(kernel_tap(cell_v0 + c, v.gy, os), a.OS, a.os_shift, v.pt.y)
+                                    : kOob;
+        const int kx_own = c < v.nu
+                               ? 8 * (v.pt.x + col_part(kernel_tap(cell_u0 + c, v.gx, os), a.OS, a.os_shift, v.pt.y))
+                               : kOob;
         const int gbase = g << a.lane_shift;
+        // this lane's first tap (row, column) and the step of L taps
+        const int nu1 = max(v.nu, 1);
+        int trow = c / nu1, tcol = c - (c / nu1) * nu1;
+        const int drow = L / nu1, dcol = L - (L / nu1) * nu1;
 
         float2 res = make_float2(0.f, 0.f);
         if (kDegrid) {
             float sr = 0.f, si = 0.f;
-            for (int b = 0; b < nb_max; ++b) {
-                const int col = col0 + (b << a.col_shift);
-                const bool col_ok = v.ok && col < v.nu;
-                const int nv_lane = col_ok ? v.nv : 0;
-                const int kbase =
-                    col_ok ? 8 * (v.pt.x + col_part(kernel_tap(cell_u0 + col, v.gx, os), a.OS, a.os_shift, v.pt.y))
-                           : kOob;
-                const int wbase = v.lv0 * stride + v.lu0 + col;
-                // rows [0, L) take their table offset from kyA, rows [L, 2L) from kyB;
-                // this lane's rows are rlo + rslot + R t (t < W keeps them below rlo + L)
-                for (int half = 0; half < 2; ++half) {
-                    const int rlo = half ? L : 0, rhi = half ? nv_max : min(nv_max, L);
-                    const int src = half ? kyB : kyA;
-                    const int bp = (gbase + rslot) << 2;
-                    for (int t0 = 0; rlo + t0 * R < rhi; t0 += kRowBatch) {
-                        float2 kk[kRowBatch], w[kRowBatch];
+            const int wbase = v.lv0 * stride + v.lu0;
+            int row = trow, col = tcol;
+            for (int j0 = 0; j0 < iters; j0 += kRowBatch) {
+                float2 kk[kRowBatch], w[kRowBatch];
 #pragma unroll
-                        for (int q = 0; q < kRowBatch; ++q) {
-                            const int r = rlo + rslot + (t0 + q) * R;
-                            const int ky = __builtin_amdgcn_ds_bpermute(bp + ((t0 + q) * R << 2), src);
-                            kk[q] = tab_load(kd_rsrc, (unsigned)(kbase + ky));
-                            w[q] = win[min(wbase + r * stride, cells - 1)];
-                            if (r >= nv_lane) w[q] = make_float2(0.f, 0.f);
-                        }
-#pragma unroll
-                        for (int q = 0; q < kRowBatch; ++q) {
-                            // grid * (k.x, conj * k.y); idle taps add exact zeros
-                            const float kyc = v.conj * kk[q].y;
-                            sr = fmaf(w[q].x, kk[q].x, fmaf(-w[q].y, kyc, sr));
-                            si = fmaf(w[q].y, kk[q].x, fmaf(w[q].x, kyc, si));
-                        }
+                for (int q = 0; q < kRowBatch; ++q) {
+                    const bool live = row < v.nv;
+                    const int ky = __builtin_amdgcn_ds_bpermute((gbase + min(row, L - 1)) << 2, ky_own);
+                    const int kx = __builtin_amdgcn_ds_bpermute((gbase + col) << 2, kx_own);
+                    kk[q] = tab_load(kd_rsrc, live ? (unsigned)(kx + ky) : (unsigned)kOob);
+                    w[q] = live ? win[wbase + row * stride + col] : make_float2(0.f, 0.f);
+                    col += dcol;
+                    row += drow;
+                    if (col >= nu1) {
+                        col -= nu1;
+                        ++row;
                     }
+                }
+#pragma unroll
+                for (int q = 0; q < kRowBatch; ++q) {
+                    // grid * (k.x, conj * k.y); idle taps add exact zeros
+                    const float kyc = v.conj * kk[q].y;
+                    sr = fmaf(w[q].x, kk[q].x, fmaf(-w[q].y, kyc, sr));
+                    si = fmaf(w[q].y, kk[q].x, fmaf(w[q].x, kyc, si));
                 }
             }
             // butterfly over the group's lanes: every lane ends with the sum
@@ -453,44 +450,39 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
             if (MODE == MODE_GRID && v.ok) res = cur_val;
             const float vr = res.x, vi = res.y, vrc = v.conj * res.x, vic = v.conj * res.y;
             const int spare_i = C * cstride + lane;
-            for (int b = 0; b < nb_max; ++b) {
-                const int col = col0 + (b << a.col_shift);
-                const bool col_ok = v.ok && col < v.nu;
-                const int nv_lane = col_ok ? v.nv : 0;
-                const int kbase =
-                    col_ok ? 8 * (v.pt.x + col_part(kernel_tap(cell_u0 + col, v.gx, os), a.OS, a.os_shift, v.pt.y))
-                           : kOob;
-                const int cbase = g * cstride + (v.lv0 + rslot) * stride + v.lu0 + col;
-                for (int half = 0; half < 2; ++half) {
-                    const int rlo = half ? L : 0, rhi = half ? nv_max : min(nv_max, L);
-                    const int src = half ? kyB : kyA;
-                    const int bp = (gbase + rslot) << 2;
-                    for (int t0 = 0; rlo + t0 * R < rhi; t0 += kRowBatch) {
-                        float2 kk[kRowBatch];
-                        int ci[kRowBatch];
+            const int cbase = g * cstride + v.lv0 * stride + v.lu0;
+            int row = trow, col = tcol;
+            for (int j0 = 0; j0 < iters; j0 += kRowBatch) {
+                float2 kk[kRowBatch];
+                int ci[kRowBatch];
 #pragma unroll
-                        for (int q = 0; q < kRowBatch; ++q) {
-                            const int r = rlo + rslot + (t0 + q) * R;
-                            const int ky = __builtin_amdgcn_ds_bpermute(bp + ((t0 + q) * R << 2), src);
-                            kk[q] = tab_load(kg_rsrc, (unsigned)(kbase + ky));
-                            ci[q] = r < nv_lane ? cbase + (rlo + (t0 + q) * R) * stride : spare_i;
-                        }
-                        // distinct rows per lane (or the lane's spare cell, which
-                        // nothing reads): reads first, then writes
-                        Acc cell[kRowBatch];
-#pragma unroll
-                        for (int q = 0; q < kRowBatch; ++q) cell[q] = acc[ci[q]];
-#pragma unroll
-                        for (int q = 0; q < kRowBatch; ++q) {
-                            // vis * (k.x, conj * k.y), std_gridding (degridgrid.c:112-118)
-                            const float px = fmaf(vr, kk[q].x, -(vic * kk[q].y));
-                            const float py = fmaf(vi, kk[q].x, vrc * kk[q].y);
-                            acc_add(cell[q], px, py);
-                        }
-#pragma unroll
-                        for (int q = kRowBatch - 1; q >= 0; --q) acc[ci[q]] = cell[q];
+                for (int q = 0; q < kRowBatch; ++q) {
+                    const bool live = row < v.nv;
+                    const int ky = __builtin_amdgcn_ds_bpermute((gbase + min(row, L - 1)) << 2, ky_own);
+                    const int kx = __builtin_amdgcn_ds_bpermute((gbase + col) << 2, kx_own);
+                    kk[q] = tab_load(kg_rsrc, live ? (unsigned)(kx + ky) : (unsigned)kOob);
+                    ci[q] = live ? cbase + row * stride + col : spare_i;
+                    col += dcol;
+                    row += drow;
+                    if (col >= nu1) {
+                        col -= nu1;
+                        ++row;
                     }
                 }
+                // distinct cells per lane (or the lane's spare cell, which
+                // nothing reads): reads first, then writes
+                Acc cell[kRowBatch];
+#pragma unroll
+                for (int q = 0; q < kRowBatch; ++q) cell[q] = acc[ci[q]];
+#pragma unroll
+                for (int q = 0; q < kRowBatch; ++q) {
+                    // vis * (k.x, conj * k.y), std_gridding (degridgrid.c:112-118)
+                    const float px = fmaf(vr, kk[q].x, -(vic * kk[q].y));
+                    const float py = fmaf(vi, kk[q].x, vrc * kk[q].y);
+                    acc_add(cell[q], px, py);
+                }
+#pragma unroll
+                for (int q = kRowBatch - 1; q >= 0; --q) acc[ci[q]] = cell[q];
             }
         }
     }
