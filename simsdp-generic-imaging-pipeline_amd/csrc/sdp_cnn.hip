@@ -721,14 +721,19 @@ int sdp_cnn_mse(size_t count, const float *d_out, const float *d_target, float *
 {
     SDP_REQUIRE(count > 0 && d_out && d_target && d_grad, "sdp_cnn_mse: bad argument");
     hipStream_t st = sdp::stream_of(s);
-    double *part = nullptr;
-    SDP_HIP_TRY(hipMalloc(&part, sizeof(double) * 1024));
+    // per-thread, per-device partial sums, allocated on the first call (the
+    // call is synchronous, so a thread never has two in flight)
+    static thread_local double *parts[64] = {};
+    int dev = 0;
+    SDP_HIP_TRY(hipGetDevice(&dev));
+    SDP_REQUIRE(dev >= 0 && dev < 64, "sdp_cnn_mse: device index out of range");
+    if (!parts[dev]) SDP_HIP_TRY(hipMalloc(&parts[dev], sizeof(double) * 1024));
+    double *part = parts[dev];
     const unsigned int nb = std::min<unsigned int>(1024, ew_grid(count));
     mse_kernel<<<nb, kT, 0, st>>>(d_out, d_target, count, d_grad, part);
     std::vector<double> h(nb);
     hipError_t e = hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(part);
     if (e != hipSuccess) return sdp::set_error(SDP_E_HIP, "sdp_cnn_mse: %s", hipGetErrorString(e));
     double sum = 0.0;
     for (double v : h) sum += v;
