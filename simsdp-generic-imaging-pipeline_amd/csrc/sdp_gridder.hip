@@ -764,7 +764,7 @@ int grid_copies(bool comp)
         env = (v == 1 || v == 2 || v == 4) ? v : 0;
     }
     if (env > 0) return env;
-    return comp ? 1 : 4;   // measured at 16.7M visibilities, 32 w-planes, h 4-16
+    return comp ? 1 : 2;   // measured at 16.7M visibilities, 32 w-planes, h 4-16
 }
 
 template <int MODE, bool COMP>
