@@ -253,7 +253,7 @@ dft_array_kernel(int n_ant, int n_chunks, const double *__restrict__ ant_uvw, co
     typedef typename Mfma<T>::v8 v8;
     // [hi/lo][antenna slot][k]: slots 0..kTile-1 the a-tile (conjugated
     // side), kTile..2 kTile-1 the b-tile (weighted side)
-    __shared__ __attribute__((aligned(16))) T stage[2][2 * kTile][kRow];
+    __shared__ __attribute__((aligned(16))) T stage[2][2][2 * kTile][kRow];   // [buffer][hi/lo][slot][k]
 
     // triangle tile index -> (ta, tb), ta <= tb
     int tb = 0, rem = blockIdx.x;
@@ -268,9 +268,11 @@ dft_array_kernel(int n_ant, int n_chunks, const double *__restrict__ ant_uvw, co
     // phasor duty: off the diagonal, thread -> (slot, 8 sources) over all
     // 2 kTile slots; on it the two sides are the same antennas, so thread
     // -> (a-side slot, 4 sources) and each phasor is written to both sides
+    // (the source part is uniform over a wavefront, so its terms are scalar
+    // loads shared by the wavefront's lanes)
     const int per = diag ? 4 : 8;
-    const int slot = diag ? (int)(threadIdx.x >> 2) : (int)(threadIdx.x >> 1);
-    const int part = diag ? (int)(threadIdx.x & 3) : (int)(threadIdx.x & 1);
+    const int slot = diag ? (int)(threadIdx.x & (kTile - 1)) : (int)(threadIdx.x & (2 * kTile - 1));
+    const int part = __builtin_amdgcn_readfirstlane(diag ? (int)(threadIdx.x >> 7) : (int)(threadIdx.x >> 8));
     const int ant = slot < kTile ? ta * kTile + slot : tb * kTile + (slot - kTile);
     const bool weighted = slot >= kTile;
     double u = 0.0, v = 0.0, w = 0.0;
@@ -291,45 +293,56 @@ dft_array_kernel(int n_ant, int n_chunks, const double *__restrict__ ant_uvw, co
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc_r[i][j] = acc_i[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int c = 0; c < n_chunks; ++c) {
-        // phasors E = exp(-i 2 pi turns), turns formed and reduced in FP64
-        T hi[16], lo[16], whi[8], wlo[8];
+    // Software pipeline over chunks of 16 sources, two LDS buffers: while
+    // the matrix cores consume chunk c from buffer c & 1, the vector ALUs
+    // form chunk c + 1's phasors, which then go to the other buffer; one
+    // barrier per chunk.  The work buffer holds one chunk of zero terms
+    // past the last, so the final iteration's look-ahead reads valid data.
+    T hi[16], lo[16], whi[8], wlo[8];
+    auto form = [&](int c) {
         if (diag)
             form_phasors<T, kHw, 4, true>(terms + c * kChunk + part * per, u, v, w, live, weighted, hi, lo, whi, wlo);
         else
             form_phasors<T, kHw, 8, false>(terms + c * kChunk + part * per, u, v, w, live, weighted, hi, lo, whi,
                                            wlo);
-        __syncthreads();   // the previous chunk's operands are consumed
+    };
+    auto store = [&](int buf) {
         if (diag) {
             const int k0 = part * 8;
-            *reinterpret_cast<v8 *>(&stage[0][slot][k0]) =
+            *reinterpret_cast<v8 *>(&stage[buf][0][slot][k0]) =
                 v8{hi[0], hi[1], hi[2], hi[3], hi[4], hi[5], hi[6], hi[7]};
-            *reinterpret_cast<v8 *>(&stage[1][slot][k0]) =
+            *reinterpret_cast<v8 *>(&stage[buf][1][slot][k0]) =
                 v8{lo[0], lo[1], lo[2], lo[3], lo[4], lo[5], lo[6], lo[7]};
-            *reinterpret_cast<v8 *>(&stage[0][kTile + slot][k0]) =
+            *reinterpret_cast<v8 *>(&stage[buf][0][kTile + slot][k0]) =
                 v8{whi[0], whi[1], whi[2], whi[3], whi[4], whi[5], whi[6], whi[7]};
-            *reinterpret_cast<v8 *>(&stage[1][kTile + slot][k0]) =
+            *reinterpret_cast<v8 *>(&stage[buf][1][kTile + slot][k0]) =
                 v8{wlo[0], wlo[1], wlo[2], wlo[3], wlo[4], wlo[5], wlo[6], wlo[7]};
         } else {
 #pragma unroll
             for (int q = 0; q < 16; q += 8) {
-                *reinterpret_cast<v8 *>(&stage[0][slot][part * 16 + q]) =
+                *reinterpret_cast<v8 *>(&stage[buf][0][slot][part * 16 + q]) =
                     v8{hi[q], hi[q + 1], hi[q + 2], hi[q + 3], hi[q + 4], hi[q + 5], hi[q + 6], hi[q + 7]};
-                *reinterpret_cast<v8 *>(&stage[1][slot][part * 16 + q]) =
+                *reinterpret_cast<v8 *>(&stage[buf][1][slot][part * 16 + q]) =
                     v8{lo[q], lo[q + 1], lo[q + 2], lo[q + 3], lo[q + 4], lo[q + 5], lo[q + 6], lo[q + 7]};
             }
         }
-        __syncthreads();
+    };
+    form(0);
+    store(0);
+    __syncthreads();
+    for (int c = 0; c < n_chunks; ++c) {
+        const int buf = c & 1;
+        form(c + 1);   // look-ahead (chunk n_chunks is the zero pad)
         v8 bb[2][2];   // [hi/lo][column group]
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
             for (int j = 0; j < 2; ++j)
-                bb[h][j] = *reinterpret_cast<const v8 *>(&stage[h][kTile + wc + j * 16 + (lane & 15)][kq]);
+                bb[h][j] = *reinterpret_cast<const v8 *>(&stage[buf][h][kTile + wc + j * 16 + (lane & 15)][kq]);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const v8 ar0 = *reinterpret_cast<const v8 *>(&stage[0][wr + i * 16 + (lane & 15)][kq]);
-            const v8 ar1 = *reinterpret_cast<const v8 *>(&stage[1][wr + i * 16 + (lane & 15)][kq]);
+            const v8 ar0 = *reinterpret_cast<const v8 *>(&stage[buf][0][wr + i * 16 + (lane & 15)][kq]);
+            const v8 ar1 = *reinterpret_cast<const v8 *>(&stage[buf][1][wr + i * 16 + (lane & 15)][kq]);
             const v8 ai0 = rotate_pairs(ar0), ai1 = rotate_pairs(ar1);
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
@@ -343,6 +356,9 @@ dft_array_kernel(int n_ant, int n_chunks, const double *__restrict__ ant_uvw, co
                 }
             }
         }
+        // buffer buf ^ 1 was last read in iteration c - 1, before its barrier
+        store(buf ^ 1);
+        __syncthreads();
     }
 
     // C[a][b], a = row (4 (lane / 16) + r), b = column (lane % 16): for a
@@ -373,7 +389,7 @@ extern "C" {
 size_t sdp_dft_array_work_bytes(int n_sources)
 {
     const size_t n_pad = (size_t)((n_sources > 0 ? n_sources : 0) + kChunk - 1) / kChunk * kChunk;
-    return array_terms_offset() + (n_pad > 0 ? n_pad : kChunk) * sizeof(SrcTerm);
+    return array_terms_offset() + (n_pad + kChunk) * sizeof(SrcTerm);   // one zero chunk past the last
 }
 
 int sdp_dft_predict_array(int n_time, int n_ant, const double *d_ant_uvw, int n_sources, const float *d_sources,
@@ -392,7 +408,7 @@ int sdp_dft_predict_array(int n_time, int n_ant, const double *d_ant_uvw, int n_
     float *scale = (float *)d_work;
     SrcTerm *terms = (SrcTerm *)((char *)d_work + array_terms_offset());
     const int fp16 = (flags & SDP_DFT_ARRAY_FP16) != 0;
-    dft_array_prep_kernel<<<1, kThreads, 0, st>>>(n_sources, n_pad, d_sources, terms, scale, fp16);
+    dft_array_prep_kernel<<<1, kThreads, 0, st>>>(n_sources, n_pad + kChunk, d_sources, terms, scale, fp16);
     SDP_LAUNCH_CHECK();
     const int nt = (n_ant + kTile - 1) / kTile;
     const dim3 grid(nt * (nt + 1) / 2, n_time);
