@@ -48,25 +48,41 @@ struct GemmArgs {
     int relu;                  // FWD: apply ReLU
 };
 
-// A[m][k] of the GEMM (0 outside the image: zero padding)
-__device__ __forceinline__ float load_a(const GemmArgs &g, int mode, int m, int k)
+// Index tables for A's im2col reads, so that the staging loops do no
+// divisions.  A[m][k] = src[b][y + s ky][x + s kx][c] with a pixel index p
+// (A's row for the forward and input-gradient products, its column for the
+// weight gradient) and a tap index q = t C + c (the other one); s = -1 for
+// the input gradient.  Pixel entries hold (b H, y, x), tap entries
+// (s ky, s kx, c); absent entries put y far outside the image.
+struct PixIdx {
+    int bh, y, x;
+};
+struct TapIdx {
+    int dy, dx, c;
+};
+
+__device__ __forceinline__ PixIdx pix_index(const GemmArgs &g, int p, int np)
 {
-    if (m >= g.M || k >= g.K) return 0.f;
-    int pix, t, c, C;
-    if (mode == G_BWD_W) {
-        t = m / g.Cin; c = m - t * g.Cin; pix = k; C = g.Cin;
-    } else if (mode == G_FWD) {
-        t = k / g.Cin; c = k - t * g.Cin; pix = m; C = g.Cin;
-    } else {
-        t = k / g.Cout; c = k - t * g.Cout; pix = m; C = g.Cout;
-    }
-    const int x = pix % g.W, rest = pix / g.W, y = rest % g.H, b = rest / g.H;
+    if (p >= np) return PixIdx{0, -(1 << 20), 0};
+    const int x = p % g.W, rest = p / g.W, y = rest % g.H, b = rest / g.H;
+    return PixIdx{b * g.H, y, x};
+}
+
+__device__ __forceinline__ TapIdx tap_index(const GemmArgs &g, int mode, int q, int nq)
+{
+    if (q >= nq) return TapIdx{-(1 << 20), 0, 0};   // with an absent pixel too, still far outside
+    const int C = mode == G_BWD_DATA ? g.Cout : g.Cin;
+    const int t = q / C, c = q - t * C;
     const int ky = t / 3 - 1, kx = t % 3 - 1;
-    // forward and weight grad read in[y + ky][x + kx]; input grad reads gout[y - ky][x - kx]
-    const int yy = mode == G_BWD_DATA ? y - ky : y + ky;
-    const int xx = mode == G_BWD_DATA ? x - kx : x + kx;
+    return mode == G_BWD_DATA ? TapIdx{-ky, -kx, c} : TapIdx{ky, kx, c};
+}
+
+__device__ __forceinline__ float load_a_idx(const GemmArgs &g, int mode, const PixIdx &p, const TapIdx &t)
+{
+    const int yy = p.y + t.dy, xx = p.x + t.dx;
     if (yy < 0 || yy >= g.H || xx < 0 || xx >= g.W) return 0.f;
-    return g.a_src[((size_t)(b * g.H + yy) * g.W + xx) * C + c];
+    const int C = mode == G_BWD_DATA ? g.Cout : g.Cin;
+    return g.a_src[((size_t)(p.bh + yy) * g.W + xx) * C + t.c];
 }
 
 // B[k][n] of the GEMM
@@ -117,10 +133,23 @@ __global__ void __launch_bounds__(kT) gemm_f32_kernel(GemmArgs g)
     int k0, k1;
     k_range(g, MODE, k0, k1);
     float acc[4][4] = {};
+    constexpr bool kRowPix = MODE != G_BWD_W;
+    __shared__ PixIdx s_pix[kRowPix ? kBM : kBK];
+    __shared__ TapIdx s_tap[kRowPix ? kBK : kBM];
+    if (kRowPix) {
+        if (threadIdx.x < kBM) s_pix[threadIdx.x] = pix_index(g, m0 + threadIdx.x, g.M);
+    } else {
+        if (threadIdx.x < kBM) s_tap[threadIdx.x] = tap_index(g, MODE, m0 + threadIdx.x, g.M);
+    }
     for (int kb = k0; kb < k1; kb += kBK) {
+        if (threadIdx.x < kBK) {
+            if (kRowPix) s_tap[threadIdx.x] = tap_index(g, MODE, kb + threadIdx.x, k1);
+            else s_pix[threadIdx.x] = pix_index(g, kb + threadIdx.x, k1);
+        }
+        __syncthreads();
         for (int i = threadIdx.x; i < kBM * kBK; i += kT) {
             const int mm = i / kBK, kk = i % kBK;
-            As[kk][mm] = (kb + kk < k1) ? load_a(g, MODE, m0 + mm, kb + kk) : 0.f;
+            As[kk][mm] = kRowPix ? load_a_idx(g, MODE, s_pix[mm], s_tap[kk]) : load_a_idx(g, MODE, s_pix[kk], s_tap[mm]);
         }
         for (int i = threadIdx.x; i < kBN * kBK; i += kT) {
             const int kk = i / kBN, nn = i % kBN;
@@ -166,10 +195,25 @@ __global__ void __launch_bounds__(kT) gemm_bf16_kernel(GemmArgs g)
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // A's row indices are fixed for the tile; its column indices per K step
+    constexpr bool kRowPix = MODE != G_BWD_W;
+    __shared__ PixIdx s_pix[kRowPix ? kBM : kBK];
+    __shared__ TapIdx s_tap[kRowPix ? kBK : kBM];
+    if (kRowPix) {
+        if (threadIdx.x < kBM) s_pix[threadIdx.x] = pix_index(g, m0 + threadIdx.x, g.M);
+    } else {
+        if (threadIdx.x < kBM) s_tap[threadIdx.x] = tap_index(g, MODE, m0 + threadIdx.x, g.M);
+    }
     for (int kb = k0; kb < k1; kb += kBK) {
+        if (threadIdx.x < kBK) {
+            if (kRowPix) s_tap[threadIdx.x] = tap_index(g, MODE, kb + threadIdx.x, k1);
+            else s_pix[threadIdx.x] = pix_index(g, kb + threadIdx.x, k1);
+        }
+        __syncthreads();
         for (int i = threadIdx.x; i < kBM * kBK; i += kT) {
             const int mm = i / kBK, kk = i % kBK;
-            As[mm][kk] = (__bf16)((kb + kk < k1) ? load_a(g, MODE, m0 + mm, kb + kk) : 0.f);
+            As[mm][kk] = (__bf16)(kRowPix ? load_a_idx(g, MODE, s_pix[mm], s_tap[kk])
+                                          : load_a_idx(g, MODE, s_pix[kk], s_tap[mm]));
         }
         for (int i = threadIdx.x; i < kBN * kBK; i += kT) {
             const int nn = i % kBN, kk = i / kBN;
