@@ -37,7 +37,7 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kMinPixelsPerChunk = kThreads * 4 * 4;   // smallest chunk (4 groups per thread)
+constexpr int kMinPixelsPerChunk = kThreads * 4 * 2;   // smallest chunk (2 groups per thread)
 
 struct Partial {
     float absmax;   // largest |r| in the workgroup that is > 0, else 0
@@ -370,15 +370,21 @@ size_t psf_pad_offset(size_t npix)
     return (partial_offset() + 2 * partial_count(npix) * sizeof(Partial) + 255) & ~(size_t)255;
 }
 
-// groups per thread (4 or 8); SDP_HOGBOM_GROUPS overrides the default for tuning
-int hogbom_groups()
+// 16-byte groups per thread per chunk (2, 4 or 8): the most that still
+// gives every CU two chunks, so small images spread over the whole GPU;
+// SDP_HOGBOM_GROUPS overrides the choice for tuning
+int hogbom_groups(size_t npix, int cus)
 {
-    static int g = 0;
-    if (g == 0) {
+    static int env = -1;
+    if (env < 0) {
         const char *e = getenv("SDP_HOGBOM_GROUPS");
-        g = (e && atoi(e) == 4) ? 4 : 8;
+        const int v = e ? atoi(e) : 0;
+        env = (v == 2 || v == 4 || v == 8) ? v : 0;
     }
-    return g;
+    if (env > 0) return env;
+    for (int g = 8; g > 2; g >>= 1)
+        if (npix / ((size_t)kThreads * 4 * g) >= 2 * (size_t)cus) return g;
+    return 2;
 }
 
 int compute_units()
@@ -440,8 +446,8 @@ int sdp_hogbom_ex(int G, int n_minor, int max_sources, float *d_residual, const 
     // more workgroups resident; the workgroup count is what the CUs hold at
     // once (no more than there are chunks), so one resident wave covers the
     // image and no second, partial wave trails it
-    const int groups = hogbom_groups();
-    auto kern = groups == 4 ? hogbom_step_kernel<4> : hogbom_step_kernel<8>;
+    const int groups = hogbom_groups(npix, compute_units());
+    auto kern = groups == 2 ? hogbom_step_kernel<2> : groups == 4 ? hogbom_step_kernel<4> : hogbom_step_kernel<8>;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
