@@ -294,10 +294,19 @@ __device__ __forceinline__ GroupVis decode_vis(const RawVis &x, const int2 *__re
     return v;
 }
 
-__device__ __forceinline__ int wave_max(int v)
+// largest v over the wavefront, for v equal on the L = 2^lane_shift lanes of
+// each group: only the exchanges between groups are needed
+__device__ __forceinline__ int groups_max(int v, int lane_shift)
 {
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
+    for (int off = 32; off >= (1 << lane_shift); off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
     return v;
+}
+
+// n / d for 0 <= n <= 64, 1 <= d <= 64: the float quotient is within 1e-5 of
+// n / d, and n / d sits at least 1 / 64 below the next integer
+__device__ __forceinline__ int small_div(int n, float rcp_d)
+{
+    return (int)((float)n * rcp_d + 0.0078125f);
 }
 
 template <bool COMP> struct AccT;
@@ -384,37 +393,45 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
         // each tap fetches both with a lane shuffle.  Offsets of absent rows
         // and columns lie past the table, so their loads read zeros.
         const int ntap = v.nu * v.nv;
-        const int iters = wave_max((ntap + L - 1) >> a.lane_shift);   // uniform trip count
+        const int iters = groups_max((ntap + L - 1) >> a.lane_shift, a.lane_shift);   // uniform trip count
         const int ky_own = c < v.nv ? 8 * row_part(kernel_tap(cell_v0 + c, v.gy, os), a.OS, a.os_shift, v.pt.y)
                                     : kOob;
         const int kx_own = c < v.nu
                                ? 8 * (v.pt.x + col_part(kernel_tap(cell_u0 + c, v.gx, os), a.OS, a.os_shift, v.pt.y))
                                : kOob;
         const int gbase = g << a.lane_shift;
-        // this lane's first tap (row, column) and the step of L taps
+        // this lane's first tap (row, column) and the step of L taps, kept
+        // as lane-shuffle byte addresses of its row and column (ra, ca) and
+        // a window cell offset (ix), advanced by additions only
         const int nu1 = max(v.nu, 1);
-        int trow = c / nu1, tcol = c - (c / nu1) * nu1;
-        const int drow = L / nu1, dcol = L - (L / nu1) * nu1;
+        const float rcp_nu = __builtin_amdgcn_rcpf((float)nu1);
+        const int trow = small_div(c, rcp_nu), tcol = c - trow * nu1;
+        const int drow = small_div(L, rcp_nu), dcol = L - drow * nu1;
+        const int rend = (gbase + v.nv) << 2, cend = (gbase + nu1) << 2;
+        const int dra = drow << 2, dca = dcol << 2, dix = drow * stride + dcol;
+        const int ra0 = (gbase + trow) << 2, ca0 = (gbase + tcol) << 2, ix0 = trow * stride + tcol;
 
         float2 res = make_float2(0.f, 0.f);
         if (kDegrid) {
             float sr = 0.f, si = 0.f;
-            const int wbase = v.lv0 * stride + v.lu0;
-            int row = trow, col = tcol;
+            int ra = ra0, ca = ca0, ix = v.lv0 * stride + v.lu0 + ix0;
             for (int j0 = 0; j0 < iters; j0 += kRowBatch) {
                 float2 kk[kRowBatch], w[kRowBatch];
 #pragma unroll
                 for (int q = 0; q < kRowBatch; ++q) {
-                    const bool live = row < v.nv;
-                    const int ky = __builtin_amdgcn_ds_bpermute((gbase + min(row, L - 1)) << 2, ky_own);
-                    const int kx = __builtin_amdgcn_ds_bpermute((gbase + col) << 2, kx_own);
+                    const bool live = ra < rend;
+                    const int ky = __builtin_amdgcn_ds_bpermute(ra, ky_own);
+                    const int kx = __builtin_amdgcn_ds_bpermute(ca, kx_own);
                     kk[q] = tab_load(kd_rsrc, live ? (unsigned)(kx + ky) : (unsigned)kOob);
-                    w[q] = live ? win[wbase + row * stride + col] : make_float2(0.f, 0.f);
-                    col += dcol;
-                    row += drow;
-                    if (col >= nu1) {
-                        col -= nu1;
-                        ++row;
+                    const float2 wv = win[min(ix, cells - 1)];   // in the window for live taps
+                    w[q] = live ? wv : make_float2(0.f, 0.f);
+                    ra += dra;
+                    ca += dca;
+                    ix += dix;
+                    if (ca >= cend) {
+                        ca -= nu1 << 2;
+                        ra += 4;
+                        ix += stride - nu1;
                     }
                 }
 #pragma unroll
@@ -450,23 +467,24 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
             if (MODE == MODE_GRID && v.ok) res = cur_val;
             const float vr = res.x, vi = res.y, vrc = v.conj * res.x, vic = v.conj * res.y;
             const int spare_i = C * cstride + lane;
-            const int cbase = g * cstride + v.lv0 * stride + v.lu0;
-            int row = trow, col = tcol;
+            int ra = ra0, ca = ca0, ix = g * cstride + v.lv0 * stride + v.lu0 + ix0;
             for (int j0 = 0; j0 < iters; j0 += kRowBatch) {
                 float2 kk[kRowBatch];
                 int ci[kRowBatch];
 #pragma unroll
                 for (int q = 0; q < kRowBatch; ++q) {
-                    const bool live = row < v.nv;
-                    const int ky = __builtin_amdgcn_ds_bpermute((gbase + min(row, L - 1)) << 2, ky_own);
-                    const int kx = __builtin_amdgcn_ds_bpermute((gbase + col) << 2, kx_own);
+                    const bool live = ra < rend;
+                    const int ky = __builtin_amdgcn_ds_bpermute(ra, ky_own);
+                    const int kx = __builtin_amdgcn_ds_bpermute(ca, kx_own);
                     kk[q] = tab_load(kg_rsrc, live ? (unsigned)(kx + ky) : (unsigned)kOob);
-                    ci[q] = live ? cbase + row * stride + col : spare_i;
-                    col += dcol;
-                    row += drow;
-                    if (col >= nu1) {
-                        col -= nu1;
-                        ++row;
+                    ci[q] = live ? ix : spare_i;
+                    ra += dra;
+                    ca += dca;
+                    ix += dix;
+                    if (ca >= cend) {
+                        ca -= nu1 << 2;
+                        ra += 4;
+                        ix += stride - nu1;
                     }
                 }
                 // distinct cells per lane (or the lane's spare cell, which
