@@ -100,8 +100,7 @@ struct PlanArgs {
 };
 
 __global__ void plan_records_kernel(const float *uvw, const int2 *supports, const int *plane_class, PlanArgs a,
-                                    VisRec *recs, uint32_t *keys, uint32_t *vals, unsigned int *key_count,
-                                    unsigned long long *key_taps)
+                                    VisRec *recs, uint32_t *keys, uint32_t *vals)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -148,8 +147,39 @@ __global__ void plan_records_kernel(const float *uvw, const int2 *supports, cons
     recs[i] = r;
     const uint32_t key = (uint32_t)(cls * a.ntiles + (va / a.tile) * a.ntx + (ua / a.tile));
     keys[i] = key;
-    atomicAdd(&key_count[key], 1u);
-    atomicAdd(&key_taps[key], (unsigned long long)(nu * nv));
+}
+
+// Per-key visibility and tap counts from the sorted keys: a run of equal
+// keys is one (class, tile) list.  The first kernel writes each run's start
+// and every record's tap count; after an inclusive scan of the tap counts the
+// second kernel, at each run's last record, forms the run's length and tap
+// total.  Dropped records (key == nkeys) sort last and are skipped.
+__global__ void key_run_start_kernel(const uint32_t *keys, const VisRec *recs, int n, int nkeys,
+                                     unsigned int *key_first, unsigned long long *taps)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = keys[i];
+    if (k >= (uint32_t)nkeys) {
+        taps[i] = 0;
+        return;
+    }
+    const uint32_t m = recs[i].meta;
+    taps[i] = (unsigned long long)((m & 0xffu) * ((m >> 8) & 0xffu));
+    if (i == 0 || keys[i - 1] != k) key_first[k] = (unsigned int)i;
+}
+
+__global__ void key_run_end_kernel(const uint32_t *keys, int n, int nkeys, const unsigned int *key_first,
+                                   const unsigned long long *tap_scan, unsigned int *key_count,
+                                   unsigned long long *key_taps)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = keys[i];
+    if (k >= (uint32_t)nkeys || (i + 1 < n && keys[i + 1] == k)) return;
+    const unsigned int f = key_first[k];
+    key_count[k] = (unsigned int)(i + 1) - f;
+    key_taps[k] = tap_scan[i] - (f > 0 ? tap_scan[f - 1] : 0ull);
 }
 
 __global__ void gather_records_kernel(const VisRec *recs, const uint32_t *order, int n, VisRec *out)
@@ -947,7 +977,7 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     const size_t n = (size_t)std::max(n_vis, 1);
     VisRec *recs_tmp = nullptr;
     uint32_t *keys = nullptr, *keys_sorted = nullptr, *vals = nullptr;
-    unsigned int *key_count = nullptr;
+    unsigned int *key_count = nullptr, *key_first = nullptr;
     unsigned long long *key_taps = nullptr;
     PLAN_TRY(hipMalloc(&recs_tmp, sizeof(VisRec) * n));
     PLAN_TRY(hipMalloc(&plan->recs, sizeof(VisRec) * n));
@@ -957,6 +987,7 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     PLAN_TRY(hipMalloc(&vals, sizeof(uint32_t) * n));
     PLAN_TRY(hipMalloc(&key_count, sizeof(unsigned int) * nkeys));
     PLAN_TRY(hipMalloc(&key_taps, sizeof(unsigned long long) * nkeys));
+    PLAN_TRY(hipMalloc(&key_first, sizeof(unsigned int) * nkeys));
     PLAN_TRY(hipMemsetAsync(key_count, 0, sizeof(unsigned int) * nkeys, st));
     PLAN_TRY(hipMemsetAsync(key_taps, 0, sizeof(unsigned long long) * nkeys, st));
 
@@ -964,7 +995,7 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
         PlanArgs a{n_vis, p.grid_size, p.oversampling, p.n_kernels, p.snap_mode, plan->tile, plan->ntx, plan->nty,
                    ntiles, nkeys, p.uv_scale, p.w_scale};
         plan_records_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(d_uvw, d_supports, d_pclass, a, recs_tmp, keys,
-                                                                    vals, key_count, key_taps);
+                                                                    vals);
         PLAN_TRY(hipGetLastError());
         int end_bit = 1;
         while ((1u << end_bit) <= (unsigned)nkeys) ++end_bit;
@@ -980,6 +1011,24 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
         PLAN_TRY(se);
         gather_records_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(recs_tmp, plan->perm, n_vis, plan->recs);
         PLAN_TRY(hipGetLastError());
+
+        // recs_tmp (16 bytes a record) is free again: its two halves hold
+        // the tap counts and their inclusive scan
+        unsigned long long *tap_count = reinterpret_cast<unsigned long long *>(recs_tmp);
+        unsigned long long *tap_scan = tap_count + n_vis;
+        key_run_start_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(keys_sorted, plan->recs, n_vis, nkeys,
+                                                                     key_first, tap_count);
+        PLAN_TRY(hipGetLastError());
+        size_t scan_bytes = 0;
+        PLAN_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, scan_bytes, tap_count, tap_scan, n_vis, st));
+        PLAN_TRY(hipMalloc(&temp, scan_bytes));
+        se = hipcub::DeviceScan::InclusiveSum(temp, scan_bytes, tap_count, tap_scan, n_vis, st);
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(temp);
+        PLAN_TRY(se);
+        key_run_end_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(keys_sorted, n_vis, nkeys, key_first, tap_scan,
+                                                                   key_count, key_taps);
+        PLAN_TRY(hipGetLastError());
     }
 
     std::vector<unsigned int> counts(nkeys);
@@ -993,6 +1042,7 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     (void)hipFree(vals);
     (void)hipFree(key_count);
     (void)hipFree(key_taps);
+    (void)hipFree(key_first);
     (void)hipFree(d_supports);
     (void)hipFree(d_pclass);
 
