@@ -53,6 +53,7 @@ constexpr int kClassMaxH[kNumClasses] = {8, 12, 16, 20, 24, 32};
 // LDS of one workgroup (one wavefront): accumulators, or accumulators plus
 // the model-grid window for the fused kernel
 constexpr size_t kMaxLds = 152 * 1024;
+constexpr int kLdsPlanes = 256;   // plane tables up to this many planes are read from LDS
 
 enum { MODE_GRID = 0, MODE_DEGRID = 1, MODE_DEGRID_GRID = 2 };
 
@@ -207,6 +208,8 @@ struct ChunkArgs {
     int copies;       // lane groups (and sub-grid copies) per wavefront
     int os_shift;     // log2 OS when OS is a power of two, else -1
     int tab_bytes;    // size of one phase-major kernel table
+    int lds_planes;   // planes whose (offset, row length) entries are copied to LDS (0: read from memory)
+    int pt_lds_off;   // byte offset of that copy in LDS
 };
 
 struct ChunkIO {
@@ -307,8 +310,8 @@ __device__ __forceinline__ RawVis load_vis(const VisRec *__restrict__ recs, cons
     return x;
 }
 
-__device__ __forceinline__ GroupVis decode_vis(const RawVis &x, const int2 *__restrict__ plane_tab, bool ok,
-                                               int tile_u0, int tile_v0)
+__device__ __forceinline__ GroupVis decode_vis(const RawVis &x, const int2 *__restrict__ plane_tab,
+                                               const int2 *lds_pt, bool pt_in_lds, bool ok, int tile_u0, int tile_v0)
 {
     GroupVis v;
     v.ok = ok;
@@ -317,7 +320,8 @@ __device__ __forceinline__ GroupVis decode_vis(const RawVis &x, const int2 *__re
     v.lv0 = (int)(x.r.start >> 16) - tile_v0;
     v.nu = ok ? (int)(x.r.meta & 0xffu) : 0;
     v.nv = ok ? (int)((x.r.meta >> 8) & 0xffu) : 0;
-    v.pt = ok ? plane_tab[(x.r.meta >> 16) & 0x7fffu] : make_int2(0, 0);
+    const int plane = (x.r.meta >> 16) & 0x7fff;
+    v.pt = !ok ? make_int2(0, 0) : pt_in_lds ? lds_pt[plane] : plane_tab[plane];
     v.conj = (x.r.meta & 0x80000000u) ? -1.0f : 1.0f;
     v.gx = x.r.gx;
     v.gy = x.r.gy;
@@ -388,6 +392,10 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
     if (kGrid) {
         for (int i = lane; i < C * cstride + kWave; i += kWave) acc_zero(acc[i]);
     }
+    // the plane table in LDS: a visibility's plane lookup then costs no memory latency
+    int2 *lds_pt = reinterpret_cast<int2 *>(reinterpret_cast<char *>(lds4) + a.pt_lds_off);
+    const bool pt_in_lds = a.lds_planes > 0;
+    for (int i = lane; i < a.lds_planes; i += kWave) lds_pt[i] = plane_tab[i];
     if (kDegrid) {
         for (int i = lane; i < S * S; i += kWave) {
             const int row = i / S, col = i - row * S;
@@ -413,7 +421,7 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
             nxt = load_vis(recs, perm, ch.y + k + C, ok_next);
             if (MODE != MODE_DEGRID && ok_next) nxt_val = (MODE == MODE_GRID ? io.vis : io.measured)[nxt.o];
         }
-        const GroupVis v = decode_vis(cur, plane_tab, ok, tile_u0, tile_v0);
+        const GroupVis v = decode_vis(cur, plane_tab, lds_pt, pt_in_lds, ok, tile_u0, tile_v0);
         const int cell_u0 = tile_u0 + v.lu0 - gc, cell_v0 = tile_v0 + v.lv0 - gc;
         // The group's L lanes walk the footprint's nu x nv taps in row-major
         // order, L at a time: lane c takes taps c, c + L, c + 2L, ...  Lane
@@ -783,7 +791,8 @@ int class_lane_shift(int h, int snap_mode)
     return shift;
 }
 
-size_t lds_bytes(int mode, int S, int copies, bool comp)
+// LDS of a chunk workgroup without the plane table (8-byte aligned)
+size_t lds_window_bytes(int mode, int S, int copies, bool comp)
 {
     const size_t cells = (size_t)S * (S + 1);
     const size_t win = mode == MODE_GRID ? 0 : ((cells + 1) & ~(size_t)1) * sizeof(float2);
@@ -792,26 +801,42 @@ size_t lds_bytes(int mode, int S, int copies, bool comp)
     return win + ((size_t)copies * (cells + 2) + kWave) * acc;
 }
 
+// planes held in LDS: all of them when there are at most kLdsPlanes
+int lds_planes(int n_kernels) { return n_kernels <= kLdsPlanes ? n_kernels : 0; }
+
+size_t lds_bytes(int mode, int S, int copies, bool comp, int n_kernels)
+{
+    return lds_window_bytes(mode, S, copies, comp) + sizeof(int2) * (size_t)lds_planes(n_kernels);
+}
+
 // sub-grid copies that fit LDS for a class (a power of two)
-int class_copies(int mode, int S, int col_shift, bool comp)
+int class_copies(int mode, int S, int col_shift, bool comp, int n_kernels)
 {
     int c = kWave >> col_shift;
-    while (c > 1 && lds_bytes(mode, S, c, comp) > kMaxLds) c >>= 1;
+    while (c > 1 && lds_bytes(mode, S, c, comp, n_kernels) > kMaxLds) c >>= 1;
     return c;
 }
 
-// copies per wavefront when gridding: more copies grid more visibilities at
-// once, fewer leave LDS for more resident wavefronts.  SDP_GRID_COPIES
-// (1, 2 or 4) overrides the default for tuning.
-int grid_copies(bool comp)
+// copies per wavefront when gridding class c: more copies grid more
+// visibilities at once, fewer leave LDS for more resident wavefronts.
+// SDP_GRID_COPIES overrides the default for tuning: one value (1, 2 or 4) for
+// every class, or a comma-separated list, one value per class.
+int grid_copies(bool comp, int c)
 {
-    static int env = -1;
-    if (env < 0) {
+    static int env[kNumClasses] = {-1};
+    if (env[0] < 0) {
+        for (int k = 0; k < kNumClasses; ++k) env[k] = 0;
         const char *e = getenv("SDP_GRID_COPIES");
-        const int v = e ? atoi(e) : 0;
-        env = (v == 1 || v == 2 || v == 4) ? v : 0;
+        int k = 0, last = 0;
+        while (e && *e && k < kNumClasses) {
+            const int v = atoi(e);
+            last = env[k++] = (v == 1 || v == 2 || v == 4) ? v : 0;
+            e = strchr(e, ',');
+            if (e) ++e;
+        }
+        for (; k < kNumClasses; ++k) env[k] = last;
     }
-    if (env > 0) return env;
+    if (env[c] > 0) return env[c];
     return comp ? 1 : 2;   // measured at 16.7M visibilities, 32 w-planes, h 4-16
 }
 
@@ -828,7 +853,7 @@ int launch_class(const sdp_grid_plan *plan, int c, const ChunkIO &io, const uint
     const int S = plan->class_S[c];
     const int col_shift = class_lane_shift(plan->class_h[c], plan->p.snap_mode);
     int copies = kWave >> col_shift;   // one visibility per group, one row at a time
-    if (MODE != MODE_DEGRID) copies = std::min(copies, std::min(grid_copies(COMP), class_copies(MODE, S, col_shift, COMP)));
+    if (MODE != MODE_DEGRID) copies = std::min(copies, std::min(grid_copies(COMP, c), class_copies(MODE, S, col_shift, COMP, plan->p.n_kernels)));
     int lane_shift = 0;
     while ((kWave >> lane_shift) > copies) ++lane_shift;   // lanes per group = 64 / copies (copies a power of 2)
     copies = kWave >> lane_shift;
@@ -837,8 +862,9 @@ int launch_class(const sdp_grid_plan *plan, int c, const ChunkIO &io, const uint
     for (int k = 0; k < 31; ++k)
         if ((1 << k) == OS) os_shift = k;
     ChunkArgs a{plan->p.grid_size, OS, plan->tile, plan->ntx, b, lane_shift, col_shift, copies, os_shift,
-                (int)(8 * plan->table_len)};
-    chunk_kernel<MODE, COMP><<<e - b, kWave, lds_bytes(MODE, S, copies, COMP), st>>>(
+                (int)(8 * plan->table_len), lds_planes(plan->p.n_kernels),
+                (int)lds_window_bytes(MODE, S, copies, COMP)};
+    chunk_kernel<MODE, COMP><<<e - b, kWave, lds_bytes(MODE, S, copies, COMP, plan->p.n_kernels), st>>>(
         plan->recs, perm, plan->plane_tab, plan->chunks, plan->slab_off, a, io);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
@@ -936,7 +962,8 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
         plan->class_h[c] = std::max(class_h[c], 0);
         plan->smax = std::max(plan->smax, plan->class_S[c]);
         if (class_h[c] >= 0 &&
-            lds_bytes(MODE_DEGRID_GRID, plan->class_S[c], 1, p.accumulate == SDP_ACC_COMPENSATED) > kMaxLds) {
+            lds_bytes(MODE_DEGRID_GRID, plan->class_S[c], 1, p.accumulate == SDP_ACC_COMPENSATED, p.n_kernels) >
+                kMaxLds) {
             const int S = plan->class_S[c], tile = plan->tile;
             delete plan;
             return sdp::set_error(SDP_E_UNSUPPORTED, "window of %d cells (tile %d, half support %d) exceeds LDS", S,
