@@ -288,43 +288,33 @@ struct GroupVis {
     uint32_t o;   // caller index
 };
 
-// the record (and caller index) of a group's next visibility, loaded one
-// iteration ahead so its latency overlaps the current visibility's taps
-struct RawVis {
-    VisRec r;
-    uint32_t o;
-};
-
-__device__ __forceinline__ RawVis load_vis(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm, int j,
-                                           bool ok)
+// A group's visibility, taken by lane shuffles from the lane that holds its
+// record (byte address src = 4 * lane): the wavefront loads kWave records at
+// once, one per lane, so their memory latency is paid once per kWave
+// visibilities instead of once per visibility.
+__device__ __forceinline__ int lane_take(int src, int x) { return __builtin_amdgcn_ds_bpermute(src, x); }
+__device__ __forceinline__ float lane_take(int src, float x)
 {
-    RawVis x;
-    x.r.gx = x.r.gy = 0.f;
-    x.r.start = 0u;
-    x.r.meta = 0u;
-    x.o = 0u;
-    if (ok) {
-        x.r = recs[j];
-        x.o = perm ? perm[j] : (uint32_t)j;
-    }
-    return x;
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(x)));
 }
 
-__device__ __forceinline__ GroupVis decode_vis(const RawVis &x, const int2 *__restrict__ plane_tab,
-                                               const int2 *lds_pt, bool pt_in_lds, bool ok, int tile_u0, int tile_v0)
+__device__ __forceinline__ GroupVis group_vis(const VisRec &hr, int2 hpt, uint32_t ho, int src, bool ok, int tile_u0,
+                                              int tile_v0)
 {
     GroupVis v;
     v.ok = ok;
-    v.o = x.o;
-    v.lu0 = (int)(x.r.start & 0xffffu) - tile_u0;
-    v.lv0 = (int)(x.r.start >> 16) - tile_v0;
-    v.nu = ok ? (int)(x.r.meta & 0xffu) : 0;
-    v.nv = ok ? (int)((x.r.meta >> 8) & 0xffu) : 0;
-    const int plane = (x.r.meta >> 16) & 0x7fff;
-    v.pt = !ok ? make_int2(0, 0) : pt_in_lds ? lds_pt[plane] : plane_tab[plane];
-    v.conj = (x.r.meta & 0x80000000u) ? -1.0f : 1.0f;
-    v.gx = x.r.gx;
-    v.gy = x.r.gy;
+    const uint32_t start = (uint32_t)lane_take(src, (int)hr.start);
+    const uint32_t meta = (uint32_t)lane_take(src, (int)hr.meta);
+    v.o = (uint32_t)lane_take(src, (int)ho);
+    v.lu0 = (int)(start & 0xffffu) - tile_u0;
+    v.lv0 = (int)(start >> 16) - tile_v0;
+    v.nu = ok ? (int)(meta & 0xffu) : 0;
+    v.nv = ok ? (int)((meta >> 8) & 0xffu) : 0;
+    const int ptx = lane_take(src, hpt.x), pty = lane_take(src, hpt.y);
+    v.pt = ok ? make_int2(ptx, pty) : make_int2(0, 0);
+    v.conj = (meta & 0x80000000u) ? -1.0f : 1.0f;
+    v.gx = lane_take(src, hr.gx);
+    v.gy = lane_take(src, hr.gy);
     return v;
 }
 
@@ -407,21 +397,37 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
     }
     __syncthreads();
 
+    // the value each visibility brings: the visibility to grid, or the
+    // measured value to subtract from (absent: degrid only)
+    const float2 *val_src = MODE == MODE_GRID ? io.vis : io.measured;
+    VisRec hr;            // this lane's record of the current kWave
+    int2 hpt;             // its plane's (table offset, row length)
+    float2 hval;          // its value
+    uint32_t ho;          // its caller index
     const int iters = (ch.z + C - 1) / C;
-    RawVis nxt = load_vis(recs, perm, ch.y + g, in_group && g < ch.z);
-    float2 nxt_val = make_float2(0.f, 0.f);   // the visibility to grid, or the measured value
-    if (MODE != MODE_DEGRID && in_group && g < ch.z) nxt_val = (MODE == MODE_GRID ? io.vis : io.measured)[nxt.o];
     for (int it = 0; it < iters; ++it) {
-        const int k = it * C + g;
-        const bool ok = in_group && k < ch.z;
-        const RawVis cur = nxt;
-        const float2 cur_val = nxt_val;
-        {
-            const bool ok_next = in_group && k + C < ch.z;
-            nxt = load_vis(recs, perm, ch.y + k + C, ok_next);
-            if (MODE != MODE_DEGRID && ok_next) nxt_val = (MODE == MODE_GRID ? io.vis : io.measured)[nxt.o];
+        const int first = it * C;                  // group 0's visibility in the chunk
+        const int slot = first & (kWave - 1);      // C divides kWave
+        if (slot == 0) {
+            const int j = first + lane;
+            hr.gx = hr.gy = 0.f;
+            hr.start = hr.meta = 0u;
+            hpt = make_int2(0, 0);
+            hval = make_float2(0.f, 0.f);
+            ho = 0u;
+            if (j < ch.z) {
+                hr = recs[ch.y + j];
+                ho = perm ? perm[ch.y + j] : (uint32_t)(ch.y + j);
+                if (val_src) hval = val_src[ho];
+                const int plane = (hr.meta >> 16) & 0x7fff;
+                hpt = pt_in_lds ? lds_pt[plane] : plane_tab[plane];
+            }
         }
-        const GroupVis v = decode_vis(cur, plane_tab, lds_pt, pt_in_lds, ok, tile_u0, tile_v0);
+        const int k = first + g;
+        const bool ok = in_group && k < ch.z;
+        const int src = (slot + g) << 2;
+        const GroupVis v = group_vis(hr, hpt, ho, src, ok, tile_u0, tile_v0);
+        const float2 cur_val = make_float2(lane_take(src, hval.x), lane_take(src, hval.y));
         const int cell_u0 = tile_u0 + v.lu0 - gc, cell_v0 = tile_v0 + v.lv0 - gc;
         // The group's L lanes walk the footprint's nu x nv taps in row-major
         // order, L at a time: lane c takes taps c, c + L, c + 2L, ...  Lane
@@ -488,10 +494,7 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
             if (MODE == MODE_DEGRID) {
                 if (v.ok && c == 0) {
                     float2 out = make_float2(sr, si);
-                    if (io.measured) {
-                        const float2 mv = io.measured[v.o];
-                        out = make_float2(mv.x - sr, mv.y - si);
-                    }
+                    if (io.measured) out = make_float2(cur_val.x - sr, cur_val.y - si);
                     io.vis_out[v.o] = out;
                 }
                 continue;
