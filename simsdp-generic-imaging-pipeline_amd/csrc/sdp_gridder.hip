@@ -36,6 +36,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -45,11 +46,10 @@
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kRowBatch = 4;   // footprint rows per lane whose kernel samples are loaded together
 // half-support limits of the support classes; a class's window is
 // tile + 2 * (largest half support present in it)
 constexpr int kNumClasses = 6;
-constexpr int kClassMaxH[kNumClasses] = {8, 12, 16, 20, 24, 32};
+constexpr int kClassMaxH[kNumClasses] = {8, 12, 16, 20, 24, 31};
 // LDS of one workgroup (one wavefront): accumulators, or accumulators plus
 // the model-grid window for the fused kernel
 constexpr size_t kMaxLds = 152 * 1024;
@@ -225,10 +225,9 @@ struct ChunkIO {
 // Lane groups.  A wavefront splits into C groups of L lanes (L a power of
 // two no smaller than the class's largest footprint extent, C = 64 / L or
 // fewer when LDS runs short).  Group g takes the chunk's visibilities
-// g, g + C, g + 2C, ... in order; within a visibility the group's lanes walk
-// the footprint's taps in row-major order, L at a time, so a group's lanes
-// never touch one cell at the same time and no lane idles while taps remain.
-// Each group accumulates into its own copy of the sub-grid; the copies are
+// g, g + C, g + 2C, ... in order; within a visibility the group's lanes
+// cover floor(L / nu) footprint rows at a time, one lane per tap, so a
+// group's lanes never touch one cell at the same time.  Each group accumulates into its own copy of the sub-grid; the copies are
 // summed in group order when the slab is written, so the result does not
 // depend on timing.  The kernel-table offsets of footprint row r and column
 // u are computed once per visibility, by lanes r and u of the group, and
@@ -326,8 +325,46 @@ __device__ __forceinline__ int groups_max(int v, int lane_shift)
     return v;
 }
 
-// n / d for 0 <= n <= 64, 1 <= d <= 64: the float quotient is within 1e-5 of
-// n / d, and n / d sits at least 1 / 64 below the next integer
+// n / d for 0 <= n <= 1024, 1 <= d <= 64: the float quotient is within 3e-4
+// of n / d, and a non-integer n / d sits at least 1 / 64 below the next
+// integer, so the added 1 / 128 floors correctly
+// LDS byte addresses: loads and stores through them need no base addition
+typedef __attribute__((address_space(3))) char lds_char_t;
+__device__ __forceinline__ uint32_t lds_addr(const void *p)
+{
+    return (uint32_t)(uintptr_t)(const lds_char_t *)p;
+}
+typedef float v4f_lds_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) v2f_t lds_v2f_t;
+typedef __attribute__((address_space(3))) v4f_lds_t lds_v4f_t;
+template <typename T> __device__ __forceinline__ T lds_load(int a);
+template <> __device__ __forceinline__ float2 lds_load<float2>(int a)
+{
+    const v2f_t v = *(const lds_v2f_t *)(uintptr_t)(uint32_t)a;
+    return make_float2(v.x, v.y);
+}
+template <> __device__ __forceinline__ float4 lds_load<float4>(int a)
+{
+    const v4f_lds_t v = *(const lds_v4f_t *)(uintptr_t)(uint32_t)a;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void lds_store(int a, const float2 &v)
+{
+    v2f_t t;
+    t.x = v.x;
+    t.y = v.y;
+    *(lds_v2f_t *)(uintptr_t)(uint32_t)a = t;
+}
+__device__ __forceinline__ void lds_store(int a, const float4 &v)
+{
+    v4f_lds_t t;
+    t.x = v.x;
+    t.y = v.y;
+    t.z = v.z;
+    t.w = v.w;
+    *(lds_v4f_t *)(uintptr_t)(uint32_t)a = t;
+}
+
 __device__ __forceinline__ int small_div(int n, float rcp_d)
 {
     return (int)((float)n * rcp_d + 0.0078125f);
@@ -375,7 +412,7 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
         __builtin_amdgcn_make_buffer_rsrc((void *)io.kd, (short)0, a.tab_bytes, 0x00020000);
 
     // LDS: [model window, S x stride float2][C sub-grid copies][one spare cell per lane]
-    const int win_cells = kDegrid ? (cells + 1) & ~1 : 0;
+    const int win_cells = kDegrid ? (cells + 2) & ~1 : 0;   // the window and a zero cell
     float2 *win = reinterpret_cast<float2 *>(lds4);
     Acc *acc = reinterpret_cast<Acc *>(win + win_cells);
     const int cstride = cells + 2;
@@ -394,6 +431,7 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
             if (gy < a.G && gx < a.G) v = io.grid[(size_t)gy * a.G + gx];
             win[row * stride + col] = v;
         }
+        if (lane == 0) win[cells] = make_float2(0.f, 0.f);
     }
     __syncthreads();
 
@@ -429,63 +467,74 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
         const GroupVis v = group_vis(hr, hpt, ho, src, ok, tile_u0, tile_v0);
         const float2 cur_val = make_float2(lane_take(src, hval.x), lane_take(src, hval.y));
         const int cell_u0 = tile_u0 + v.lu0 - gc, cell_v0 = tile_v0 + v.lv0 - gc;
-        // The group's L lanes walk the footprint's nu x nv taps in row-major
-        // order, L at a time: lane c takes taps c, c + L, c + 2L, ...  Lane
-        // r (r < nv) holds the kernel-table byte offset of footprint row r
-        // (std_gridding's abs((int)ROUND((v - grid_pos) * OS)), degridgrid.c:110,
-        // in the phase-major layout) and lane u (u < nu) that of column u;
-        // each tap fetches both with a lane shuffle.  Offsets of absent rows
-        // and columns lie past the table, so their loads read zeros.
-        const int ntap = v.nu * v.nv;
-        const int iters = groups_max((ntap + L - 1) >> a.lane_shift, a.lane_shift);   // uniform trip count
+        // Lane r (r < nv) of the group holds the kernel-table byte offset of
+        // footprint row r (std_gridding's abs((int)ROUND((v - grid_pos) * OS)),
+        // degridgrid.c:110, in the phase-major layout) and lane u (u < nu) that
+        // of column u.  Offsets of absent rows and columns lie past the table,
+        // so their loads read zeros.
         const int ky_own = c < v.nv ? 8 * row_part(kernel_tap(cell_v0 + c, v.gy, os), a.OS, a.os_shift, v.pt.y)
                                     : kOob;
         const int kx_own = c < v.nu
                                ? 8 * (v.pt.x + col_part(kernel_tap(cell_u0 + c, v.gx, os), a.OS, a.os_shift, v.pt.y))
                                : kOob;
+        // The group's L lanes cover R = L / nu footprint rows at a time: lane
+        // c takes column c % nu of rows c / nu, c / nu + R, c / nu + 2R, ...
+        // (lanes past R * nu idle), so its column, and that column's table
+        // offset, stay fixed and each step advances the row by R.  The row
+        // offset comes by a lane shuffle from the row's lane (byte address ra).
         const int gbase = g << a.lane_shift;
-        // this lane's first tap (row, column) and the step of L taps, kept
-        // as lane-shuffle byte addresses of its row and column (ra, ca) and
-        // a window cell offset (ix), advanced by additions only
-        const int nu1 = max(v.nu, 1);
+        const int nu1 = max(v.nu, 1);                     // nu <= 2h + 1 <= L
         const float rcp_nu = __builtin_amdgcn_rcpf((float)nu1);
-        const int trow = small_div(c, rcp_nu), tcol = c - trow * nu1;
-        const int drow = small_div(L, rcp_nu), dcol = L - drow * nu1;
-        const int rend = (gbase + v.nv) << 2, cend = (gbase + nu1) << 2;
-        const int dra = drow << 2, dca = dcol << 2, dix = drow * stride + dcol;
-        const int ra0 = (gbase + trow) << 2, ca0 = (gbase + tcol) << 2, ix0 = trow * stride + tcol;
+        const int R = max(small_div(L, rcp_nu), 1);
+        const int r0 = small_div(c, rcp_nu), col = c - __mul24(r0, nu1);
+        const int rend = (gbase + v.nv) << 2;
+        const int ra0 = r0 < R ? (gbase + r0) << 2 : rend;
+        // the largest step count over the groups: wavefront-uniform
+        const int steps = __builtin_amdgcn_readfirstlane(
+            groups_max(small_div(v.nv + R - 1, __builtin_amdgcn_rcpf((float)R)), a.lane_shift));
+        const int kx = __builtin_amdgcn_ds_bpermute((gbase + col) << 2, kx_own);
+        const int dra = R << 2;
+        const int ix0 = __mul24(v.lv0 + r0, stride) + v.lu0 + col;   // window cell of the first tap
+        const uint32_t lds0 = lds_addr(lds4);
 
         float2 res = make_float2(0.f, 0.f);
         if (kDegrid) {
+            // window byte offsets; dead taps read the zero cell past the window
+            const int dwb = __mul24(R, stride) * (int)sizeof(float2);
+            const int zero_b = (int)lds0 + cells * (int)sizeof(float2);
             float sr = 0.f, si = 0.f;
-            int ra = ra0, ca = ca0, ix = v.lv0 * stride + v.lu0 + ix0;
-            for (int j0 = 0; j0 < iters; j0 += kRowBatch) {
-                float2 kk[kRowBatch], w[kRowBatch];
+            int ra = ra0, wb = (int)lds0 + ix0 * (int)sizeof(float2);
+            auto batch = [&](auto nb) {
+                constexpr int N = decltype(nb)::value;
+                float2 kk[N];
+                int wi[N];
 #pragma unroll
-                for (int q = 0; q < kRowBatch; ++q) {
+                for (int q = 0; q < N; ++q) {
                     const bool live = ra < rend;
                     const int ky = __builtin_amdgcn_ds_bpermute(ra, ky_own);
-                    const int kx = __builtin_amdgcn_ds_bpermute(ca, kx_own);
                     kk[q] = tab_load(kd_rsrc, live ? (unsigned)(kx + ky) : (unsigned)kOob);
-                    const float2 wv = win[min(ix, cells - 1)];   // in the window for live taps
-                    w[q] = live ? wv : make_float2(0.f, 0.f);
+                    wi[q] = live ? wb : zero_b;
                     ra += dra;
-                    ca += dca;
-                    ix += dix;
-                    if (ca >= cend) {
-                        ca -= nu1 << 2;
-                        ra += 4;
-                        ix += stride - nu1;
-                    }
+                    wb += dwb;
                 }
+                float2 w[N];
 #pragma unroll
-                for (int q = 0; q < kRowBatch; ++q) {
+                for (int q = 0; q < N; ++q) w[q] = lds_load<float2>(wi[q]);
+#pragma unroll
+                for (int q = 0; q < N; ++q) {
                     // grid * (k.x, conj * k.y); idle taps add exact zeros
                     const float kyc = v.conj * kk[q].y;
                     sr = fmaf(w[q].x, kk[q].x, fmaf(-w[q].y, kyc, sr));
                     si = fmaf(w[q].y, kk[q].x, fmaf(w[q].x, kyc, si));
                 }
+            };
+            int j = 0;
+            for (; j + 4 <= steps; j += 4) batch(std::integral_constant<int, 4>{});
+            if (j + 2 <= steps) {
+                batch(std::integral_constant<int, 2>{});
+                j += 2;
             }
+            if (j < steps) batch(std::integral_constant<int, 1>{});
             // butterfly over the group's lanes: every lane ends with the sum
             for (int off = L >> 1; off > 0; off >>= 1) {
                 sr += __shfl_xor(sr, off, kWave);
@@ -507,42 +556,47 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
         if (kGrid) {
             if (MODE == MODE_GRID && v.ok) res = cur_val;
             const float vr = res.x, vi = res.y, vrc = v.conj * res.x, vic = v.conj * res.y;
-            const int spare_i = C * cstride + lane;
-            int ra = ra0, ca = ca0, ix = g * cstride + v.lv0 * stride + v.lu0 + ix0;
-            for (int j0 = 0; j0 < iters; j0 += kRowBatch) {
-                float2 kk[kRowBatch];
-                int ci[kRowBatch];
+            // byte offsets from the start of LDS; dead taps use the lane's
+            // spare cell, which nothing reads
+            const int acc0 = (int)lds_addr(acc);
+            const int spare_b = acc0 + (C * cstride + lane) * (int)sizeof(Acc);
+            const int dab = __mul24(R, stride) * (int)sizeof(Acc);
+            int ra = ra0, ab = acc0 + (g * cstride + ix0) * (int)sizeof(Acc);
+            auto batch = [&](auto nb) {
+                constexpr int N = decltype(nb)::value;
+                float2 kk[N];
+                int ci[N];
 #pragma unroll
-                for (int q = 0; q < kRowBatch; ++q) {
+                for (int q = 0; q < N; ++q) {
                     const bool live = ra < rend;
                     const int ky = __builtin_amdgcn_ds_bpermute(ra, ky_own);
-                    const int kx = __builtin_amdgcn_ds_bpermute(ca, kx_own);
                     kk[q] = tab_load(kg_rsrc, live ? (unsigned)(kx + ky) : (unsigned)kOob);
-                    ci[q] = live ? ix : spare_i;
+                    ci[q] = live ? ab : spare_b;
                     ra += dra;
-                    ca += dca;
-                    ix += dix;
-                    if (ca >= cend) {
-                        ca -= nu1 << 2;
-                        ra += 4;
-                        ix += stride - nu1;
-                    }
+                    ab += dab;
                 }
-                // distinct cells per lane (or the lane's spare cell, which
-                // nothing reads): reads first, then writes
-                Acc cell[kRowBatch];
+                // distinct cells per lane (or the lane's spare cell): reads
+                // first, then writes
+                Acc cell[N];
 #pragma unroll
-                for (int q = 0; q < kRowBatch; ++q) cell[q] = acc[ci[q]];
+                for (int q = 0; q < N; ++q) cell[q] = lds_load<Acc>(ci[q]);
 #pragma unroll
-                for (int q = 0; q < kRowBatch; ++q) {
+                for (int q = 0; q < N; ++q) {
                     // vis * (k.x, conj * k.y), std_gridding (degridgrid.c:112-118)
                     const float px = fmaf(vr, kk[q].x, -(vic * kk[q].y));
                     const float py = fmaf(vi, kk[q].x, vrc * kk[q].y);
                     acc_add(cell[q], px, py);
                 }
 #pragma unroll
-                for (int q = kRowBatch - 1; q >= 0; --q) acc[ci[q]] = cell[q];
+                for (int q = N - 1; q >= 0; --q) lds_store(ci[q], cell[q]);
+            };
+            int j = 0;
+            for (; j + 4 <= steps; j += 4) batch(std::integral_constant<int, 4>{});
+            if (j + 2 <= steps) {
+                batch(std::integral_constant<int, 2>{});
+                j += 2;
             }
+            if (j < steps) batch(std::integral_constant<int, 1>{});
         }
     }
     if (kGrid) {
@@ -785,10 +839,14 @@ void plan_free(sdp_grid_plan *p)
 }
 
 // lanes per visibility for a class: a power of two >= the largest footprint
-// extent (2h half-open, 2h + 1 inclusive), at least 16, at most 64
+// extent, at least 16, at most 64.  The extent is 2h + 1 in both snap modes:
+// std_gridding's [ceilf(p - h), ceilf(p + h)) spans 2h cells in exact
+// arithmetic, but p - h rounded to float can fall on the integer below and
+// make it 2h + 1 (the reference loops over those cells too).
 int class_lane_shift(int h, int snap_mode)
 {
-    const int extent = snap_mode == SDP_SNAP_STD ? 2 * h : 2 * h + 1;
+    (void)snap_mode;
+    const int extent = 2 * h + 1;
     int shift = 4;
     while (shift < 6 && (1 << shift) < extent) ++shift;
     return shift;
@@ -798,7 +856,7 @@ int class_lane_shift(int h, int snap_mode)
 size_t lds_window_bytes(int mode, int S, int copies, bool comp)
 {
     const size_t cells = (size_t)S * (S + 1);
-    const size_t win = mode == MODE_GRID ? 0 : ((cells + 1) & ~(size_t)1) * sizeof(float2);
+    const size_t win = mode == MODE_GRID ? 0 : ((cells + 2) & ~(size_t)1) * sizeof(float2);
     if (mode == MODE_DEGRID) return win;
     const size_t acc = comp ? sizeof(float4) : sizeof(float2);
     return win + ((size_t)copies * (cells + 2) + kWave) * acc;
@@ -928,6 +986,11 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
         if (h > kClassMaxH[kNumClasses - 1])
             return sdp::set_error(SDP_E_UNSUPPORTED, "half support %d of plane %d exceeds %d", h, k,
                                   kClassMaxH[kNumClasses - 1]);
+        // a footprint row or column (up to 2h + 1 cells) must fit one
+        // wavefront's lanes
+        if (2 * h + 1 > kWave)
+            return sdp::set_error(SDP_E_UNSUPPORTED, "half support %d of plane %d: footprint wider than %d cells", h,
+                                  k, kWave);
         H = std::max(H, h);
         pclass[k] = class_of(h);
         class_h[pclass[k]] = std::max(class_h[pclass[k]], h);

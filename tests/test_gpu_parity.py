@@ -3,6 +3,8 @@
 Bars: integer / index / copy work is bit-exact; floating-point work is within
 the tolerance written in each test, relative to the data's peak.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -182,6 +184,74 @@ def test_degrid_weighted_accuracy_and_adjointness(hip, actors, oracle, case, rng
     lhs = np.vdot(c(gx), c(grid.astype(np.float64)))
     rhs = np.vdot(c(x.astype(np.float64)), c(got.astype(np.float64)))
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+def wide_footprint_case(G=1024, seed=3):
+    """Random visibilities plus ones whose float footprint spans 2h + 1 cells:
+    gx - h, rounded to float just past a power of two, falls on the integer
+    below, so std_gridding's [ceilf(gx - h), ceilf(gx + h)) (degridgrid.c:93,103)
+    holds one cell more than 2h.  One visibility per power-of-two edge and
+    per plane, at half supports 8, 12 and 16 (the largest of each support
+    class); half of them are wide in both axes."""
+    f32 = np.float32
+    d = sdp_synth.small_case(n_vis=600, grid_size=G, h_list=[8, 12, 16], seed=seed)
+    # uv_scale 1: a visibility's u is its float grid position
+    rand = d.uvw.copy()
+    rand[:, :2] *= np.float32(d.uv_scale)
+    d = dataclasses.replace(d, cell_size=1.0 / G, uvw=rand)
+    s = d.uv_scale
+
+    def extent(u, h):
+        g = f32(np.float64(f32(u)) * s)
+        lo, hi = np.ceil(f32(g - f32(h))), np.ceil(f32(g + f32(h)))
+        return int(hi - lo), int(lo) + G // 2, int(hi) + G // 2
+
+    def find(m, h):
+        # gx = -(m - h) + one ulp: gx - h = -(m - ulp) rounds (to even) onto -m
+        u = f32(-(m - h)) + np.spacing(f32(m - h))
+        n, lo, hi = extent(u, h)
+        return u if n == 2 * h + 1 and lo >= 0 and hi <= G else None
+
+    rows = []
+    for p, h in enumerate([8, 12, 16]):
+        w = f32(p * p / d.w_scale)
+        e = 32
+        while e + h < G // 2:
+            for m in range(e + 1, e + h):
+                u = find(m, h)
+                if u is not None:
+                    rows.append((u, u if len(rows) % 2 else f32(0.1 * e), w, h))
+                    break
+            e *= 2
+    wide = np.array([r[:3] for r in rows], np.float32)
+    for (u, v, _, h) in rows:
+        assert extent(u, h)[0] == 2 * h + 1
+    uvw = np.concatenate([wide, d.uvw]).astype(np.float32)
+    rng = np.random.default_rng(seed)
+    vis = rng.integers(-3, 4, size=(uvw.shape[0], 2)).astype(np.float32)
+    return dataclasses.replace(d, uvw=uvw, vis=vis), len(rows)
+
+
+def test_footprints_one_cell_wider_than_2h(hip, actors, oracle):
+    """Footprints of 2h + 1 cells at each support class's largest h: gridding
+    is bit-exact (integer kernels and visibilities) and kernel-weighted
+    degridding within 2e-6 of the FP64 sum over the same cells."""
+    d, n_wide = wide_footprint_case()
+    assert n_wide >= 12
+    G, cfg = d.grid_size, cfg_of(hip, d)
+    kern = integer_kernels(d.kernels.shape[0])
+    got = actors.std_gridding(G, d.oversampling, kern, d.supports, d.uvw, d.vis, cfg)
+    ref = oracle.std_gridding(G, d.oversampling, kern, d.supports, d.uvw, d.vis, cfg)
+    np.testing.assert_array_equal(got, ref)
+    grid = np.random.default_rng(4).normal(size=(G * G, 2)).astype(np.float32)
+    actors.set_degrid_mode(hip.DEGRID_WEIGHTED)
+    try:
+        deg = actors.std_degridding(G, d.oversampling, d.degrid_kernels, d.supports, grid, d.uvw, cfg)
+    finally:
+        actors.set_degrid_mode(hip.DEGRID_NEAREST)
+    ref = oracle.degrid_weighted_f64(G, d.oversampling, d.degrid_kernels, d.supports, grid, d.uvw, d.uv_scale,
+                                     d.w_scale)
+    assert np.abs(deg - ref).max() <= 2e-6 * np.abs(ref).max()
 
 
 def test_degrid_fused_subtract(hip, case, rng):
