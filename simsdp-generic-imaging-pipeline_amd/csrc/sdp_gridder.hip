@@ -47,9 +47,10 @@ namespace {
 
 constexpr int kWave = 64;
 // half-support limits of the support classes; a class's window is
-// tile + 2 * (largest half support present in it)
+// tile + 2 * (largest half support present in it).  Footprints span up to
+// 2h + 1 cells, so h <= 7 fits 16-lane groups and h <= 15 32-lane groups.
 constexpr int kNumClasses = 6;
-constexpr int kClassMaxH[kNumClasses] = {8, 12, 16, 20, 24, 31};
+constexpr int kClassMaxH[kNumClasses] = {7, 11, 15, 20, 24, 31};
 // LDS of one workgroup (one wavefront): accumulators, or accumulators plus
 // the model-grid window for the fused kernel
 constexpr size_t kMaxLds = 152 * 1024;
