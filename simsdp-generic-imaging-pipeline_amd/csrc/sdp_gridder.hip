@@ -84,8 +84,12 @@ __device__ __forceinline__ void two_sum_acc(float &s, float &c, float b)
 
 __device__ __forceinline__ int kernel_tap(int cell_abs, float pos, float os)
 {
-    // abs((int)ROUND((v - grid_pos) * OVERSAMPLING_FACTOR)), degridgrid.c:101,110
-    return abs((int)roundf(((float)cell_abs - pos) * os));
+    // abs((int)ROUND((v - grid_pos) * OVERSAMPLING_FACTOR)), degridgrid.c:101,110.
+    // ROUND takes halves away from zero, so this is roundf(|x|) =
+    // trunc(|x|) + (|x| - trunc(|x|) >= 0.5); the difference is exact.
+    const float y = fabsf(((float)cell_abs - pos) * os);
+    const float t = truncf(y);
+    return (int)t + (y - t >= 0.5f ? 1 : 0);
 }
 
 __device__ __forceinline__ float wave_sum(float v)
@@ -242,16 +246,26 @@ struct ChunkIO {
 // value, into the order [fy][fx][iy][ix] with ky = iy OS + fy, kx = ix OS + fx:
 // a visibility's taps along a row are then consecutive.  Same samples, same
 // products: results are those of the reference layout bit for bit.
-// os_shift >= 0: OS = 1 << os_shift (no division); otherwise divide by OS
+// os_shift >= 0: OS = 1 << os_shift < 4096 (shifts and 24-bit products:
+// fy * OS < 2^24, and the sum stays below the plane's (hp1 * OS)^2 < 2^26
+// samples); otherwise divide by OS
 __device__ __forceinline__ int row_part(int ky, int OS, int os_shift, int hp1)
 {
-    const int iy = os_shift >= 0 ? ky >> os_shift : ky / OS;
+    if (os_shift >= 0) {
+        const unsigned iy = (unsigned)ky >> os_shift, fy = (unsigned)ky & (unsigned)(OS - 1);
+        return (int)(__umul24(__umul24(fy, OS), __umul24(hp1, hp1)) + __umul24(iy, hp1));
+    }
+    const int iy = ky / OS;
     const int fy = ky - iy * OS;
     return fy * OS * hp1 * hp1 + iy * hp1;
 }
 __device__ __forceinline__ int col_part(int kx, int OS, int os_shift, int hp1)
 {
-    const int ix = os_shift >= 0 ? kx >> os_shift : kx / OS;
+    if (os_shift >= 0) {
+        const unsigned ix = (unsigned)kx >> os_shift, fx = (unsigned)kx & (unsigned)(OS - 1);
+        return (int)(__umul24(fx, __umul24(hp1, hp1)) + ix);
+    }
+    const int ix = kx / OS;
     const int fx = kx - ix * OS;
     return fx * hp1 * hp1 + ix;
 }
@@ -921,7 +935,7 @@ int launch_class(const sdp_grid_plan *plan, int c, const ChunkIO &io, const uint
     copies = kWave >> lane_shift;
     const int OS = plan->p.oversampling;
     int os_shift = -1;
-    for (int k = 0; k < 31; ++k)
+    for (int k = 0; k < 12; ++k)
         if ((1 << k) == OS) os_shift = k;
     ChunkArgs a{plan->p.grid_size, OS, plan->tile, plan->ntx, b, lane_shift, col_shift, copies, os_shift,
                 (int)(8 * plan->table_len), lds_planes(plan->p.n_kernels),
