@@ -102,6 +102,7 @@ __device__ __forceinline__ float wave_sum(float v)
 
 struct PlanArgs {
     int n, G, OS, n_kernels, snap_mode, tile, ntx, nty, ntiles, nkeys;
+    int plane_bits;   // sort key = (class, tile) key << plane_bits | plane
     double uv_scale, w_scale;
 };
 
@@ -112,7 +113,7 @@ __global__ void plan_records_kernel(const float *uvw, const int2 *supports, cons
     if (i >= a.n) return;
     const float x = uvw[3 * (size_t)i], y = uvw[3 * (size_t)i + 1], z = uvw[3 * (size_t)i + 2];
     const int gc = a.G / 2;
-    keys[i] = (uint32_t)a.nkeys;   // dropped: sorts last
+    keys[i] = (uint32_t)a.nkeys << a.plane_bits;   // dropped: sorts last
     vals[i] = (uint32_t)i;
 
     const float wabs = fabsf((float)((double)z * a.w_scale));
@@ -152,37 +153,40 @@ __global__ void plan_records_kernel(const float *uvw, const int2 *supports, cons
     r.meta = (uint32_t)nu | ((uint32_t)nv << 8) | ((uint32_t)plane << 16) | ((z < 0.0f) ? 0x80000000u : 0u);
     recs[i] = r;
     const uint32_t key = (uint32_t)(cls * a.ntiles + (va / a.tile) * a.ntx + (ua / a.tile));
-    keys[i] = key;
+    // within a (class, tile) list, records in plane order: neighbours share
+    // a kernel table and a footprint size
+    keys[i] = key << a.plane_bits | (uint32_t)plane;
 }
 
 // Per-key visibility and tap counts from the sorted keys: a run of equal
 // keys is one (class, tile) list.  The first kernel writes each run's start
 // and every record's tap count; after an inclusive scan of the tap counts the
 // second kernel, at each run's last record, forms the run's length and tap
-// total.  Dropped records (key == nkeys) sort last and are skipped.
-__global__ void key_run_start_kernel(const uint32_t *keys, const VisRec *recs, int n, int nkeys,
+// total.  Dropped records (key == nkeys) sort last and are skipped.  The
+// sort keys carry the plane in their low plane_bits bits.
+__global__ void key_run_start_kernel(const uint32_t *keys, const VisRec *recs, int n, int nkeys, int plane_bits,
                                      unsigned int *key_first, unsigned long long *taps)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t k = keys[i];
+    const uint32_t k = keys[i] >> plane_bits;
     if (k >= (uint32_t)nkeys) {
         taps[i] = 0;
         return;
     }
     const uint32_t m = recs[i].meta;
     taps[i] = (unsigned long long)((m & 0xffu) * ((m >> 8) & 0xffu));
-    if (i == 0 || keys[i - 1] != k) key_first[k] = (unsigned int)i;
+    if (i == 0 || (keys[i - 1] >> plane_bits) != k) key_first[k] = (unsigned int)i;
 }
 
-__global__ void key_run_end_kernel(const uint32_t *keys, int n, int nkeys, const unsigned int *key_first,
-                                   const unsigned long long *tap_scan, unsigned int *key_count,
-                                   unsigned long long *key_taps)
+__global__ void key_run_end_kernel(const uint32_t *keys, int n, int nkeys, int plane_bits,
+                                   const unsigned int *key_first, const unsigned long long *tap_scan,
+                                   unsigned int *key_count, unsigned long long *key_taps)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t k = keys[i];
-    if (k >= (uint32_t)nkeys || (i + 1 < n && keys[i + 1] == k)) return;
+    const uint32_t k = keys[i] >> plane_bits;
+    if (k >= (uint32_t)nkeys || (i + 1 < n && (keys[i + 1] >> plane_bits) == k)) return;
     const unsigned int f = key_first[k];
     key_count[k] = (unsigned int)(i + 1) - f;
     key_taps[k] = tap_scan[i] - (f > 0 ? tap_scan[f - 1] : 0ull);
@@ -1101,12 +1105,17 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
 
     if (n_vis > 0) {
         PlanArgs a{n_vis, p.grid_size, p.oversampling, p.n_kernels, p.snap_mode, plan->tile, plan->ntx, plan->nty,
-                   ntiles, nkeys, p.uv_scale, p.w_scale};
+                   ntiles, nkeys, 0, p.uv_scale, p.w_scale};
+        // plane order inside each (class, tile) list when the combined key,
+        // dropped records' nkeys << bits included, fits 32 bits
+        int pb = 0;
+        while ((1 << pb) < p.n_kernels) ++pb;
+        if (((unsigned long long)nkeys << pb) < (1ull << 32)) a.plane_bits = pb;
         plan_records_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(d_uvw, d_supports, d_pclass, a, recs_tmp, keys,
                                                                     vals);
         PLAN_TRY(hipGetLastError());
         int end_bit = 1;
-        while ((1u << end_bit) <= (unsigned)nkeys) ++end_bit;
+        while (end_bit < 32 && (1ull << end_bit) <= ((unsigned long long)nkeys << a.plane_bits)) ++end_bit;
         size_t temp_bytes = 0;
         PLAN_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys, keys_sorted, vals, plan->perm, n_vis,
                                                     0, end_bit, st));
@@ -1125,7 +1134,7 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
         unsigned long long *tap_count = reinterpret_cast<unsigned long long *>(recs_tmp);
         unsigned long long *tap_scan = tap_count + n_vis;
         key_run_start_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(keys_sorted, plan->recs, n_vis, nkeys,
-                                                                     key_first, tap_count);
+                                                                     a.plane_bits, key_first, tap_count);
         PLAN_TRY(hipGetLastError());
         size_t scan_bytes = 0;
         PLAN_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, scan_bytes, tap_count, tap_scan, n_vis, st));
@@ -1134,8 +1143,8 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
         (void)hipStreamSynchronize(st);
         (void)hipFree(temp);
         PLAN_TRY(se);
-        key_run_end_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(keys_sorted, n_vis, nkeys, key_first, tap_scan,
-                                                                   key_count, key_taps);
+        key_run_end_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(keys_sorted, n_vis, nkeys, a.plane_bits,
+                                                                   key_first, tap_scan, key_count, key_taps);
         PLAN_TRY(hipGetLastError());
     }
 
