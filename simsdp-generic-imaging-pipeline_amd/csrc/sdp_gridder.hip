@@ -299,12 +299,21 @@ __device__ __forceinline__ float2 tab_load(__amdgpu_buffer_rsrc_t rs, unsigned o
 }
 
 struct GroupVis {
-    bool ok;
-    int nu, nv, lu0, lv0;
-    int2 pt;      // (table offset, table row length (h+1)*OS)
+    int nu, nv, lu0, lv0;   // footprint size and first cell relative to the tile
+    int R;                  // footprint rows per step of the group's walk
+    float rcp_nu;           // 1 / max(nu, 1)
+    int2 pt;                // (table offset, h + 1)
     float conj, gx, gy;
-    uint32_t o;   // caller index
+    uint32_t o;             // caller index
 };
+
+// A record's walk geometry, formed once by the lane that holds it: nu, nv,
+// R = L / nu rows per step, the step count (largest over the C records the
+// groups take together) and the conjugation flag, 7 bits each (all <= 65).
+__device__ __forceinline__ uint32_t pack_geo(int nu, int nv, int R, int steps, bool conj)
+{
+    return (uint32_t)nu | (uint32_t)nv << 7 | (uint32_t)R << 14 | (uint32_t)steps << 21 | (conj ? 1u << 31 : 0u);
+}
 
 // A group's visibility, taken by lane shuffles from the lane that holds its
 // record (byte address src = 4 * lane): the wavefront loads kWave records at
@@ -316,31 +325,23 @@ __device__ __forceinline__ float lane_take(int src, float x)
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(x)));
 }
 
-__device__ __forceinline__ GroupVis group_vis(const VisRec &hr, int2 hpt, uint32_t ho, int src, bool ok, int tile_u0,
-                                              int tile_v0)
+__device__ __forceinline__ GroupVis group_vis(const VisRec &hr, uint32_t hgeo, float hrcp, int2 hpt, uint32_t ho,
+                                              int src)
 {
     GroupVis v;
-    v.ok = ok;
-    const uint32_t start = (uint32_t)lane_take(src, (int)hr.start);
-    const uint32_t meta = (uint32_t)lane_take(src, (int)hr.meta);
+    const uint32_t start = (uint32_t)lane_take(src, (int)hr.start);   // tile-relative
+    const uint32_t geo = (uint32_t)lane_take(src, (int)hgeo);
     v.o = (uint32_t)lane_take(src, (int)ho);
-    v.lu0 = (int)(start & 0xffffu) - tile_u0;
-    v.lv0 = (int)(start >> 16) - tile_v0;
-    v.nu = ok ? (int)(meta & 0xffu) : 0;
-    v.nv = ok ? (int)((meta >> 8) & 0xffu) : 0;
-    const int ptx = lane_take(src, hpt.x), pty = lane_take(src, hpt.y);
-    v.pt = ok ? make_int2(ptx, pty) : make_int2(0, 0);
-    v.conj = (meta & 0x80000000u) ? -1.0f : 1.0f;
+    v.lu0 = (int)(start & 0xffffu);
+    v.lv0 = (int)(start >> 16);
+    v.nu = (int)(geo & 0x7fu);
+    v.nv = (int)((geo >> 7) & 0x7fu);
+    v.R = (int)((geo >> 14) & 0x7fu);
+    v.rcp_nu = lane_take(src, hrcp);
+    v.pt = make_int2(lane_take(src, hpt.x), lane_take(src, hpt.y));
+    v.conj = (geo & 0x80000000u) ? -1.0f : 1.0f;
     v.gx = lane_take(src, hr.gx);
     v.gy = lane_take(src, hr.gy);
-    return v;
-}
-
-// largest v over the wavefront, for v equal on the L = 2^lane_shift lanes of
-// each group: only the exchanges between groups are needed
-__device__ __forceinline__ int groups_max(int v, int lane_shift)
-{
-    for (int off = 32; off >= (1 << lane_shift); off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
     return v;
 }
 
@@ -420,7 +421,6 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
     const int lane = threadIdx.x;
     const int L = 1 << a.lane_shift, C = a.copies;
     const int g = lane >> a.lane_shift, c = lane & (L - 1);
-    const bool in_group = g < C;
     const int tile_u0 = (ch.x % a.ntx) * a.tile, tile_v0 = (ch.x / a.ntx) * a.tile;
     const int gc = a.G / 2;
     const float os = (float)a.OS;
@@ -457,8 +457,10 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
     // the value each visibility brings: the visibility to grid, or the
     // measured value to subtract from (absent: degrid only)
     const float2 *val_src = MODE == MODE_GRID ? io.vis : io.measured;
-    VisRec hr;            // this lane's record of the current kWave
-    int2 hpt;             // its plane's (table offset, row length)
+    VisRec hr;            // this lane's record of the current kWave (start tile-relative)
+    uint32_t hgeo;        // its walk geometry
+    float hrcp;           // 1 / max(nu, 1)
+    int2 hpt;             // its plane's (table offset, h + 1)
     float2 hval;          // its value
     uint32_t ho;          // its caller index
     const int iters = (ch.z + C - 1) / C;
@@ -478,12 +480,23 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
                 if (val_src) hval = val_src[ho];
                 const int plane = (hr.meta >> 16) & 0x7fff;
                 hpt = pt_in_lds ? lds_pt[plane] : plane_tab[plane];
+                hr.start -= (uint32_t)tile_u0 | (uint32_t)tile_v0 << 16;   // the first cell lies in the tile
             }
+            // The group's L lanes cover R = L / nu footprint rows at a time
+            // (nu <= 2h + 1 <= L); the groups step together, as often as the
+            // largest ceil(nv / R) of the C records taken at once.
+            const int nu = (int)(hr.meta & 0xffu), nv = (int)((hr.meta >> 8) & 0xffu);
+            hrcp = __builtin_amdgcn_rcpf((float)max(nu, 1));
+            const int R = max(small_div(L, hrcp), 1);
+            int st = small_div(nv + R - 1, __builtin_amdgcn_rcpf((float)R));
+            for (int off = 1; off < C; off <<= 1) st = max(st, __shfl_xor(st, off, kWave));
+            hgeo = pack_geo(nu, nv, R, st, (hr.meta & 0x80000000u) != 0u);
         }
-        const int k = first + g;
-        const bool ok = in_group && k < ch.z;
         const int src = (slot + g) << 2;
-        const GroupVis v = group_vis(hr, hpt, ho, src, ok, tile_u0, tile_v0);
+        const GroupVis v = group_vis(hr, hgeo, hrcp, hpt, ho, src);
+        const bool ok = v.nu > 0;                  // records past the chunk are empty
+        // the step count, equal on every lane
+        const int steps = (int)(((uint32_t)__builtin_amdgcn_readfirstlane(lane_take(src, (int)hgeo)) >> 21) & 0x7fu);
         const float2 cur_val = make_float2(lane_take(src, hval.x), lane_take(src, hval.y));
         const int cell_u0 = tile_u0 + v.lu0 - gc, cell_v0 = tile_v0 + v.lv0 - gc;
         // Lane r (r < nv) of the group holds the kernel-table byte offset of
@@ -502,15 +515,10 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
         // offset, stay fixed and each step advances the row by R.  The row
         // offset comes by a lane shuffle from the row's lane (byte address ra).
         const int gbase = g << a.lane_shift;
-        const int nu1 = max(v.nu, 1);                     // nu <= 2h + 1 <= L
-        const float rcp_nu = __builtin_amdgcn_rcpf((float)nu1);
-        const int R = max(small_div(L, rcp_nu), 1);
-        const int r0 = small_div(c, rcp_nu), col = c - __mul24(r0, nu1);
+        const int R = v.R;
+        const int r0 = small_div(c, v.rcp_nu), col = c - __mul24(r0, max(v.nu, 1));
         const int rend = (gbase + v.nv) << 2;
         const int ra0 = r0 < R ? (gbase + r0) << 2 : rend;
-        // the largest step count over the groups: wavefront-uniform
-        const int steps = __builtin_amdgcn_readfirstlane(
-            groups_max(small_div(v.nv + R - 1, __builtin_amdgcn_rcpf((float)R)), a.lane_shift));
         const int kx = __builtin_amdgcn_ds_bpermute((gbase + col) << 2, kx_own);
         const int dra = R << 2;
         const int ix0 = __mul24(v.lv0 + r0, stride) + v.lu0 + col;   // window cell of the first tap
@@ -560,20 +568,20 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
                 si += __shfl_xor(si, off, kWave);
             }
             if (MODE == MODE_DEGRID) {
-                if (v.ok && c == 0) {
+                if (ok && c == 0) {
                     float2 out = make_float2(sr, si);
                     if (io.measured) out = make_float2(cur_val.x - sr, cur_val.y - si);
                     io.vis_out[v.o] = out;
                 }
                 continue;
             }
-            if (v.ok) {
+            if (ok) {
                 res = make_float2(cur_val.x - sr, cur_val.y - si);
                 if (io.vis_out && c == 0) io.vis_out[v.o] = res;
             }
         }
         if (kGrid) {
-            if (MODE == MODE_GRID && v.ok) res = cur_val;
+            if (MODE == MODE_GRID && ok) res = cur_val;
             const float vr = res.x, vi = res.y, vrc = v.conj * res.x, vic = v.conj * res.y;
             // byte offsets from the start of LDS; dead taps use the lane's
             // spare cell, which nothing reads
