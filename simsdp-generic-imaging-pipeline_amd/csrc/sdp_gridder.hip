@@ -529,7 +529,7 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
             // window byte offsets; dead taps read the zero cell past the window
             const int dwb = __mul24(R, stride) * (int)sizeof(float2);
             const int zero_b = (int)lds0 + cells * (int)sizeof(float2);
-            float sr = 0.f, si = 0.f;
+            float sxx = 0.f, syy = 0.f, syx = 0.f, sxy = 0.f;
             int ra = ra0, wb = (int)lds0 + ix0 * (int)sizeof(float2);
             auto batch = [&](auto nb) {
                 constexpr int N = decltype(nb)::value;
@@ -549,10 +549,11 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
                 for (int q = 0; q < N; ++q) w[q] = lds_load<float2>(wi[q]);
 #pragma unroll
                 for (int q = 0; q < N; ++q) {
-                    // grid * (k.x, conj * k.y); idle taps add exact zeros
-                    const float kyc = v.conj * kk[q].y;
-                    sr = fmaf(w[q].x, kk[q].x, fmaf(-w[q].y, kyc, sr));
-                    si = fmaf(w[q].y, kk[q].x, fmaf(w[q].x, kyc, si));
+                    // the four real products of grid * k; idle taps add exact zeros
+                    sxx = fmaf(w[q].x, kk[q].x, sxx);
+                    syy = fmaf(w[q].y, kk[q].y, syy);
+                    syx = fmaf(w[q].y, kk[q].x, syx);
+                    sxy = fmaf(w[q].x, kk[q].y, sxy);
                 }
             };
             int j = 0;
@@ -562,6 +563,8 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
                 j += 2;
             }
             if (j < steps) batch(std::integral_constant<int, 1>{});
+            // grid * (k.x, conj * k.y), this lane's share
+            float sr = fmaf(-v.conj, syy, sxx), si = fmaf(v.conj, sxy, syx);
             // butterfly over the group's lanes: every lane ends with the sum
             for (int off = L >> 1; off > 0; off >>= 1) {
                 sr += __shfl_xor(sr, off, kWave);
@@ -609,7 +612,9 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
                 for (int q = 0; q < N; ++q) cell[q] = lds_load<Acc>(ci[q]);
 #pragma unroll
                 for (int q = 0; q < N; ++q) {
-                    // vis * (k.x, conj * k.y), std_gridding (degridgrid.c:112-118)
+                    // vis * (k.x, conj * k.y), std_gridding (degridgrid.c:112-118),
+                    // rounded before it is added: the running sum takes one
+                    // rounding per tap
                     const float px = fmaf(vr, kk[q].x, -(vic * kk[q].y));
                     const float py = fmaf(vi, kk[q].x, vrc * kk[q].y);
                     acc_add(cell[q], px, py);
