@@ -910,27 +910,38 @@ int class_copies(int mode, int S, int col_shift, bool comp, int n_kernels)
     return c;
 }
 
-// copies per wavefront when gridding class c: more copies grid more
-// visibilities at once, fewer leave LDS for more resident wavefronts.
-// SDP_GRID_COPIES overrides the default for tuning: one value (1, 2 or 4) for
-// every class, or a comma-separated list, one value per class.
-int grid_copies(bool comp, int c)
+// A per-class override for tuning: environment variable `name` holds one
+// value (1, 2, 4 or 8) for every class or a comma-separated list, one value
+// per class; 0 when unset.
+int class_env(const char *name, int c)
 {
-    static int env[kNumClasses] = {-1};
-    if (env[0] < 0) {
-        for (int k = 0; k < kNumClasses; ++k) env[k] = 0;
-        const char *e = getenv("SDP_GRID_COPIES");
-        int k = 0, last = 0;
-        while (e && *e && k < kNumClasses) {
-            const int v = atoi(e);
-            last = env[k++] = (v == 1 || v == 2 || v == 4) ? v : 0;
-            e = strchr(e, ',');
-            if (e) ++e;
-        }
-        for (; k < kNumClasses; ++k) env[k] = last;
+    const char *e = getenv(name);
+    int v = 0;
+    for (int k = 0; e && *e && k <= c; ++k) {
+        v = atoi(e);
+        e = strchr(e, ',');
+        if (e) ++e;
     }
-    if (env[c] > 0) return env[c];
-    return comp ? 1 : 2;   // measured at 16.7M visibilities, 32 w-planes, h 4-16
+    return (v == 1 || v == 2 || v == 4 || v == 8) ? v : 0;
+}
+
+// lane groups per wavefront for class c: more groups take more visibilities
+// at once (less per-visibility work per group), fewer give each visibility
+// more lanes (more footprint rows per step) and, when gridding, leave LDS
+// for more resident wavefronts.  Measured at 16.7M visibilities, 32
+// w-planes, h 4-16: gridding takes 2 groups for h <= 7 and 1 above, degridding
+// as many as the class's footprint width allows.  SDP_GRID_COPIES and
+// SDP_DEGRID_GROUPS override these for tuning.
+int class_groups(int mode, bool comp, int c, int col_shift)
+{
+    const int widest = kWave >> col_shift;
+    if (mode == MODE_DEGRID) {
+        const int v = class_env("SDP_DEGRID_GROUPS", c);
+        return v ? std::min(v, widest) : widest;
+    }
+    const int v = class_env("SDP_GRID_COPIES", c);
+    if (v) return std::min(v, widest);
+    return std::min(comp || c > 0 ? 1 : 2, widest);
 }
 
 template <int MODE, bool COMP>
@@ -945,8 +956,8 @@ int launch_class(const sdp_grid_plan *plan, int c, const ChunkIO &io, const uint
     const int b = plan->class_chunk_begin[c], e = plan->class_chunk_begin[c + 1];
     const int S = plan->class_S[c];
     const int col_shift = class_lane_shift(plan->class_h[c], plan->p.snap_mode);
-    int copies = kWave >> col_shift;   // one visibility per group, one row at a time
-    if (MODE != MODE_DEGRID) copies = std::min(copies, std::min(grid_copies(COMP, c), class_copies(MODE, S, col_shift, COMP, plan->p.n_kernels)));
+    int copies = class_groups(MODE, COMP, c, col_shift);
+    if (MODE != MODE_DEGRID) copies = std::min(copies, class_copies(MODE, S, col_shift, COMP, plan->p.n_kernels));
     int lane_shift = 0;
     while ((kWave >> lane_shift) > copies) ++lane_shift;   // lanes per group = 64 / copies (copies a power of 2)
     copies = kWave >> lane_shift;
