@@ -348,6 +348,28 @@ __device__ __forceinline__ GroupVis group_vis(const VisRec &hr, uint32_t hgeo, f
 // n / d for 0 <= n <= 1024, 1 <= d <= 64: the float quotient is within 3e-4
 // of n / d, and a non-integer n / d sits at least 1 / 64 below the next
 // integer, so the added 1 / 128 floors correctly
+// x plus lane x ^ m's x by a DPP move (no LDS): CTRL is a quad permutation
+// or a row (16-lane) mirror
+template <int CTRL> __device__ __forceinline__ float dpp_add(float x)
+{
+    return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+
+// Butterfly sums over lane groups of 2^lane_shift lanes: within 16 lanes by
+// DPP (quad swaps 1 and 2, half-row and row mirrors), beyond by shuffles.
+// Every lane of a group ends with the group's sums, in a fixed order.
+__device__ __forceinline__ void group_sum(float &x, float &y, int lane_shift)
+{
+    if (lane_shift >= 1) { x = dpp_add<0xb1>(x); y = dpp_add<0xb1>(y); }
+    if (lane_shift >= 2) { x = dpp_add<0x4e>(x); y = dpp_add<0x4e>(y); }
+    if (lane_shift >= 3) { x = dpp_add<0x141>(x); y = dpp_add<0x141>(y); }
+    if (lane_shift >= 4) { x = dpp_add<0x140>(x); y = dpp_add<0x140>(y); }
+    for (int off = 16; off < (1 << lane_shift); off <<= 1) {
+        x += __shfl_xor(x, off, kWave);
+        y += __shfl_xor(y, off, kWave);
+    }
+}
+
 // LDS byte addresses: loads and stores through them need no base addition
 typedef __attribute__((address_space(3))) char lds_char_t;
 __device__ __forceinline__ uint32_t lds_addr(const void *p)
@@ -565,11 +587,8 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
             if (j < steps) batch(std::integral_constant<int, 1>{});
             // grid * (k.x, conj * k.y), this lane's share
             float sr = fmaf(-v.conj, syy, sxx), si = fmaf(v.conj, sxy, syx);
-            // butterfly over the group's lanes: every lane ends with the sum
-            for (int off = L >> 1; off > 0; off >>= 1) {
-                sr += __shfl_xor(sr, off, kWave);
-                si += __shfl_xor(si, off, kWave);
-            }
+            // sum over the group's lanes; every lane ends with the sum
+            group_sum(sr, si, a.lane_shift);
             if (MODE == MODE_DEGRID) {
                 if (ok && c == 0) {
                     float2 out = make_float2(sr, si);
