@@ -325,23 +325,36 @@ __device__ __forceinline__ float lane_take(int src, float x)
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(x)));
 }
 
+// With one group per wavefront (ONE) the holding lane `slot` is the same for
+// every lane: its fields are read into scalar registers, and what is formed
+// from them is scalar work.
+template <bool ONE> __device__ __forceinline__ int take(int src, int slot, int x)
+{
+    return ONE ? __builtin_amdgcn_readlane(x, slot) : lane_take(src, x);
+}
+template <bool ONE> __device__ __forceinline__ float take(int src, int slot, float x)
+{
+    return __int_as_float(take<ONE>(src, slot, __float_as_int(x)));
+}
+
+template <bool ONE>
 __device__ __forceinline__ GroupVis group_vis(const VisRec &hr, uint32_t hgeo, float hrcp, int2 hpt, uint32_t ho,
-                                              int src)
+                                              int src, int slot)
 {
     GroupVis v;
-    const uint32_t start = (uint32_t)lane_take(src, (int)hr.start);   // tile-relative
-    const uint32_t geo = (uint32_t)lane_take(src, (int)hgeo);
-    v.o = (uint32_t)lane_take(src, (int)ho);
+    const uint32_t start = (uint32_t)take<ONE>(src, slot, (int)hr.start);   // tile-relative
+    const uint32_t geo = (uint32_t)take<ONE>(src, slot, (int)hgeo);
+    v.o = (uint32_t)take<ONE>(src, slot, (int)ho);
     v.lu0 = (int)(start & 0xffffu);
     v.lv0 = (int)(start >> 16);
     v.nu = (int)(geo & 0x7fu);
     v.nv = (int)((geo >> 7) & 0x7fu);
     v.R = (int)((geo >> 14) & 0x7fu);
-    v.rcp_nu = lane_take(src, hrcp);
-    v.pt = make_int2(lane_take(src, hpt.x), lane_take(src, hpt.y));
+    v.rcp_nu = take<ONE>(src, slot, hrcp);
+    v.pt = make_int2(take<ONE>(src, slot, hpt.x), take<ONE>(src, slot, hpt.y));
     v.conj = (geo & 0x80000000u) ? -1.0f : 1.0f;
-    v.gx = lane_take(src, hr.gx);
-    v.gy = lane_take(src, hr.gy);
+    v.gx = take<ONE>(src, slot, hr.gx);
+    v.gy = take<ONE>(src, slot, hr.gy);
     return v;
 }
 
@@ -429,7 +442,7 @@ __device__ __forceinline__ void acc_add(float2 &c, float px, float py)
 __device__ __forceinline__ void acc_zero(float4 &c) { c = make_float4(0.f, 0.f, 0.f, 0.f); }
 __device__ __forceinline__ void acc_zero(float2 &c) { c = make_float2(0.f, 0.f); }
 
-template <int MODE, bool COMP>
+template <int MODE, bool COMP, bool ONE>
 __global__ void __launch_bounds__(kWave)
 chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm, const int2 *__restrict__ plane_tab,
              const int4 *__restrict__ chunks, const long long *__restrict__ slab_off, ChunkArgs a, ChunkIO io)
@@ -441,8 +454,8 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
     const int4 ch = chunks[a.chunk_base + blockIdx.x];   // tile, first record, count, window S
     const int S = ch.w, stride = S + 1, cells = S * stride;
     const int lane = threadIdx.x;
-    const int L = 1 << a.lane_shift, C = a.copies;
-    const int g = lane >> a.lane_shift, c = lane & (L - 1);
+    const int L = ONE ? kWave : 1 << a.lane_shift, C = ONE ? 1 : a.copies;
+    const int g = ONE ? 0 : lane >> a.lane_shift, c = ONE ? lane : lane & (L - 1);
     const int tile_u0 = (ch.x % a.ntx) * a.tile, tile_v0 = (ch.x / a.ntx) * a.tile;
     const int gc = a.G / 2;
     const float os = (float)a.OS;
@@ -515,11 +528,11 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
             hgeo = pack_geo(nu, nv, R, st, (hr.meta & 0x80000000u) != 0u);
         }
         const int src = (slot + g) << 2;
-        const GroupVis v = group_vis(hr, hgeo, hrcp, hpt, ho, src);
+        const GroupVis v = group_vis<ONE>(hr, hgeo, hrcp, hpt, ho, src, slot);
         const bool ok = v.nu > 0;                  // records past the chunk are empty
         // the step count, equal on every lane
-        const int steps = (int)(((uint32_t)__builtin_amdgcn_readfirstlane(lane_take(src, (int)hgeo)) >> 21) & 0x7fu);
-        const float2 cur_val = make_float2(lane_take(src, hval.x), lane_take(src, hval.y));
+        const int steps = (int)(((uint32_t)__builtin_amdgcn_readfirstlane(take<ONE>(src, slot, (int)hgeo)) >> 21) & 0x7fu);
+        const float2 cur_val = make_float2(take<ONE>(src, slot, hval.x), take<ONE>(src, slot, hval.y));
         const int cell_u0 = tile_u0 + v.lu0 - gc, cell_v0 = tile_v0 + v.lv0 - gc;
         // Lane r (r < nv) of the group holds the kernel-table byte offset of
         // footprint row r (std_gridding's abs((int)ROUND((v - grid_pos) * OS)),
@@ -968,8 +981,10 @@ int launch_class(const sdp_grid_plan *plan, int c, const ChunkIO &io, const uint
 {
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)chunk_kernel<MODE, COMP>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)kMaxLds);
+        (void)hipFuncSetAttribute((const void *)chunk_kernel<MODE, COMP, false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+        (void)hipFuncSetAttribute((const void *)chunk_kernel<MODE, COMP, true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
         attr_set = true;
     }
     const int b = plan->class_chunk_begin[c], e = plan->class_chunk_begin[c + 1];
@@ -987,8 +1002,13 @@ int launch_class(const sdp_grid_plan *plan, int c, const ChunkIO &io, const uint
     ChunkArgs a{plan->p.grid_size, OS, plan->tile, plan->ntx, b, lane_shift, col_shift, copies, os_shift,
                 (int)(8 * plan->table_len), lds_planes(plan->p.n_kernels),
                 (int)lds_window_bytes(MODE, S, copies, COMP)};
-    chunk_kernel<MODE, COMP><<<e - b, kWave, lds_bytes(MODE, S, copies, COMP, plan->p.n_kernels), st>>>(
-        plan->recs, perm, plan->plane_tab, plan->chunks, plan->slab_off, a, io);
+    const size_t lds = lds_bytes(MODE, S, copies, COMP, plan->p.n_kernels);
+    if (copies == 1)
+        chunk_kernel<MODE, COMP, true><<<e - b, kWave, lds, st>>>(plan->recs, perm, plan->plane_tab, plan->chunks,
+                                                                  plan->slab_off, a, io);
+    else
+        chunk_kernel<MODE, COMP, false><<<e - b, kWave, lds, st>>>(plan->recs, perm, plan->plane_tab, plan->chunks,
+                                                                   plan->slab_off, a, io);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }
