@@ -1025,15 +1025,69 @@ const float2 *relayout(const sdp_grid_plan *plan, const float *d_kernels, int sl
     return out;
 }
 
+// This thread's helper stream for the current device (default priority) and
+// its fork and join events; created on first use.
+int side_stream(hipStream_t *side, hipEvent_t *fork, hipEvent_t *join)
+{
+    struct Side {
+        int dev = -1;
+        hipStream_t s = nullptr;
+        hipEvent_t fork = nullptr, join = nullptr;
+    };
+    thread_local Side t;
+    int dev = 0;
+    SDP_HIP_TRY(hipGetDevice(&dev));
+    if (t.s == nullptr || t.dev != dev) {
+        Side n;
+        n.dev = dev;
+        SDP_HIP_TRY(hipStreamCreateWithFlags(&n.s, hipStreamNonBlocking));
+        SDP_HIP_TRY(hipEventCreateWithFlags(&n.fork, hipEventDisableTiming));
+        SDP_HIP_TRY(hipEventCreateWithFlags(&n.join, hipEventDisableTiming));
+        t = n;   // a previous device's objects stay with that device
+    }
+    *side = t.s;
+    *fork = t.fork;
+    *join = t.join;
+    return SDP_OK;
+}
+
 template <int MODE>
 int launch_chunks(const sdp_grid_plan *plan, const ChunkIO &io, const uint32_t *perm, hipStream_t st)
 {
     const bool comp = plan->p.accumulate == SDP_ACC_COMPENSATED;
+    // The class with the most work items runs on the caller's stream; the
+    // others, whose launches are too small to fill the GPU on their own, run
+    // beside it on a helper stream forked from and joined back to it (slabs
+    // of different classes are disjoint).
+    int big = -1, n_used = 0;
+    for (int c = 0; c < kNumClasses; ++c) {
+        const int n = plan->class_chunk_begin[c + 1] - plan->class_chunk_begin[c];
+        if (n <= 0) continue;
+        ++n_used;
+        if (big < 0 || n > plan->class_chunk_begin[big + 1] - plan->class_chunk_begin[big]) big = c;
+    }
+    hipStream_t side = st;
+    hipEvent_t join = nullptr;
+    if (n_used > 1) {
+        hipEvent_t fork = nullptr;
+        if (side_stream(&side, &fork, &join) == SDP_OK) {
+            SDP_HIP_TRY(hipEventRecord(fork, st));
+            SDP_HIP_TRY(hipStreamWaitEvent(side, fork, 0));
+        } else {
+            side = st;
+            join = nullptr;
+        }
+    }
     for (int c = 0; c < kNumClasses; ++c) {
         if (plan->class_chunk_begin[c + 1] <= plan->class_chunk_begin[c]) continue;
-        const int rc = (comp || MODE == MODE_DEGRID) ? launch_class<MODE, true>(plan, c, io, perm, st)
-                                                     : launch_class<MODE, false>(plan, c, io, perm, st);
+        hipStream_t s = c == big ? st : side;
+        const int rc = (comp || MODE == MODE_DEGRID) ? launch_class<MODE, true>(plan, c, io, perm, s)
+                                                     : launch_class<MODE, false>(plan, c, io, perm, s);
         if (rc != SDP_OK) return rc;
+    }
+    if (join) {
+        SDP_HIP_TRY(hipEventRecord(join, side));
+        SDP_HIP_TRY(hipStreamWaitEvent(st, join, 0));
     }
     return SDP_OK;
 }
