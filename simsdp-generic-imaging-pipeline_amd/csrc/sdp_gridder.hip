@@ -766,8 +766,15 @@ merge_region_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ k
     __shared__ int s_b[kRegionCover];           // (by + 512) | (bx + 512) << 10
     const int R = a.R;
     const int oy0 = blockIdx.y * R, ox0 = blockIdx.x * R;
-    const int rows_per_pass = kMergeThreads / R;
-    const int col = threadIdx.x % R, row0 = threadIdx.x / R;
+    // At R = 32 each wavefront owns a band of 8 rows (two rows of 32 lanes,
+    // its cells 2 rows apart) and skips the additions of slabs that miss its
+    // band; smaller regions (G < 32) take rows in passes of kMergeThreads / R.
+    const bool banded = R == kRegion;
+    const int wave = threadIdx.x / 64;
+    const int rows_per_pass = banded ? 2 : kMergeThreads / R;
+    const int col = threadIdx.x % R;
+    const int row0 = banded ? (kRegion / 4) * wave + (threadIdx.x % 64) / kRegion : threadIdx.x / R;
+    const int band_lo = banded ? (kRegion / 4) * wave : 0, band_hi = banded ? band_lo + kRegion / 4 : R;
     float sr[kCells], si[kCells], cr[kCells], ci[kCells];
 #pragma unroll
     for (int q = 0; q < kCells; ++q) sr[q] = si[q] = cr[q] = ci[q] = 0.f;
@@ -786,9 +793,34 @@ merge_region_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ k
         s_b[k] = (oy0 - ty * a.tile + 512) | ((ox0 - tx * a.tile + 512) << 10);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        s_base[0] = 0;
-        for (int k = 0; k < nt; ++k) s_base[k + 1] += s_base[k];
+    {
+        // s_base[1 .. nt] hold the covering tiles' key counts: turn them into
+        // running totals, 4 entries per thread then a scan over the threads
+        static_assert(kRegionCover <= 4 * kMergeThreads, "4 covering tiles per thread");
+        __shared__ int s_wave[kMergeThreads / 64];
+        const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+        int loc[4], sum = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = 4 * t + j;
+            sum += k < nt ? s_base[k + 1] : 0;
+            loc[j] = sum;
+        }
+        int x = sum;   // inclusive scan over the wavefront
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) s_wave[w] = x;
+        __syncthreads();
+        int before = x - sum;
+        for (int v = 0; v < w; ++v) before += s_wave[v];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = 4 * t + j;
+            if (k < nt) s_base[k + 1] = before + loc[j];
+        }
+        if (t == 0) s_base[0] = 0;
     }
     __syncthreads();
     const int total = s_base[nt];
@@ -808,11 +840,14 @@ merge_region_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ k
         __syncthreads();
         for (int e0 = 0; e0 < n; e0 += kRegionBatch) {
             float2 v[kRegionBatch][kCells];
+            unsigned hits = 0;   // slabs reaching this wavefront's band (uniform)
 #pragma unroll
             for (int u = 0; u < kRegionBatch; ++u) {
                 const int e = e0 + u;
                 const int geo = e < n ? s_geo[e] : 0;
                 const int S = geo & 1023, by = ((geo >> 10) & 1023) - 512, bx = ((geo >> 20) & 1023) - 512;
+                // slab rows ly = by + r in [0, S): region rows r in [-by, S - by)
+                if (e < n && max(-by, band_lo) < min(S - by, band_hi)) hits |= 1u << u;
                 const int lx = bx + col;
                 const bool colok = e < n && lx >= 0 && lx < S;
                 const float2 *slab = kslabs + (e < n ? s_off[e] : 0) + lx;
@@ -823,13 +858,16 @@ merge_region_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ k
                 }
             }
             // absent values are zeros: adding them leaves sum and compensation unchanged
+            hits = __builtin_amdgcn_readfirstlane(hits);
 #pragma unroll
-            for (int u = 0; u < kRegionBatch; ++u)
+            for (int u = 0; u < kRegionBatch; ++u) {
+                if (!(hits & (1u << u))) continue;
 #pragma unroll
                 for (int q = 0; q < kCells; ++q) {
                     two_sum_acc(sr[q], cr[q], v[u][q].x);
                     two_sum_acc(si[q], ci[q], v[u][q].y);
                 }
+            }
         }
         __syncthreads();
     }
