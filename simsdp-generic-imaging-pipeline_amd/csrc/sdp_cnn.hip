@@ -242,6 +242,70 @@ __global__ void __launch_bounds__(kT) gemm_bf16_kernel(GemmArgs g)
                          acc[i][j][r], blockIdx.z);
 }
 
+// Forward convolution with at most 16 output channels (the full-resolution
+// layers and the head): a 64 x 64 tile would leave 3/4 or more of each
+// product empty, so a workgroup takes 256 pixels x 16 channels, each
+// wavefront 64 pixels.  The input has a power-of-two channel count >= 8: a
+// K step's 8-channel groups are 32 contiguous bytes of one tap, read as two
+// 16-byte loads and stored to LDS as one 16-byte BF16 vector.
+constexpr int kNarrowM = 256;
+static_assert(kNarrowM == kT, "one pixel index per thread");
+
+__global__ void __launch_bounds__(kT) gemm_fwd_narrow_kernel(GemmArgs g, int cin_shift)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 As[kNarrowM][kBK + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[16][kBK + 8];
+    __shared__ PixIdx s_pix[kNarrowM];
+    const int m0 = blockIdx.x * kNarrowM;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    s_pix[threadIdx.x] = pix_index(g, m0 + threadIdx.x, g.M);
+    f32x4 acc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kb = 0; kb < g.K; kb += kBK) {
+        __syncthreads();   // the pixel table is ready; the previous step's products have read LDS
+#pragma unroll
+        for (int j = 0; j < kBK / 8; ++j) {
+            const int q = j * kT + threadIdx.x;           // (pixel, 8-channel group) of this K step
+            const int m = q >> 2, kg = (q & 3) * 8, k = kb + kg;
+            bf16x8 v;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = (__bf16)0.f;
+            if (k < g.K) {
+                const int t = k >> cin_shift, c0 = k & (g.Cin - 1);   // tap (ky, kx) = (t / 3, t % 3)
+                const int dy = t < 3 ? -1 : (t < 6 ? 0 : 1), dx = t - 3 * (dy + 1) - 1;
+                const PixIdx p = s_pix[m];
+                const int yy = p.y + dy, xx = p.x + dx;
+                if (yy >= 0 && yy < g.H && xx >= 0 && xx < g.W) {
+                    const float4 *src =
+                        reinterpret_cast<const float4 *>(g.a_src + ((size_t)(p.bh + yy) * g.W + xx) * g.Cin + c0);
+                    const float4 u0 = src[0], u1 = src[1];
+                    v[0] = (__bf16)u0.x; v[1] = (__bf16)u0.y; v[2] = (__bf16)u0.z; v[3] = (__bf16)u0.w;
+                    v[4] = (__bf16)u1.x; v[5] = (__bf16)u1.y; v[6] = (__bf16)u1.z; v[7] = (__bf16)u1.w;
+                }
+            }
+            *reinterpret_cast<bf16x8 *>(&As[m][kg]) = v;
+        }
+        for (int i = threadIdx.x; i < 16 * kBK; i += kT) {
+            const int nn = i & 15, kk = i >> 4;
+            Bs[nn][kk] = (__bf16)((kb + kk < g.K && nn < g.N) ? g.b_src[(size_t)(kb + kk) * g.Cout + nn] : 0.f);
+        }
+        __syncthreads();
+        const int kq = (lane >> 4) * 8;
+        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&Bs[lane & 15][kq]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&As[wave * 64 + i * 16 + (lane & 15)][kq]);
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            epilogue(g, G_FWD, m0 + wave * 64 + i * 16 + (lane >> 4) * 4 + r, lane & 15, acc[i][r], 0);
+}
+
 // weight gradient: sum the K-split partials in split order
 __global__ void sum_splits_kernel(const float *part, int splits, size_t n, float *out)
 {
@@ -490,8 +554,13 @@ int run_gemm(const sdp_cnn *n, int mode, GemmArgs g, hipStream_t st)
     }
     switch (mode) {
     case G_FWD:
-        if (bf16) gemm_bf16_kernel<G_FWD><<<grid, kT, 0, st>>>(g);
-        else gemm_f32_kernel<G_FWD><<<grid, kT, 0, st>>>(g);
+        if (bf16 && g.N <= 16 && g.Cin >= 8 && (g.Cin & (g.Cin - 1)) == 0) {
+            gemm_fwd_narrow_kernel<<<(g.M + kNarrowM - 1) / kNarrowM, kT, 0, st>>>(g, __builtin_ctz(g.Cin));
+        } else if (bf16) {
+            gemm_bf16_kernel<G_FWD><<<grid, kT, 0, st>>>(g);
+        } else {
+            gemm_f32_kernel<G_FWD><<<grid, kT, 0, st>>>(g);
+        }
         break;
     case G_BWD_DATA:
         if (bf16) gemm_bf16_kernel<G_BWD_DATA><<<grid, kT, 0, st>>>(g);

@@ -67,12 +67,15 @@ def test_bf16_forward_and_gradients_track_oracle(hip):
         assert cos > 0.999, (k, cos)
 
 
-def test_bf16_full_size_network_matches_fp32(hip):
-    """4 levels, 32 base channels (1.75M parameters) on 64 x 64 patches:
-    every GEMM shape of the real network, BF16 against FP32."""
+@pytest.mark.parametrize("levels,base", [(4, 32), (3, 16)])
+def test_bf16_full_size_network_matches_fp32(hip, levels, base):
+    """4 levels, 32 base channels (1.75M parameters), and the 3-level,
+    16-channel network the learned cycle uses (whose full-resolution layers
+    take the 16-channel forward kernel), on 64 x 64 patches: every GEMM
+    shape of the real networks, BF16 against FP32."""
     outs, grads = [], []
     for prec in (hip.CNN_FP32, hip.CNN_BF16):
-        net, params, x, t = setup(hip, 4, 32, 64, 2, prec, seed=9)
+        net, params, x, t = setup(hip, levels, base, 64, 2, prec, seed=9)
         y, loss, g, _ = run(hip, net, x, t)
         outs.append(y)
         grads.append(net.get_grads())
