@@ -242,26 +242,30 @@ __global__ void __launch_bounds__(kT) gemm_bf16_kernel(GemmArgs g)
                          acc[i][j][r], blockIdx.z);
 }
 
-// Forward convolution with at most 16 output channels (the full-resolution
-// layers and the head): a 64 x 64 tile would leave 3/4 or more of each
-// product empty, so a workgroup takes 256 pixels x 16 channels, each
+// Forward convolution with at most BN = 16 or 32 output channels (the
+// upper levels and the head): a 64 x 64 tile would leave half or more of
+// each product empty, so a workgroup takes 256 pixels x BN channels, each
 // wavefront 64 pixels.  The input has a power-of-two channel count >= 8: a
 // K step's 8-channel groups are 32 contiguous bytes of one tap, read as two
 // 16-byte loads and stored to LDS as one 16-byte BF16 vector.
 constexpr int kNarrowM = 256;
 static_assert(kNarrowM == kT, "one pixel index per thread");
 
+template <int BN>
 __global__ void __launch_bounds__(kT) gemm_fwd_narrow_kernel(GemmArgs g, int cin_shift)
 {
+    constexpr int NB = BN / 16;   // 16-column blocks
     __shared__ __attribute__((aligned(16))) __bf16 As[kNarrowM][kBK + 8];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[16][kBK + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[BN][kBK + 8];
     __shared__ PixIdx s_pix[kNarrowM];
     const int m0 = blockIdx.x * kNarrowM;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     s_pix[threadIdx.x] = pix_index(g, m0 + threadIdx.x, g.M);
-    f32x4 acc[4];
+    f32x4 acc[4][NB];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int kb = 0; kb < g.K; kb += kBK) {
         __syncthreads();   // the pixel table is ready; the previous step's products have read LDS
 #pragma unroll
@@ -286,24 +290,31 @@ __global__ void __launch_bounds__(kT) gemm_fwd_narrow_kernel(GemmArgs g, int cin
             }
             *reinterpret_cast<bf16x8 *>(&As[m][kg]) = v;
         }
-        for (int i = threadIdx.x; i < 16 * kBK; i += kT) {
-            const int nn = i & 15, kk = i >> 4;
+        for (int i = threadIdx.x; i < BN * kBK; i += kT) {
+            const int nn = i % BN, kk = i / BN;
             Bs[nn][kk] = (__bf16)((kb + kk < g.K && nn < g.N) ? g.b_src[(size_t)(kb + kk) * g.Cout + nn] : 0.f);
         }
         __syncthreads();
         const int kq = (lane >> 4) * 8;
-        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&Bs[lane & 15][kq]);
+        bf16x8 b[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) b[j] = *reinterpret_cast<const bf16x8 *>(&Bs[j * 16 + (lane & 15)][kq]);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&As[wave * 64 + i * 16 + (lane & 15)][kq]);
-            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
         }
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-            epilogue(g, G_FWD, m0 + wave * 64 + i * 16 + (lane >> 4) * 4 + r, lane & 15, acc[i][r], 0);
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                epilogue(g, G_FWD, m0 + wave * 64 + i * 16 + (lane >> 4) * 4 + r, j * 16 + (lane & 15), acc[i][j][r],
+                         0);
 }
 
 // weight gradient: sum the K-split partials in split order
@@ -554,8 +565,10 @@ int run_gemm(const sdp_cnn *n, int mode, GemmArgs g, hipStream_t st)
     }
     switch (mode) {
     case G_FWD:
-        if (bf16 && g.N <= 16 && g.Cin >= 8 && (g.Cin & (g.Cin - 1)) == 0) {
-            gemm_fwd_narrow_kernel<<<(g.M + kNarrowM - 1) / kNarrowM, kT, 0, st>>>(g, __builtin_ctz(g.Cin));
+        if (bf16 && g.N <= 32 && g.Cin >= 8 && (g.Cin & (g.Cin - 1)) == 0) {
+            const int blocks = (g.M + kNarrowM - 1) / kNarrowM, sh = __builtin_ctz(g.Cin);
+            if (g.N <= 16) gemm_fwd_narrow_kernel<16><<<blocks, kT, 0, st>>>(g, sh);
+            else gemm_fwd_narrow_kernel<32><<<blocks, kT, 0, st>>>(g, sh);
         } else if (bf16) {
             gemm_bf16_kernel<G_FWD><<<grid, kT, 0, st>>>(g);
         } else {
