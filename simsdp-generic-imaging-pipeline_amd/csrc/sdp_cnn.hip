@@ -377,6 +377,27 @@ __global__ void pool_back_add_mask_kernel(const float *g_low, const float *skip_
 }
 
 // nearest 2x upsample (NHWC), input H x W -> 2H x 2W
+// The same two maps for C % 4 == 0 and fewer than 2^31 16-byte groups: one
+// group of 4 channels per thread, 32-bit index arithmetic.
+__global__ void avgpool2_vec_kernel(const float4 *in, unsigned B, unsigned H, unsigned W, unsigned C4, float4 *out)
+{
+    const unsigned W2 = W / 2, H2 = H / 2, n = B * H2 * W2 * C4;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const unsigned c = i % C4, px = i / C4, x = px % W2, r = px / W2, y = r % H2, b = r / H2;
+        const float4 *p = in + ((b * H + 2 * y) * W + 2 * x) * C4 + c;
+        const float4 a = p[0], e = p[C4], u = p[W * C4], v = p[W * C4 + C4];
+        out[i] = make_float4(0.25f * ((a.x + e.x) + (u.x + v.x)), 0.25f * ((a.y + e.y) + (u.y + v.y)),
+                             0.25f * ((a.z + e.z) + (u.z + v.z)), 0.25f * ((a.w + e.w) + (u.w + v.w)));
+    }
+}
+__global__ void upsample2_vec_kernel(const float4 *in, unsigned B, unsigned H, unsigned W, unsigned C4, float4 *out)
+{
+    const unsigned W2 = 2 * W, H2 = 2 * H, n = B * H2 * W2 * C4;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const unsigned c = i % C4, px = i / C4, x = px % W2, r = px / W2, y = r % H2, b = r / H2;
+        out[i] = in[((b * H + y / 2) * W + x / 2) * C4 + c];
+    }
+}
 __global__ void upsample2_kernel(const float *in, int B, int H, int W, int C, float *out)
 {
     const size_t n = (size_t)B * (2 * H) * (2 * W) * C;
@@ -775,8 +796,12 @@ int sdp_cnn_forward(sdp_cnn *n, const float *d_in, float *d_out, int B, sdp_stre
         if (int e = conv_forward(n, enc_a(l), B, H, W, xin, n->a[l], nullptr, 1, st)) return e;
         if (int e = conv_forward(n, enc_b(l), B, H, W, n->a[l], n->e[l], nullptr, 1, st)) return e;
         if (l < L - 1) {
-            avgpool2_kernel<<<ew_grid((size_t)B * (H / 2) * (W / 2) * n->ch(l)), kT, 0, st>>>(n->e[l], B, H, W,
-                                                                                           n->ch(l), n->x[l + 1]);
+            const size_t np = (size_t)B * (H / 2) * (W / 2) * n->ch(l);
+            if (n->ch(l) % 4 == 0 && (size_t)B * H * W * n->ch(l) / 4 < (1u << 31))
+                avgpool2_vec_kernel<<<ew_grid(np / 4), kT, 0, st>>>((const float4 *)n->e[l], B, H, W, n->ch(l) / 4,
+                                                                   (float4 *)n->x[l + 1]);
+            else
+                avgpool2_kernel<<<ew_grid(np), kT, 0, st>>>(n->e[l], B, H, W, n->ch(l), n->x[l + 1]);
             SDP_LAUNCH_CHECK();
             xin = n->x[l + 1];
         }
@@ -784,8 +809,12 @@ int sdp_cnn_forward(sdp_cnn *n, const float *d_in, float *d_out, int B, sdp_stre
     const float *dcur = n->e[L - 1];
     for (int l = L - 2; l >= 0; --l) {
         const int H = n->hl(l), W = n->wl(l);
-        upsample2_kernel<<<ew_grid((size_t)B * H * W * n->ch(l + 1)), kT, 0, st>>>(dcur, B, H / 2, W / 2,
-                                                                                 n->ch(l + 1), n->up[l]);
+        const size_t nu = (size_t)B * H * W * n->ch(l + 1);
+        if (n->ch(l + 1) % 4 == 0 && nu / 4 < (1u << 31))
+            upsample2_vec_kernel<<<ew_grid(nu / 4), kT, 0, st>>>((const float4 *)dcur, B, H / 2, W / 2,
+                                                                n->ch(l + 1) / 4, (float4 *)n->up[l]);
+        else
+            upsample2_kernel<<<ew_grid(nu), kT, 0, st>>>(dcur, B, H / 2, W / 2, n->ch(l + 1), n->up[l]);
         SDP_LAUNCH_CHECK();
         if (int e = conv_forward(n, dec_u(n, l), B, H, W, n->up[l], n->s[l], n->e[l], 1, st)) return e;
         if (int e = conv_forward(n, dec_d(n, l), B, H, W, n->s[l], n->d[l], nullptr, 1, st)) return e;
