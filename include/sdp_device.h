@@ -125,7 +125,8 @@ typedef struct sdp_grid_plan sdp_grid_plan;
 /* Builds the per-visibility records (exact reference index arithmetic) and
  * the tile-sorted order for n_vis coordinates.  supports is a HOST array of
  * n_kernels (half_support, sample_offset) int pairs (kernel_host_set_up
- * layout, top.c:252-257). */
+ * layout, top.c:252-257); half supports up to 31 (a footprint, up to 2h + 1
+ * cells across, must fit a 64-lane wavefront), else SDP_E_UNSUPPORTED. */
 int sdp_grid_plan_create(sdp_grid_plan **plan, const sdp_grid_params *p, const int *supports,
                          const float *d_uvw, int n_vis, sdp_stream s);
 int sdp_grid_plan_destroy(sdp_grid_plan *plan);
