@@ -102,6 +102,11 @@ void execute_inv_convolution_correction_actor(int GRID_SIZE, IN PRECISION* dirty
 
 /* ---- visibility arithmetic (include/gains_apply_run.h) ----------------- */
 
+/* replaces include/gains_apply_run.h:14 (src/CPU/gains_apply_run.c:4-17): the
+ * first actor of the FFT graph's correct_to_finegrid subgraph
+ * (Algo/correct_to_finegrid.pi:11-24) */
+void reciprocal_transform_actor(int NUM_RECEIVERS, IN PRECISION2* gains_in, IN Config *config,
+        OUT PRECISION2* gains_out);
 /* replaces include/gains_apply_run.h:23 (src/CPU/gains_apply_run.c:70-96) */
 void subtract_from_measurements(int NUM_RECEIVERS, int NUM_BASELINES, int NUM_VISIBILITIES,
         IN PRECISION2* measured_vis, IN PRECISION2* visibilities_in, IN PRECISION2* gains,

@@ -210,6 +210,9 @@ int sdp_vis_add(int n, const float *d_a, const float *d_b, float *d_out, sdp_str
 int sdp_vis_subtract_gains(int n_vis, int n_baselines, const float *d_measured, const float *d_pred,
                            const float *d_gains, const int *d_pairs, int apply_only,
                            float *d_out, sdp_stream s);
+/* out[i] = 1 / in[i] for n complex gains, as conj(z) / |z|^2 in float
+ * (reciprocal_transform_actor, gains_apply_run.c:4-17); in == out allowed */
+int sdp_gains_reciprocal(int n, const float *d_in, float *d_out, sdp_stream s);
 /* out[i] = sum_{j<slices} grids[j*G*G + i] in slice order */
 int sdp_merge_grids(int grid_size, int slices, const float *d_grids, float *d_out, sdp_stream s);
 

@@ -583,6 +583,19 @@ void oracle_subtract_from_measurements(int NUM_RECEIVERS, int NUM_BASELINES, int
     }
 }
 
+/* reciprocal_transform_actor, gains_apply_run.c:4-17: each gain's
+ * reciprocal as conj(z) / (x^2 + y^2), every operation in float */
+void oracle_reciprocal_transform(int NUM_RECEIVERS, const float2 *gains_in, float2 *gains_out)
+{
+    for (int i = 0; i < NUM_RECEIVERS; i++) {
+        const float2 z = gains_in[i];
+        const float re = z.x / (z.x * z.x + z.y * z.y);
+        const float im = z.y / (z.x * z.x + z.y * z.y);
+        gains_out[i].x = re;
+        gains_out[i].y = -im;
+    }
+}
+
 /* apply_gains_only, gains_apply_run.c:98-118 */
 void oracle_apply_gains_only(int NUM_RECEIVERS, int NUM_BASELINES, int NUM_VISIBILITIES,
         const float2 *measured_vis, const float2 *gains, const int2 *receiver_pairs,

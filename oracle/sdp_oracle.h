@@ -83,6 +83,7 @@ void oracle_subtract_from_measurements_finegrid(int NUM_VISIBILITIES, const floa
 void oracle_subtract_from_measurements(int NUM_RECEIVERS, int NUM_BASELINES, int NUM_VISIBILITIES,
         const float2 *measured_vis, const float2 *visibilities_in, const float2 *gains,
         const int2 *receiver_pairs, float2 *visibilities_out);
+void oracle_reciprocal_transform(int NUM_RECEIVERS, const float2 *gains_in, float2 *gains_out);
 void oracle_apply_gains_only(int NUM_RECEIVERS, int NUM_BASELINES, int NUM_VISIBILITIES,
         const float2 *measured_vis, const float2 *gains, const int2 *receiver_pairs,
         float2 *visibilities_out);

@@ -51,6 +51,7 @@ def load():
             "oracle_subtract_from_measurements_finegrid": [I, P, P, P, P],
             "oracle_subtract_from_measurements": [I, I, I, P, P, P, P, P],
             "oracle_apply_gains_only": [I, I, I, P, P, P, P],
+            "oracle_reciprocal_transform": [I, P, P],
             "oracle_add_visibilities": [I, P, P, P],
             "oracle_merge_gridded_visibilities": [I, I, P, P],
             "oracle_split_fftdegrid_counts": [I, I, I, P],
@@ -231,6 +232,14 @@ def subtract_from_measurements(measured, pred, gains, pairs):
     out = np.empty_like(measured)
     load().oracle_subtract_from_measurements(gains.shape[0], pairs.shape[0], measured.shape[0], _p(measured),
                                              _p(pred), _p(gains), _p(pairs), _p(out))
+    return out
+
+
+def reciprocal_transform(gains):
+    """reciprocal_transform_actor, gains_apply_run.c:4-17."""
+    gains = _c(gains, np.float32)
+    out = np.empty_like(gains)
+    load().oracle_reciprocal_transform(gains.shape[0], _p(gains), _p(out))
     return out
 
 

@@ -357,6 +357,18 @@ static void gains_actor(int NUM_RECEIVERS, int NUM_BASELINES, int NUM_VISIBILITI
     unlock();
 }
 
+void reciprocal_transform_actor(int NUM_RECEIVERS, PRECISION2 *gains_in, Config *config, PRECISION2 *gains_out)
+{
+    (void)config;
+    const size_t b = sizeof(float2) * (size_t)NUM_RECEIVERS;
+    lock();
+    float *d_i = upload(B_IN0, gains_in, b);
+    float *d_o = dev(B_OUT0, b);
+    CHECK(sdp_gains_reciprocal(NUM_RECEIVERS, d_i, d_o, NULL));
+    download(gains_out, d_o, b);
+    unlock();
+}
+
 void subtract_from_measurements(int NUM_RECEIVERS, int NUM_BASELINES, int NUM_VISIBILITIES,
         PRECISION2 *measured_vis, PRECISION2 *visibilities_in, PRECISION2 *gains, int2 *receiver_pairs,
         Config *config, PRECISION2 *visibilities_out)

@@ -107,6 +107,18 @@ __device__ __forceinline__ float2 cmul_apply(float2 a, float2 b)
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// reciprocal_transform_actor (gains_apply_run.c:4-17): out = conj(z) / |z|^2
+// = 1 / z, in the reference's float order (|z|^2 formed for each part)
+__global__ void reciprocal_kernel(const float2 *in, int n, float2 *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 z = in[i];
+    const float re = z.x / (z.x * z.x + z.y * z.y);
+    const float im = z.y / (z.x * z.x + z.y * z.y);
+    out[i] = make_float2(re, -im);
+}
+
 // subtract_from_measurements / apply_gains_only (gains_apply_run.c:79-92,105-115)
 __global__ void gains_kernel(const float2 *measured, const float2 *pred, const float2 *gains, const int2 *pairs,
                              int n, int n_baselines, int apply_only, float2 *out)
@@ -290,6 +302,14 @@ int sdp_vis_subtract_gains(int n, int n_baselines, const float *d_measured, cons
     gains_kernel<<<grid_for(n), kBlock, 0, sdp::stream_of(s)>>>((const float2 *)d_measured, (const float2 *)d_pred,
                                                                 (const float2 *)d_gains, (const int2 *)d_pairs, n,
                                                                 n_baselines, apply_only, (float2 *)d_out);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_gains_reciprocal(int n, const float *d_in, float *d_out, sdp_stream s)
+{
+    if (n <= 0) return SDP_OK;
+    reciprocal_kernel<<<grid_for(n), kBlock, 0, sdp::stream_of(s)>>>((const float2 *)d_in, n, (float2 *)d_out);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }

@@ -92,12 +92,6 @@ __device__ __forceinline__ int kernel_tap(int cell_abs, float pos, float os)
     return (int)t + (y - t >= 0.5f ? 1 : 0);
 }
 
-__device__ __forceinline__ float wave_sum(float v)
-{
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return v;
-}
-
 // ---- plan construction ---------------------------------------------------
 
 struct PlanArgs {
@@ -213,7 +207,6 @@ __global__ void permute_kernel(const float2 *in, const uint32_t *perm, int n, fl
 struct ChunkArgs {
     int G, OS, tile, ntx, chunk_base;
     int lane_shift;   // log2 of the lanes per visibility (64 / copies)
-    int col_shift;    // log2 of the footprint columns a group covers at once
     int copies;       // lane groups (and sub-grid copies) per wavefront
     int os_shift;     // log2 OS when OS is a power of two, else -1
     int tab_bytes;    // size of one phase-major kernel table
@@ -1037,7 +1030,7 @@ int launch_class(const sdp_grid_plan *plan, int c, const ChunkIO &io, const uint
     int os_shift = -1;
     for (int k = 0; k < 12; ++k)
         if ((1 << k) == OS) os_shift = k;
-    ChunkArgs a{plan->p.grid_size, OS, plan->tile, plan->ntx, b, lane_shift, col_shift, copies, os_shift,
+    ChunkArgs a{plan->p.grid_size, OS, plan->tile, plan->ntx, b, lane_shift, copies, os_shift,
                 (int)(8 * plan->table_len), lds_planes(plan->p.n_kernels),
                 (int)lds_window_bytes(MODE, S, copies, COMP)};
     const size_t lds = lds_bytes(MODE, S, copies, COMP, plan->p.n_kernels);

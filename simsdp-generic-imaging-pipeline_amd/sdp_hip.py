@@ -165,6 +165,7 @@ def load() -> ctypes.CDLL:
         "sdp_convolution_correct": (c_int, [c_int, P, P, P, P]),
         "sdp_vis_subtract": (c_int, [c_int, P, P, P, P]), "sdp_vis_add": (c_int, [c_int, P, P, P, P]),
         "sdp_vis_subtract_gains": (c_int, [c_int, c_int, P, P, P, P, c_int, P, P]),
+        "sdp_gains_reciprocal": (c_int, [c_int, P, P, P]),
         "sdp_merge_grids": (c_int, [c_int, c_int, P, P, P]),
         "sdp_hogbom_work_bytes": (c_size_t, [c_int]),
         "sdp_hogbom": (c_int, [c_int, c_int, c_int, P, P, c_double, c_double, c_double, c_double, c_double,
@@ -790,6 +791,7 @@ class Actors:
             "execute_inv_convolution_correction_actor": [c_int, P, P, cfg, P],
             "subtract_from_measurements": [c_int, c_int, c_int, P, P, P, P, cfg, P],
             "apply_gains_only": [c_int, c_int, c_int, P, P, P, cfg, P],
+            "reciprocal_transform_actor": [c_int, P, cfg, P],
             "subtract_from_measurements_finegrid": [c_int, P, P, ip, P],
             "merge_gridded_visibilities": [c_int, c_int, P, P],
             "split_uvgrid": [c_int, c_int, P, P],
@@ -893,6 +895,12 @@ class Actors:
         out = np.empty_like(measured)
         self.lib.subtract_from_measurements(gains.shape[0], pairs.shape[0], measured.shape[0], _ptr(measured),
                                             _ptr(pred), _ptr(gains), _ptr(pairs), ctypes.byref(cfg), _ptr(out))
+        return out
+
+    def reciprocal_transform_actor(self, gains, cfg):
+        gains = np.ascontiguousarray(gains, dtype=np.float32)
+        out = np.empty_like(gains)
+        self.lib.reciprocal_transform_actor(gains.shape[0], _ptr(gains), ctypes.byref(cfg), _ptr(out))
         return out
 
     def apply_gains_only(self, measured, gains, pairs, cfg):

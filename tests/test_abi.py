@@ -48,7 +48,8 @@ def test_actor_prototypes_cover_reference_symbols():
               "CUFFT_EXECUTE_INVERSE_C2C_actor", "fft_shift_complex_to_real_actor",
               "fft_shift_real_to_complex_actor", "execute_convolution_correction_actor",
               "execute_inv_convolution_correction_actor", "subtract_from_measurements_finegrid",
-              "subtract_from_measurements", "split_fftdegrid_visibilities", "split_uvgrid",
+              "subtract_from_measurements", "apply_gains_only", "reciprocal_transform_actor",
+              "split_fftdegrid_visibilities", "split_uvgrid",
               "merge_gridded_visibilities", "correction_set_up", "psf_host_set_up", "config_struct_set_up_sequel",
               "kernel_host_set_up"):
         assert n in names, n

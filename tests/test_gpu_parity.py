@@ -302,6 +302,17 @@ def test_oversampling_not_a_power_of_two(hip, actors, oracle, rng, OS):
     assert np.abs(deg - ref).max() <= 2e-6 * np.abs(ref).max()
 
 
+def test_reciprocal_transform_bit_exact(hip, actors, oracle, rng):
+    """The correct_to_finegrid subgraph's first actor: each receiver gain's
+    reciprocal, bit for bit (IEEE float division on both sides)."""
+    g = rng.normal(size=(512, 2)).astype(np.float32)
+    g[0] = (0.0, 0.0)                                     # 0 / 0: NaN on both sides
+    cfg = hip.make_config(64, 1e-4)
+    got = actors.reciprocal_transform_actor(g, cfg)
+    ref = oracle.reciprocal_transform(g)
+    np.testing.assert_array_equal(got, ref)
+
+
 def test_degrid_fused_subtract(hip, case, rng):
     d = case
     G = d.grid_size

@@ -336,6 +336,21 @@ def test_visibility_arithmetic(oracle, rng):
     assert np.all(d[5:] == 0)
 
 
+def test_reciprocal_transform(oracle, rng):
+    """reciprocal_transform_actor returns 1 / z: known answers, z * (1 / z) = 1
+    to float rounding, and the FP64 complex reciprocal."""
+    z = np.array([[2.0, 0.0], [0.0, 1.0], [3.0, 4.0], [-1.0, -1.0]], np.float32)
+    got = oracle.reciprocal_transform(z)
+    np.testing.assert_array_equal(got, np.array([[0.5, 0.0], [0.0, -1.0], [np.float32(3) / np.float32(25),
+                                                 -(np.float32(4) / np.float32(25))], [-0.5, 0.5]], np.float32))
+    z = rng.normal(size=(1000, 2)).astype(np.float32)
+    got = oracle.reciprocal_transform(z)
+    zc = z[:, 0].astype(np.float64) + 1j * z[:, 1]
+    gc = got[:, 0].astype(np.float64) + 1j * got[:, 1]
+    assert np.abs(zc * gc - 1).max() < 1e-6
+    np.testing.assert_allclose(gc, 1 / zc, rtol=1e-6)
+
+
 def test_merge_and_split(oracle, rng):
     G, S = 8, 4
     grids = rng.normal(size=(S * G * G, 2)).astype(np.float32)
