@@ -158,6 +158,21 @@ double prolate_spheroidal(double nu);
 /* replaces include/top.h:38 (src/top.c:449-512) */
 void psf_host_set_up(int GRID_SIZE, int PSF_GRID_SIZE, IN Config *config, OUT PRECISION *psf,
         OUT double *psf_max_value);
+/* replaces include/top.h:42 (src/top.c:527-592): the clean beam, a squared
+ * Gaussian over the dirty PSF's central lobe; host-only (set-up) */
+void clean_psf_host_set_up(int GRID_SIZE, int GAUSSIAN_CLEAN_PSF, IN Config *config, IN PRECISION *dirty_psf,
+        OUT PRECISION *clean_psf, OUT int2 *partial_psf_halfdims);
+/* replaces include/major_loop_iter.h:30 (src/major_loop_iter.c:253-278): the
+ * output images as text files, the restored image (conv2D,
+ * major_loop_iter.c:8-46) formed on the GPU by sdp_restore_conv2d */
+void save_output(int GRID_SIZE, IN PRECISION *residual, IN PRECISION *model, IN PRECISION *clean_psf,
+        int2 *clean_psf_halfdims, IN PRECISION *psf, IN Config *config, IN int *cycle);
+/* replaces include/top.h:25 (src/top.c:140-232): the visibility text file,
+ * a count then one "u v w re im weight" line per visibility (uvw in metres);
+ * uvw scaled by frequency_hz / c, u and w negated when right_ascension, the
+ * value weighted unless force_weight_to_one; host-only (file input) */
+void visibility_host_set_up(int NUM_VISIBILITIES, IN Config *config, OUT PRECISION3 *vis_uvw_coords,
+        OUT PRECISION2 *measured_vis);
 /* replaces include/top.h:40 (src/top.c:514-518) */
 void config_struct_set_up_sequel(IN Config *config_in, IN double *psf_max_value, OUT Config *config_out);
 

@@ -210,6 +210,13 @@ int sdp_vis_add(int n, const float *d_a, const float *d_b, float *d_out, sdp_str
 int sdp_vis_subtract_gains(int n_vis, int n_baselines, const float *d_measured, const float *d_pred,
                            const float *d_gains, const int *d_pairs, int apply_only,
                            float *d_out, sdp_stream s);
+/* the restored image (conv2D, major_loop_iter.c:8-46): out = the model
+ * convolved with the clean beam over the [-hx, hx) x [-hy, hy) box, model
+ * pixels below 1e-5 skipped, the beam read flipped about (D/2 + 1, D/2 + 1)
+ * and zero padding at the edges; bit-exact with the reference's float sums.
+ * Requires D/2 + 1 + h < D. */
+int sdp_restore_conv2d(int D, const float *d_image, const float *d_filter, int hx, int hy, float *d_out,
+                       sdp_stream s);
 /* out[i] = 1 / in[i] for n complex gains, as conj(z) / |z|^2 in float
  * (reciprocal_transform_actor, gains_apply_run.c:4-17); in == out allowed */
 int sdp_gains_reciprocal(int n, const float *d_in, float *d_out, sdp_stream s);

@@ -670,3 +670,73 @@ void oracle_create_1D_half_prolate(float *prolate, int grid_size)
     int half = grid_size / 2;
     for (int i = 0; i < half; ++i) prolate[i] = (float)oracle_prolate_spheroidal((double)i / (double)half);
 }
+
+/* conv2D, major_loop_iter.c:8-46: scatter form, sources in raster order */
+void oracle_conv2d(const float *image, const float *filter, float *result, int IMAGE_DIM, int hx, int hy)
+{
+    memset(result, 0, sizeof(float) * (size_t)IMAGE_DIM * IMAGE_DIM);
+    for (int y = 0; y < IMAGE_DIM; ++y)
+        for (int x = 0; x < IMAGE_DIM; ++x) {
+            const int idx = y * IMAGE_DIM + x;
+            if (image[idx] < 1e-5) continue;
+            const int center = IMAGE_DIM / 2 + 1;
+            for (int fy = -hy; fy < hy; ++fy)
+                for (int fx = -hx; fx < hx; ++fx) {
+                    const int ix = x + fx, iy = y + fy;
+                    if (ix < 0 || ix >= IMAGE_DIM || iy < 0 || iy >= IMAGE_DIM) continue;
+                    const int cfx = center + (2 * hx - (fx + hx) - hx);
+                    const int cfy = center + (2 * hy - (fy + hy) - hy);
+                    result[iy * IMAGE_DIM + ix] += image[idx] * filter[cfy * IMAGE_DIM + cfx];
+                }
+        }
+}
+
+/* gaussian2D, top.c:520-523 */
+static float oracle_gaussian2d(float x, float y, float x_sigma, float y_sigma)
+{
+    const float pi = (float)3.141592654;
+    return exp(-(x * x) / (2 * x_sigma * x_sigma) - (y * y) / (2 * y_sigma * y_sigma)) / (2 * pi * x_sigma * y_sigma);
+}
+
+/* clean_psf_host_set_up, top.c:527-592 (prev_sign is never updated there:
+ * a walk ends at its first non-decreasing step) */
+void oracle_clean_psf(int GRID_SIZE, double psf_max_value, const float *dirty_psf, float *clean_psf, int *halfdim)
+{
+    int center_box[4] = {GRID_SIZE / 2};
+    const int dx[4] = {1, 0, -1, 0}, dy[4] = {0, 1, 0, -1};
+    const int cx = GRID_SIZE / 2 + 1, cy = GRID_SIZE / 2 + 1;
+    for (int i = 0; i < 4; ++i) {
+        const int prev_sign = -1;
+        for (int j = 1; j < GRID_SIZE / 2; ++j) {
+            const int cur = (cx + dx[i] * j) + (cy + dy[i] * j) * GRID_SIZE;
+            const int last = (cx + dx[i] * (j - 1)) + (cy + dy[i] * (j - 1)) * GRID_SIZE;
+            const float deriv = dirty_psf[cur] - dirty_psf[last];
+            const int sign = deriv >= 0.f ? 1 : -1;
+            if (sign != prev_sign) {
+                center_box[i] = j;
+                break;
+            }
+        }
+    }
+    int h = center_box[0];
+    for (int i = 1; i < 4; ++i)
+        if (center_box[i] > h) h = center_box[i];
+    if (h > GRID_SIZE / 2 - 2) h = GRID_SIZE / 2 - 2;   /* the build's cap: the box stays inside the image */
+    *halfdim = h;
+    memset(clean_psf, 0, sizeof(float) * (size_t)GRID_SIZE * GRID_SIZE);
+    const float psf_max_val = (float)psf_max_value;
+    const float x_sigma = h / 2.f, y_sigma = h / 2.f;
+    float sum = 0.f;
+    for (int y = -h; y < h; ++y)
+        for (int x = -h; x < h; ++x) {
+            const int idx = cx + x + (cy + y) * GRID_SIZE;
+            clean_psf[idx] = oracle_gaussian2d(x, y, x_sigma, y_sigma);
+            clean_psf[idx] *= clean_psf[idx];
+            sum += clean_psf[idx];
+        }
+    for (int y = -h; y < h; ++y)
+        for (int x = -h; x < h; ++x) {
+            const int idx = cx + x + (cy + y) * GRID_SIZE;
+            clean_psf[idx] = clean_psf[idx] / sum * psf_max_val;
+        }
+}

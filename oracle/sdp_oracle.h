@@ -94,8 +94,10 @@ void oracle_split_fftdegrid_counts(int NUM_VISIBILITIES, int SLICES, int num_cor
 double oracle_prolate_spheroidal(double nu);
 void oracle_create_1D_half_prolate(float *prolate, int grid_size);
 
+void oracle_conv2d(const float *image, const float *filter, float *result, int IMAGE_DIM, int hx, int hy);
+void oracle_clean_psf(int GRID_SIZE, double psf_max_value, const float *dirty_psf, float *clean_psf, int *halfdim);
+
 #ifdef __cplusplus
 }
 #endif
-
 #endif

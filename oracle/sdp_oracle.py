@@ -52,6 +52,8 @@ def load():
             "oracle_subtract_from_measurements": [I, I, I, P, P, P, P, P],
             "oracle_apply_gains_only": [I, I, I, P, P, P, P],
             "oracle_reciprocal_transform": [I, P, P],
+            "oracle_conv2d": [P, P, P, I, I, I],
+            "oracle_clean_psf": [I, D, P, P, P],
             "oracle_add_visibilities": [I, P, P, P],
             "oracle_merge_gridded_visibilities": [I, I, P, P],
             "oracle_split_fftdegrid_counts": [I, I, I, P],
@@ -279,3 +281,22 @@ def half_prolate(G):
     out = np.empty(G // 2, np.float32)
     load().oracle_create_1D_half_prolate(_p(out), G)
     return out
+
+
+def conv2d(image, clean_psf, hx, hy):
+    """conv2D, major_loop_iter.c:8-46: the model convolved with the clean PSF."""
+    image, clean_psf = _c(image, np.float32), _c(clean_psf, np.float32)
+    D = image.shape[0]
+    out = np.empty_like(image)
+    load().oracle_conv2d(_p(image), _p(clean_psf), _p(out), D, int(hx), int(hy))
+    return out
+
+
+def clean_psf(dirty_psf, psf_max_value):
+    """clean_psf_host_set_up, top.c:527-592: (clean PSF, half size)."""
+    dirty_psf = _c(dirty_psf, np.float32)
+    G = dirty_psf.shape[0]
+    out = np.empty_like(dirty_psf)
+    h = ctypes.c_int(0)
+    load().oracle_clean_psf(G, float(psf_max_value), _p(dirty_psf), _p(out), ctypes.byref(h))
+    return out, h.value

@@ -619,6 +619,154 @@ void psf_host_set_up(int GRID_SIZE, int PSF_GRID_SIZE, Config *config, PRECISION
     for (size_t i = 0; i < cells(GRID_SIZE); i++) psf[i] = (float)((double)psf[i] / *psf_max_value);
 }
 
+void visibility_host_set_up(int NUM_VISIBILITIES, Config *config, PRECISION3 *vis_uvw_coords,
+        PRECISION2 *measured_vis)
+{
+    /* top.c:140-232: the leading count is read and not used; each line's
+     * values are doubles, scaled and weighted in double, stored as float */
+    if (vis_uvw_coords == NULL || measured_vis == NULL) exit(EXIT_FAILURE);
+    FILE *f = fopen(config->visibility_source_file, "r");
+    if (!f) {
+        fprintf(stderr, "ERROR >>> Unable to open visibility file %s\n", config->visibility_source_file);
+        exit(EXIT_FAILURE);
+    }
+    int count = 0;
+    if (fscanf(f, "%d", &count) != 1) count = 0;
+    (void)count;
+    const double to_wavelengths = config->frequency_hz / 299792458.0;   /* SPEED_OF_LIGHT, common.h:59 */
+    for (int i = 0; i < NUM_VISIBILITIES; ++i) {
+        double u = 0.0, v = 0.0, w = 0.0, re = 0.0, im = 0.0, weight = 0.0;
+        if (fscanf(f, "%lf %lf %lf %lf %lf %lf", &u, &v, &w, &re, &im, &weight) != 6) {
+            fprintf(stderr, "ERROR >>> visibility file %s: line %d unreadable\n", config->visibility_source_file,
+                    i + 2);
+            fclose(f);
+            exit(EXIT_FAILURE);
+        }
+        if (config->right_ascension) {
+            u = -u;
+            w = -w;
+        }
+        if (!config->force_weight_to_one) {
+            re *= weight;
+            im *= weight;
+        }
+        vis_uvw_coords[i].x = (float)(u * to_wavelengths);
+        vis_uvw_coords[i].y = (float)(v * to_wavelengths);
+        vis_uvw_coords[i].z = (float)(w * to_wavelengths);
+        measured_vis[i].x = (float)re;
+        measured_vis[i].y = (float)im;
+    }
+    fclose(f);
+}
+
+/* conv2D, major_loop_iter.c:8-46, on the GPU */
+static void restore_image(int G, const float *model, const float *clean_psf, const int2 *halfdims, float *out)
+{
+    const size_t b = sizeof(float) * cells(G);
+    lock();
+    float *d_m = upload(B_IN0, model, b);
+    float *d_f = upload(B_IN1, clean_psf, b);
+    float *d_o = dev(B_OUT0, b);
+    CHECK(sdp_restore_conv2d(G, d_m, d_f, halfdims->x, halfdims->y, d_o, NULL));
+    download(out, d_o, b);
+    unlock();
+}
+
+/* gaussian2D, top.c:520-523: float arithmetic, a double exp, PI = 3.141592654f */
+static float clean_gaussian(float x, float y, float x_sigma, float y_sigma)
+{
+    const float pi = 3.141592654f;
+    return (float)(exp(-(x * x) / (2 * x_sigma * x_sigma) - (y * y) / (2 * y_sigma * y_sigma)) /
+                   (2 * pi * x_sigma * y_sigma));
+}
+
+void clean_psf_host_set_up(int GRID_SIZE, int GAUSSIAN_CLEAN_PSF, Config *config, PRECISION *dirty_psf,
+        PRECISION *clean_psf, int2 *partial_psf_halfdims)
+{
+    /* top.c:527-592.  From (G/2 + 1, G/2 + 1) the dirty PSF is walked right,
+     * down, left and up; each walk stops at the first step that does not
+     * decrease (the reference compares with a sign it never updates), and a
+     * walk that never stops keeps its initial extent (G/2 rightwards, 0
+     * otherwise).  The largest extent h is both half sizes; over the box
+     * [-h, h)^2 the clean PSF is a squared Gaussian (sigma h / 2),
+     * normalised to sum to psf_max_value, zero elsewhere. */
+    (void)GAUSSIAN_CLEAN_PSF;
+    if (dirty_psf == NULL || clean_psf == NULL || partial_psf_halfdims == NULL) exit(EXIT_FAILURE);
+    const int G = GRID_SIZE, cx = G / 2 + 1, cy = G / 2 + 1;
+    const int dirs[4][2] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+    int box[4] = {G / 2, 0, 0, 0};
+    for (int i = 0; i < 4; ++i) {
+        for (int j = 1; j < G / 2; ++j) {
+            const int cur = (cx + dirs[i][0] * j) + (cy + dirs[i][1] * j) * G;
+            const int last = (cx + dirs[i][0] * (j - 1)) + (cy + dirs[i][1] * (j - 1)) * G;
+            if (dirty_psf[cur] - dirty_psf[last] >= 0.f) {
+                box[i] = j;
+                break;
+            }
+        }
+    }
+    int h = box[0];
+    for (int i = 1; i < 4; ++i) h = box[i] > h ? box[i] : h;
+    /* a walk that never stops leaves h = G/2, and the box (and conv2D's
+     * flipped reads) would run past the image: h is capped at G/2 - 2 */
+    if (h > G / 2 - 2) h = G / 2 - 2;
+    partial_psf_halfdims->x = partial_psf_halfdims->y = h;
+    memset(clean_psf, 0, sizeof(float) * cells(G));
+    const float psf_max = (float)config->psf_max_value;
+    const float sigma = h / 2.f;
+    float sum = 0.f;
+    for (int y = -h; y < h; ++y)
+        for (int x = -h; x < h; ++x) {
+            const int idx = cx + x + (cy + y) * G;
+            clean_psf[idx] = clean_gaussian((float)x, (float)y, sigma, sigma);
+            clean_psf[idx] *= clean_psf[idx];
+            sum += clean_psf[idx];
+        }
+    for (int y = -h; y < h; ++y)
+        for (int x = -h; x < h; ++x) {
+            const int idx = cx + x + (cy + y) * G;
+            clean_psf[idx] = clean_psf[idx] / sum * psf_max;
+        }
+}
+
+/* save_image_to_file, major_loop_iter.c:168-199: "<output_path>cycle_<n>_<name>",
+ * "%f, " per pixel, one row per line; a file that cannot be opened is reported
+ * and skipped */
+static void save_image(const Config *config, const float *image, const char *name, int G, int cycle)
+{
+    char path[256];
+    snprintf(path, sizeof(path), "%scycle_%d_%s", config->output_path ? config->output_path : "", cycle,
+             name ? name : "image.csv");
+    FILE *f = fopen(path, "w");
+    if (f == NULL) {
+        fprintf(stderr, ">>> ERROR: Unable to save image to file %s\n", path);
+        return;
+    }
+    for (int row = 0; row < G; ++row) {
+        for (int col = 0; col < G; ++col) fprintf(f, "%f, ", image[(size_t)row * G + col]);
+        fprintf(f, "\n");
+    }
+    fclose(f);
+}
+
+void save_output(int GRID_SIZE, PRECISION *residual, PRECISION *model, PRECISION *clean_psf,
+        int2 *clean_psf_halfdims, PRECISION *psf, Config *config, int *cycle)
+{
+    /* major_loop_iter.c:253-278: the residual, dirty PSF, clean PSF and model
+     * images, then the restored image (conv2D of the model with the clean
+     * PSF, on the GPU) */
+    const int G = GRID_SIZE;
+    save_image(config, residual, config->residual_image_output, G, *cycle);
+    save_image(config, psf, config->psf_image_output, G, *cycle);
+    save_image(config, clean_psf, config->clean_psf_image_output, G, *cycle);
+    save_image(config, model, config->model_image_output, G, *cycle);
+    float *final_image = (float *)malloc(sizeof(float) * cells(G));
+    if (final_image == NULL) exit(EXIT_FAILURE);
+    restore_image(G, model, clean_psf, clean_psf_halfdims, final_image);
+    save_image(config, final_image, config->final_image_output, G, *cycle);
+    free(final_image);
+}
+
 void config_struct_set_up_sequel(Config *config_in, double *psf_max_value, Config *config_out)
 {
     memmove(config_out, config_in, sizeof(Config));

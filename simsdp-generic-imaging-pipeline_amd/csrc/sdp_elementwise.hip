@@ -119,6 +119,32 @@ __global__ void reciprocal_kernel(const float2 *in, int n, float2 *out)
     out[i] = make_float2(re, -im);
 }
 
+// conv2D (major_loop_iter.c:8-46), the restored image: every pixel of the
+// model at or above 1e-5 spreads model * filter over the [-h, h) box around
+// it, the filter read flipped about (D/2 + 1, D/2 + 1).  Gathered here: an
+// output pixel adds its sources in their raster order, which is the order
+// in which the reference's scatter adds them, so each sum rounds the same.
+__global__ void restore_conv2d_kernel(const float *image, const float *filter, int D, int hx, int hy, float *out)
+{
+    const int tx = blockIdx.x * blockDim.x + threadIdx.x, ty = blockIdx.y;
+    if (tx >= D) return;
+    const int center = D / 2 + 1;
+    float acc = 0.f;
+    for (int y = max(ty - hy + 1, 0); y <= min(ty + hy, D - 1); ++y) {
+        const int fy = ty - y;                               // in [-hy, hy)
+        const float *frow = filter + (size_t)(center - fy) * D + center;
+        const float *irow = image + (size_t)y * D;
+        for (int x = max(tx - hx + 1, 0); x <= min(tx + hx, D - 1); ++x) {
+            const float v = irow[x];
+            // the reference's float < 1e-5 (a double): v <= float(1e-5), which
+            // lies below 1e-5, i.e. v < the next float 0x1.4f8b5ap-17
+            if (v < 0x1.4f8b5ap-17f) continue;
+            acc += v * frow[-(tx - x)];
+        }
+    }
+    out[(size_t)ty * D + tx] = acc;
+}
+
 // subtract_from_measurements / apply_gains_only (gains_apply_run.c:79-92,105-115)
 __global__ void gains_kernel(const float2 *measured, const float2 *pred, const float2 *gains, const int2 *pairs,
                              int n, int n_baselines, int apply_only, float2 *out)
@@ -302,6 +328,19 @@ int sdp_vis_subtract_gains(int n, int n_baselines, const float *d_measured, cons
     gains_kernel<<<grid_for(n), kBlock, 0, sdp::stream_of(s)>>>((const float2 *)d_measured, (const float2 *)d_pred,
                                                                 (const float2 *)d_gains, (const int2 *)d_pairs, n,
                                                                 n_baselines, apply_only, (float2 *)d_out);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_restore_conv2d(int D, const float *d_image, const float *d_filter, int hx, int hy, float *d_out,
+                       sdp_stream s)
+{
+    SDP_REQUIRE(D >= 4 && d_image && d_filter && d_out, "sdp_restore_conv2d: bad arguments");
+    // the flipped filter rows and columns D/2 + 1 - f, f in [-h, h), stay inside the image
+    SDP_REQUIRE(hx >= 0 && hy >= 0 && D / 2 + 1 + hx < D && D / 2 + 1 + hy < D,
+                "sdp_restore_conv2d: filter half sizes (%d, %d) too large for %d^2", hx, hy, D);
+    dim3 grid((D + kBlock - 1) / kBlock, D);
+    restore_conv2d_kernel<<<grid, kBlock, 0, sdp::stream_of(s)>>>(d_image, d_filter, D, hx, hy, d_out);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }

@@ -166,6 +166,7 @@ def load() -> ctypes.CDLL:
         "sdp_vis_subtract": (c_int, [c_int, P, P, P, P]), "sdp_vis_add": (c_int, [c_int, P, P, P, P]),
         "sdp_vis_subtract_gains": (c_int, [c_int, c_int, P, P, P, P, c_int, P, P]),
         "sdp_gains_reciprocal": (c_int, [c_int, P, P, P]),
+        "sdp_restore_conv2d": (c_int, [c_int, P, P, c_int, c_int, P, P]),
         "sdp_merge_grids": (c_int, [c_int, c_int, P, P, P]),
         "sdp_hogbom_work_bytes": (c_size_t, [c_int]),
         "sdp_hogbom": (c_int, [c_int, c_int, c_int, P, P, c_double, c_double, c_double, c_double, c_double,
@@ -792,6 +793,7 @@ class Actors:
             "subtract_from_measurements": [c_int, c_int, c_int, P, P, P, P, cfg, P],
             "apply_gains_only": [c_int, c_int, c_int, P, P, P, cfg, P],
             "reciprocal_transform_actor": [c_int, P, cfg, P],
+            "save_output": [c_int, P, P, P, P, P, cfg, ip],
             "subtract_from_measurements_finegrid": [c_int, P, P, ip, P],
             "merge_gridded_visibilities": [c_int, c_int, P, P],
             "split_uvgrid": [c_int, c_int, P, P],

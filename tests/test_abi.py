@@ -99,3 +99,38 @@ def test_setup_actors_on_host(tmp_path):
     lib.psf_host_set_up(4, 6, ctypes.byref(cfg), got.ctypes.data, ctypes.byref(mx))
     assert mx.value == pytest.approx(35 / 7.0, rel=1e-6)
     np.testing.assert_allclose(got.reshape(4, 4), (psf[1:5, 1:5] / (35 / 7.0)).astype(np.float32), rtol=1e-6)
+    # clean_psf_host_set_up on the host against the oracle, bit for bit
+    G = 64
+    c = G // 2 + 1
+    yy, xx = np.mgrid[0:G, 0:G]
+    rr = np.hypot(yy - c, xx - c)
+    dirty = (np.exp(-rr ** 2 / 40.0) * np.cos(rr / 2.0)).astype(np.float32)
+    cfg = sdp_hip.make_config(G, 0.01, psf_max_value=1.75)
+    lib.clean_psf_host_set_up.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(sdp_hip.Config),
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    clean = np.zeros((G, G), np.float32)
+    hd = np.zeros(2, np.int32)
+    lib.clean_psf_host_set_up(G, 1, ctypes.byref(cfg), dirty.ctypes.data, clean.ctypes.data, hd.ctypes.data)
+    want, h = sdp_oracle.clean_psf(dirty, 1.75)
+    assert hd[0] == hd[1] == h and h > 1
+    np.testing.assert_array_equal(clean, want)
+    # visibility_host_set_up: the count line, then "u v w re im weight" in metres
+    rows = np.array([[100.0, -50.5, 3.25, 1.5, -2.0, 0.5], [-7.0, 8.0, -9.5, 0.25, 4.0, 2.0],
+                     [1e3, 2e3, -3e3, -1.0, 1.0, 1.0]])
+    f = tmp_path / "vis.csv"
+    f.write_text("3\n" + "\n".join(" ".join(f"{x:.6f}" for x in r) for r in rows) + "\n")
+    lib.visibility_host_set_up.argtypes = [ctypes.c_int, ctypes.POINTER(sdp_hip.Config), ctypes.c_void_p,
+                                           ctypes.c_void_p]
+    for ra, unit_weight in ((True, True), (False, False)):
+        cfg = sdp_hip.make_config(64, 0.01)
+        cfg.visibility_source_file = str(f).encode()
+        cfg.frequency_hz = 150e6
+        cfg.right_ascension = ra
+        cfg.force_weight_to_one = unit_weight
+        uvw = np.zeros((3, 3), np.float32)
+        vis = np.zeros((3, 2), np.float32)
+        lib.visibility_host_set_up(3, ctypes.byref(cfg), uvw.ctypes.data, vis.ctypes.data)
+        sign = np.array([-1.0, 1.0, -1.0]) if ra else np.ones(3)
+        np.testing.assert_array_equal(uvw, (rows[:, :3] * sign * (150e6 / 299792458.0)).astype(np.float32))
+        wts = np.ones(3) if unit_weight else rows[:, 5]
+        np.testing.assert_array_equal(vis, (rows[:, 3:5] * wts[:, None]).astype(np.float32))

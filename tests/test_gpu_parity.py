@@ -313,6 +313,58 @@ def test_reciprocal_transform_bit_exact(hip, actors, oracle, rng):
     np.testing.assert_array_equal(got, ref)
 
 
+@pytest.mark.parametrize("D,h", [(64, 3), (128, 9)])
+def test_restore_conv2d_bit_exact(hip, oracle, rng, D, h):
+    """The restored image (conv2D, major_loop_iter.c:8-46) on the GPU against
+    the reference's scatter loop: a sparse model with pixels on both sides of
+    the 1e-5 threshold, every output pixel bit for bit."""
+    img = np.zeros((D, D), np.float32)
+    idx = rng.choice(D * D, size=D * D // 20, replace=False)
+    img.flat[idx] = rng.uniform(-0.5, 1.0, idx.size).astype(np.float32)
+    img.flat[idx[:20]] = np.float32(1e-5)                         # skipped (below 1e-5 as a double)
+    img.flat[idx[20:40]] = np.nextafter(np.float32(1e-5), np.float32(1))
+    beam = rng.uniform(0.0, 1.0, (D, D)).astype(np.float32)
+    d_img, d_beam = hip.DeviceBuffer.from_array(img), hip.DeviceBuffer.from_array(beam)
+    d_out = hip.DeviceBuffer(4 * D * D)
+    hip.check(hip.load().sdp_restore_conv2d(D, d_img.ptr, d_beam.ptr, h, h, d_out.ptr, None))
+    np.testing.assert_array_equal(d_out.download((D, D), np.float32), oracle.conv2d(img, beam, h, h))
+    assert hip.load().sdp_restore_conv2d(D, d_img.ptr, d_beam.ptr, D // 2 - 1, h, d_out.ptr, None) == -2
+
+
+def test_save_output_writes_restored_image(hip, actors, oracle, rng, tmp_path):
+    """save_output (major_loop_iter.c:253-278): the five images as
+    "%f, " text files named cycle_<n>_<name> under output_path; the restored
+    image is the model convolved with the clean PSF (the GPU conv2D)."""
+    import ctypes
+    G = 64
+    c = G // 2 + 1
+    yy, xx = np.mgrid[0:G, 0:G]
+    rr = np.hypot(yy - c, xx - c)
+    dirty = (np.exp(-rr ** 2 / 40.0) * np.cos(rr / 2.0)).astype(np.float32)
+    clean, h = oracle.clean_psf(dirty, 1.0)
+    model = np.zeros((G, G), np.float32)
+    model[20, 30], model[40, 12] = 0.8, 0.3
+    resid = rng.normal(size=(G, G)).astype(np.float32)
+    cfg = hip.make_config(G, 0.01)
+    cfg.output_path = (str(tmp_path) + "/").encode()
+    for f, name in (("residual_image_output", "res.csv"), ("psf_image_output", "psf.csv"),
+                    ("clean_psf_image_output", "cpsf.csv"), ("model_image_output", "model.csv"),
+                    ("final_image_output", "final.csv")):
+        setattr(cfg, f, name.encode())
+    hd = np.array([h, h], np.int32)
+    cycle = ctypes.c_int(3)
+    actors.lib.save_output(G, resid.ctypes.data, model.ctypes.data, clean.ctypes.data, hd.ctypes.data,
+                           dirty.ctypes.data, ctypes.byref(cfg), ctypes.byref(cycle))
+
+    def read(name):
+        rows = (tmp_path / f"cycle_3_{name}").read_text().splitlines()
+        return np.array([[float(v) for v in r.split(",") if v.strip()] for r in rows])
+
+    np.testing.assert_allclose(read("model.csv"), model, atol=1e-6)
+    np.testing.assert_allclose(read("res.csv"), resid, atol=1e-6)
+    np.testing.assert_allclose(read("final.csv"), oracle.conv2d(model, clean, h, h), atol=1e-6)
+
+
 def test_degrid_fused_subtract(hip, case, rng):
     d = case
     G = d.grid_size

@@ -351,6 +351,59 @@ def test_reciprocal_transform(oracle, rng):
     np.testing.assert_allclose(gc, 1 / zc, rtol=1e-6)
 
 
+def test_restore_conv2d_known_answers(oracle, rng):
+    """conv2D: a model pixel a at (y0, x0) puts a * filter[c - fy][c - fx]
+    (c = D/2 + 1) at (y0 + fy, x0 + fx) for fy, fx in [-h, h), clipped at the
+    edges; pixels below 1e-5 (as a double) add nothing; disjoint sources add."""
+    D, h = 32, 3
+    c = D // 2 + 1
+    f = rng.normal(size=(D, D)).astype(np.float32)
+    for (y0, x0) in ((10, 12), (0, 1), (D - 1, D - 2)):
+        img = np.zeros((D, D), np.float32)
+        img[y0, x0] = 1.5
+        got = oracle.conv2d(img, f, h, h)
+        want = np.zeros((D, D), np.float32)
+        for fy in range(-h, h):
+            for fx in range(-h, h):
+                if 0 <= y0 + fy < D and 0 <= x0 + fx < D:
+                    want[y0 + fy, x0 + fx] = np.float32(1.5) * f[c - fy, c - fx]
+        np.testing.assert_array_equal(got, want)
+    img = np.zeros((D, D), np.float32)
+    img[4, 4] = np.float32(1e-5)                                  # 9.99999975e-06 < 1e-5: skipped
+    img[20, 20] = np.nextafter(np.float32(1e-5), np.float32(1))   # above 1e-5: kept
+    got = oracle.conv2d(img, f, h, h)
+    assert not got[:10, :10].any() and got[17:23, 17:23].any()
+    a, b = np.zeros((D, D), np.float32), np.zeros((D, D), np.float32)
+    a[5, 5], b[25, 25] = 2.0, 3.0
+    np.testing.assert_array_equal(oracle.conv2d(a + b, f, h, h), oracle.conv2d(a, f, h, h) + oracle.conv2d(b, f, h, h))
+
+
+def test_clean_psf_box_and_normalisation(oracle):
+    """clean_psf_host_set_up: the box half size is the longest walk from
+    (G/2 + 1, G/2 + 1) to its first non-decreasing step; the squared Gaussian
+    over [-h, h)^2 sums to psf_max, is symmetric about the centre, and is
+    zero outside the box."""
+    G = 64
+    c = G // 2 + 1
+    # decreasing for 5 steps rightwards, 3 downwards, 4 leftwards, 2 upwards, then
+    # rising: the walks stop at steps 6, 4, 5 and 3
+    psf = np.zeros((G, G), np.float32)
+    def walk(n):
+        return np.concatenate([1.0 - 0.1 * np.arange(n + 1), 1.0 - 0.1 * n + 0.05 * np.arange(1, G)])
+    psf[c, c:] = walk(5)[:G - c]
+    psf[c:, c] = walk(3)[:G - c]
+    psf[c, :c + 1] = walk(4)[:c + 1][::-1]
+    psf[:c + 1, c] = walk(2)[:c + 1][::-1]
+    cp, h = oracle.clean_psf(psf, 2.5)
+    assert h == 6
+    assert abs(float(cp.sum(dtype=np.float64)) - 2.5) < 1e-5
+    box = cp[c - h:c + h, c - h:c + h]
+    assert np.count_nonzero(cp) == np.count_nonzero(box) == (2 * h) ** 2
+    inner = cp[c - h + 1:c + h, c - h + 1:c + h]
+    np.testing.assert_array_equal(inner, inner[::-1, ::-1])
+    assert cp[c, c] == cp.max()
+
+
 def test_merge_and_split(oracle, rng):
     G, S = 8, 4
     grids = rng.normal(size=(S * G * G, 2)).astype(np.float32)
