@@ -15,6 +15,7 @@
 #include "sdp_oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -739,4 +740,98 @@ void oracle_clean_psf(int GRID_SIZE, double psf_max_value, const float *dirty_ps
             const int idx = cx + x + (cy + y) * GRID_SIZE;
             clean_psf[idx] = clean_psf[idx] / sum * psf_max_val;
         }
+}
+
+/* The reference's only intra-node parallelism (splitmerge.c:14-56): the
+ * visibilities cut into SLICES equal runs, each gridded into its own grid
+ * (or degridded) by its own thread, the grids then summed in slice order
+ * (merge_gridded_visibilities).  For the CPU baseline's multi-core mode. */
+typedef struct {
+    int G, n, nk, tks, os;
+    const float2 *kernels;
+    const int2 *supports;
+    const float3 *uvw;
+    const float2 *vis;
+    const Config *config;
+    float2 *grid;          /* gridding: this slice's grid */
+    const float2 *in_grid; /* degridding: the grid read */
+    float2 *out_vis;       /* degridding: this slice's outputs */
+    size_t c0, c1;         /* merge: cell range */
+    const float2 *grids;
+    int slices;
+    float2 *merged;
+} slice_job;
+
+static void *grid_slice(void *p)
+{
+    slice_job *j = (slice_job *)p;
+    const int zero = 0;
+    oracle_std_gridding(j->G, j->n, j->nk, j->tks, j->os, 0, &zero, &j->n, j->kernels, j->supports, j->uvw, j->vis,
+                        j->config, NULL, j->grid);
+    return NULL;
+}
+
+static void *degrid_slice(void *p)
+{
+    slice_job *j = (slice_job *)p;
+    oracle_std_degridding(j->G, j->n, j->nk, j->tks, j->os, j->kernels, j->supports, j->in_grid, j->uvw, &j->n,
+                          j->config, j->out_vis);
+    return NULL;
+}
+
+static void *merge_slice(void *p)
+{
+    slice_job *j = (slice_job *)p;
+    const size_t n = (size_t)j->G * j->G;
+    for (size_t i = j->c0; i < j->c1; ++i) {
+        float2 s = {0, 0};
+        for (int k = 0; k < j->slices; ++k) {
+            s.x += j->grids[(size_t)k * n + i].x;
+            s.y += j->grids[(size_t)k * n + i].y;
+        }
+        j->merged[i] = s;
+    }
+    return NULL;
+}
+
+int oracle_sliced_grid_degrid(int SLICES, int GRID_SIZE, int NUM_VISIBILITIES, int NUM_KERNELS,
+        int TOTAL_KERNEL_SAMPLES, int OVERSAMPLING_FACTOR, const float2 *grid_kernels, const float2 *degrid_kernels,
+        const int2 *kernel_supports, const float3 *uvw, const float2 *vis, const Config *config,
+        float2 *output_grid, float2 *output_vis, double *seconds)
+{
+    if (SLICES < 1 || SLICES > 256) return -1;
+    const size_t cells = (size_t)GRID_SIZE * GRID_SIZE;
+    float2 *grids = (float2 *)calloc(cells * (size_t)SLICES, sizeof(float2));
+    slice_job *jobs = (slice_job *)calloc((size_t)SLICES, sizeof(slice_job));
+    pthread_t *th = (pthread_t *)calloc((size_t)SLICES, sizeof(pthread_t));
+    if (!grids || !jobs || !th) {
+        free(grids); free(jobs); free(th);
+        return -1;
+    }
+    struct timespec t0, t1, t2;
+    for (int k = 0; k < SLICES; ++k) {
+        const int b = (int)((long long)NUM_VISIBILITIES * k / SLICES);
+        const int e = (int)((long long)NUM_VISIBILITIES * (k + 1) / SLICES);
+        slice_job j = {GRID_SIZE, e - b, NUM_KERNELS, TOTAL_KERNEL_SAMPLES, OVERSAMPLING_FACTOR, grid_kernels,
+                       kernel_supports, uvw + b, vis + b, config, grids + cells * (size_t)k, output_grid,
+                       output_vis + b, cells * (size_t)k / SLICES, cells * (size_t)(k + 1) / SLICES, grids, SLICES,
+                       output_grid};
+        jobs[k] = j;
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int k = 0; k < SLICES; ++k) pthread_create(&th[k], NULL, grid_slice, &jobs[k]);
+    for (int k = 0; k < SLICES; ++k) pthread_join(th[k], NULL);
+    for (int k = 0; k < SLICES; ++k) pthread_create(&th[k], NULL, merge_slice, &jobs[k]);
+    for (int k = 0; k < SLICES; ++k) pthread_join(th[k], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    for (int k = 0; k < SLICES; ++k) jobs[k].kernels = degrid_kernels;
+    for (int k = 0; k < SLICES; ++k) pthread_create(&th[k], NULL, degrid_slice, &jobs[k]);
+    for (int k = 0; k < SLICES; ++k) pthread_join(th[k], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &t2);
+    seconds[0] = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+    seconds[1] = (double)(t2.tv_sec - t1.tv_sec) + 1e-9 * (double)(t2.tv_nsec - t1.tv_nsec);
+    free(grids);
+    free(jobs);
+    free(th);
+    return 0;
 }

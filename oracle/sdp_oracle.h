@@ -94,6 +94,13 @@ void oracle_split_fftdegrid_counts(int NUM_VISIBILITIES, int SLICES, int num_cor
 double oracle_prolate_spheroidal(double nu);
 void oracle_create_1D_half_prolate(float *prolate, int grid_size);
 
+/* std_gridding then std_degridding over SLICES threads (splitmerge.c's
+ * split, per-slice grids, slice-order merge); seconds[0] gridding with the
+ * merge, seconds[1] degridding; 0 on success */
+int oracle_sliced_grid_degrid(int SLICES, int GRID_SIZE, int NUM_VISIBILITIES, int NUM_KERNELS,
+        int TOTAL_KERNEL_SAMPLES, int OVERSAMPLING_FACTOR, const float2 *grid_kernels, const float2 *degrid_kernels,
+        const int2 *kernel_supports, const float3 *uvw, const float2 *vis, const Config *config,
+        float2 *output_grid, float2 *output_vis, double *seconds);
 void oracle_conv2d(const float *image, const float *filter, float *result, int IMAGE_DIM, int hx, int hy);
 void oracle_clean_psf(int GRID_SIZE, double psf_max_value, const float *dirty_psf, float *clean_psf, int *halfdim);
 

@@ -53,6 +53,7 @@ def load():
             "oracle_apply_gains_only": [I, I, I, P, P, P, P],
             "oracle_reciprocal_transform": [I, P, P],
             "oracle_conv2d": [P, P, P, I, I, I],
+            "oracle_sliced_grid_degrid": [I, I, I, I, I, I, P, P, P, P, P, P, P, P, P],
             "oracle_clean_psf": [I, D, P, P, P],
             "oracle_add_visibilities": [I, P, P, P],
             "oracle_merge_gridded_visibilities": [I, I, P, P],
@@ -63,7 +64,8 @@ def load():
         for name, args in sigs.items():
             f = getattr(_lib, name)
             f.argtypes = args
-            f.restype = D if name == "oracle_prolate_spheroidal" else (I if name == "oracle_fft2d_f64" else None)
+            f.restype = (D if name == "oracle_prolate_spheroidal" else
+                         I if name in ("oracle_fft2d_f64", "oracle_sliced_grid_degrid") else None)
     return _lib
 
 
@@ -300,3 +302,20 @@ def clean_psf(dirty_psf, psf_max_value):
     h = ctypes.c_int(0)
     load().oracle_clean_psf(G, float(psf_max_value), _p(dirty_psf), _p(out), ctypes.byref(h))
     return out, h.value
+
+
+def sliced_grid_degrid(slices, G, OS, grid_kernels, degrid_kernels, supports, uvw, vis, cfg):
+    """std_gridding then std_degridding of the gridded result over `slices`
+    threads, as splitmerge.c splits and merges: (grid, visibilities,
+    (grid seconds, degrid seconds))."""
+    gk, dk = _c(grid_kernels, np.float32), _c(degrid_kernels, np.float32)
+    supports, uvw, vis = _c(supports, np.int32), _c(uvw, np.float32), _c(vis, np.float32)
+    grid = np.empty((G * G, 2), np.float32)
+    out = np.empty((uvw.shape[0], 2), np.float32)
+    sec = np.zeros(2, np.float64)
+    rc = load().oracle_sliced_grid_degrid(int(slices), G, uvw.shape[0], supports.shape[0], gk.shape[0], OS, _p(gk),
+                                          _p(dk), _p(supports), _p(uvw), _p(vis), config_bytes(cfg), _p(grid),
+                                          _p(out), _p(sec))
+    if rc != 0:
+        raise MemoryError("oracle_sliced_grid_degrid failed")
+    return grid, out, (float(sec[0]), float(sec[1]))

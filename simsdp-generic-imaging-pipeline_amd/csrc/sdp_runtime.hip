@@ -23,11 +23,19 @@ __global__ void zero_kernel(uint4 *p, size_t n16, unsigned char *tail, size_t nt
     for (; i < n16; i += stride) p[i] = make_uint4(0u, 0u, 0u, 0u);
     if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0;
 }
-// Plain 16-byte copy (graph safe, and the reference copy for HBM counters).
+// Plain 16-byte copy (graph safe, and the reference copy for HBM counters
+// and bandwidth): four loads in flight per thread before their stores.
 __global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
     for (; i < n16; i += stride) dst[i] = src[i];
 }
 }  // namespace
