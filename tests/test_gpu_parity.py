@@ -302,6 +302,55 @@ def test_oversampling_not_a_power_of_two(hip, actors, oracle, rng, OS):
     assert np.abs(deg - ref).max() <= 2e-6 * np.abs(ref).max()
 
 
+def test_no_visibilities(hip, actors, oracle):
+    """Zero visibilities: the grid is all zeros (degridgrid.c:73 clears it
+    before the loop), degridding returns an empty set in both modes."""
+    d = sdp_synth.small_case(n_vis=1, grid_size=64, n_planes=4, oversampling=16, seed=30)
+    G, cfg = d.grid_size, cfg_of(hip, d)
+    uvw, vis = np.ascontiguousarray(d.uvw[:0]), np.ascontiguousarray(d.vis[:0])
+    got = actors.std_gridding(G, d.oversampling, d.kernels, d.supports, uvw, vis, cfg)
+    ref = oracle.std_gridding(G, d.oversampling, d.kernels, d.supports, uvw, vis, cfg)
+    assert not np.any(ref)
+    np.testing.assert_array_equal(got, ref)
+    grid = np.ones((G * G, 2), np.float32)
+    for mode in (hip.DEGRID_NEAREST, hip.DEGRID_WEIGHTED):
+        actors.set_degrid_mode(mode)
+        try:
+            deg = actors.std_degridding(G, d.oversampling, d.degrid_kernels, d.supports, grid, uvw, cfg)
+        finally:
+            actors.set_degrid_mode(hip.DEGRID_NEAREST)
+        assert deg.shape == (0, 2)
+
+
+def test_visibilities_off_the_grid(hip, actors, oracle, rng):
+    """uv spread over 1.5x the grid: about half the footprints lie partly or
+    wholly outside [-G/2, G/2), whose cells the reference skips
+    (degridgrid.c:95-109).  Gridding bit-exact (integer kernels and
+    visibilities); weighted degridding within 2e-6 of the FP64 sum, whose
+    zero padding gives exactly 0 for a visibility that reaches no cell."""
+    d = sdp_synth.small_case(n_vis=4000, grid_size=128, n_planes=4, oversampling=16, seed=31, fill=1.5)
+    G, cfg = d.grid_size, cfg_of(hip, d)
+    gx = np.abs(d.uvw[:, :2].astype(np.float64) * d.uv_scale).max(axis=1)
+    outside = gx > G / 2 + 8
+    assert outside.sum() > 500 and (~outside).sum() > 1500
+    kern = integer_kernels(d.kernels.shape[0])
+    vis = rng.integers(-3, 4, size=(d.n_vis, 2)).astype(np.float32)
+    got = actors.std_gridding(G, d.oversampling, kern, d.supports, d.uvw, vis, cfg)
+    ref = oracle.std_gridding(G, d.oversampling, kern, d.supports, d.uvw, vis, cfg)
+    assert np.count_nonzero(ref) > 1000
+    np.testing.assert_array_equal(got, ref)
+    grid = rng.normal(size=(G * G, 2)).astype(np.float32)
+    actors.set_degrid_mode(hip.DEGRID_WEIGHTED)
+    try:
+        deg = actors.std_degridding(G, d.oversampling, d.degrid_kernels, d.supports, grid, d.uvw, cfg)
+    finally:
+        actors.set_degrid_mode(hip.DEGRID_NEAREST)
+    ref = oracle.degrid_weighted_f64(G, d.oversampling, d.degrid_kernels, d.supports, grid, d.uvw, d.uv_scale,
+                                     d.w_scale)
+    assert np.all(deg[outside] == 0.0) and np.all(ref[outside] == 0.0)
+    assert np.abs(deg - ref).max() <= 2e-6 * np.abs(ref).max()
+
+
 def test_reciprocal_transform_bit_exact(hip, actors, oracle, rng):
     """The correct_to_finegrid subgraph's first actor: each receiver gain's
     reciprocal, bit for bit (IEEE float division on both sides)."""
