@@ -25,7 +25,7 @@
 //           LDS order serialises visibilities: no atomics, deterministic.
 //           The copies are summed in group order (TwoSum) into the chunk's
 //           slab.  The merge then sums a (class, tile)'s pieces into one
-//           slab (reduce_pieces_kernel) and, per 32 x 32 block of grid cells,
+//           slab (reduce_pieces_kernel) and, per 16 x 16 block of grid cells,
 //           the slabs covering each cell in a fixed order (compensated),
 //           writing the whole grid once (merge_region_kernel).
 //   degrid  The wavefront stages the grid's S x S window in LDS; lanes sweep
@@ -731,13 +731,13 @@ reduce_pieces_kernel(const float2 *__restrict__ slabs, const long long *__restri
 }
 
 // Merge by output region: one workgroup per R x R block of grid cells
-// (R = 32, or G when smaller).  Its cells are covered by the key slabs (one
+// (R = RG, 16 or 32, or G when smaller).  Its cells are covered by the key slabs (one
 // per (class, tile), pieces already summed) of the tiles whose windows
 // [tile origin, tile origin + S) reach into the block; every cell is the
 // compensated sum of those slabs' values in (tile row, tile column, class)
 // order, and every cell is written.  Consecutive lanes read consecutive
 // cells of a slab row, and the loads of 8 slabs are in flight together.
-constexpr int kRegion = 32;
+constexpr int kRegion = 16;         // default region side (cells)
 constexpr int kRegionList = 512;    // slab descriptors staged per pass
 constexpr int kRegionCover = 1024;  // covering tiles per region (checked by the plan)
 constexpr int kRegionBatch = 8;     // slabs whose loads are issued together
@@ -746,12 +746,14 @@ struct RegionArgs {
     int G, tile, ntx, nty, smax, R;
 };
 
+template <int RG>
 __global__ void __launch_bounds__(kMergeThreads)
 merge_region_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ key_S,
                     const long long *__restrict__ key_off, const int *__restrict__ tile_key_begin, RegionArgs a,
                     float2 *__restrict__ grid)
 {
-    constexpr int kCells = kRegion * kRegion / kMergeThreads;   // cells per thread at R = 32
+    constexpr int kCells = RG * RG / kMergeThreads;   // cells per thread at R = RG
+    static_assert(kCells >= 1 && RG % 4 == 0 && kWave % RG == 0, "region side");
     __shared__ long long s_off[kRegionList];
     __shared__ int s_geo[kRegionList];          // S | (by + 512) << 10 | (bx + 512) << 20
     __shared__ int s_base[kRegionCover + 1];    // first list position of each covering tile
@@ -759,15 +761,16 @@ merge_region_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ k
     __shared__ int s_b[kRegionCover];           // (by + 512) | (bx + 512) << 10
     const int R = a.R;
     const int oy0 = blockIdx.y * R, ox0 = blockIdx.x * R;
-    // At R = 32 each wavefront owns a band of 8 rows (two rows of 32 lanes,
-    // its cells 2 rows apart) and skips the additions of slabs that miss its
-    // band; smaller regions (G < 32) take rows in passes of kMergeThreads / R.
-    const bool banded = R == kRegion;
+    // At R = RG each of the 4 wavefronts owns a band of RG / 4 rows (64 / RG
+    // rows of RG lanes at a time, its cells 64 / RG rows apart) and skips the
+    // additions of slabs that miss its band; smaller regions (G < RG) take
+    // rows in passes of kMergeThreads / R.
+    const bool banded = R == RG;
     const int wave = threadIdx.x / 64;
-    const int rows_per_pass = banded ? 2 : kMergeThreads / R;
+    const int rows_per_pass = banded ? kWave / RG : kMergeThreads / R;
     const int col = threadIdx.x % R;
-    const int row0 = banded ? (kRegion / 4) * wave + (threadIdx.x % 64) / kRegion : threadIdx.x / R;
-    const int band_lo = banded ? (kRegion / 4) * wave : 0, band_hi = banded ? band_lo + kRegion / 4 : R;
+    const int row0 = banded ? (RG / 4) * wave + (threadIdx.x % 64) / RG : threadIdx.x / R;
+    const int band_lo = banded ? (RG / 4) * wave : 0, band_hi = banded ? band_lo + RG / 4 : R;
     float sr[kCells], si[kCells], cr[kCells], ci[kCells];
 #pragma unroll
     for (int q = 0; q < kCells; ++q) sr[q] = si[q] = cr[q] = ci[q] = 0.f;
@@ -909,7 +912,8 @@ struct sdp_grid_plan {
     int *key_S = nullptr;
     long long *key_off = nullptr;  // into key_base (see key_base_of)
     int *tile_key_begin = nullptr;
-    int region = 0;                    // merge region edge
+    int region = 0;                    // merge region edge (min(region_rg, G))
+    int region_rg = kRegion;           // merge kernel's region side, 16 or 32
     int4 *multi = nullptr;         // multi-piece keys: first chunk, pieces, S*S, -, then their output starts
     float2 *key_slabs = nullptr;   // contiguous with slabs: slabs followed by the reduced key slabs
 };
@@ -1109,18 +1113,20 @@ int launch_chunks(const sdp_grid_plan *plan, const ChunkIO &io, const uint32_t *
             join = nullptr;
         }
     }
-    for (int c = 0; c < kNumClasses; ++c) {
+    int rc = SDP_OK;
+    for (int c = 0; c < kNumClasses && rc == SDP_OK; ++c) {
         if (plan->class_chunk_begin[c + 1] <= plan->class_chunk_begin[c]) continue;
         hipStream_t s = c == big ? st : side;
-        const int rc = (comp || MODE == MODE_DEGRID) ? launch_class<MODE, true>(plan, c, io, perm, s)
-                                                     : launch_class<MODE, false>(plan, c, io, perm, s);
-        if (rc != SDP_OK) return rc;
+        rc = (comp || MODE == MODE_DEGRID) ? launch_class<MODE, true>(plan, c, io, perm, s)
+                                           : launch_class<MODE, false>(plan, c, io, perm, s);
     }
+    // joined on the error path too, so a capturing caller's stream is never
+    // left with an unjoined fork
     if (join) {
         SDP_HIP_TRY(hipEventRecord(join, side));
         SDP_HIP_TRY(hipStreamWaitEvent(st, join, 0));
     }
-    return SDP_OK;
+    return rc;
 }
 
 }  // namespace
@@ -1206,7 +1212,11 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     const int ntiles = plan->ntx * plan->nty;
     const int nkeys = kNumClasses * ntiles;
     {
-        plan->region = std::min(kRegion, p.grid_size);
+        // SDP_MERGE_REGION (16 or 32) overrides the region side for tuning
+        const char *e = getenv("SDP_MERGE_REGION");
+        const int rg = e ? atoi(e) : 0;
+        plan->region_rg = (rg == 16 || rg == 32) ? rg : kRegion;
+        plan->region = std::min(plan->region_rg, p.grid_size);
         const int rspan = (plan->smax + plan->region + 2 * plan->tile - 1) / plan->tile;
         if (rspan * rspan > kRegionCover) {
             const int tile = plan->tile;
@@ -1229,16 +1239,31 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     PLAN_TRY(hipMemcpyAsync(plan->plane_tab, tab.data(), sizeof(int2) * p.n_kernels, hipMemcpyHostToDevice, st));
     int2 *d_supports = nullptr;
     int *d_pclass = nullptr;
+    VisRec *recs_tmp = nullptr;
+    uint32_t *keys = nullptr, *keys_sorted = nullptr, *vals = nullptr;
+    unsigned int *key_count = nullptr, *key_first = nullptr;
+    unsigned long long *key_taps = nullptr;
+    // the set-up buffers above: freed once the per-key counts are read back,
+    // or by this guard when an error returns early
+    struct SetupBuffers {
+        void **slot[9];
+        void release()
+        {
+            for (void **q : slot)
+                if (*q) {
+                    (void)hipFree(*q);
+                    *q = nullptr;
+                }
+        }
+        ~SetupBuffers() { release(); }
+    } setup{{(void **)&d_supports, (void **)&d_pclass, (void **)&recs_tmp, (void **)&keys, (void **)&keys_sorted,
+             (void **)&vals, (void **)&key_count, (void **)&key_first, (void **)&key_taps}};
     PLAN_TRY(hipMalloc(&d_supports, sizeof(int2) * p.n_kernels));
     PLAN_TRY(hipMemcpyAsync(d_supports, sup.data(), sizeof(int2) * p.n_kernels, hipMemcpyHostToDevice, st));
     PLAN_TRY(hipMalloc(&d_pclass, sizeof(int) * p.n_kernels));
     PLAN_TRY(hipMemcpyAsync(d_pclass, pclass.data(), sizeof(int) * p.n_kernels, hipMemcpyHostToDevice, st));
 
     const size_t n = (size_t)std::max(n_vis, 1);
-    VisRec *recs_tmp = nullptr;
-    uint32_t *keys = nullptr, *keys_sorted = nullptr, *vals = nullptr;
-    unsigned int *key_count = nullptr, *key_first = nullptr;
-    unsigned long long *key_taps = nullptr;
     PLAN_TRY(hipMalloc(&recs_tmp, sizeof(VisRec) * n));
     PLAN_TRY(hipMalloc(&plan->recs, sizeof(VisRec) * n));
     PLAN_TRY(hipMalloc(&plan->perm, sizeof(uint32_t) * n));
@@ -1301,15 +1326,7 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     PLAN_TRY(hipMemcpyAsync(counts.data(), key_count, sizeof(unsigned int) * nkeys, hipMemcpyDeviceToHost, st));
     PLAN_TRY(hipMemcpyAsync(taps.data(), key_taps, sizeof(unsigned long long) * nkeys, hipMemcpyDeviceToHost, st));
     PLAN_TRY(hipStreamSynchronize(st));
-    (void)hipFree(recs_tmp);
-    (void)hipFree(keys);
-    (void)hipFree(keys_sorted);
-    (void)hipFree(vals);
-    (void)hipFree(key_count);
-    (void)hipFree(key_taps);
-    (void)hipFree(key_first);
-    (void)hipFree(d_supports);
-    (void)hipFree(d_pclass);
+    setup.release();
 
     // work items, class-major then tile-major, each (class, tile) list cut
     // into pieces of about chunk_taps taps
@@ -1489,8 +1506,12 @@ int sdp_grid_merge(const sdp_grid_plan *plan, float *d_grid, sdp_stream s)
     const int R = plan->region;
     RegionArgs ra{G, plan->tile, plan->ntx, plan->nty, std::max(plan->smax, 1), R};
     dim3 rg((G + R - 1) / R, (G + R - 1) / R);
-    merge_region_kernel<<<rg, kMergeThreads, 0, st>>>(plan->key_slabs, plan->key_S, plan->key_off,
-                                                     plan->tile_key_begin, ra, (float2 *)d_grid);
+    if (plan->region_rg == 32)
+        merge_region_kernel<32><<<rg, kMergeThreads, 0, st>>>(plan->key_slabs, plan->key_S, plan->key_off,
+                                                             plan->tile_key_begin, ra, (float2 *)d_grid);
+    else
+        merge_region_kernel<16><<<rg, kMergeThreads, 0, st>>>(plan->key_slabs, plan->key_S, plan->key_off,
+                                                             plan->tile_key_begin, ra, (float2 *)d_grid);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }

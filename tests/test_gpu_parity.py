@@ -67,10 +67,10 @@ def test_grid_float_accuracy(hip, actors, oracle, case):
     np.testing.assert_allclose(got, ref32, atol=2e-6 * peak)
 
 
-@pytest.mark.parametrize("G,tile", [(80, 4), (96, 8), (150, 16)])
+@pytest.mark.parametrize("G,tile", [(24, 4), (80, 4), (96, 8), (150, 16)])
 def test_grid_sizes_off_the_merge_block(hip, actors, oracle, G, tile):
-    """Grid sizes that are not multiples of the merge's 32-cell output blocks
-    (and one that is not a power of two at all): every cell of the grid,
+    """Grid sizes that are not multiples of the merge's 16- or 32-cell output
+    blocks (and one that is not a power of two at all): every cell of the grid,
     including the partial blocks at its far edges, matches the FP64 grid."""
     d = sdp_synth.small_case(n_vis=4000, grid_size=G, n_planes=3, oversampling=16, seed=G)
     uvw = hip.DeviceBuffer.from_array(d.uvw)
