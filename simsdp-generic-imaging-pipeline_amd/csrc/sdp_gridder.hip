@@ -686,6 +686,10 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
 }
 
 constexpr int kMergeThreads = 256;
+// A key's pieces summed by one workgroup of the first reduce pass: a key with
+// more (a dense tile can have a thousand) is split into groups whose partial
+// slabs a second pass sums, so no workgroup's serial chain of loads is long.
+constexpr int kGroupPieces = 64;
 
 // Sum the slabs of a (class, tile) key's pieces, in piece order, into the
 // key's slab.  One workgroup per key with more than one piece; each thread
@@ -915,7 +919,11 @@ struct sdp_grid_plan {
     int *tile_key_begin = nullptr;
     int region = 0;                    // merge region edge (min(region_rg, G))
     int region_rg = kRegion;           // merge kernel's region side, 16 or 32
-    int4 *multi = nullptr;         // multi-piece keys: first chunk, pieces, S*S, -, then their output starts
+    int4 *multi = nullptr;         // first reduce pass: first chunk, pieces, S*S, -, then their output starts
+    int n_multi2 = 0;
+    int4 *multi2 = nullptr;        // second pass (keys of more than kGroupPieces pieces): first partial, count,
+                                   // S*S, -, then their output starts
+    long long *part_off = nullptr; // the partial slabs the second pass reads
     float2 *key_slabs = nullptr;   // contiguous with slabs: slabs followed by the reduced key slabs
 };
 
@@ -935,6 +943,8 @@ void plan_free(sdp_grid_plan *p)
     (void)hipFree(p->key_off);
     (void)hipFree(p->tile_key_begin);
     (void)hipFree(p->multi);
+    (void)hipFree(p->multi2);
+    (void)hipFree(p->part_off);
     delete p;
 }
 
@@ -1370,20 +1380,38 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     for (int t = 0; t < ntiles; ++t) plan->n_tiles_used += tile_used[t];
 
     // key slabs: single-piece keys use their chunk's slab; multi-piece keys
-    // get a reduced slab placed after the chunk slabs
+    // get a reduced slab placed after the chunk slabs.  A key of more than
+    // kGroupPieces pieces is reduced in two passes: its pieces in groups of
+    // kGroupPieces into partial slabs (after its key slab), then the partials
+    // in group order into the key slab.
     const int nk = (int)key_S.size();
     std::vector<long long> key_off(nk);
-    std::vector<int4> multi;
-    std::vector<long long> multi_out;
+    std::vector<int4> multi, multi2;
+    std::vector<long long> multi_out, multi2_out, part_off;
     long long extra = 0;
     for (int k = 0; k < nk; ++k) {
         const long long S2 = (long long)key_S[k] * key_S[k];
-        if (key_pieces[k] == 1) {
-            key_off[k] = slab_off[key_first_chunk[k]];
-        } else {
-            key_off[k] = cells + extra;
-            multi.push_back(make_int4((int)key_first_chunk[k], (int)key_pieces[k], (int)S2, 0));
+        const long long pieces = key_pieces[k], c0 = key_first_chunk[k];
+        if (pieces == 1) {
+            key_off[k] = slab_off[c0];
+            continue;
+        }
+        key_off[k] = cells + extra;
+        const long long out = cells + extra;
+        extra += S2;
+        if (pieces <= kGroupPieces) {
+            multi.push_back(make_int4((int)c0, (int)pieces, (int)S2, 0));
+            multi_out.push_back(out);
+            continue;
+        }
+        const long long groups = (pieces + kGroupPieces - 1) / kGroupPieces;
+        multi2.push_back(make_int4((int)part_off.size(), (int)groups, (int)S2, 0));
+        multi2_out.push_back(out);
+        for (long long g = 0; g < groups; ++g) {
+            const long long b = g * kGroupPieces;
+            multi.push_back(make_int4((int)(c0 + b), (int)std::min<long long>(kGroupPieces, pieces - b), (int)S2, 0));
             multi_out.push_back(cells + extra);
+            part_off.push_back(cells + extra);
             extra += S2;
         }
     }
@@ -1403,6 +1431,7 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     }
     plan->n_keys_used = nk;
     plan->n_multi = (int)multi.size();
+    plan->n_multi2 = (int)multi2.size();
 
     PLAN_TRY(hipMalloc(&plan->tile_key_begin, sizeof(int) * (ntiles + 1)));
     PLAN_TRY(hipMemcpyAsync(plan->tile_key_begin, tile_key_begin.data(), sizeof(int) * (ntiles + 1),
@@ -1425,6 +1454,17 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
                                     st));
             PLAN_TRY(hipMemcpyAsync((char *)plan->multi + sizeof(int4) * plan->n_multi, multi_out.data(),
                                     sizeof(long long) * plan->n_multi, hipMemcpyHostToDevice, st));
+        }
+        if (plan->n_multi2 > 0) {
+            const size_t bytes = sizeof(int4) * plan->n_multi2 + sizeof(long long) * plan->n_multi2;
+            PLAN_TRY(hipMalloc(&plan->multi2, bytes));
+            PLAN_TRY(hipMemcpyAsync(plan->multi2, multi2.data(), sizeof(int4) * plan->n_multi2,
+                                    hipMemcpyHostToDevice, st));
+            PLAN_TRY(hipMemcpyAsync((char *)plan->multi2 + sizeof(int4) * plan->n_multi2, multi2_out.data(),
+                                    sizeof(long long) * plan->n_multi2, hipMemcpyHostToDevice, st));
+            PLAN_TRY(hipMalloc(&plan->part_off, sizeof(long long) * part_off.size()));
+            PLAN_TRY(hipMemcpyAsync(plan->part_off, part_off.data(), sizeof(long long) * part_off.size(),
+                                    hipMemcpyHostToDevice, st));
         }
         plan->slab_bytes = sizeof(float2) * (size_t)(cells + extra);
         PLAN_TRY(hipMalloc(&plan->slabs, plan->slab_bytes));
@@ -1518,6 +1558,11 @@ int sdp_grid_merge(const sdp_grid_plan *plan, float *d_grid, sdp_stream s)
             reduce_pieces_kernel<2, 16><<<plan->n_multi, kMergeThreads, 0, st>>>(plan->slabs, plan->slab_off,
                                                                                 plan->multi, plan->key_slabs);
         SDP_LAUNCH_CHECK();
+        if (plan->n_multi2 > 0) {   // the partial slabs of keys split into groups
+            reduce_pieces_kernel<2, 16><<<plan->n_multi2, kMergeThreads, 0, st>>>(plan->slabs, plan->part_off,
+                                                                                 plan->multi2, plan->key_slabs);
+            SDP_LAUNCH_CHECK();
+        }
     }
     const int R = plan->region;
     RegionArgs ra{G, plan->tile, plan->ntx, plan->nty, std::max(plan->smax, 1), R};
