@@ -745,13 +745,12 @@ reduce_pieces_kernel(const float2 *__restrict__ slabs, const long long *__restri
 constexpr int kRegion = 16;         // default region side (cells)
 constexpr int kRegionList = 512;    // slab descriptors staged per pass
 constexpr int kRegionCover = 1024;  // covering tiles per region (checked by the plan)
-constexpr int kRegionBatch = 8;     // slabs whose loads are issued together
 
 struct RegionArgs {
     int G, tile, ntx, nty, smax, R;
 };
 
-template <int RG>
+template <int RG, int kRegionBatch>   // kRegionBatch: slabs whose loads are issued together
 __global__ void __launch_bounds__(kMergeThreads)
 merge_region_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ key_S,
                     const long long *__restrict__ key_off, const int *__restrict__ tile_key_begin, RegionArgs a,
@@ -1389,6 +1388,9 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     std::vector<int4> multi, multi2;
     std::vector<long long> multi_out, multi2_out, part_off;
     long long extra = 0;
+    // SDP_GROUP_PIECES (8 to 1024) overrides kGroupPieces for tuning
+    const char *gp_env = getenv("SDP_GROUP_PIECES");
+    const long long gp = gp_env && atoi(gp_env) >= 8 && atoi(gp_env) <= 1024 ? atoi(gp_env) : kGroupPieces;
     for (int k = 0; k < nk; ++k) {
         const long long S2 = (long long)key_S[k] * key_S[k];
         const long long pieces = key_pieces[k], c0 = key_first_chunk[k];
@@ -1399,17 +1401,17 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
         key_off[k] = cells + extra;
         const long long out = cells + extra;
         extra += S2;
-        if (pieces <= kGroupPieces) {
+        if (pieces <= gp) {
             multi.push_back(make_int4((int)c0, (int)pieces, (int)S2, 0));
             multi_out.push_back(out);
             continue;
         }
-        const long long groups = (pieces + kGroupPieces - 1) / kGroupPieces;
+        const long long groups = (pieces + gp - 1) / gp;
         multi2.push_back(make_int4((int)part_off.size(), (int)groups, (int)S2, 0));
         multi2_out.push_back(out);
         for (long long g = 0; g < groups; ++g) {
-            const long long b = g * kGroupPieces;
-            multi.push_back(make_int4((int)(c0 + b), (int)std::min<long long>(kGroupPieces, pieces - b), (int)S2, 0));
+            const long long b = g * gp;
+            multi.push_back(make_int4((int)(c0 + b), (int)std::min<long long>(gp, pieces - b), (int)S2, 0));
             multi_out.push_back(cells + extra);
             part_off.push_back(cells + extra);
             extra += S2;
@@ -1567,12 +1569,13 @@ int sdp_grid_merge(const sdp_grid_plan *plan, float *d_grid, sdp_stream s)
     const int R = plan->region;
     RegionArgs ra{G, plan->tile, plan->ntx, plan->nty, std::max(plan->smax, 1), R};
     dim3 rg((G + R - 1) / R, (G + R - 1) / R);
+    // 8 slabs' loads in flight per batch: 16 and 32 measured no faster
     if (plan->region_rg == 32)
-        merge_region_kernel<32><<<rg, kMergeThreads, 0, st>>>(plan->key_slabs, plan->key_S, plan->key_off,
-                                                             plan->tile_key_begin, ra, (float2 *)d_grid);
+        merge_region_kernel<32, 8><<<rg, kMergeThreads, 0, st>>>(plan->key_slabs, plan->key_S, plan->key_off,
+                                                                plan->tile_key_begin, ra, (float2 *)d_grid);
     else
-        merge_region_kernel<16><<<rg, kMergeThreads, 0, st>>>(plan->key_slabs, plan->key_S, plan->key_off,
-                                                             plan->tile_key_begin, ra, (float2 *)d_grid);
+        merge_region_kernel<16, 8><<<rg, kMergeThreads, 0, st>>>(plan->key_slabs, plan->key_S, plan->key_off,
+                                                                plan->tile_key_begin, ra, (float2 *)d_grid);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }
