@@ -69,7 +69,10 @@ void dft_actor(int NUM_VISIBILITIES, int NUM_MAX_SOURCES, IN PRECISION3 *sources
 
 /* ---- Hogbom minor cycle (include/deconvolution_run.h) ------------------- */
 
-/* replaces include/deconvolution_run.h:14 (src/CPU/deconvolution_run.c:197-312) */
+/* replaces include/deconvolution_run.h:14 (src/CPU/deconvolution_run.c:197-312).
+ * The noise stop uses the reference's own float running average (sdp_hogbom
+ * in sdp_device.h).  Unlike the reference, it stops after MAX_SOURCES
+ * components instead of writing past sources_out. */
 void hogbom_clean(int GRID_SIZE, int NUM_MINOR_CYCLES, int MAX_SOURCES, IN PRECISION* residual,
         IN PRECISION* partial_psf, int2* partial_psf_halfdims, IN Config* config,
         IN PRECISION* current_model, OUT int* num_sources_out, OUT PRECISION3* sources_out,

@@ -252,8 +252,20 @@ int sdp_hogbom(int grid_size, int n_minor, int max_sources, float *d_residual, c
                int *d_num_sources, void *d_work, sdp_stream s);
 /* flags for sdp_hogbom_ex.  By default the PSF is read x-mirrored about its
  * centre exactly as the reference indexes it (deconvolution_run.c:269);
- * SDP_HOGBOM_PSF_AS_STORED subtracts it in its stored orientation. */
-enum { SDP_HOGBOM_PSF_AS_STORED = 1 };
+ * SDP_HOGBOM_PSF_AS_STORED subtracts it in its stored orientation.
+ * The noise stop (deconvolution_run.c:238) compares the peak with
+ * noise_factor * gain * the reference's float running "average" (:226),
+ * which drifts from the true mean of |r_1..r_{N-1}| as N grows (0.7% at
+ * 4096^2).  When noise_factor * gain > 1/2 (below that the stop cannot
+ * fire) sdp_hogbom runs that recurrence sequentially on one wavefront before
+ * every iteration, so the stop matches the reference bit for bit at the
+ * cost of about 0.1 s per iteration at 2048^2; SDP_HOGBOM_DOUBLE_MEAN
+ * uses the parallel double sum / (N - 1) instead, which can decide
+ * differently only when the peak is within that drift of the threshold.
+ * Both entry points also stop after max_sources components (the reference
+ * keeps going and writes past its sources array); the first max_sources
+ * iterations are the reference's. */
+enum { SDP_HOGBOM_PSF_AS_STORED = 1, SDP_HOGBOM_DOUBLE_MEAN = 2 };
 int sdp_hogbom_ex(int grid_size, int n_minor, int max_sources, float *d_residual, const float *d_psf,
                   double psf_max, double loop_gain, double noise_factor, double weak_source_percent,
                   double cell_size, const float *d_model_in, float *d_model_out, float *d_sources,

@@ -30,7 +30,7 @@ def conv3x3(x, w, b):
     out = np.broadcast_to(b, (n, h, wd, co)).astype(np.float64).copy()
     for ky in range(3):
         for kx in range(3):
-            out += np.einsum("nyxi,io->nyxo", xp[:, ky:ky + h, kx:kx + wd], w[ky, kx])
+            out += (xp[:, ky:ky + h, kx:kx + wd].reshape(-1, ci) @ w[ky, kx]).reshape(n, h, wd, co)
     return out
 
 
@@ -41,10 +41,12 @@ def conv3x3_backward(x, w, gout):
     xp[:, 1:-1, 1:-1] = x
     gw = np.zeros_like(w, dtype=np.float64)
     gxp = np.zeros_like(xp)
+    co = gout.shape[3]
+    g2 = gout.reshape(-1, co)
     for ky in range(3):
         for kx in range(3):
-            gw[ky, kx] = np.einsum("nyxi,nyxo->io", xp[:, ky:ky + h, kx:kx + wd], gout)
-            gxp[:, ky:ky + h, kx:kx + wd] += np.einsum("nyxo,io->nyxi", gout, w[ky, kx])
+            gw[ky, kx] = xp[:, ky:ky + h, kx:kx + wd].reshape(-1, ci).T @ g2
+            gxp[:, ky:ky + h, kx:kx + wd] += (g2 @ np.asarray(w[ky, kx], np.float64).T).reshape(n, h, wd, ci)
     gb = gout.sum(axis=(0, 1, 2))
     return gxp[:, 1:-1, 1:-1], gw, gb
 

@@ -316,6 +316,58 @@ void oracle_gridding_actor(int GRID_SIZE, int NUM_VISIBILITIES, int NUM_KERNELS,
     }
 }
 
+/* gridding_actor's index arithmetic (gridding_run.c:63-123: snap to 1/16,
+ * inclusive range, roundf plane, OS fixed at 16) with the products and the
+ * cell sums in double: the exact sum the float grid approximates */
+void oracle_gridding_actor_f64(int GRID_SIZE, int NUM_VISIBILITIES, const float2 *kernels,
+        const int2 *kernel_supports, const float3 *vis_uvw_coords, const float2 *visibilities,
+        double uv_scale, double w_scale, double *grid)
+{
+    memset(grid, 0, sizeof(double) * 2 * (size_t)GRID_SIZE * GRID_SIZE);
+    const int oversampling = 16;
+    const float os = (float)oversampling;
+    const int half_grid_size = GRID_SIZE / 2;
+    for (int vi = 0; vi < NUM_VISIBILITIES; vi++) {
+        const float3 c = vis_uvw_coords[vi];
+        const int plane_index = (int)roundf(sqrtf(fabsf((float)((double)c.z * w_scale))));
+        const float gx = (float)((double)c.x * uv_scale);
+        const float gy = (float)((double)c.y * uv_scale);
+        const int h = kernel_supports[plane_index].x;
+        const double conj = (c.z < 0.0) ? -1.0 : 1.0;
+        const float sx = roundf(gx * os) / os;
+        const float sy = roundf(gy * os) / os;
+        const float maxx = floorf(sx + (float)h), maxy = floorf(sy + (float)h);
+        for (int gv = (int)ceilf(sy - (float)h); (float)gv <= maxy; ++gv) {
+            if (gv < -half_grid_size || gv >= half_grid_size) continue;
+            int ky = abs((int)roundf(((float)gv - sy) * os));
+            for (int gu = (int)ceilf(sx - (float)h); (float)gu <= maxx; ++gu) {
+                if (gu < -half_grid_size || gu >= half_grid_size) continue;
+                int kx = abs((int)roundf(((float)gu - sx) * os));
+                float2 k = kernels[kernel_supports[plane_index].y + ky * (h + 1) * oversampling + kx];
+                double kr = k.x, ki = (double)k.y * conj;
+                double vr = visibilities[vi].x, vim = visibilities[vi].y;
+                size_t g = (size_t)(gv + half_grid_size) * GRID_SIZE + (gu + half_grid_size);
+                grid[2 * g] += vr * kr - vim * ki;
+                grid[2 * g + 1] += vim * kr + vr * ki;
+            }
+        }
+    }
+}
+
+/* The "average" hogbom_clean computes in its peak scan
+ * (deconvolution_run.c:223-227): m = |r[0]|, then m += (|r[i]| - m) / i in
+ * float for i = 1 .. n-1.  In exact arithmetic this is the mean of
+ * |r[1..n-1]|; in float its rounding drifts with n. */
+float oracle_running_mean(const float *r, long n)
+{
+    float avg = 0.f;
+    for (long idx = 0; idx < n; ++idx) {
+        float val = fabsf(r[idx]);
+        avg = idx == 0 ? val : avg + (val - avg) / (float)idx;
+    }
+    return avg;
+}
+
 /* dft_actor, dft_run.c:3-47 (approximation branch, :21-32) */
 void oracle_dft_actor(int NUM_VISIBILITIES, int NUM_MAX_SOURCES, const float3 *sources,
         const float3 *vis_uvw_coords, const int *num_sources, const Config *config, float2 *visibilities)

@@ -53,6 +53,13 @@ void oracle_gridding_actor(int GRID_SIZE, int NUM_VISIBILITIES, int NUM_KERNELS,
         const float2 *kernels, const int2 *kernel_supports, const float3 *vis_uvw_coords,
         const float2 *visibilities, const Config *config, float2 *uv_grid);
 
+/* gridding_actor's indices with double products and sums */
+void oracle_gridding_actor_f64(int GRID_SIZE, int NUM_VISIBILITIES, const float2 *kernels,
+        const int2 *kernel_supports, const float3 *vis_uvw_coords, const float2 *visibilities,
+        double uv_scale, double w_scale, double *grid /* 2*G*G */);
+/* the float recurrence hogbom_clean's peak scan calls its average */
+float oracle_running_mean(const float *r, long n);
+
 void oracle_dft_actor(int NUM_VISIBILITIES, int NUM_MAX_SOURCES, const float3 *sources,
         const float3 *vis_uvw_coords, const int *num_sources, const Config *config, float2 *visibilities);
 /* same formula evaluated in double from the float inputs */

@@ -40,6 +40,8 @@ def load():
             "oracle_std_degridding": [I, I, I, I, I, P, P, P, P, P, P, P],
             "oracle_degrid_weighted_f64": [I, I, I, P, P, P, P, D, D, P],
             "oracle_gridding_actor": [I, I, I, I, P, P, P, P, P, P],
+            "oracle_gridding_actor_f64": [I, I, P, P, P, P, D, D, P],
+            "oracle_running_mean": [P, ctypes.c_long],
             "oracle_dft_actor": [I, I, P, P, P, P, P],
             "oracle_dft_f64": [I, I, P, P, P],
             "oracle_hogbom_clean": [I, I, I, P, P, P, P, P, P, P, P],
@@ -65,6 +67,7 @@ def load():
             f = getattr(_lib, name)
             f.argtypes = args
             f.restype = (D if name == "oracle_prolate_spheroidal" else
+                         ctypes.c_float if name == "oracle_running_mean" else
                          I if name in ("oracle_fft2d_f64", "oracle_sliced_grid_degrid") else None)
     return _lib
 
@@ -151,6 +154,22 @@ def gridding_actor(G, kernels, supports, uvw, vis, cfg):
     return out
 
 
+def gridding_actor_f64(G, kernels, supports, uvw, vis, uv_scale, w_scale):
+    """gridding_actor's index arithmetic with double products and sums"""
+    kernels, supports = _c(kernels, np.float32), _c(supports, np.int32)
+    uvw, vis = _c(uvw, np.float32), _c(vis, np.float32)
+    out = np.empty((G * G, 2), np.float64)
+    load().oracle_gridding_actor_f64(G, uvw.shape[0], _p(kernels), _p(supports), _p(uvw), _p(vis), uv_scale, w_scale,
+                                     _p(out))
+    return out
+
+
+def running_mean(r):
+    """hogbom_clean's float "average" recurrence, deconvolution_run.c:223-227"""
+    r = _c(r, np.float32).ravel()
+    return float(load().oracle_running_mean(_p(r), r.size))
+
+
 def dft_actor(sources, uvw, cfg):
     """dft_actor, dft_run.c:3-47"""
     sources, uvw = _c(sources, np.float32), _c(uvw, np.float32)
@@ -172,7 +191,8 @@ def hogbom_clean(G, n_minor, max_sources, residual, psf_padded, cfg, model):
     residual, psf_padded, model = _c(residual, np.float32), _c(psf_padded, np.float32), _c(model, np.float32)
     out_model = np.empty(G * G, np.float32)
     out_res = np.empty(G * G, np.float32)
-    sources = np.zeros((max(max_sources, 1), 3), np.float32)
+    # the reference writes one source per iteration whatever MAX_SOURCES is
+    sources = np.zeros((max(max_sources, n_minor, 1), 3), np.float32)
     ns = ctypes.c_int(0)
     load().oracle_hogbom_clean(G, n_minor, max_sources, _p(residual), _p(psf_padded), config_bytes(cfg), _p(model),
                                ctypes.byref(ns), _p(sources), _p(out_model), _p(out_res))

@@ -5,10 +5,10 @@
 // things:
 //   1. reduce the per-workgroup partials the previous launch wrote (largest
 //      |residual|, first index on ties as the reference's strict '>'
-//      row-major scan, :222-233; sum of |residual| over pixels 1..N-1, which
-//      the reference's running mean, :226, equals over N-1), in a fixed
-//      order, so every workgroup reaches the same component and the same stop
-//      decision (:238-243); workgroup 0 records the component (:247-256);
+//      row-major scan, :222-233; sum of |residual| over pixels 1..N-1) in a
+//      fixed order, so every workgroup reaches the same component and the
+//      same stop decision (:238-243); workgroup 0 records the component
+//      (:247-256);
 //   2. subtract that component's PSF from its pixels (:266-273);
 //   3. scan its pixels and write its partial for the next launch.
 // State and partials are double-buffered by launch parity, so no workgroup
@@ -29,6 +29,21 @@
 // ((gain * max_val) * psf) * psf_max, then subtracted from the float residual
 // in double and rounded to float, so the residual, model and component list
 // match the reference bit for bit whenever the stop decision does.
+//
+// The noise stop compares the peak with noise_factor * gain * "average",
+// where the reference's average (:226) is the float recurrence m = |r0|,
+// m += (|r_i| - m) / i.  In exact arithmetic that is the mean of
+// |r_1 .. r_{N-1}|, but in float it drifts as N grows (about 2e-6 of the
+// mean at 128^2, 1e-4 at 2048^2 and 0.7% at 4096^2 on noise-like images),
+// so no parallel sum reproduces it.  The stop can only fire when
+// noise_factor * gain exceeds 1/2 (the recurrence never exceeds the largest
+// |r| by more than a few ulps); then, by default, one wavefront runs the
+// reference's recurrence itself before each iteration
+// (hogbom_running_mean_kernel: sequential, about 0.1 s per iteration at
+// 2048^2), and the decision is the reference's bit for bit.
+// SDP_HOGBOM_DOUBLE_MEAN takes the parallel double sum over N - 1 instead,
+// which agrees with the reference except when the peak lies within that
+// drift of the threshold.
 #include <algorithm>
 #include <cstdlib>
 
@@ -51,11 +66,11 @@ struct Partial {
 struct State {
     int stop, nsrc;
     float src0;
-    int pad;
+    float mean;   // the reference's running "average" of the current residual (exact-mean mode)
 };
 
 struct HogbomArgs {
-    int G, n_minor, max_sources, nblocks, mirror, vec;
+    int G, n_minor, max_sources, nblocks, mirror, vec, exact_mean;
     int psf_pad_bytes;
     double psf_max, gain, noise, weak;
 };
@@ -76,11 +91,47 @@ __global__ void hogbom_init_kernel(State *st, const float *model_in, float *mode
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    if (i < 2) st[i] = State{0, 0, 0.f, 0};
+    if (i < 2) st[i] = State{0, 0, 0.f, 0.f};
     for (size_t k = i; k < n_model; k += stride) model_out[k] = model_in[k];
     const size_t total = npix + 2 * kPsfPad;
     for (size_t k = i; k < total; k += stride)
         psf_pad[k] = (k >= (size_t)kPsfPad && k <= npix + kPsfPad) ? psf[k - kPsfPad] : 0.f;
+}
+
+// deconvolution_run.c:223-227: avg = |r0|, then avg += (|r_i| - avg) / i in
+// float, i converted to float, for i = 1 .. npix-1 in row-major order.  One
+// wavefront: each step loads 64 consecutive pixels, one per lane, and every
+// lane runs the same sequential recurrence over them, taking pixel j from
+// lane j by readlane, while the next 64 pixels are already loading.
+// Skipped once a stop has fired.
+__global__ void __launch_bounds__(64) hogbom_running_mean_kernel(const float *__restrict__ r, int npix, State *slot)
+{
+    if (slot->stop) return;
+    const int lane = threadIdx.x;
+    float avg = 0.f;
+    float next = lane < npix ? r[lane] : 0.f;
+    for (int base = 0; base < npix; base += 64) {
+        const float cur = next;
+        const int q = base + 64 + lane;
+        next = q < npix ? r[q] : 0.f;
+        const int n = min(64, npix - base);
+        const int bits = __float_as_int(cur);
+        if (n == 64) {
+#pragma unroll
+            for (int j = 0; j < 64; ++j) {
+                const float val = fabsf(__int_as_float(__builtin_amdgcn_readlane(bits, j)));
+                const int idx = base + j;
+                avg = idx == 0 ? val : avg + (val - avg) / (float)idx;
+            }
+        } else {
+            for (int j = 0; j < n; ++j) {
+                const float val = fabsf(__int_as_float(__builtin_amdgcn_readlane(bits, j)));
+                const int idx = base + j;
+                avg = idx == 0 ? val : avg + (val - avg) / (float)idx;
+            }
+        }
+    }
+    if (lane == 0) slot->mean = avg;
 }
 
 constexpr unsigned kOob = 0x80000000u;   // a buffer offset past any PSF copy: reads as zero
@@ -184,13 +235,13 @@ hogbom_step_kernel(int iter, HogbomArgs a, float *__restrict__ r, const float *_
                 max_idx = 0;
                 peak = prev[0].v0;
             }
-            const float mean = (float)(red_s[0] / (double)(npix - 1));
+            const float mean = a.exact_mean ? in.mean : (float)(red_s[0] / (double)(npix - 1));
             const int nsrc = in.nsrc;
             const bool extracting_noise = (double)curr_max < a.noise * (double)mean * a.gain;
             const bool weak_source = nsrc > 0 && (double)curr_max < (double)fabsf(in.src0) * a.weak;
             if (extracting_noise || weak_source || nsrc >= a.max_sources) {
                 s_go = 0;
-                if (blockIdx.x == 0) *out = State{1, nsrc, in.src0, 0};
+                if (blockIdx.x == 0) *out = State{1, nsrc, in.src0, 0.f};
             } else {
                 const float max_val = (float)((double)peak / a.psf_max);
                 s_go = 1;
@@ -204,7 +255,7 @@ hogbom_step_kernel(int iter, HogbomArgs a, float *__restrict__ r, const float *_
                     sources[3 * nsrc + 2] = flux;
                     model[max_idx] = (float)((double)model[max_idx] + (double)max_val * a.gain);
                     *num_sources = nsrc + 1;
-                    *out = State{0, nsrc + 1, nsrc == 0 ? flux : in.src0, 0};
+                    *out = State{0, nsrc + 1, nsrc == 0 ? flux : in.src0, 0.f};
                 }
             }
         }
@@ -442,6 +493,8 @@ int sdp_hogbom_ex(int G, int n_minor, int max_sources, float *d_residual, const 
     a.max_sources = max_sources;
     a.mirror = (flags & SDP_HOGBOM_PSF_AS_STORED) ? 0 : 1;
     a.vec = (G % 4) == 0;
+    // the noise stop can fire only when noise_factor * gain > 1/2
+    a.exact_mean = !(flags & SDP_HOGBOM_DOUBLE_MEAN) && noise_factor * loop_gain > 0.5;
     // 16-byte groups per thread per chunk: 8 keeps more loads in flight, 4
     // more workgroups resident; the workgroup count is what the CUs hold at
     // once (no more than there are chunks), so one resident wave covers the
@@ -466,6 +519,11 @@ int sdp_hogbom_ex(int G, int n_minor, int max_sources, float *d_residual, const 
         state, d_model_in, d_model_out, d_model_in == d_model_out ? 0 : npix, d_psf, psf_pad, npix);
     SDP_LAUNCH_CHECK();
     for (int it = 0; it <= n_minor; ++it) {
+        if (a.exact_mean && it > 0) {
+            // the residual the previous launch left, for launch it's decision
+            hogbom_running_mean_kernel<<<1, 64, 0, st>>>(d_residual, (int)npix, state + (it & 1));
+            SDP_LAUNCH_CHECK();
+        }
         kern<<<a.nblocks, kThreads, 0, st>>>(it, a, d_residual, d_psf, psf_pad, d_model_out, d_sources,
                                              d_num_sources, state, parts);
         SDP_LAUNCH_CHECK();

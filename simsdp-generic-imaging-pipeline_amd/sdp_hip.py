@@ -39,6 +39,7 @@ ACC_BLOCKED, ACC_COMPENSATED = 0, 1
 FFT_FORWARD, FFT_INVERSE = -1, 1
 FFT_SHIFT_IN, FFT_SHIFT_OUT, FFT_REAL_IN, FFT_REAL_OUT, FFT_CORRECT_OUT, FFT_CORRECT_IN = 1, 2, 4, 8, 16, 32
 HOGBOM_PSF_AS_STORED = 1
+HOGBOM_DOUBLE_MEAN = 2
 # major-cycle pipeline (sdp_pipeline.h)
 PIPE_GRAPH, PIPE_REFERENCE_PREDICT, PIPE_PSF_MIRRORED, PIPE_PROFILE = 1, 2, 4, 8
 STAGES = ("predict_fft", "degrid_subtract", "grid", "inverse_fft", "clean")

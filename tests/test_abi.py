@@ -134,3 +134,115 @@ def test_setup_actors_on_host(tmp_path):
         np.testing.assert_array_equal(uvw, (rows[:, :3] * sign * (150e6 / 299792458.0)).astype(np.float32))
         wts = np.ones(3) if unit_weight else rows[:, 5]
         np.testing.assert_array_equal(vis, (rows[:, 3:5] * wts[:, None]).astype(np.float32))
+
+
+def _split_lib():
+    lib = ctypes.CDLL(sdp_hip.LIB_PATH)
+    P, I, CP = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(sdp_hip.Config)
+    lib.split_visibilities.argtypes = [CP, I, I, P, P, P, P, P, P]
+    lib.split_fftdegrid_visibilities.argtypes = [I, I, I, CP, P, P, P, P, P, P, P, P, P]
+    lib.split_uvgrid.argtypes = [I, I, P, P]
+    lib.merge_configs.argtypes = [P, P]
+    return lib
+
+
+def _config_bytes(c):
+    return bytes(memoryview(c))
+
+
+@pytest.mark.parametrize("n_vis,slices,n_corrected", [(1000, 4, 1000), (1003, 4, 1003), (1003, 4, 777),
+                                                      (10, 3, 0), (7, 8, 7)])
+def test_split_actors_on_host(n_vis, slices, n_corrected):
+    """split_visibilities, split_fftdegrid_visibilities, split_uvgrid and
+    merge_configs (host copies, no GPU) against a restatement of
+    splitmerge.c:3-56: copies of the inputs, one Config and one major-cycle
+    counter per slice, and NUM_VISIBILITIES / SLICES corrected visibilities
+    per slice while they last (a short last slice, then empty ones)."""
+    import numpy as np
+    lib = _split_lib()
+    rng = np.random.default_rng(n_vis + slices)
+    vis = rng.normal(size=(n_vis, 2)).astype(np.float32)
+    uvw = rng.normal(size=(n_vis, 3)).astype(np.float32)
+    cfg = sdp_hip.make_config(32, 0.01, num_kernels=3)
+    cfg.num_visibilities = n_vis
+    # split_visibilities: splitmerge.c:3-12
+    out_vis, out_uvw = np.zeros_like(vis), np.zeros_like(uvw)
+    cfgs = (sdp_hip.Config * slices)()
+    one = sdp_hip.Config()
+    lib.split_visibilities(ctypes.byref(cfg), n_vis, slices, vis.ctypes.data, uvw.ctypes.data, out_vis.ctypes.data,
+                           out_uvw.ctypes.data, ctypes.addressof(cfgs), ctypes.byref(one))
+    np.testing.assert_array_equal(out_vis, vis)
+    np.testing.assert_array_equal(out_uvw, uvw)
+    assert all(_config_bytes(c) == _config_bytes(cfg) for c in cfgs) and _config_bytes(one) == _config_bytes(cfg)
+    # split_fftdegrid_visibilities: splitmerge.c:30-48
+    out_vis, out_uvw = np.zeros_like(vis), np.zeros_like(uvw)
+    cfgs = (sdp_hip.Config * slices)()
+    num_vis = np.full(slices, -1, np.int32)
+    maj_out = np.full(slices, -1, np.int32)
+    lib.split_fftdegrid_visibilities(n_vis, slices, 32, ctypes.byref(cfg), uvw.ctypes.data, vis.ctypes.data,
+                                     ctypes.byref(ctypes.c_int(n_corrected)), ctypes.byref(ctypes.c_int(3)),
+                                     out_vis.ctypes.data, out_uvw.ctypes.data, ctypes.addressof(cfgs),
+                                     num_vis.ctypes.data, maj_out.ctypes.data)
+    np.testing.assert_array_equal(out_vis, vis)
+    np.testing.assert_array_equal(out_uvw, uvw)
+    assert all(_config_bytes(c) == _config_bytes(cfg) for c in cfgs)
+    assert list(maj_out) == [3] * slices
+    ideal, left, want = n_vis // slices, n_corrected, []
+    for _ in range(slices):
+        cur = min(ideal, left)
+        want.append(cur)
+        left -= cur
+    assert list(num_vis) == want
+    assert sum(want) == min(n_corrected, ideal * slices)
+    # split_uvgrid: splitmerge.c:50-56, one copy of the grid per slice
+    G = 16
+    grid = rng.normal(size=(G * G, 2)).astype(np.float32)
+    grids = np.zeros((slices, G * G, 2), np.float32)
+    lib.split_uvgrid(G, slices, grid.ctypes.data, grids.ctypes.data)
+    for s in range(slices):
+        np.testing.assert_array_equal(grids[s], grid)
+    # merge_configs: splitmerge.c:26-28, the first slice's Config
+    cfgs = (sdp_hip.Config * 2)(sdp_hip.make_config(64, 0.5), sdp_hip.make_config(32, 0.25))
+    out = sdp_hip.Config()
+    lib.merge_configs(ctypes.addressof(cfgs), ctypes.byref(out))
+    assert _config_bytes(out) == _config_bytes(cfgs[0]) and out.grid_size == 64
+
+
+def test_kernel_csv_loaders(tmp_path):
+    """kernel_host_set_up / degridding_kernel_host_set_up read the files the
+    Config names (top.c:235-379): one half support per line, plane p's
+    samples at offset sum_{q<p} ((h_q + 1) * OS)^2, the real and imaginary
+    files holding whitespace-separated samples plane after plane, each
+    value parsed as a double and stored as float."""
+    import numpy as np
+    lib = ctypes.CDLL(sdp_hip.LIB_PATH)
+    CP = ctypes.POINTER(sdp_hip.Config)
+    OS = 4
+    hs = [2, 3, 1]
+    counts = [((h + 1) * OS) ** 2 for h in hs]          # 144, 256, 64
+    total = sum(counts)
+    rng = np.random.default_rng(5)
+    re = rng.normal(size=total) * 10.0 ** rng.integers(-6, 3, total)
+    im = rng.normal(size=total)
+    for name, which in (("kernel_host_set_up", ""), ("degridding_kernel_host_set_up", "degridding_")):
+        d = tmp_path / name
+        d.mkdir()
+        (d / "sup.csv").write_text("".join(f"{h}\n" for h in hs))
+        # the real file one sample per line, the imaginary one space-separated rows of 16
+        (d / "re.csv").write_text("\n".join(repr(float(v)) for v in re) + "\n")
+        rows = [" ".join(f"{v:.12e}" for v in im[i:i + 16]) for i in range(0, total, 16)]
+        (d / "im.csv").write_text("\n".join(rows) + "\n")
+        cfg = sdp_hip.make_config(64, 0.01, oversampling=OS, num_kernels=len(hs))
+        setattr(cfg, which + "kernel_support_file", str(d / "sup.csv").encode())
+        setattr(cfg, which + "kernel_real_file", str(d / "re.csv").encode())
+        setattr(cfg, which + "kernel_imag_file", str(d / "im.csv").encode())
+        sup = np.full((len(hs), 2), -1, np.int32)
+        ker = np.full((total + 8, 2), np.nan, np.float32)
+        f = getattr(lib, name)
+        f.argtypes = [ctypes.c_int, ctypes.c_int, CP, ctypes.c_void_p, ctypes.c_void_p]
+        f(len(hs), total, ctypes.byref(cfg), sup.ctypes.data, ker.ctypes.data)
+        np.testing.assert_array_equal(sup[:, 0], hs)
+        np.testing.assert_array_equal(sup[:, 1], [0, counts[0], counts[0] + counts[1]])
+        np.testing.assert_array_equal(ker[:total, 0], re.astype(np.float32))
+        np.testing.assert_array_equal(ker[:total, 1], np.array([float(f"{v:.12e}") for v in im], np.float32))
+        assert np.isnan(ker[total:]).all()                 # nothing written past the last plane

@@ -113,3 +113,25 @@ def test_training_reduces_loss(hip):
         net.backward(d_g)
         net.adam(2e-3)
     assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])
+
+
+def test_fp32_configured_network_at_256_matches_oracle(hip):
+    """The configured proposal network, 4 levels x 32 base channels
+    (1,753,313 parameters), on one 256 x 256 (residual, PSF) patch in FP32
+    mode: forward pass within 1e-5 and every parameter gradient within 1e-4
+    of its largest magnitude of the FP64 oracle."""
+    levels, base, H = 4, 32, 256
+    net, params, x, t = setup(hip, levels, base, H, 1, hip.CNN_FP32, seed=17)
+    assert sum(v.size for v in params.values()) == 1_753_313
+    y, loss, grads, _ = run(hip, net, x, t)
+    yr, act = C.forward(params, x, levels)
+    lr, gy = C.mse(yr, t)
+    gr = C.backward(params, act, gy, levels)
+    e_fwd = np.abs(y - yr).max() / np.abs(yr).max()
+    e_grad = max(np.abs(grads[k] - gr[k]).max() / (np.abs(gr[k]).max() + 1e-30) for k in gr)
+    print(f"4x32 at 256^2, FP32: forward {e_fwd:.2e}, worst gradient {e_grad:.2e} of its largest magnitude")
+    assert e_fwd <= 1e-5
+    assert loss == pytest.approx(lr, rel=1e-5)
+    for k in gr:
+        assert np.abs(grads[k] - gr[k]).max() <= 1e-4 * np.abs(gr[k]).max() + 1e-9, k
+    net.close()

@@ -506,7 +506,7 @@ def test_finegrid_bit_exact(hip, actors, oracle, case):
 
 # ---------------------------------------------------------------- FFT
 
-@pytest.mark.parametrize("G", [16, 32, 64, 128, 256, 512, 1024, 2048])
+@pytest.mark.parametrize("G", [16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192])
 def test_fft_against_fp64(actors, rng, G):
     x = rng.normal(size=(G * G, 2)).astype(np.float32)
     z = (x[:, 0] + 1j * x[:, 1]).astype(np.complex128).reshape(G, G)
@@ -650,6 +650,73 @@ def test_hogbom_quirks(hip, oracle, rng):
         np.testing.assert_array_equal(d_src.download((8, 3), np.float32)[:ns], rsrc)
         np.testing.assert_array_equal(d_mo.download(G * G, np.float32), rmodel)
         np.testing.assert_array_equal(d_res.download(G * G, np.float32), rres)
+
+
+def run_device_hogbom(hip, G, n_minor, max_sources, r, psf_padded, cfg, flags=0):
+    d_res = hip.DeviceBuffer.from_array(r)
+    d_psf = hip.DeviceBuffer.from_array(psf_padded)
+    d_m = hip.DeviceBuffer.from_array(np.zeros(G * G, np.float32))
+    d_mo = hip.DeviceBuffer(4 * G * G)
+    d_src = hip.DeviceBuffer(12 * max(max_sources, 1))
+    d_n = hip.DeviceBuffer(16)
+    d_w = hip.DeviceBuffer(hip.hogbom_work_bytes(G))
+    hip.hogbom_ex(G, n_minor, max_sources, d_res, d_psf, cfg.psf_max_value, cfg.loop_gain, cfg.noise_factor,
+                  cfg.weak_source_percent_img, cfg.cell_size, d_m, d_mo, d_src, d_n, d_w, flags)
+    ns = int(d_n.download(1, np.int32)[0])
+    return (ns, d_src.download((max(max_sources, 1), 3), np.float32)[:ns], d_mo.download(G * G, np.float32),
+            d_res.download(G * G, np.float32))
+
+
+@pytest.mark.parametrize("G", [256, 2048])
+def test_hogbom_noise_stop_uses_the_reference_running_mean(hip, actors, oracle, G):
+    """A noise image whose threshold noise_factor * gain * mean is set
+    between the reference's float running average and the exact mean of
+    |r_1..r_{N-1}|, so the two means disagree on whether to stop at once.
+    The device path (its sequential recurrence) and the actor agree with
+    the oracle bit for bit; the parallel double-sum option decides the
+    other way, which shows that the case sits inside the drift band."""
+    rng = np.random.default_rng(G)
+    r = rng.standard_normal(G * G).astype(np.float32)
+    r[(G // 3) * G + G // 5] = 6.0
+    m_ref = oracle.running_mean(r)
+    m_exact = np.float32(np.abs(r[1:].astype(np.float64)).sum() / (G * G - 1))
+    assert m_ref != m_exact
+    peak = float(np.abs(r).max())
+    gain = 0.1
+    nf = peak / (gain * 0.5 * (float(m_ref) + float(m_exact)))
+    psf = np.zeros(G * G + 1, np.float32)
+    psf[(G // 2) * G + G // 2] = 1.0
+    cfg = hip.make_config(G, 1e-4, psf_max_value=1.0, noise_factor=nf, loop_gain=gain)
+    n_minor = 3
+    rns, rsrc, rmodel, rres = oracle.hogbom_clean(G, n_minor, 8, r, psf, cfg, np.zeros(G * G, np.float32))
+    ns, src, model, res = run_device_hogbom(hip, G, n_minor, 8, r, psf, cfg)
+    assert ns == rns
+    np.testing.assert_array_equal(src, rsrc)
+    np.testing.assert_array_equal(model, rmodel)
+    np.testing.assert_array_equal(res, rres)
+    ans, asrc, amodel = actors.hogbom_clean(G, n_minor, 8, r, psf[:-1].copy(), cfg, np.zeros(G * G, np.float32))
+    assert ans == rns
+    np.testing.assert_array_equal(asrc, rsrc)
+    np.testing.assert_array_equal(amodel, rmodel)
+    dns, _, _, _ = run_device_hogbom(hip, G, n_minor, 8, r, psf, cfg, flags=hip.HOGBOM_DOUBLE_MEAN)
+    assert (dns == 0) != (rns == 0)
+
+
+def test_hogbom_stops_after_max_sources(hip, oracle, rng):
+    """n_minor > max_sources: the reference would keep cleaning and write
+    past its sources array; the library stops after max_sources components,
+    which are the reference's first max_sources iterations bit for bit."""
+    G = 64
+    res, psf = dirty_case(G, rng)
+    cfg = hip.make_config(G, 1e-4, psf_max_value=1.0)
+    ns, src, model, r = run_device_hogbom(hip, G, 40, 12, res, psf, cfg)
+    rns, rsrc, rmodel, rres = oracle.hogbom_clean(G, 12, 12, res, psf, cfg, np.zeros(G * G, np.float32))
+    assert ns == rns == 12
+    np.testing.assert_array_equal(src, rsrc)
+    np.testing.assert_array_equal(model, rmodel)
+    np.testing.assert_array_equal(r, rres)
+    full = oracle.hogbom_clean(G, 40, 12, res, psf, cfg, np.zeros(G * G, np.float32))
+    assert full[0] == 40 and np.array_equal(full[1][:12], src)
 
 
 # ---------------------------------------------------------------- DFT

@@ -431,3 +431,169 @@ def test_merge_and_split(oracle, rng):
     np.testing.assert_allclose(oracle.merge(G, S, grids), grids.reshape(S, G * G, 2).sum(0), rtol=1e-6)
     assert list(oracle.split_counts(10, 4, 10)) == [2, 2, 2, 2]
     assert list(oracle.split_counts(12, 4, 7)) == [3, 3, 1, 0]
+
+
+# ---------------------------------------------------------------- gridding_actor (gridding_run.c:63-133)
+
+def _actor_taps(grid, G):
+    """{(v, u): kernel index} of a grid made with index_kernel and vis = 1."""
+    g = grid.reshape(G, G, 2)
+    return {(v - G // 2, u - G // 2): int(g[v, u, 0]) for v, u in zip(*np.nonzero(g[..., 1]))}
+
+
+def test_gridding_actor_snaps_to_sixteenths_and_takes_the_inclusive_range(oracle):
+    """gx = 10.02 snaps to roundf(160.32) / 16 = 10.0 (gridding_run.c:79-82),
+    and the inclusive range [ceil(s - h), floor(s + h)] (:84-92, :101-110)
+    then holds 2h + 1 = 7 columns, 7..13, with |round((u - 10) * 16)| =
+    48 32 16 0 16 32 48.  gy = -3.5 is already on a sixteenth: rows -6..-1
+    (floor(-0.5) = -1), 6 of them.  Without the snap the columns would be
+    8..13 with offsets 32 16 0 16 32 48; with std_gridding's half-open range
+    7..12."""
+    G, h = 64, 3
+    kern, sup = index_kernel(h, 16)
+    grid = oracle.gridding_actor(G, kern, sup, np.array([[10.02, -3.5, 0.0]], np.float32),
+                                 np.array([[1.0, 0.0]], np.float32), cfg_for(G))
+    kx = {7: 48, 8: 32, 9: 16, 10: 0, 11: 16, 12: 32, 13: 48}
+    ky = {-6: 40, -5: 24, -4: 8, -3: 8, -2: 24, -1: 40}
+    want = {(v, u): a * 64 + b for v, a in ky.items() for u, b in kx.items()}
+    got = _actor_taps(grid, G)
+    assert got == want
+    no_snap = {(v, u): ky[v] * 64 + abs(round((u - 10.02) * 16)) for v in ky for u in range(8, 14)}
+    half_open = {k: x for k, x in want.items() if k[1] <= 12}
+    assert got != no_snap and got != half_open
+    assert np.all(grid.reshape(G, G, 2)[..., 1][grid.reshape(G, G, 2)[..., 1] != 0] == 0.5)
+
+
+def test_gridding_actor_plane_rounds_half_away_from_zero(oracle):
+    """plane = (int)roundf(sqrtf(|w| * w_scale)) (gridding_run.c:67): with
+    w_scale = 1, w = +-6.25 gives sqrtf = 2.5, plane 3 (banker's rounding
+    or truncation would give 2); w = 2.2 gives 1.48, plane 1; w = -0.2
+    gives 0.447, plane 0.  Negative w conjugates the kernel (:76)."""
+    G, OS = 64, 16
+    hs = [1, 1, 1, 1]
+    sup = np.array([[1, p * 1024] for p in range(4)], np.int32)   # ((1 + 1) * 16)^2 = 1024 samples a plane
+    n = 4 * 1024
+    kern = np.stack([np.arange(n, dtype=np.float32), np.full(n, 0.25, np.float32)], axis=1)
+    cfg = cfg_for(G, n_kernels=4, max_w=9.0)                      # w_scale = (4 - 1)^2 / 9 = 1
+    assert cfg.w_scale == 1.0
+    for w, plane in ((6.25, 3), (-6.25, 3), (2.2, 1), (-0.2, 0), (2.25, 2)):
+        g = oracle.gridding_actor(G, kern, sup, np.array([[0.0, 0.0, w]], np.float32),
+                                  np.array([[1.0, 0.0]], np.float32), cfg).reshape(G, G, 2)
+        nz = g[..., 1] != 0
+        assert np.count_nonzero(nz) == 9, w                        # s = 0: columns and rows -1..1
+        idx = g[..., 0][nz]
+        # tap (0, 0) reads sample 0 of its plane, whose real part is the plane offset
+        assert np.all((idx >= plane * 1024) & (idx < (plane + 1) * 1024)), (w, plane)
+        assert np.all(g[..., 1][nz] == (-0.25 if w < 0 else 0.25)), w
+
+
+def test_gridding_actor_fixes_oversampling_at_16(oracle):
+    """gridding_actor passes 16 to gridding_CPU whatever config->oversampling
+    says (gridding_run.c:43): with oversampling = 8 in the Config, gx =
+    10.0625 still snaps to 161 / 16 = 10.0625 (8 would give 10.125), the
+    columns 8..13 take |round((u - 10.0625) * 16)| = 33 17 1 15 31 47 and
+    rows step by (h + 1) * 16 = 64 samples."""
+    G, h = 64, 3
+    kern, sup = index_kernel(h, 16)
+    cfg = cfg_for(G)
+    cfg.oversampling = 8
+    grid = oracle.gridding_actor(G, kern, sup, np.array([[10.0625, 0.0, 0.0]], np.float32),
+                                 np.array([[1.0, 0.0]], np.float32), cfg)
+    kx = {8: 33, 9: 17, 10: 1, 11: 15, 12: 31, 13: 47}
+    ky = {-3: 48, -2: 32, -1: 16, 0: 0, 1: 16, 2: 32, 3: 48}
+    assert _actor_taps(grid, G) == {(v, u): a * 64 + b for v, a in ky.items() for u, b in kx.items()}
+
+
+def test_gridding_actor_float_sum_tracks_its_fp64_twin(oracle, rng):
+    """The float grid against the same index arithmetic summed in double:
+    the two differ by float rounding only (within 1e-5 of the peak); their
+    tap sets are identical."""
+    d = __import__("sdp_synth").small_case(n_vis=4000, grid_size=128, n_planes=4, oversampling=16, seed=21)
+    cfg = cfg_for(d.grid_size, cell=d.cell_size, n_kernels=d.supports.shape[0], max_w=d.max_w)
+    f32 = oracle.gridding_actor(d.grid_size, d.kernels, d.supports, d.uvw, d.vis, cfg)
+    f64 = oracle.gridding_actor_f64(d.grid_size, d.kernels, d.supports, d.uvw, d.vis, d.uv_scale, d.w_scale)
+    peak = np.abs(f64).max()
+    assert peak > 0 and np.abs(f32 - f64).max() <= 1e-5 * peak
+    np.testing.assert_array_equal(f32 != 0, f64 != 0)
+
+
+# ---------------------------------------------------------------- Hogbom's "average"
+
+def test_running_mean_recurrence_excludes_pixel_zero(oracle):
+    """m = |r0|, m += (|r_i| - m) / i (deconvolution_run.c:226) is the mean
+    of |r_1..r_{n-1}|: on [5, 2, 4, 6, 8] every step is exact, 5 -> 2 -> 3
+    -> 4 -> 5 = mean(2, 4, 6, 8); the mean over all five would be 5 too, so
+    also [100, 2, 4, 6, 8] -> 5, not 24."""
+    assert oracle.running_mean(np.array([5, 2, 4, 6, 8], np.float32)) == 5.0
+    assert oracle.running_mean(np.array([100, -2, 4, -6, 8], np.float32)) == 5.0
+    assert oracle.running_mean(np.array([-3.5], np.float32)) == 3.5
+
+
+def test_hogbom_noise_stop_excludes_pixel_zero(oracle):
+    """Pixel 0 holds 100, every other pixel 1, noise_factor * gain = 80.
+    The reference's average excludes pixel 0 (mean 1, threshold 80), so
+    CLEAN takes pixel 0 three times (100, 90, 81) and stops at 72.9 < 80;
+    a mean that counted pixel 0 (355 / 256 = 1.39, threshold 111) would stop
+    at once with no component."""
+    G = 16
+    cfg = cfg_for(G, psf_max_value=1.0, noise_factor=800.0)
+    res = np.ones(G * G, np.float32)
+    res[0] = 100.0
+    ns, src, model, r = oracle.hogbom_clean(G, 10, 10, res, delta_psf(G), cfg, np.zeros(G * G, np.float32))
+    assert ns == 3
+    np.testing.assert_allclose(src[:, 2], [10.0, 9.0, 8.1], rtol=1e-6)
+    assert np.all(src[:, :2] == -8 * cfg.cell_size)            # pixel (0, 0) in radians
+    assert r[0] == pytest.approx(72.9, rel=1e-6)
+
+
+def test_running_mean_drift_at_4096(oracle):
+    """The float recurrence drifts from the exact mean as the image grows:
+    its increments (|r_i| - m) / i fall below half an ulp of m for i near
+    2^24.  Measured on half-normal noise (three seeds): within 2e-6 of the
+    exact mean at 128^2, 3e-5 at 1024^2, 1.2e-4 at 2048^2, and at 4096^2
+    the float value sits 0.70% below it.  A stop decision within that band of the threshold is where
+    a mean computed any other way can disagree with the reference."""
+    rng = np.random.default_rng(0)
+    small = rng.standard_normal(128 * 128).astype(np.float32)
+    exact = np.abs(small[1:].astype(np.float64)).mean()
+    assert oracle.running_mean(small) == pytest.approx(exact, rel=5e-6)
+    big = rng.standard_normal(4096 * 4096).astype(np.float32)
+    exact = np.abs(big[1:].astype(np.float64)).mean()
+    drift = oracle.running_mean(big) / exact - 1.0
+    assert -0.02 < drift < -0.001, drift
+
+
+# ---------------------------------------------------------------- taper against the spheroidal function
+
+def test_prolate_matches_the_spheroidal_function():
+    """The taper's rational approximation (top.c:408-447, Schwab 1984) against
+    the function it approximates, computed here from its differential
+    equation: the alpha = 1, c = 3 pi (support 6) spheroidal function psi
+    solves (1 - x^2) psi'' - 4 x psi' + (chi - c^2 x^2) psi = 0, whose
+    Gegenbauer C_k^(3/2) expansion turns into a matrix eigenproblem
+    (diag k (k + 3) + c^2 X^2) a = chi a, X the multiplication-by-x
+    operator from the three-term recurrence; the smallest chi gives psi.
+    The approximation's own error is 3.33e-7 (nu < 0.75) and 3.66e-7
+    (nu >= 0.75) of psi; a change of the second-to-last printed digit of
+    12 of its 14 coefficients, or of the last digit of 7 of them, pushes
+    it past the bars."""
+    import sdp_oracle
+    from scipy.special import eval_gegenbauer
+    c, lam, K = 3 * np.pi, 1.5, 60
+    X = np.zeros((K + 1, K + 1))
+    for k in range(K + 1):
+        if k + 1 <= K:
+            X[k + 1, k] = (k + 1) / (2 * (k + lam))
+        if k >= 1:
+            X[k - 1, k] = (k + 2 * lam - 1) / (2 * (k + lam))
+    M = (np.diag([k * (k + 3.0) for k in range(K + 1)]) + c * c * X @ X)[:K - 2, :K - 2]
+    w, v = np.linalg.eig(M)
+    a = v[:, np.argmin(w.real)].real
+    nu = np.linspace(0.0, 0.9999, 4000)
+    psi = sum(a[k] * eval_gegenbauer(k, lam, nu) for k in range(len(a)))
+    psi /= sum(a[k] * eval_gegenbauer(k, lam, 0.0) for k in range(len(a)))
+    assert np.min(w.real) == pytest.approx(7.7154048, abs=1e-6)   # chi_0 of alpha = 1, c = 3 pi
+    got = np.array([sdp_oracle.prolate_spheroidal(x) for x in nu])
+    rel = np.abs(got - psi) / psi
+    assert rel[nu < 0.75].max() <= 3.40e-7
+    assert rel[nu >= 0.75].max() <= 3.70e-7
