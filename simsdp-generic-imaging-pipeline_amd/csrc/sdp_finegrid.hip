@@ -2,7 +2,8 @@
 // src/CPU/degridgrid.c:12-65) and the nearest-cell value std_degridding
 // returns (src/CPU/degridgrid.c:194-200): index arithmetic that must match
 // the reference bit for bit, one thread per visibility.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 
@@ -173,13 +174,13 @@ extern "C" int sdp_finegrid_simplify(int n_vis, int G, int OS, double uv_scale, 
     SIMP_TRY(hipMalloc(&fine, 12 * n));
     simplify_keys_kernel<<<sdp_div_up(n, 256), 256, 0, st>>>(n_vis, G, OS, uv_scale, w_scale, d_uvw_in, keys, vals, fine);
     SIMP_TRY(hipGetLastError());
-    SIMP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, keys, skeys, vals, svals, n_vis, 0, 32, st));
-    SIMP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, first, pos, n_vis, st));
+    SIMP_TRY(rocprim::radix_sort_pairs(nullptr, tb1, keys, skeys, vals, svals, (size_t)n_vis, 0, 32, st));
+    SIMP_TRY(rocprim::exclusive_scan(nullptr, tb2, first, pos, 0, (size_t)n_vis, rocprim::plus<int>(), st));
     SIMP_TRY(hipMalloc(&temp, std::max(tb1, tb2)));
-    SIMP_TRY(hipcub::DeviceRadixSort::SortPairs(temp, tb1, keys, skeys, vals, svals, n_vis, 0, 32, st));
+    SIMP_TRY(rocprim::radix_sort_pairs(temp, tb1, keys, skeys, vals, svals, (size_t)n_vis, 0, 32, st));
     simplify_heads_kernel<<<sdp_div_up(n, 256), 256, 0, st>>>(n_vis, skeys, svals, first);
     SIMP_TRY(hipGetLastError());
-    SIMP_TRY(hipcub::DeviceScan::ExclusiveSum(temp, tb2, first, pos, n_vis, st));
+    SIMP_TRY(rocprim::exclusive_scan(temp, tb2, first, pos, 0, (size_t)n_vis, rocprim::plus<int>(), st));
     simplify_sum_kernel<<<sdp_div_up(n, 256), 256, 0, st>>>(n_vis, skeys, svals, pos, (const float2 *)d_vis_in, fine,
                                                              (float2 *)d_vis_out, d_uvw_out);
     SIMP_TRY(hipGetLastError());

@@ -16,6 +16,30 @@
 //   S = tile + 2 * (largest half support in the class).  Each (class, tile)
 //   list is cut into work items ("chunks") of about `chunk_taps` taps.
 //
+// Exact-footprint classes and the register gridder
+//   A visibility whose footprint is exactly 2h x 2h cells (not clipped at the
+//   grid edge, not widened to 2h + 1 by float rounding) with 1 <= h <= 16,
+//   and whose per-tap kernel offsets |round((u - p) OS)| are |d OS - c| for
+//   the footprint's d = 0 .. 2h-1 (checked tap by tap against the reference
+//   expression when the plan is built; half-way cases fail it), joins the
+//   class of its own h.  Its sort key carries its first cell's position
+//   (t_u, t_v) inside the uv tile.  Gridding these classes needs no index
+//   arithmetic per tap: an "unfolded" copy of each plane's table,
+//   [fy][fx][jy][jx] with jy, jx in [-h, h] holding the sample at
+//   (|jy OS + fy|, |jx OS + fx|), puts tap (d_u, d_v) of a visibility at
+//   base + d_v (2h + 1) + d_u, one base per visibility formed by the plan.
+//   One wavefront covers 64 of the W^2 = (2h)^2 footprint positions (a, b)
+//   of a work item; position (a, b) takes footprint tap
+//   ((a - t_u) mod W, (b - t_v) mod W) of every visibility, which lands on
+//   window column a or a + W and row b or b + W: each window cell belongs to
+//   exactly one lane, whatever the visibility.  While (t_u, t_v) stays the
+//   same (visibilities are sorted by it) a lane's tap offset and target cell
+//   stay fixed, so a tap is one buffer load (the per-visibility base as the
+//   scalar offset) and four FMAs into a register; a change of (t_u, t_v)
+//   adds the register into the lane's cell (LDS private to the lane).  The
+//   lanes then write the work item's window to its slab.  Degridding treats
+//   these classes like any other (their window is tile + 2h - 1).
+//
 // Kernels: one wavefront per chunk.
 //   grid    The wavefront is split into lane groups, each with a private
 //           S x S LDS sub-grid copy.  A group takes its share of the chunk's
@@ -33,7 +57,8 @@
 //   degrid+subtract+grid (major cycle): both of the above per visibility
 //           in one pass: predict, subtract from the measured value, grid the
 //           residual into the sub-grid.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <type_traits>
@@ -51,6 +76,11 @@ constexpr int kWave = 64;
 // 2h + 1 cells, so h <= 7 fits 16-lane groups and h <= 15 32-lane groups.
 constexpr int kNumClasses = 6;
 constexpr int kClassMaxH[kNumClasses] = {7, 11, 15, 20, 24, 31};
+// exact-footprint classes follow: class kNumClasses + h - 1 for h = 1 .. kFastMaxH
+constexpr int kFastMaxH = 16;
+constexpr int kAllClasses = kNumClasses + kFastMaxH;
+__host__ __device__ constexpr int fast_class(int h) { return kNumClasses + h - 1; }
+constexpr bool is_fast(int c) { return c >= kNumClasses; }
 // LDS of one workgroup (one wavefront): accumulators, or accumulators plus
 // the model-grid window for the fused kernel
 constexpr size_t kMaxLds = 152 * 1024;
@@ -96,12 +126,32 @@ __device__ __forceinline__ int kernel_tap(int cell_abs, float pos, float os)
 
 struct PlanArgs {
     int n, G, OS, n_kernels, snap_mode, tile, ntx, nty, ntiles, nkeys;
-    int plane_bits;   // sort key = (class, tile) key << plane_bits | plane
+    int plane_bits;   // sort key = (class, tile) key << plane_bits | low field (plane, or t_u + t_v tile)
     double uv_scale, w_scale;
 };
 
-__global__ void plan_records_kernel(const float *uvw, const int2 *supports, const int *plane_class, PlanArgs a,
-                                    VisRec *recs, uint32_t *keys, uint32_t *vals)
+// The exact-footprint test of one axis: first cell c0 (signed grid
+// coordinate) of a 2h-cell footprint at position p.  With c the reference's
+// tap offset of the first cell, every tap d must read |d OS - c| (the
+// reference expression, kernel_tap, evaluated for each d), and the signed
+// offsets d OS - c = j OS + f (0 <= f < OS) must keep j in [-h, h]; then
+// *j0 = j at d = 0 and *f the phase.
+__device__ bool exact_axis(int c0, float p, int h, int OS, float os, int *j0, int *f)
+{
+    const int c = kernel_tap(c0, p, os);
+    for (int d = 1; d < 2 * h; ++d)
+        if (kernel_tap(c0 + d, p, os) != abs(d * OS - c)) return false;
+    const int j = -((c + OS - 1) / OS);   // floor(-c / OS), c >= 0
+    *j0 = j;
+    *f = -c - j * OS;
+    return j >= -h && j + 2 * h - 1 <= h;
+}
+
+// plane_utab[p]: the plane's offset in the unfolded table, or -1 when its
+// visibilities never join an exact-footprint class
+__global__ void plan_records_kernel(const float *uvw, const int2 *supports, const int *plane_class,
+                                    const int *plane_utab, PlanArgs a, VisRec *recs, uint2 *frecs, uint32_t *keys,
+                                    uint32_t *vals)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -109,6 +159,7 @@ __global__ void plan_records_kernel(const float *uvw, const int2 *supports, cons
     const int gc = a.G / 2;
     keys[i] = (uint32_t)a.nkeys << a.plane_bits;   // dropped: sorts last
     vals[i] = (uint32_t)i;
+    frecs[i] = make_uint2(0u, 0u);
 
     const float wabs = fabsf((float)((double)z * a.w_scale));
     const float root = __fsqrt_rn(wabs);
@@ -119,7 +170,7 @@ __global__ void plan_records_kernel(const float *uvw, const int2 *supports, cons
         plane = (int)roundf(root);                   // gridding_run.c:67
     if (!(root == root) || plane < 0 || plane >= a.n_kernels) return;
     const int h = supports[plane].x;
-    const int cls = plane_class[plane];
+    int cls = plane_class[plane];
     if (cls < 0) return;
 
     float gx = (float)((double)x * a.uv_scale);
@@ -146,44 +197,123 @@ __global__ void plan_records_kernel(const float *uvw, const int2 *supports, cons
     r.start = (uint32_t)ua | ((uint32_t)va << 16);
     r.meta = (uint32_t)nu | ((uint32_t)nv << 8) | ((uint32_t)plane << 16) | ((z < 0.0f) ? 0x80000000u : 0u);
     recs[i] = r;
-    const uint32_t key = (uint32_t)(cls * a.ntiles + (va / a.tile) * a.ntx + (ua / a.tile));
     // within a (class, tile) list, records in plane order: neighbours share
     // a kernel table and a footprint size
-    keys[i] = key << a.plane_bits | (uint32_t)plane;
+    uint32_t low = (uint32_t)plane;
+    uint2 fr = make_uint2(0u, 0u);
+    const int ut = plane_utab[plane];
+    if (ut >= 0 && nu == 2 * h && nv == 2 * h && ua == (int)lo_x + gc && va == (int)lo_y + gc) {
+        int jx, fx, jy, fy;
+        const float os = (float)a.OS;
+        if (exact_axis((int)lo_x, gx, h, a.OS, os, &jx, &fx) && exact_axis((int)lo_y, gy, h, a.OS, os, &jy, &fy)) {
+            const int n = 2 * h + 1;
+            const int tu = ua % a.tile, tv = va % a.tile;
+            cls = fast_class(h);
+            // exact-footprint lists in (t_u, t_v, conj) order: a lane's tap
+            // stays put, and the conjugation is one per run
+            const uint32_t conj = r.meta >> 31;
+            low = (uint32_t)(tu + tv * a.tile) << 1 | conj;
+            fr.x = 8u * (uint32_t)(ut + ((fy * a.OS + fx) * n + jy + h) * n + jx + h);   // bytes
+            fr.y = (uint32_t)tu | (uint32_t)tv << 5 | conj << 10;
+        }
+    }
+    frecs[i] = fr;
+    const uint32_t key = (uint32_t)(cls * a.ntiles + (va / a.tile) * a.ntx + (ua / a.tile));
+    keys[i] = key << a.plane_bits | (low & ((1u << a.plane_bits) - 1u));
 }
 
-// Per-key visibility and tap counts from the sorted keys: a run of equal
-// keys is one (class, tile) list.  The first kernel writes each run's start
-// and every record's tap count; after an inclusive scan of the tap counts the
-// second kernel, at each run's last record, forms the run's length and tap
-// total.  Dropped records (key == nkeys) sort last and are skipped.  The
-// sort keys carry the plane in their low plane_bits bits.
-__global__ void key_run_start_kernel(const uint32_t *keys, const VisRec *recs, int n, int nkeys, int plane_bits,
-                                     unsigned int *key_first, unsigned long long *taps)
+// The (class, tile) lists of the sorted keys as runs: a run of equal keys
+// is one list.  The first kernel flags each run's first record and writes
+// every record's tap count; after an exclusive scan of the flags (run
+// numbers) and an inclusive scan of the tap counts, the second records each
+// run's key and first position at its start, and the third its length and
+// tap total at its end.  Dropped records (key == nkeys) sort last and are
+// skipped.  The sort keys carry a low field of plane_bits bits.
+struct KeyRun {
+    uint32_t key, first, count, pad;
+    unsigned long long taps;
+};
+
+__global__ void run_flags_kernel(const uint32_t *keys, const VisRec *recs, int n, int nkeys, int plane_bits,
+                                 uint32_t *flags, unsigned long long *taps)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = keys[i] >> plane_bits;
     if (k >= (uint32_t)nkeys) {
         taps[i] = 0;
+        flags[i] = 0;
         return;
     }
     const uint32_t m = recs[i].meta;
     taps[i] = (unsigned long long)((m & 0xffu) * ((m >> 8) & 0xffu));
-    if (i == 0 || (keys[i - 1] >> plane_bits) != k) key_first[k] = (unsigned int)i;
+    flags[i] = (i == 0 || (keys[i - 1] >> plane_bits) != k) ? 1u : 0u;
 }
 
-__global__ void key_run_end_kernel(const uint32_t *keys, int n, int nkeys, int plane_bits,
-                                   const unsigned int *key_first, const unsigned long long *tap_scan,
-                                   unsigned int *key_count, unsigned long long *key_taps)
+__global__ void run_start_kernel(const uint32_t *keys, const uint32_t *flags, const uint32_t *run_no, int n,
+                                 int plane_bits, KeyRun *runs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flags[i]) return;
+    KeyRun r;
+    r.key = keys[i] >> plane_bits;
+    r.first = (uint32_t)i;
+    r.count = 0;
+    r.pad = 0;
+    r.taps = 0;
+    runs[run_no[i]] = r;
+}
+
+__global__ void run_end_kernel(const uint32_t *keys, const uint32_t *flags, const uint32_t *run_no, int n,
+                               int nkeys, int plane_bits, const unsigned long long *tap_scan, KeyRun *runs)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = keys[i] >> plane_bits;
     if (k >= (uint32_t)nkeys || (i + 1 < n && (keys[i + 1] >> plane_bits) == k)) return;
-    const unsigned int f = key_first[k];
-    key_count[k] = (unsigned int)(i + 1) - f;
-    key_taps[k] = tap_scan[i] - (f > 0 ? tap_scan[f - 1] : 0ull);
+    KeyRun *r = &runs[run_no[i] + flags[i] - 1];
+    const uint32_t f = r->first;
+    r->count = (uint32_t)(i + 1) - f;
+    r->taps = tap_scan[i] - (f > 0 ? tap_scan[f - 1] : 0ull);
+}
+
+// the exact-footprint records follow the same permutation
+__global__ void gather_frecs_kernel(const uint2 *frecs, const uint32_t *order, int n, uint2 *out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = frecs[order[i]];
+}
+
+// Runs of records (sorted order) with one (class, tile) key and one
+// (t_u, t_v, conj) field: each record's bits 11-31 receive the records left
+// in its run from it on (capped at 2^21 - 1), so the register gridder takes
+// a run in one stretch.  Run starts are flagged, numbered by an inclusive
+// scan of the flags, each run's end is recorded at its last record, and
+// every record then reads its run's end.
+constexpr uint32_t kRunField = 0x7ffu;   // t_u | t_v << 5 | conj << 10
+constexpr uint32_t kRunCap = (1u << 21) - 1u;
+__global__ void fast_run_flags_kernel(const uint32_t *keys, const uint2 *frecs, int n, int plane_bits,
+                                      uint32_t *flags)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flags[i] = (i == 0 || (keys[i - 1] >> plane_bits) != (keys[i] >> plane_bits) ||
+                ((frecs[i - 1].y ^ frecs[i].y) & kRunField) != 0u) ? 1u : 0u;
+}
+
+__global__ void fast_run_ends_kernel(const uint32_t *flags, const uint32_t *run_id, int n, uint32_t *run_end)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i + 1 == n || flags[i + 1]) run_end[run_id[i] - 1] = (uint32_t)(i + 1);
+}
+
+__global__ void fast_run_rem_kernel(const uint32_t *run_id, const uint32_t *run_end, int n, uint2 *frecs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t rem = min(run_end[run_id[i] - 1] - (uint32_t)i, kRunCap);
+    frecs[i].y = (frecs[i].y & kRunField) | rem << 11;
 }
 
 __global__ void gather_records_kernel(const VisRec *recs, const uint32_t *order, int n, VisRec *out)
@@ -685,6 +815,188 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
     }
 }
 
+// ---- the register gridder of the exact-footprint classes -----------------
+
+// One workgroup row per plane (blockIdx.y); tab holds (quadrant offset, h,
+// unfolded offset, -).  Output sample ((fy OS + fx) n + jy + h) n + jx + h,
+// n = 2h + 1, is the quadrant sample (|jy OS + fy|, |jx OS + fx|): a
+// visibility's taps are then a (2h) x (2h) block of rows n samples apart.
+__global__ void unfold_kernel(const float2 *__restrict__ in, const int4 *__restrict__ tab, int OS,
+                              float2 *__restrict__ out)
+{
+    const int4 t = tab[blockIdx.y];
+    const int h = t.y, n = 2 * h + 1, row = (h + 1) * OS;
+    const int total = OS * OS * n * n;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int jx = i % n - h;
+        int r = i / n;
+        const int jy = r % n - h;
+        r /= n;
+        const int fx = r % OS, fy = r / OS;
+        const int kx = abs(jx * OS + fx), ky = abs(jy * OS + fy);   // < (h + 1) OS
+        out[t.z + i] = in[t.x + ky * row + kx];
+    }
+}
+
+// a sample at byte offset voff + soff of the unfolded table (soff uniform:
+// the visibility's base); offsets past the table read zero
+__device__ __forceinline__ float2 utab_load(__amdgpu_buffer_rsrc_t rs, unsigned voff, unsigned soff)
+{
+    const v2f_t v = __builtin_bit_cast(v2f_t, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+    return make_float2(v.x, v.y);
+}
+
+// vis * (k.x, conj k.y) added to a lane's running sum: re += vr kx - cvi ky,
+// im += vi kx + cvr ky (cvr, cvi: the conjugation sign applied to vr, vi)
+__device__ __forceinline__ void tap_add(float2 &t, float2 k, float vr, float vi, float cvr, float cvi)
+{
+    t.x = fmaf(k.x, vr, t.x);
+    t.x = fmaf(k.y, -cvi, t.x);
+    t.y = fmaf(k.x, vi, t.y);
+    t.y = fmaf(k.y, cvr, t.y);
+}
+__device__ __forceinline__ void tap_add(float4 &t, float2 k, float vr, float vi, float cvr, float cvi)
+{
+    // the product rounded first, then a TwoSum into (sum, error)
+    two_sum_acc(t.x, t.z, fmaf(vr, k.x, -(cvi * k.y)));
+    two_sum_acc(t.y, t.w, fmaf(vi, k.x, cvr * k.y));
+}
+__device__ __forceinline__ void acc_merge(float2 &c, const float2 &t)
+{
+    c.x += t.x;
+    c.y += t.y;
+}
+__device__ __forceinline__ void acc_merge(float4 &c, const float4 &t)
+{
+    two_sum_acc(c.x, c.z, t.x);
+    two_sum_acc(c.y, c.w, t.y);
+    c.z += t.z;
+    c.w += t.w;
+}
+__device__ __forceinline__ float2 acc_value(const float2 &c) { return c; }
+__device__ __forceinline__ float2 acc_value(const float4 &c) { return make_float2(c.x + c.z, c.y + c.w); }
+
+// vis * (k.x, conj k.y) for conj = -1: re += vr kx + vi ky, im += vi kx - vr ky
+__device__ __forceinline__ void tap_add_conj(float2 &t, float2 k, float vr, float vi)
+{
+    t.x = fmaf(k.x, vr, t.x);
+    t.x = fmaf(k.y, vi, t.x);
+    t.y = fmaf(k.x, vi, t.y);
+    t.y = fmaf(k.y, -vr, t.y);
+}
+__device__ __forceinline__ void tap_add_conj(float4 &t, float2 k, float vr, float vi)
+{
+    two_sum_acc(t.x, t.z, fmaf(vr, k.x, vi * k.y));
+    two_sum_acc(t.y, t.w, fmaf(vi, k.x, -(vr * k.y)));
+}
+
+// len records of one run: per record its base (bytes, the scalar offset of
+// the table load) and its visibility.  Records go in groups of 8 whose table
+// loads are issued one group ahead of the group being summed.
+template <bool CONJ, bool PERM, typename Acc>
+__device__ __forceinline__ void fast_run(Acc &run, __amdgpu_buffer_rsrc_t rs, unsigned voff, const uint2 *fr,
+                                         const float2 *vv, const uint32_t *pp, const float2 *vis, int len)
+{
+    constexpr int U = 8;
+    auto add = [&](float2 k, float2 v) {
+        if (CONJ) tap_add_conj(run, k, v.x, v.y);
+        else tap_add(run, k, v.x, v.y, v.x, v.y);
+    };
+    const int groups = len / U;
+    if (groups > 0) {
+        float2 k[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) k[q] = utab_load(rs, voff, fr[q].x);
+        for (int g = 0; g < groups; ++g) {
+            const int q0 = g * U;
+            float2 kn[U];
+            if (g + 1 < groups) {
+#pragma unroll
+                for (int q = 0; q < U; ++q) kn[q] = utab_load(rs, voff, fr[q0 + U + q].x);
+            }
+            float2 v[U];
+#pragma unroll
+            for (int q = 0; q < U; ++q) v[q] = PERM ? vis[pp[q0 + q]] : vv[q0 + q];
+#pragma unroll
+            for (int q = 0; q < U; ++q) add(k[q], v[q]);
+            if (g + 1 < groups) {
+#pragma unroll
+                for (int q = 0; q < U; ++q) k[q] = kn[q];
+            }
+        }
+    }
+    for (int q0 = groups * U; q0 < len; ++q0) add(utab_load(rs, voff, fr[q0].x), PERM ? vis[pp[q0]] : vv[q0]);
+}
+
+// One wavefront per item (work item, part w): footprint positions
+// p = 64 w + lane < W^2, (a, b) = (p % W, p / W).  kFastWaves independent
+// wavefronts share a workgroup (no barriers; each its own LDS cells), so the
+// CU's workgroup slots do not cap the resident wavefronts.
+constexpr int kFastWaves = 4;
+template <bool COMP, bool PERM>
+__global__ void __launch_bounds__(kWave * kFastWaves)
+fast_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restrict__ perm, const int4 *__restrict__ chunks,
+                 const int2 *__restrict__ items, const long long *__restrict__ slab_off,
+                 const float2 *__restrict__ vis, const float2 *__restrict__ utab, int tab_bytes, int tile,
+                 float2 *__restrict__ slabs)
+{
+    using Acc = typename AccT<COMP>::T;
+    __shared__ Acc wave_cells[kFastWaves][4 * kWave];   // lane-private: cell q of lane l at [q * 64 + l]
+    // the wavefront's index, uniform: its item, records and visibilities are
+    // then read through scalar loads
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    Acc *cells = wave_cells[wave];
+    const int2 item = items[blockIdx.x * kFastWaves + wave];
+    if (item.x < 0) return;             // padding after the last item
+    const int4 ch = chunks[item.x];     // tile, first record, count, window S
+    const int S = ch.w, W = S - tile + 1, n = W + 1;
+    const int lane = threadIdx.x % kWave;
+    const int p = item.y * kWave + lane;
+    const bool active = p < W * W;
+    const int a = active ? p % W : 0, b = active ? p / W : 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc_zero(cells[q * kWave + lane]);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)utab, (short)0, tab_bytes, 0x00020000);
+    const uint2 *fr = frecs + ch.y;
+    const float2 *vv = PERM ? nullptr : vis + ch.y;
+    const uint32_t *pp = PERM ? perm + ch.y : nullptr;
+    Acc run;
+    acc_zero(run);
+    bool started = false;    // a run has been summed into `run`
+    unsigned voff = kOob;    // this lane's tap in the current run (bytes from the visibility's base)
+    int cell = 0;
+    int j = 0;
+    while (j < ch.z) {
+        // a run: records j .. j + len - 1 share (t_u, t_v) and the conjugation
+        const uint2 r0 = fr[j];
+        const int len = max(1, min((int)(r0.y >> 11), ch.z - j));
+        if (started) acc_merge(cells[cell * kWave + lane], run);
+        {
+            const int tu = (int)(r0.y & 0x1fu), tv = (int)((r0.y >> 5) & 0x1fu);
+            const int du = a >= tu ? a - tu : a - tu + W, dv = b >= tv ? b - tv : b - tv + W;
+            voff = active ? (unsigned)(8 * (dv * n + du)) : (unsigned)kOob;
+            cell = (a < tu ? 1 : 0) + (b < tv ? 2 : 0);
+            started = true;
+            acc_zero(run);
+        }
+        if (r0.y & 0x400u)
+            fast_run<true, PERM>(run, rs, voff, fr + j, PERM ? nullptr : vv + j, PERM ? pp + j : nullptr, vis, len);
+        else
+            fast_run<false, PERM>(run, rs, voff, fr + j, PERM ? nullptr : vv + j, PERM ? pp + j : nullptr, vis, len);
+        j += len;
+    }
+    if (started) acc_merge(cells[cell * kWave + lane], run);
+    // cell q of lane (a, b) is window cell (b + (q >> 1) W, a + (q & 1) W)
+    if (active) {
+        float2 *slab = slabs + slab_off[item.x];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int u = a + (q & 1) * W, v = b + (q >> 1) * W;
+            if (u < S && v < S) slab[v * S + u] = acc_value(cells[q * kWave + lane]);
+        }
+    }
+}
+
 constexpr int kMergeThreads = 256;
 // A key's pieces summed by one workgroup of the first reduce pass: a key with
 // more (a dense tile can have a thousand) is split into groups whose partial
@@ -897,10 +1209,18 @@ struct sdp_grid_plan {
     int n = 0, n_kept = 0, tile = 0, chunk_taps = 0, ntx = 0, nty = 0, H = 0, smax = 0;
     int n_chunks = 0, n_tiles_used = 0, n_class_launches = 0;
     long long total_taps = 0;
-    int class_S[kNumClasses] = {};
-    int class_h[kNumClasses] = {};
-    int class_chunk_begin[kNumClasses + 1] = {};
+    int class_S[kAllClasses] = {};
+    int class_h[kAllClasses] = {};
+    int class_chunk_begin[kAllClasses + 1] = {};
     VisRec *recs = nullptr;        // plan order
+    uint2 *frecs = nullptr;        // plan order: exact-footprint records (unfolded base, t_u | t_v << 8 | conj)
+    int n_unfold = 0;              // planes with an unfolded table
+    int4 *unfold_tab = nullptr;    // per such plane: quadrant offset, h, unfolded offset
+    long long utab_len = 0;        // samples of the unfolded gridding table
+    float2 *utab = nullptr;
+    int n_items = 0;               // register-gridder items: (chunk, 64-position part)
+    int2 *items = nullptr;
+    long long n_fast = 0;          // visibilities in exact-footprint classes
     uint32_t *perm = nullptr;      // plan position -> caller index
     int2 *plane_tab = nullptr;     // (sample offset, h+1) per plane
     long long table_len = 0;       // samples in a kernel table (all planes)
@@ -932,6 +1252,10 @@ void plan_free(sdp_grid_plan *p)
 {
     if (!p) return;
     (void)hipFree(p->recs);
+    (void)hipFree(p->frecs);
+    (void)hipFree(p->unfold_tab);
+    (void)hipFree(p->utab);
+    (void)hipFree(p->items);
     (void)hipFree(p->perm);
     (void)hipFree(p->plane_tab);
     (void)hipFree(p->ktab);
@@ -1021,8 +1345,40 @@ int class_groups(int mode, bool comp, int c, int col_shift)
     return std::min(comp || c > 0 ? 1 : 2, widest);
 }
 
+// a contiguous range of work items launched together: chunks [b, e), the
+// largest window S and half support h among them, and the support class
+// whose tuning overrides apply
+struct LaunchRange {
+    int b, e, S, h, env_c;
+    bool fast;
+};
+
+// The launches of the LDS kernels: one per support class, and the
+// exact-footprint classes grouped by the lane-group width their largest
+// footprint needs (h <= 7, <= 15, 16).
+std::vector<LaunchRange> lds_ranges(const sdp_grid_plan *plan, bool with_fast)
+{
+    std::vector<LaunchRange> out;
+    for (int c = 0; c < kAllClasses; ++c) {
+        const int b = plan->class_chunk_begin[c], e = plan->class_chunk_begin[c + 1];
+        if (e <= b || (is_fast(c) && !with_fast)) continue;
+        const int h = plan->class_h[c], S = plan->class_S[c];
+        if (is_fast(c) && !out.empty() && out.back().fast && out.back().e == b &&
+            class_lane_shift(out.back().h, plan->p.snap_mode) == class_lane_shift(h, plan->p.snap_mode)) {
+            LaunchRange &r = out.back();
+            r.e = e;
+            r.S = std::max(r.S, S);
+            r.h = std::max(r.h, h);
+            continue;
+        }
+        out.push_back(LaunchRange{b, e, S, h, is_fast(c) ? class_of(h) : c, is_fast(c)});
+    }
+    return out;
+}
+
 template <int MODE, bool COMP>
-int launch_class(const sdp_grid_plan *plan, int c, const ChunkIO &io, const uint32_t *perm, hipStream_t st)
+int launch_class(const sdp_grid_plan *plan, const LaunchRange &rg, const ChunkIO &io, const uint32_t *perm,
+                 hipStream_t st)
 {
     static bool attr_set = false;
     if (!attr_set) {
@@ -1032,9 +1388,8 @@ int launch_class(const sdp_grid_plan *plan, int c, const ChunkIO &io, const uint
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
         attr_set = true;
     }
-    const int b = plan->class_chunk_begin[c], e = plan->class_chunk_begin[c + 1];
-    const int S = plan->class_S[c];
-    const int col_shift = class_lane_shift(plan->class_h[c], plan->p.snap_mode);
+    const int b = rg.b, e = rg.e, S = rg.S, c = rg.env_c;
+    const int col_shift = class_lane_shift(rg.h, plan->p.snap_mode);
     int copies = class_groups(MODE, COMP, c, col_shift);
     if (MODE != MODE_DEGRID) copies = std::min(copies, class_copies(MODE, S, col_shift, COMP, plan->p.n_kernels));
     int lane_shift = 0;
@@ -1058,12 +1413,46 @@ int launch_class(const sdp_grid_plan *plan, int c, const ChunkIO &io, const uint
     return SDP_OK;
 }
 
+// the register gridder over every exact-footprint work item: one launch
+int launch_fast(const sdp_grid_plan *plan, const float2 *vis, const uint32_t *perm, hipStream_t st)
+{
+    const bool comp = plan->p.accumulate == SDP_ACC_COMPENSATED;
+    const int tb = (int)(8 * plan->utab_len);
+    if (comp && perm)
+        fast_grid_kernel<true, true><<<plan->n_items / kFastWaves, kWave * kFastWaves, 0, st>>>(plan->frecs, perm, plan->chunks, plan->items,
+                                                                     plan->slab_off, vis, plan->utab, tb, plan->tile,
+                                                                     plan->slabs);
+    else if (comp)
+        fast_grid_kernel<true, false><<<plan->n_items / kFastWaves, kWave * kFastWaves, 0, st>>>(plan->frecs, perm, plan->chunks, plan->items,
+                                                                      plan->slab_off, vis, plan->utab, tb, plan->tile,
+                                                                      plan->slabs);
+    else if (perm)
+        fast_grid_kernel<false, true><<<plan->n_items / kFastWaves, kWave * kFastWaves, 0, st>>>(plan->frecs, perm, plan->chunks, plan->items,
+                                                                      plan->slab_off, vis, plan->utab, tb, plan->tile,
+                                                                      plan->slabs);
+    else
+        fast_grid_kernel<false, false><<<plan->n_items / kFastWaves, kWave * kFastWaves, 0, st>>>(plan->frecs, perm, plan->chunks, plan->items,
+                                                                       plan->slab_off, vis, plan->utab, tb,
+                                                                       plan->tile, plan->slabs);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+// the unfolded copy of the gridding table
+void unfold(const sdp_grid_plan *plan, const float *d_kernels, hipStream_t st)
+{
+    const int OS = plan->p.oversampling, n = 2 * kFastMaxH + 1;
+    const int bx = (int)std::min<long long>(128, ((long long)OS * OS * n * n + 255) / 256);
+    unfold_kernel<<<dim3(bx, plan->n_unfold), 256, 0, st>>>((const float2 *)d_kernels, plan->unfold_tab, OS,
+                                                            plan->utab);
+}
+
 // phase-major copy of a kernel table into the plan (slot 0 gridding, 1 degridding)
 const float2 *relayout(const sdp_grid_plan *plan, const float *d_kernels, int slot, hipStream_t st)
 {
     float2 *out = plan->ktab + (size_t)slot * (size_t)plan->table_len;
     int maxn = 1;
-    for (int c = 0; c < kNumClasses; ++c) maxn = std::max(maxn, (plan->class_h[c] + 1) * plan->p.oversampling);
+    for (int c = 0; c < kAllClasses; ++c) maxn = std::max(maxn, (plan->class_h[c] + 1) * plan->p.oversampling);
     const int bx = (int)std::min<long long>(64, ((long long)maxn * maxn + 255) / 256);
     relayout_kernel<<<dim3(bx, plan->p.n_kernels), 256, 0, st>>>((const float2 *)d_kernels, plan->plane_tab,
                                                                   plan->p.oversampling, out);
@@ -1100,20 +1489,21 @@ template <int MODE>
 int launch_chunks(const sdp_grid_plan *plan, const ChunkIO &io, const uint32_t *perm, hipStream_t st)
 {
     const bool comp = plan->p.accumulate == SDP_ACC_COMPENSATED;
-    // The class with the most work items runs on the caller's stream; the
-    // others, whose launches are too small to fill the GPU on their own, run
-    // beside it on a helper stream forked from and joined back to it (slabs
-    // of different classes are disjoint).
-    int big = -1, n_used = 0;
-    for (int c = 0; c < kNumClasses; ++c) {
-        const int n = plan->class_chunk_begin[c + 1] - plan->class_chunk_begin[c];
-        if (n <= 0) continue;
-        ++n_used;
-        if (big < 0 || n > plan->class_chunk_begin[big + 1] - plan->class_chunk_begin[big]) big = c;
-    }
+    // gridding takes the exact-footprint classes in one register-gridder launch
+    const bool fast = MODE == MODE_GRID && plan->n_items > 0;
+    const std::vector<LaunchRange> ranges = lds_ranges(plan, MODE != MODE_GRID);
+    // The launch with the most work runs on the caller's stream (the
+    // register gridder when it runs); the others, too small to fill the GPU
+    // on their own, run beside it on a helper stream forked from and joined
+    // back to it (slabs of different launches are disjoint).
+    int big = -1;
+    if (!fast)
+        for (size_t i = 0; i < ranges.size(); ++i)
+            if (big < 0 || ranges[i].e - ranges[i].b > ranges[big].e - ranges[big].b) big = (int)i;
+    const size_t n_launch = ranges.size() + (fast ? 1 : 0);
     hipStream_t side = st;
     hipEvent_t join = nullptr;
-    if (n_used > 1) {
+    if (n_launch > 1) {
         hipEvent_t fork = nullptr;
         if (side_stream(&side, &fork, &join) == SDP_OK) {
             SDP_HIP_TRY(hipEventRecord(fork, st));
@@ -1124,11 +1514,11 @@ int launch_chunks(const sdp_grid_plan *plan, const ChunkIO &io, const uint32_t *
         }
     }
     int rc = SDP_OK;
-    for (int c = 0; c < kNumClasses && rc == SDP_OK; ++c) {
-        if (plan->class_chunk_begin[c + 1] <= plan->class_chunk_begin[c]) continue;
-        hipStream_t s = c == big ? st : side;
-        rc = (comp || MODE == MODE_DEGRID) ? launch_class<MODE, true>(plan, c, io, perm, s)
-                                           : launch_class<MODE, false>(plan, c, io, perm, s);
+    if (fast) rc = launch_fast(plan, io.vis, perm, st);
+    for (size_t i = 0; i < ranges.size() && rc == SDP_OK; ++i) {
+        hipStream_t s = (int)i == big ? st : side;
+        rc = (comp || MODE == MODE_DEGRID) ? launch_class<MODE, true>(plan, ranges[i], io, perm, s)
+                                           : launch_class<MODE, false>(plan, ranges[i], io, perm, s);
     }
     // joined on the error path too, so a capturing caller's stream is never
     // left with an unjoined fork
@@ -1160,7 +1550,8 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
 
     std::vector<int2> tab(p.n_kernels), sup(p.n_kernels);
     std::vector<int> pclass(p.n_kernels);
-    int class_h[kNumClasses] = {-1, -1, -1, -1, -1, -1};
+    int class_h[kAllClasses];
+    for (int c = 0; c < kAllClasses; ++c) class_h[c] = -1;
     int H = 0;
     long long table_len = 0;
     for (int k = 0; k < p.n_kernels; ++k) {
@@ -1206,8 +1597,32 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     plan->chunk_taps = p.chunk > 0 ? p.chunk : (1 << 17);
     plan->ntx = (p.grid_size + plan->tile - 1) / plan->tile;
     plan->nty = plan->ntx;
-    for (int c = 0; c < kNumClasses; ++c) {
-        plan->class_S[c] = class_h[c] >= 0 ? plan->tile + 2 * class_h[c] : 0;
+    // Exact-footprint classes (register gridder, std_gridding footprints
+    // only): planes with 1 <= h <= 16 whose 2h-wide footprint spans the tile
+    // (2h + 1 >= tile), each with an unfolded table of OS^2 (2h + 1)^2
+    // samples.  SDP_GRID_FAST=0 turns them off (every visibility then takes
+    // the LDS kernels).
+    std::vector<int> plane_utab(p.n_kernels, -1);
+    std::vector<int4> unfold_tab;
+    {
+        const char *e = getenv("SDP_GRID_FAST");
+        const bool on = !(e && atoi(e) == 0) && p.snap_mode == SDP_SNAP_STD;
+        long long len = 0;
+        for (int k = 0; on && k < p.n_kernels; ++k) {
+            const int h = supports[2 * k];
+            if (h < 1 || h > kFastMaxH || 2 * h + 1 < plan->tile) continue;
+            const long long nn = 2LL * h + 1;
+            if ((len + (long long)p.oversampling * p.oversampling * nn * nn) * 8 >= (1LL << 31)) break;
+            plane_utab[k] = (int)len;
+            unfold_tab.push_back(make_int4(supports[2 * k + 1], h, (int)len, 0));
+            len += (long long)p.oversampling * p.oversampling * nn * nn;
+            class_h[fast_class(h)] = h;
+        }
+        plan->utab_len = len;
+        plan->n_unfold = (int)unfold_tab.size();
+    }
+    for (int c = 0; c < kAllClasses; ++c) {
+        plan->class_S[c] = class_h[c] >= 0 ? plan->tile + 2 * class_h[c] - (is_fast(c) ? 1 : 0) : 0;
         plan->class_h[c] = std::max(class_h[c], 0);
         plan->smax = std::max(plan->smax, plan->class_S[c]);
         if (class_h[c] >= 0 &&
@@ -1219,8 +1634,13 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
                                   tile, class_h[c]);
         }
     }
+    if ((long long)plan->ntx * plan->nty * kAllClasses >= (1LL << 31)) {
+        const int tile = plan->tile;
+        delete plan;
+        return sdp::set_error(SDP_E_UNSUPPORTED, "tile %d too small for grid size %d", tile, p.grid_size);
+    }
     const int ntiles = plan->ntx * plan->nty;
-    const int nkeys = kNumClasses * ntiles;
+    const int nkeys = kAllClasses * ntiles;
     {
         // SDP_MERGE_REGION (16 or 32) overrides the region side for tuning
         const char *e = getenv("SDP_MERGE_REGION");
@@ -1247,16 +1667,22 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     PLAN_TRY(hipMalloc(&plan->ktab, sizeof(float2) * 2 * (size_t)std::max(table_len, 1LL)));
     PLAN_TRY(hipMalloc(&plan->plane_tab, sizeof(int2) * p.n_kernels));
     PLAN_TRY(hipMemcpyAsync(plan->plane_tab, tab.data(), sizeof(int2) * p.n_kernels, hipMemcpyHostToDevice, st));
+    if (plan->n_unfold > 0) {
+        PLAN_TRY(hipMalloc(&plan->unfold_tab, sizeof(int4) * plan->n_unfold));
+        PLAN_TRY(hipMemcpyAsync(plan->unfold_tab, unfold_tab.data(), sizeof(int4) * plan->n_unfold,
+                                hipMemcpyHostToDevice, st));
+        PLAN_TRY(hipMalloc(&plan->utab, sizeof(float2) * (size_t)plan->utab_len));
+    }
     int2 *d_supports = nullptr;
-    int *d_pclass = nullptr;
+    int *d_pclass = nullptr, *d_putab = nullptr;
     VisRec *recs_tmp = nullptr;
+    uint2 *frecs_tmp = nullptr;
     uint32_t *keys = nullptr, *keys_sorted = nullptr, *vals = nullptr;
-    unsigned int *key_count = nullptr, *key_first = nullptr;
-    unsigned long long *key_taps = nullptr;
-    // the set-up buffers above: freed once the per-key counts are read back,
-    // or by this guard when an error returns early
+    KeyRun *d_runs = nullptr;
+    // the set-up buffers above: freed once the key runs are read back, or by
+    // this guard when an error returns early
     struct SetupBuffers {
-        void **slot[9];
+        void **slot[10];
         void release()
         {
             for (void **q : slot)
@@ -1266,75 +1692,120 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
                 }
         }
         ~SetupBuffers() { release(); }
-    } setup{{(void **)&d_supports, (void **)&d_pclass, (void **)&recs_tmp, (void **)&keys, (void **)&keys_sorted,
-             (void **)&vals, (void **)&key_count, (void **)&key_first, (void **)&key_taps}};
+    } setup{{(void **)&d_supports, (void **)&d_pclass, (void **)&d_putab, (void **)&recs_tmp, (void **)&frecs_tmp,
+             (void **)&keys, (void **)&keys_sorted, (void **)&vals, (void **)&d_runs, (void **)&d_runs}};
     PLAN_TRY(hipMalloc(&d_supports, sizeof(int2) * p.n_kernels));
     PLAN_TRY(hipMemcpyAsync(d_supports, sup.data(), sizeof(int2) * p.n_kernels, hipMemcpyHostToDevice, st));
     PLAN_TRY(hipMalloc(&d_pclass, sizeof(int) * p.n_kernels));
     PLAN_TRY(hipMemcpyAsync(d_pclass, pclass.data(), sizeof(int) * p.n_kernels, hipMemcpyHostToDevice, st));
+    PLAN_TRY(hipMalloc(&d_putab, sizeof(int) * p.n_kernels));
+    PLAN_TRY(hipMemcpyAsync(d_putab, plane_utab.data(), sizeof(int) * p.n_kernels, hipMemcpyHostToDevice, st));
 
     const size_t n = (size_t)std::max(n_vis, 1);
     PLAN_TRY(hipMalloc(&recs_tmp, sizeof(VisRec) * n));
+    PLAN_TRY(hipMalloc(&frecs_tmp, sizeof(uint2) * n));
     PLAN_TRY(hipMalloc(&plan->recs, sizeof(VisRec) * n));
+    PLAN_TRY(hipMalloc(&plan->frecs, sizeof(uint2) * n));
     PLAN_TRY(hipMalloc(&plan->perm, sizeof(uint32_t) * n));
     PLAN_TRY(hipMalloc(&keys, sizeof(uint32_t) * n));
     PLAN_TRY(hipMalloc(&keys_sorted, sizeof(uint32_t) * n));
     PLAN_TRY(hipMalloc(&vals, sizeof(uint32_t) * n));
-    PLAN_TRY(hipMalloc(&key_count, sizeof(unsigned int) * nkeys));
-    PLAN_TRY(hipMalloc(&key_taps, sizeof(unsigned long long) * nkeys));
-    PLAN_TRY(hipMalloc(&key_first, sizeof(unsigned int) * nkeys));
-    PLAN_TRY(hipMemsetAsync(key_count, 0, sizeof(unsigned int) * nkeys, st));
-    PLAN_TRY(hipMemsetAsync(key_taps, 0, sizeof(unsigned long long) * nkeys, st));
 
+    std::vector<KeyRun> runs;
     if (n_vis > 0) {
         PlanArgs a{n_vis, p.grid_size, p.oversampling, p.n_kernels, p.snap_mode, plan->tile, plan->ntx, plan->nty,
                    ntiles, nkeys, 0, p.uv_scale, p.w_scale};
-        // plane order inside each (class, tile) list when the combined key,
-        // dropped records' nkeys << bits included, fits 32 bits
-        int pb = 0;
+        // The low field of the sort key: the plane (slow classes; neighbours
+        // then share a kernel table) and (t_u, t_v) (exact-footprint classes,
+        // whose lanes keep their tap while it stays the same), when the
+        // combined key, dropped records' nkeys << bits included, fits 32
+        // bits; (t_u, t_v) alone when only that fits.
+        int pb = 0, tb = 0;
         while ((1 << pb) < p.n_kernels) ++pb;
-        if (((unsigned long long)nkeys << pb) < (1ull << 32)) a.plane_bits = pb;
-        plan_records_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(d_uvw, d_supports, d_pclass, a, recs_tmp, keys,
-                                                                    vals);
+        if (plan->n_unfold > 0) {
+            while ((1 << tb) < plan->tile * plan->tile) ++tb;
+            ++tb;   // and the conjugation flag
+        }
+        auto fits = [&](int bits) { return ((unsigned long long)nkeys << bits) < (1ull << 32); };
+        if (fits(std::max(pb, tb))) a.plane_bits = std::max(pb, tb);
+        else if (tb > 0 && fits(tb)) a.plane_bits = tb;
+        plan_records_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(d_uvw, d_supports, d_pclass, d_putab, a,
+                                                                    recs_tmp, frecs_tmp, keys, vals);
         PLAN_TRY(hipGetLastError());
         int end_bit = 1;
         while (end_bit < 32 && (1ull << end_bit) <= ((unsigned long long)nkeys << a.plane_bits)) ++end_bit;
         size_t temp_bytes = 0;
-        PLAN_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys, keys_sorted, vals, plan->perm, n_vis,
-                                                    0, end_bit, st));
+        PLAN_TRY(rocprim::radix_sort_pairs(nullptr, temp_bytes, keys, keys_sorted, vals, plan->perm, (size_t)n_vis,
+                                           0, (unsigned)end_bit, st));
         void *temp = nullptr;
         PLAN_TRY(hipMalloc(&temp, temp_bytes));
-        hipError_t se = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_sorted, vals, plan->perm,
-                                                           n_vis, 0, end_bit, st);
+        hipError_t se = rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_sorted, vals, plan->perm,
+                                                  (size_t)n_vis, 0, (unsigned)end_bit, st);
         (void)hipStreamSynchronize(st);
         (void)hipFree(temp);
         PLAN_TRY(se);
         gather_records_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(recs_tmp, plan->perm, n_vis, plan->recs);
         PLAN_TRY(hipGetLastError());
+        gather_frecs_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(frecs_tmp, plan->perm, n_vis, plan->frecs);
+        PLAN_TRY(hipGetLastError());
+        if (plan->n_unfold > 0) {
+            // run lengths of the sorted exact-footprint records; the unsorted
+            // keys, the values and the unsorted records' buffer are free
+            uint32_t *fl = keys, *rid = vals, *rend = reinterpret_cast<uint32_t *>(recs_tmp);
+            fast_run_flags_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(keys_sorted, plan->frecs, n_vis,
+                                                                          a.plane_bits, fl);
+            PLAN_TRY(hipGetLastError());
+            size_t bb = 0;
+            PLAN_TRY(rocprim::inclusive_scan(nullptr, bb, fl, rid, (size_t)n_vis, rocprim::plus<uint32_t>(), st));
+            PLAN_TRY(hipMalloc(&temp, bb));
+            se = rocprim::inclusive_scan(temp, bb, fl, rid, (size_t)n_vis, rocprim::plus<uint32_t>(), st);
+            (void)hipStreamSynchronize(st);
+            (void)hipFree(temp);
+            PLAN_TRY(se);
+            fast_run_ends_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(fl, rid, n_vis, rend);
+            PLAN_TRY(hipGetLastError());
+            fast_run_rem_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(rid, rend, n_vis, plan->frecs);
+            PLAN_TRY(hipGetLastError());
+        }
 
         // recs_tmp (16 bytes a record) is free again: its two halves hold
-        // the tap counts and their inclusive scan
+        // the tap counts and their inclusive scan; the unsorted keys and
+        // values hold the run flags and run numbers
         unsigned long long *tap_count = reinterpret_cast<unsigned long long *>(recs_tmp);
         unsigned long long *tap_scan = tap_count + n_vis;
-        key_run_start_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(keys_sorted, plan->recs, n_vis, nkeys,
-                                                                     a.plane_bits, key_first, tap_count);
+        uint32_t *flags = keys, *run_no = vals;
+        run_flags_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(keys_sorted, plan->recs, n_vis, nkeys, a.plane_bits,
+                                                                 flags, tap_count);
         PLAN_TRY(hipGetLastError());
-        size_t scan_bytes = 0;
-        PLAN_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, scan_bytes, tap_count, tap_scan, n_vis, st));
-        PLAN_TRY(hipMalloc(&temp, scan_bytes));
-        se = hipcub::DeviceScan::InclusiveSum(temp, scan_bytes, tap_count, tap_scan, n_vis, st);
+        size_t b1 = 0, b2 = 0;
+        PLAN_TRY(rocprim::inclusive_scan(nullptr, b1, tap_count, tap_scan, (size_t)n_vis,
+                                         rocprim::plus<unsigned long long>(), st));
+        PLAN_TRY(rocprim::exclusive_scan(nullptr, b2, flags, run_no, 0u, (size_t)n_vis, rocprim::plus<uint32_t>(),
+                                         st));
+        PLAN_TRY(hipMalloc(&temp, std::max(b1, b2)));
+        se = rocprim::inclusive_scan(temp, b1, tap_count, tap_scan, (size_t)n_vis,
+                                     rocprim::plus<unsigned long long>(), st);
+        if (se == hipSuccess)
+            se = rocprim::exclusive_scan(temp, b2, flags, run_no, 0u, (size_t)n_vis, rocprim::plus<uint32_t>(), st);
+        uint32_t last[2] = {0, 0};
+        if (se == hipSuccess) se = hipMemcpyAsync(&last[0], run_no + n_vis - 1, 4, hipMemcpyDeviceToHost, st);
+        if (se == hipSuccess) se = hipMemcpyAsync(&last[1], flags + n_vis - 1, 4, hipMemcpyDeviceToHost, st);
         (void)hipStreamSynchronize(st);
         (void)hipFree(temp);
         PLAN_TRY(se);
-        key_run_end_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(keys_sorted, n_vis, nkeys, a.plane_bits,
-                                                                   key_first, tap_scan, key_count, key_taps);
-        PLAN_TRY(hipGetLastError());
+        const size_t n_runs = (size_t)last[0] + last[1];
+        if (n_runs > 0) {
+            PLAN_TRY(hipMalloc(&d_runs, sizeof(KeyRun) * n_runs));
+            run_start_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(keys_sorted, flags, run_no, n_vis,
+                                                                     a.plane_bits, d_runs);
+            PLAN_TRY(hipGetLastError());
+            run_end_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(keys_sorted, flags, run_no, n_vis, nkeys,
+                                                                   a.plane_bits, tap_scan, d_runs);
+            PLAN_TRY(hipGetLastError());
+            runs.resize(n_runs);
+            PLAN_TRY(hipMemcpyAsync(runs.data(), d_runs, sizeof(KeyRun) * n_runs, hipMemcpyDeviceToHost, st));
+        }
     }
-
-    std::vector<unsigned int> counts(nkeys);
-    std::vector<unsigned long long> taps(nkeys);
-    PLAN_TRY(hipMemcpyAsync(counts.data(), key_count, sizeof(unsigned int) * nkeys, hipMemcpyDeviceToHost, st));
-    PLAN_TRY(hipMemcpyAsync(taps.data(), key_taps, sizeof(unsigned long long) * nkeys, hipMemcpyDeviceToHost, st));
     PLAN_TRY(hipStreamSynchronize(st));
     setup.release();
 
@@ -1342,22 +1813,19 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     // into pieces of about chunk_taps taps
     std::vector<int4> chunks;
     std::vector<long long> slab_off;
-    std::vector<std::vector<int>> tile_keys(ntiles);    // per tile: key indices, class order
-    std::vector<int> key_S;
+    std::vector<int> key_S, key_tile;
     std::vector<long long> key_first_chunk, key_pieces;
-    std::vector<char> tile_used(ntiles, 0);
     long long cells = 0;
-    int first = 0;
-    for (int c = 0; c < kNumClasses; ++c) {
+    int kept = 0;
+    size_t ri = 0;
+    for (int c = 0; c < kAllClasses; ++c) {
         plan->class_chunk_begin[c] = (int)chunks.size();
-        for (int t = 0; t < ntiles; ++t) {
-            const int key = c * ntiles + t;
-            const int cnt = (int)counts[key];
-            if (cnt == 0) continue;
-            tile_used[t] = 1;
-            long long pieces = (long long)((taps[key] + plan->chunk_taps - 1) / plan->chunk_taps);
+        for (; ri < runs.size() && (int)(runs[ri].key / (uint32_t)ntiles) == c; ++ri) {
+            const KeyRun &r = runs[ri];
+            const int t = (int)(r.key % (uint32_t)ntiles), cnt = (int)r.count, first = (int)r.first;
+            long long pieces = (long long)((r.taps + plan->chunk_taps - 1) / plan->chunk_taps);
             pieces = std::max(1LL, std::min(pieces, (long long)cnt));
-            tile_keys[t].push_back((int)key_S.size());
+            key_tile.push_back(t);
             key_S.push_back(plan->class_S[c]);
             key_first_chunk.push_back((long long)chunks.size());
             key_pieces.push_back(pieces);
@@ -1367,16 +1835,26 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
                 slab_off.push_back(cells);
                 cells += (long long)plan->class_S[c] * plan->class_S[c];
             }
-            first += cnt;
+            kept += cnt;
+            plan->total_taps += (long long)r.taps;
+            if (is_fast(c)) plan->n_fast += cnt;
         }
     }
-    plan->class_chunk_begin[kNumClasses] = (int)chunks.size();
-    for (int c = 0; c < kNumClasses; ++c)
+    plan->class_chunk_begin[kAllClasses] = (int)chunks.size();
+    for (int c = 0; c < kAllClasses; ++c)
         plan->n_class_launches += plan->class_chunk_begin[c + 1] > plan->class_chunk_begin[c];
-    for (int k = 0; k < nkeys; ++k) plan->total_taps += (long long)taps[k];
-    plan->n_kept = first;
+    plan->n_kept = kept;
     plan->n_chunks = (int)chunks.size();
-    for (int t = 0; t < ntiles; ++t) plan->n_tiles_used += tile_used[t];
+    // register-gridder items: each exact-footprint work item in parts of 64
+    // footprint positions
+    std::vector<int2> items;
+    for (int c = kNumClasses; c < kAllClasses; ++c) {
+        const int W = 2 * plan->class_h[c], parts = (W * W + kWave - 1) / kWave;
+        for (int k = plan->class_chunk_begin[c]; k < plan->class_chunk_begin[c + 1]; ++k)
+            for (int w = 0; w < parts; ++w) items.push_back(make_int2(k, w));
+    }
+    while (items.size() % kFastWaves) items.push_back(make_int2(-1, 0));   // whole workgroups
+    plan->n_items = (int)items.size();
 
     // key slabs: single-piece keys use their chunk's slab; multi-piece keys
     // get a reduced slab placed after the chunk slabs.  A key of more than
@@ -1417,13 +1895,17 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
             extra += S2;
         }
     }
-    std::vector<int> tile_key_begin(ntiles + 1, 0), tile_key_list;
-    tile_key_list.reserve(nk);
+    // each tile's keys in class order (runs are class-major), tiles in order
+    std::vector<int> tile_key_begin(ntiles + 1, 0), tile_key_list(nk);
+    for (int k = 0; k < nk; ++k) ++tile_key_begin[key_tile[k] + 1];
     for (int t = 0; t < ntiles; ++t) {
-        tile_key_begin[t] = (int)tile_key_list.size();
-        tile_key_list.insert(tile_key_list.end(), tile_keys[t].begin(), tile_keys[t].end());
+        plan->n_tiles_used += tile_key_begin[t + 1] > 0;
+        tile_key_begin[t + 1] += tile_key_begin[t];
     }
-    tile_key_begin[ntiles] = (int)tile_key_list.size();
+    {
+        std::vector<int> fill(tile_key_begin.begin(), tile_key_begin.end() - 1);
+        for (int k = 0; k < nk; ++k) tile_key_list[fill[key_tile[k]]++] = k;
+    }
     // key tables in tile order so that a tile's keys are contiguous
     std::vector<int> kS_t(nk);
     std::vector<long long> kOff_t(nk);
@@ -1467,6 +1949,11 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
             PLAN_TRY(hipMalloc(&plan->part_off, sizeof(long long) * part_off.size()));
             PLAN_TRY(hipMemcpyAsync(plan->part_off, part_off.data(), sizeof(long long) * part_off.size(),
                                     hipMemcpyHostToDevice, st));
+        }
+        if (plan->n_items > 0) {
+            PLAN_TRY(hipMalloc(&plan->items, sizeof(int2) * plan->n_items));
+            PLAN_TRY(hipMemcpyAsync(plan->items, items.data(), sizeof(int2) * plan->n_items, hipMemcpyHostToDevice,
+                                    st));
         }
         plan->slab_bytes = sizeof(float2) * (size_t)(cells + extra);
         PLAN_TRY(hipMalloc(&plan->slabs, plan->slab_bytes));
@@ -1529,7 +2016,9 @@ int sdp_grid_accumulate(const sdp_grid_plan *plan, const float *d_kernels, const
     SDP_REQUIRE(d_kernels && d_vis, "sdp_grid_accumulate: null argument");
     hipStream_t st = sdp::stream_of(s);
     ChunkIO io{};
-    io.kg = relayout(plan, d_kernels, 0, st);
+    // the LDS kernels read the phase-major table, the register gridder the unfolded one
+    if (plan->class_chunk_begin[kNumClasses] > 0) io.kg = relayout(plan, d_kernels, 0, st);
+    if (plan->n_items > 0) unfold(plan, d_kernels, st);
     SDP_LAUNCH_CHECK();
     io.vis = (const float2 *)d_vis;
     io.slabs = plan->slabs;
