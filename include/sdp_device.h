@@ -295,6 +295,17 @@ enum { SDP_DFT_ARRAY_BF16 = 0, SDP_DFT_ARRAY_FP16 = 1, SDP_DFT_ARRAY_NO_SPLIT = 
 size_t sdp_dft_array_work_bytes(int n_sources);
 int sdp_dft_predict_array(int n_time, int n_ant, const double *d_ant_uvw, int n_sources, const float *d_sources,
                           float *d_vis_out, int flags, void *d_work, sdp_stream s);
+/* No counterpart in the reference: sdp_dft_predict_array for callers that
+ * cannot allocate (code under graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Its work buffer is a fixed 2.0 MiB device array
+ * of the library, one per GPU, held until the process ends; the call itself
+ * allocates nothing.  At most SDP_DFT_NOALLOC_MAX_SOURCES sources (larger
+ * counts return SDP_E_ARG).  All calls on one GPU share that array: calls
+ * that overlap in time on one GPU (different streams or threads) race, so
+ * serialise them.  Results are those of sdp_dft_predict_array bit for bit. */
+#define SDP_DFT_NOALLOC_MAX_SOURCES 65536
+int sdp_dft_predict_array_noalloc(int n_time, int n_ant, const double *d_ant_uvw, int n_sources,
+                                  const float *d_sources, float *d_vis_out, int flags, sdp_stream s);
 
 /* ---- learned minor-cycle proposal ---------------------------------------
  * No reference counterpart (the reference has no learned component).  An

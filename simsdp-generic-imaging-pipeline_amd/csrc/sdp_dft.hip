@@ -30,6 +30,7 @@
 // them in LDS; each of the 8 waves then runs 4 x 2 x 2 x 3 MFMAs for its
 // 64 x 32 block.  scale is a power of two that keeps FP16 operands in range.
 #include <cstdlib>
+#include <mutex>
 
 #include "sdp_internal.h"
 
@@ -382,6 +383,36 @@ dft_array_kernel(int n_ant, int n_chunks, const double *__restrict__ ant_uvw, co
 
 size_t array_terms_offset() { return 256; }
 
+// The allocation-free entry point's work buffer: the scale and the source
+// terms of up to SDP_DFT_NOALLOC_MAX_SOURCES sources (plus the zero chunk),
+// 2.0 MiB held by each GPU the library runs on until the process ends.  Host
+// code never uses this name as an address: each device's copy is reached
+// through the pointer hipGetSymbolAddress returns with that device current.
+constexpr size_t kScratchBytes = 256 + (size_t)(SDP_DFT_NOALLOC_MAX_SOURCES + kChunk) * sizeof(SrcTerm);
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's scratch address, looked up on its first use
+int scratch_address(void **out)
+{
+    int dev = 0;
+    SDP_HIP_TRY(hipGetDevice(&dev));
+    SDP_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the scratch table", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return sdp::set_error(SDP_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SDP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -423,6 +454,19 @@ int sdp_dft_predict_array(int n_time, int n_ant, const double *d_ant_uvw, int n_
     kern<<<grid, kArrayThreads, 0, st>>>(n_ant, n_pad / kChunk, d_ant_uvw, terms, scale, (float2 *)d_vis_out, split);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
+}
+
+int sdp_dft_predict_array_noalloc(int n_time, int n_ant, const double *d_ant_uvw, int n_sources,
+                                  const float *d_sources, float *d_vis_out, int flags, sdp_stream s)
+{
+    SDP_REQUIRE(n_sources >= 0 && n_sources <= SDP_DFT_NOALLOC_MAX_SOURCES,
+                "sdp_dft_predict_array_noalloc: %d sources, at most %d", n_sources, SDP_DFT_NOALLOC_MAX_SOURCES);
+    static_assert(256 + (size_t)(SDP_DFT_NOALLOC_MAX_SOURCES + kChunk) * sizeof(SrcTerm) == kScratchBytes,
+                  "scratch holds the work buffer of the largest call");
+    void *work = nullptr;
+    if (int e = scratch_address(&work)) return e;
+    // every byte the kernels read is written by this call's prep kernel
+    return sdp_dft_predict_array(n_time, n_ant, d_ant_uvw, n_sources, d_sources, d_vis_out, flags, work, s);
 }
 
 int sdp_dft_predict(int n_vis, int n_sources, const float *d_sources, const float *d_uvw, float *d_vis_out, int mode,

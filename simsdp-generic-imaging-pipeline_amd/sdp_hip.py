@@ -211,6 +211,7 @@ def load() -> ctypes.CDLL:
         "sdp_cnn_describe": (c_int, [P, P]),
         "sdp_dft_array_work_bytes": (c_size_t, [c_int]),
         "sdp_dft_predict_array": (c_int, [c_int, c_int, P, c_int, P, P, c_int, P, P]),
+        "sdp_dft_predict_array_noalloc": (c_int, [c_int, c_int, P, c_int, P, P, c_int, P]),
         "sdp_cnn_patch_peaks": (c_int, [c_int, c_int, c_int, P, P, ctypes.POINTER(c_int), P]),
         "sdp_cnn_gather_patches": (c_int, [c_int, c_int, c_int, P, P, P, P, P, P]),
         "sdp_cnn_scatter_add": (c_int, [c_int, c_int, c_int, c_int, P, P, P, c_float, c_float, P, c_float, P, P]),
