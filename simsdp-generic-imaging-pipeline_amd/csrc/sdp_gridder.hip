@@ -997,6 +997,174 @@ fast_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restrict__ p
     }
 }
 
+// ---- fused degrid + subtract + grid of the exact-footprint classes --------
+
+// flag = 0 when some degridding sample is not the conjugate of its gridding
+// sample (the caller set it to 1 first)
+__global__ void conj_check_kernel(const float2 *__restrict__ kg, const float2 *__restrict__ kd, long long n,
+                                  int *flag)
+{
+    bool ok = true;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        ok = ok && kd[i].x == kg[i].x && kd[i].y == -kg[i].y;
+    if (!ok) atomicAnd(flag, 0);
+}
+
+__global__ void set_flag_kernel(int *flag, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v;
+}
+
+// waves per workgroup and LDS cells per wave of the fused kernel: a wave
+// covers all W^2 footprint positions of its work item in P <= PMAX parts of
+// 64, four cells per part and lane
+template <int PMAX> constexpr int fused_waves() { return PMAX <= 4 ? 4 : PMAX <= 8 ? 2 : 1; }
+template <bool COMP, int PMAX> constexpr size_t fused_lds()
+{
+    return (size_t)fused_waves<PMAX>() * PMAX * 4 * kWave * (COMP ? sizeof(float4) : sizeof(float2));
+}
+
+struct FusedArgs {
+    int chunk_b, chunk_e, tile, G, ntx, tab_bytes;
+};
+
+// One wavefront per work item: for each of its visibilities (in plan
+// order) the prediction sum over the footprint -- each lane's taps read the
+// model grid at the lane's own cells, held in registers for the run -- is
+// reduced over the wavefront, subtracted from the measured value, and the
+// residual gridded with the same table samples into the lane's running sums
+// (as fast_grid_kernel).  When the degridding table is the conjugate of the
+// gridding table (*conj_flag) each tap reads one sample for both.
+template <bool COMP, int PMAX>
+__global__ void __launch_bounds__(kWave * fused_waves<PMAX>())
+fast_degrid_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restrict__ perm,
+                        const int4 *__restrict__ chunks, const long long *__restrict__ slab_off,
+                        const float2 *__restrict__ measured, const float2 *__restrict__ model,
+                        float2 *__restrict__ vis_out, const float2 *__restrict__ utab_g,
+                        const float2 *__restrict__ utab_d, const int *__restrict__ conj_flag, FusedArgs fa,
+                        float2 *__restrict__ slabs)
+{
+    using Acc = typename AccT<COMP>::T;
+    extern __shared__ float4 lds4[];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int ci = fa.chunk_b + blockIdx.x * fused_waves<PMAX>() + wave;
+    if (ci >= fa.chunk_e) return;
+    Acc *cells = reinterpret_cast<Acc *>(lds4) + (size_t)wave * PMAX * 4 * kWave;
+    const int4 ch = chunks[ci];
+    const int S = ch.w, tile = fa.tile, W = S - tile + 1, n = W + 1, P = (W * W + kWave - 1) / kWave;
+    const int tile_u0 = (ch.x % fa.ntx) * tile, tile_v0 = (ch.x / fa.ntx) * tile;
+    const int lane = threadIdx.x % kWave;
+    for (int q = 0; q < PMAX * 4; ++q) acc_zero(cells[q * kWave + lane]);
+    const __amdgpu_buffer_rsrc_t rg =
+        __builtin_amdgcn_make_buffer_rsrc((void *)utab_g, (short)0, fa.tab_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rd =
+        __builtin_amdgcn_make_buffer_rsrc((void *)utab_d, (short)0, fa.tab_bytes, 0x00020000);
+    const bool own_d = *conj_flag == 0;   // a degridding table of its own
+    const uint2 *fr = frecs + ch.y;
+    Acc run[PMAX];
+    unsigned voff[PMAX];
+    int cell[PMAX];
+    float2 g[PMAX];
+    int pa[PMAX], pb[PMAX];   // the lane's footprint position (a, b) in each part; a = -1 when unused
+#pragma unroll
+    for (int k = 0; k < PMAX; ++k) {
+        acc_zero(run[k]);
+        voff[k] = (unsigned)kOob;
+        cell[k] = 0;
+        g[k] = make_float2(0.f, 0.f);
+        const int p = k * kWave + lane;
+        const bool act = k < P && p < W * W;
+        pa[k] = act ? p % W : -1;
+        pb[k] = act ? p / W : 0;
+    }
+    bool started = false;
+    int j = 0;
+    while (j < ch.z) {
+        const uint2 r0 = fr[j];
+        const int len = max(1, min((int)(r0.y >> 11), ch.z - j));
+        const int tu = (int)(r0.y & 0x1fu), tv = (int)((r0.y >> 5) & 0x1fu);
+        const bool conj = (r0.y & 0x400u) != 0u;
+#pragma unroll
+        for (int k = 0; k < PMAX; ++k) {
+            if (started) acc_merge(cells[(k * 4 + cell[k]) * kWave + lane], run[k]);
+            const bool act = pa[k] >= 0;
+            const int a = act ? pa[k] : 0, b = pb[k];
+            const int du = a >= tu ? a - tu : a - tu + W, dv = b >= tv ? b - tv : b - tv + W;
+            voff[k] = act ? (unsigned)(8 * (dv * n + du)) : (unsigned)kOob;
+            cell[k] = (a < tu ? 1 : 0) + (b < tv ? 2 : 0);
+            // the model value at the lane's cell for this run
+            const int gu = tile_u0 + a + (a < tu ? W : 0), gv = tile_v0 + b + (b < tv ? W : 0);
+            g[k] = (act && gu < fa.G && gv < fa.G) ? model[(size_t)gv * fa.G + gu] : make_float2(0.f, 0.f);
+            acc_zero(run[k]);
+        }
+        started = true;
+        const float cs = conj ? -1.f : 1.f;
+        // U visibilities at a time: their loads issued together and their
+        // wavefront reductions interleaved
+        auto step = [&](auto ucount, int i0) {
+            constexpr int U = decltype(ucount)::value;
+            float2 kk[U][PMAX], kd[U][PMAX];
+            uint32_t o[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t base = fr[j + i0 + u].x;
+                o[u] = perm ? perm[ch.y + j + i0 + u] : (uint32_t)(ch.y + j + i0 + u);
+#pragma unroll
+                for (int k = 0; k < PMAX; ++k) kk[u][k] = utab_load(rg, voff[k], base);
+                if (own_d) {
+#pragma unroll
+                    for (int k = 0; k < PMAX; ++k) kd[u][k] = utab_load(rd, voff[k], base);
+                }
+            }
+            float sr[U], si[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                // model * (kd.x, conj kd.y) over the lane's taps
+                float sxx = 0.f, syy = 0.f, syx = 0.f, sxy = 0.f;
+#pragma unroll
+                for (int k = 0; k < PMAX; ++k) {
+                    const float2 q = own_d ? kd[u][k] : make_float2(kk[u][k].x, -kk[u][k].y);
+                    sxx = fmaf(g[k].x, q.x, sxx);
+                    syy = fmaf(g[k].y, q.y, syy);
+                    syx = fmaf(g[k].y, q.x, syx);
+                    sxy = fmaf(g[k].x, q.y, sxy);
+                }
+                sr[u] = fmaf(-cs, syy, sxx);
+                si[u] = fmaf(cs, sxy, syx);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) group_sum(sr[u], si[u], 6);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float2 m = measured[o[u]];
+                const float vr = m.x - sr[u], vi = m.y - si[u];
+                if (vis_out && lane == 0) vis_out[o[u]] = make_float2(vr, vi);
+                const float cvr = cs * vr, cvi = cs * vi;
+#pragma unroll
+                for (int k = 0; k < PMAX; ++k) tap_add(run[k], kk[u][k], vr, vi, cvr, cvi);
+            }
+        };
+        constexpr int UU = PMAX <= 4 ? 4 : PMAX <= 8 ? 2 : 1;
+        int i = 0;
+        for (; i + UU <= len; i += UU) step(std::integral_constant<int, UU>{}, i);
+        for (; i < len; ++i) step(std::integral_constant<int, 1>{}, i);
+        j += len;
+    }
+    float2 *slab = slabs + slab_off[ci];
+#pragma unroll
+    for (int k = 0; k < PMAX; ++k) {
+        if (started) acc_merge(cells[(k * 4 + cell[k]) * kWave + lane], run[k]);
+        if (pa[k] >= 0) {
+            const int a = pa[k], b = pb[k];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int u = a + (q & 1) * W, v = b + (q >> 1) * W;
+                if (u < S && v < S) slab[v * S + u] = acc_value(cells[(k * 4 + q) * kWave + lane]);
+            }
+        }
+    }
+}
+
 constexpr int kMergeThreads = 256;
 // A key's pieces summed by one workgroup of the first reduce pass: a key with
 // more (a dense tile can have a thousand) is split into groups whose partial
@@ -1218,6 +1386,8 @@ struct sdp_grid_plan {
     int4 *unfold_tab = nullptr;    // per such plane: quadrant offset, h, unfolded offset
     long long utab_len = 0;        // samples of the unfolded gridding table
     float2 *utab = nullptr;
+    float2 *utab_d = nullptr;      // the unfolded degridding table (fused degrid + grid)
+    int *conj_flag = nullptr;      // 1 when the degridding table is the conjugate of the gridding table
     int n_items = 0;               // register-gridder items: (chunk, 64-position part)
     int2 *items = nullptr;
     long long n_fast = 0;          // visibilities in exact-footprint classes
@@ -1255,6 +1425,8 @@ void plan_free(sdp_grid_plan *p)
     (void)hipFree(p->frecs);
     (void)hipFree(p->unfold_tab);
     (void)hipFree(p->utab);
+    (void)hipFree(p->utab_d);
+    (void)hipFree(p->conj_flag);
     (void)hipFree(p->items);
     (void)hipFree(p->perm);
     (void)hipFree(p->plane_tab);
@@ -1438,13 +1610,62 @@ int launch_fast(const sdp_grid_plan *plan, const float2 *vis, const uint32_t *pe
     return SDP_OK;
 }
 
-// the unfolded copy of the gridding table
-void unfold(const sdp_grid_plan *plan, const float *d_kernels, hipStream_t st)
+// an unfolded copy of a kernel table (the gridding table into utab, the
+// degridding table into utab_d)
+void unfold(const sdp_grid_plan *plan, const float *d_kernels, float2 *out, hipStream_t st)
 {
     const int OS = plan->p.oversampling, n = 2 * kFastMaxH + 1;
     const int bx = (int)std::min<long long>(128, ((long long)OS * OS * n * n + 255) / 256);
-    unfold_kernel<<<dim3(bx, plan->n_unfold), 256, 0, st>>>((const float2 *)d_kernels, plan->unfold_tab, OS,
-                                                            plan->utab);
+    unfold_kernel<<<dim3(bx, plan->n_unfold), 256, 0, st>>>((const float2 *)d_kernels, plan->unfold_tab, OS, out);
+}
+
+template <bool COMP, int PMAX>
+int launch_fused_group(const sdp_grid_plan *plan, int b, int e, const ChunkIO &io, const uint32_t *perm,
+                       hipStream_t st)
+{
+    if (e <= b) return SDP_OK;
+    constexpr int waves = fused_waves<PMAX>();
+    constexpr size_t lds = fused_lds<COMP, PMAX>();
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void *)fast_degrid_grid_kernel<COMP, PMAX>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        attr_set = true;
+    }
+    FusedArgs fa{b, e, plan->tile, plan->p.grid_size, plan->ntx, (int)(8 * plan->utab_len)};
+    fast_degrid_grid_kernel<COMP, PMAX><<<(e - b + waves - 1) / waves, kWave * waves, lds, st>>>(
+        plan->frecs, perm, plan->chunks, plan->slab_off, io.measured, io.grid, io.vis_out, plan->utab, plan->utab_d,
+        plan->conj_flag, fa, plan->slabs);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+// the fused kernel's launches: exact-footprint classes grouped by the parts
+// of 64 footprint positions their work items need, P = ceil((2h)^2 / 64):
+// h <= 4 one part, 5 two, 6 three, 7-8 four, 9 six, 10-11 eight, 12-16 up
+// to sixteen (a group's kernel runs all PMAX parts; unused ones read zeros)
+constexpr int kFusedGroups = 7;
+constexpr int kFusedLo[kFusedGroups] = {1, 5, 6, 7, 9, 10, 12}, kFusedHi[kFusedGroups] = {4, 5, 6, 8, 9, 11, 16};
+template <bool COMP>
+int launch_fused(const sdp_grid_plan *plan, int group, const ChunkIO &io, const uint32_t *perm, hipStream_t st)
+{
+    const int b = plan->class_chunk_begin[fast_class(kFusedLo[group])];
+    const int e = plan->class_chunk_begin[fast_class(kFusedHi[group]) + 1];
+    switch (group) {
+    case 0: return launch_fused_group<COMP, 1>(plan, b, e, io, perm, st);
+    case 1: return launch_fused_group<COMP, 2>(plan, b, e, io, perm, st);
+    case 2: return launch_fused_group<COMP, 3>(plan, b, e, io, perm, st);
+    case 3: return launch_fused_group<COMP, 4>(plan, b, e, io, perm, st);
+    case 4: return launch_fused_group<COMP, 6>(plan, b, e, io, perm, st);
+    case 5: return launch_fused_group<COMP, 8>(plan, b, e, io, perm, st);
+    default: return launch_fused_group<COMP, 16>(plan, b, e, io, perm, st);
+    }
+}
+
+int fused_group_chunks(const sdp_grid_plan *plan, int group)
+{
+    return plan->class_chunk_begin[fast_class(kFusedHi[group]) + 1] -
+           plan->class_chunk_begin[fast_class(kFusedLo[group])];
 }
 
 // phase-major copy of a kernel table into the plan (slot 0 gridding, 1 degridding)
@@ -1489,18 +1710,25 @@ template <int MODE>
 int launch_chunks(const sdp_grid_plan *plan, const ChunkIO &io, const uint32_t *perm, hipStream_t st)
 {
     const bool comp = plan->p.accumulate == SDP_ACC_COMPENSATED;
-    // gridding takes the exact-footprint classes in one register-gridder launch
+    // gridding takes the exact-footprint classes in one register-gridder
+    // launch, degrid + grid in up to four fused launches
     const bool fast = MODE == MODE_GRID && plan->n_items > 0;
-    const std::vector<LaunchRange> ranges = lds_ranges(plan, MODE != MODE_GRID);
+    int fused[kFusedGroups] = {};
+    int n_fused = 0;
+    if (MODE == MODE_DEGRID_GRID && plan->n_items > 0)
+        for (int q = 0; q < kFusedGroups; ++q) n_fused += (fused[q] = fused_group_chunks(plan, q)) > 0;
+    const std::vector<LaunchRange> ranges = lds_ranges(plan, MODE == MODE_DEGRID || (MODE == MODE_DEGRID_GRID &&
+                                                                                     plan->n_items == 0));
     // The launch with the most work runs on the caller's stream (the
     // register gridder when it runs); the others, too small to fill the GPU
     // on their own, run beside it on a helper stream forked from and joined
     // back to it (slabs of different launches are disjoint).
     int big = -1;
-    if (!fast)
+    if (n_fused == 0 && !fast) {
         for (size_t i = 0; i < ranges.size(); ++i)
             if (big < 0 || ranges[i].e - ranges[i].b > ranges[big].e - ranges[big].b) big = (int)i;
-    const size_t n_launch = ranges.size() + (fast ? 1 : 0);
+    }
+    const size_t n_launch = ranges.size() + (fast ? 1 : 0) + n_fused;
     hipStream_t side = st;
     hipEvent_t join = nullptr;
     if (n_launch > 1) {
@@ -1515,6 +1743,10 @@ int launch_chunks(const sdp_grid_plan *plan, const ChunkIO &io, const uint32_t *
     }
     int rc = SDP_OK;
     if (fast) rc = launch_fast(plan, io.vis, perm, st);
+    // the fused launches each fill the GPU: one after another on the caller's stream
+    for (int q = 0; q < kFusedGroups && rc == SDP_OK; ++q)
+        if (fused[q] > 0)
+            rc = comp ? launch_fused<true>(plan, q, io, perm, st) : launch_fused<false>(plan, q, io, perm, st);
     for (size_t i = 0; i < ranges.size() && rc == SDP_OK; ++i) {
         hipStream_t s = (int)i == big ? st : side;
         rc = (comp || MODE == MODE_DEGRID) ? launch_class<MODE, true>(plan, ranges[i], io, perm, s)
@@ -1672,6 +1904,8 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
         PLAN_TRY(hipMemcpyAsync(plan->unfold_tab, unfold_tab.data(), sizeof(int4) * plan->n_unfold,
                                 hipMemcpyHostToDevice, st));
         PLAN_TRY(hipMalloc(&plan->utab, sizeof(float2) * (size_t)plan->utab_len));
+        PLAN_TRY(hipMalloc(&plan->utab_d, sizeof(float2) * (size_t)plan->utab_len));
+        PLAN_TRY(hipMalloc(&plan->conj_flag, sizeof(int)));
     }
     int2 *d_supports = nullptr;
     int *d_pclass = nullptr, *d_putab = nullptr;
@@ -2018,7 +2252,7 @@ int sdp_grid_accumulate(const sdp_grid_plan *plan, const float *d_kernels, const
     ChunkIO io{};
     // the LDS kernels read the phase-major table, the register gridder the unfolded one
     if (plan->class_chunk_begin[kNumClasses] > 0) io.kg = relayout(plan, d_kernels, 0, st);
-    if (plan->n_items > 0) unfold(plan, d_kernels, st);
+    if (plan->n_items > 0) unfold(plan, d_kernels, plan->utab, st);
     SDP_LAUNCH_CHECK();
     io.vis = (const float2 *)d_vis;
     io.slabs = plan->slabs;
@@ -2112,8 +2346,17 @@ int sdp_degrid_grid_accumulate(const sdp_grid_plan *plan, const float *d_degrid_
     if (plan->n_chunks > 0) {
         SDP_REQUIRE(d_degrid_kernels && d_grid_kernels, "sdp_degrid_grid: null kernels");
         ChunkIO io{};
-        io.kd = relayout(plan, d_degrid_kernels, 1, st);
-        io.kg = relayout(plan, d_grid_kernels, 0, st);
+        if (plan->class_chunk_begin[kNumClasses] > 0) {   // the LDS kernels' phase-major tables
+            io.kd = relayout(plan, d_degrid_kernels, 1, st);
+            io.kg = relayout(plan, d_grid_kernels, 0, st);
+        }
+        if (plan->n_items > 0) {
+            // the fused kernel's unfolded tables, and whether one serves both
+            unfold(plan, d_grid_kernels, plan->utab, st);
+            unfold(plan, d_degrid_kernels, plan->utab_d, st);
+            set_flag_kernel<<<1, 64, 0, st>>>(plan->conj_flag, 1);
+            conj_check_kernel<<<256, 256, 0, st>>>(plan->utab, plan->utab_d, plan->utab_len, plan->conj_flag);
+        }
         SDP_LAUNCH_CHECK();
         io.grid = (const float2 *)d_model_grid;
         io.measured = (const float2 *)d_vis_measured;

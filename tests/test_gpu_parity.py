@@ -455,18 +455,26 @@ def test_degrid_fused_subtract(hip, case, rng):
     np.testing.assert_array_equal(delta.download((d.n_vis, 2), np.float32), d.vis - p)
 
 
+@pytest.mark.parametrize("tables", ["conjugate", "own"])
 @pytest.mark.parametrize("order", ["original", "plan"])
-def test_degrid_grid_fused_equals_separate(hip, case, rng, order):
+def test_degrid_grid_fused_equals_separate(hip, case, rng, order, tables):
     """The fused major-cycle pass gives the residual visibilities and residual
     grid of degrid(+subtract) followed by grid.  The fused kernel may split a
     footprint's rows over more lanes than the separate kernels do, so sums
-    are grouped differently: equal to float rounding (1e-6 of the peak)."""
+    are grouped differently: equal to float rounding (1e-6 of the peak).
+    With the degridding table the conjugate of the gridding table each tap
+    reads one sample for both; with a table of its own ("own": the
+    conjugate table perturbed in one sample of every plane) it reads both."""
     d = case
     G = d.grid_size
     o = hip.ORDER_ORIGINAL if order == "original" else hip.ORDER_PLAN
     uvw = hip.DeviceBuffer.from_array(d.uvw)
     gk = hip.DeviceBuffer.from_array(d.kernels)
-    dk = hip.DeviceBuffer.from_array(d.degrid_kernels)
+    dtab = d.degrid_kernels.copy()
+    if tables == "own":
+        dtab[d.supports[:, 1] + 5] *= np.float32(0.75)
+        dtab *= np.float32(1.25)
+    dk = hip.DeviceBuffer.from_array(dtab)
     model = hip.DeviceBuffer.from_array(rng.normal(size=(G * G, 2)).astype(np.float32))
     meas = hip.DeviceBuffer.from_array(d.vis)
     plan = hip.GridPlan(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, uvw, d.n_vis)
