@@ -23,6 +23,7 @@ sys.path.insert(0, os.path.join(REPO, "simsdp-generic-imaging-pipeline_amd"))
 
 METRIC = "Mvis/s grid+degrid; major-cycle s to fixed residual RMS, learned vs Hogbom"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TFLOPS = 78.6   # FP32 FMA without packed math: 256 CUs x 64 lanes x 2 flops x 2.4 GHz
 
 
 def log(*a):
@@ -272,10 +273,13 @@ def dft_precision_variant(hip, d, n_src=4096):
         got = d_out.download((n_pairs, 2), np.float32)
         products = 1 if flags & hip.DFT_ARRAY_NO_SPLIT else 3
         mfma_flops = n_time * nt * (nt + 1) // 2 * 128 * 128 * ((n_src + 15) // 16 * 32) * 2 * 2 * products
+        alg_flops = 8.0 * N * n_src   # one complex multiply-add per visibility-source pair
         out[name] = {"ms": round(ms, 3), "gpairs_per_s": round(N * n_src / ms / 1e6, 1),
                      "max_err": float(f"{np.abs(got - ref).max() / gsum:.3g}"),
-                     "mfma_tflops": round(mfma_flops / ms / 1e9, 1),
-                     "mfma_frac": round(mfma_flops / ms / 1e9 / 2500.0, 4)}
+                     "alg_tflops": round(alg_flops / ms / 1e9, 1),
+                     "mfma_frac": round(alg_flops / ms / 1e9 / 2500.0, 4),
+                     "mfma_issued_tflops": round(mfma_flops / ms / 1e9, 1),
+                     "mfma_issued_frac": round(mfma_flops / ms / 1e9 / 2500.0, 4)}
         log(f"array DFT {name}: {ms:.3f} ms, error {out[name]['max_err']}")
     uvw = (ant[:, pairs[:, 1]] - ant[:, pairs[:, 0]]).reshape(-1, 3).astype(np.float32)
     d_uvw = hip.DeviceBuffer.from_array(uvw)
@@ -336,7 +340,7 @@ def batched_channels(hip, n_channels=64, passes=2):
     _, stage = prof.residuals()
     prof.close()
     st = dict(zip(hip.STAGES, stage))
-    gd_ms = st["degrid_subtract"] + st["grid"]
+    gd_ms = st["degrid_subtract_grid"] + st["grid_merge"]
     alg = total * 40 + n_channels * 16 * G * G
     out = {"workload": f"{n_channels} channels {m.freqs[0] / 1e6:.0f}-{m.freqs[-1] / 1e6:.0f} MHz, {total} vis "
                        f"({int(m.n_vis.min())}-{int(m.n_vis.max())} per channel), {G}^2 grids, "
@@ -452,49 +456,46 @@ def main():
     d_grid = hip.DeviceBuffer(8 * G * G)
     info["classes"] = plan.class_launches()
 
-    def step(timed):
-        if timed:
+    # one step: the major cycle's fused degrid + subtract + grid pass (into
+    # the work items' sub-grids), then their merge into the residual grid
+    def run_steps(pl, steps, warmup):
+        for _ in range(warmup):
+            pl.degrid_grid_accumulate(d_dk, d_gk, d_model, d_meas_p, d_residual_vis=d_delta, order=hip.ORDER_PLAN)
+            pl.merge(d_grid)
+        hip.synchronize()
+        acc = {"degrid_subtract_grid": 0.0, "grid_merge": 0.0}
+        t0 = time.perf_counter()
+        for _ in range(steps):
             ev[0].record()
-        plan.degrid(d_dk, d_model, d_delta, d_measured=d_meas_p, order=hip.ORDER_PLAN)
-        if timed:
+            pl.degrid_grid_accumulate(d_dk, d_gk, d_model, d_meas_p, d_residual_vis=d_delta, order=hip.ORDER_PLAN)
             ev[1].record()
-        plan.accumulate(d_gk, d_delta, order=hip.ORDER_PLAN)
-        if timed:
+            pl.merge(d_grid)
             ev[2].record()
-        plan.merge(d_grid)
-        if timed:
-            ev[3].record()
+            ev[2].sync()
+            acc["degrid_subtract_grid"] += ev[0].elapsed_ms(ev[1])
+            acc["grid_merge"] += ev[1].elapsed_ms(ev[2])
+        hip.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps, {k: v / steps for k, v in acc.items()}
 
-    for _ in range(args.warmup):
-        step(False)
-    hip.synchronize()
-    acc = {"degrid_subtract": 0.0, "grid_accumulate": 0.0, "grid_merge": 0.0}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(True)
-        ev[3].sync()
-        acc["degrid_subtract"] += ev[0].elapsed_ms(ev[1])
-        acc["grid_accumulate"] += ev[1].elapsed_ms(ev[2])
-        acc["grid_merge"] += ev[2].elapsed_ms(ev[3])
-    hip.synchronize()
-    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    k_ms = {k: v / args.steps for k, v in acc.items()}
-    ms_per_step = wall_ms
+    ms_per_step, k_ms = run_steps(plan, args.steps, args.warmup)
     log(f"step {ms_per_step:.3f} ms (events: {k_ms})")
 
     # algorithmic bytes per step of each phase
     alg = {
         # model grid read, uvw read, measured visibility read, residual visibility written
-        "degrid_subtract": 8 * G * G + N * (12 + 8 + 8),
-        "grid_accumulate": N * (12 + 8),           # uvw + residual visibility read
+        "degrid_subtract_grid": 8 * G * G + N * (12 + 8 + 8),
         "grid_merge": 8 * G * G,                   # residual grid written once
     }
     dom = max(k_ms, key=k_ms.get)
     traffic = measured_traffic(dom) if args.config == "config2" else None
-    launches = int(info["classes"]) if dom != "grid_merge" else 1 + (1 if info["chunks"] > info["tiles"] else 0)
     achieved = alg[dom] / (k_ms[dom] * 1e-3) / 1e9
     b_gd = N * 40 + 16 * G * G
     gd_ms = sum(k_ms.values())
+    taps = plan.total_taps()
+    # the fused pass's arithmetic: a complex multiply-add per tap for the
+    # prediction and another for the gridding, 8 flops each, against the
+    # non-packed FP32 FMA rate (256 CUs x 64 lanes x 2 flops x 2.4 GHz)
+    valu_tflops = 16.0 * taps / (k_ms["degrid_subtract_grid"] * 1e-3) / 1e12
 
     extras = {}
     if not args.no_extras:
@@ -515,16 +516,33 @@ def main():
         extras["plan_ms"] = round(plan_s * 1e3, 1)
         extras["grid_degrid_hbm_frac"] = round(b_gd / (gd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
         extras["kernel_ms"] = {k: round(v, 4) for k, v in k_ms.items()}
-        # the fused degrid + subtract + grid kernel on the same data, for comparison
-        ev[0].record()
-        plan.degrid_grid_accumulate(d_dk, d_gk, d_model, d_meas_p, order=hip.ORDER_PLAN)
-        ev[1].record()
-        ev[1].sync()
-        extras["fused_degrid_grid_ms"] = round(ev[0].elapsed_ms(ev[1]), 4)
+        # the same step as separate degrid + subtract, grid accumulate and merge passes
+        sep = {}
+        for rep in range(3):
+            ev[0].record()
+            plan.degrid(d_dk, d_model, d_delta, d_measured=d_meas_p, order=hip.ORDER_PLAN)
+            ev[1].record()
+            plan.accumulate(d_gk, d_delta, order=hip.ORDER_PLAN)
+            ev[2].record()
+            plan.merge(d_grid)
+            ev[3].record()
+            ev[3].sync()
+            if rep:
+                for k, (a, b) in (("degrid_subtract", (0, 1)), ("grid_accumulate", (1, 2)), ("grid_merge", (2, 3))):
+                    sep[k] = sep.get(k, 0.0) + ev[a].elapsed_ms(ev[b]) / 2
+        extras["separate_passes_ms"] = {k: round(v, 4) for k, v in sep.items()}
+        # the same step with TwoSum-compensated sums inside the work items
+        t = time.perf_counter()
+        cplan = hip.GridPlan(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, d_uvw, N, tile=tile, chunk=chunk,
+                             accumulate=hip.ACC_COMPENSATED)
+        c_ms, c_k = run_steps(cplan, 3, 1)
+        cplan.close()
+        extras["compensated_step"] = {"ms_per_step": round(c_ms, 4), "value": round(N / (c_ms * 1e-3) / 1e6, 2),
+                                      "kernel_ms": {k: round(v, 4) for k, v in c_k.items()},
+                                      "s": round(time.perf_counter() - t, 2)}
         extras["autotune"] = {"tile": tile, "chunk_taps": chunk, "pass_ms": round(tune_ms, 3), "s": round(tune_s, 2)}
-        taps = plan.total_taps()
         extras["taps_per_step"] = taps
-        extras["gtaps_per_s"] = round(2 * taps / ((k_ms["degrid_subtract"] + k_ms["grid_accumulate"]) * 1e-3) / 1e9, 1)
+        extras["gtaps_per_s"] = round(2 * taps / (k_ms["degrid_subtract_grid"] * 1e-3) / 1e9, 1)
         extras["plan"] = info
         extras["major_cycle"] = major_cycle(hip, sdp_synth, d, d_uvw, d_meas, d_gk, d_dk, tile, chunk)
         if "enu" in d.extra:
@@ -551,16 +569,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{d.name}: degrid(w-proj, weighted)+subtract+grid of {N} vis on a {G}^2 grid, "
+        "config": {"workload": f"{d.name}: degrid(w-proj, weighted)+subtract+grid (one fused pass, then the merge) of {N} vis on a {G}^2 grid, "
                                f"{d.supports.shape[0]} w-planes h {d.supports[:, 0].min()}-{d.supports[:, 0].max()}, "
                                f"OS {d.oversampling}",
                    "n_vis": N, "grid": G, "w_planes": int(d.supports.shape[0]), "oversampling": d.oversampling,
                    "accumulate": args.accumulate},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_per": "step",
-                     "alg_bytes_per_step": alg[dom], "launches_per_step": launches,
-                     "ms_per_step": round(k_ms[dom], 4),
-                     "copy_gbs": copy_gbs, "frac_of_copy": round(achieved / copy_gbs, 5) if copy_gbs else None},
+                     "alg_bytes_per_step": alg[dom], "ms_per_step": round(k_ms[dom], 4),
+                     "copy_gbs": copy_gbs, "frac_of_copy": round(achieved / copy_gbs, 5) if copy_gbs else None,
+                     "valu": {"kernel": "degrid_subtract_grid", "achieved": round(valu_tflops, 2), "peak": VALU_PEAK_TFLOPS,
+                              "unit": "TFLOP/s", "frac": round(valu_tflops / VALU_PEAK_TFLOPS, 4),
+                              "flops_per_tap": 8, "taps_per_step": 2 * taps,
+                              "note": "prediction and gridding taps, 8 flops each, against non-packed FP32 FMA"}},
     }
     out.update(extras)
     if not args.no_cpu:

@@ -171,7 +171,7 @@ int sdp_degrid_grid_accumulate(const sdp_grid_plan *plan, const float *d_degrid_
                                int order, float *d_residual_vis, sdp_stream s);
 
 /* Autotuner: builds a plan for every (tile, taps-per-work-item) candidate
- * on these coordinates, times one degrid + grid + merge pass of each with
+ * on these coordinates, times one sdp_degrid_grid_accumulate + merge pass of each with
  * HIP events and returns the fastest pair (and its time).  Candidates whose
  * windows do not fit LDS are skipped.  Allocates scratch for the timing. */
 int sdp_grid_autotune(const sdp_grid_params *p, const int *supports, const float *d_uvw, int n_vis,

@@ -81,7 +81,9 @@ int sdp_pipeline_destroy(sdp_pipeline *p);
  * NULL.  Synchronous. */
 int sdp_pipeline_cycle(sdp_pipeline *p, double *rms_residual, double *rms_after, float *ms);
 
-/* With SDP_PIPE_PROFILE: milliseconds of each stage of the last cycle,
+/* With SDP_PIPE_PROFILE: milliseconds of each stage of the last cycle
+ * (SDP_STAGE_DEGRID_SUBTRACT: the fused degrid + subtract + grid pass into
+ * the work items' sub-grids; SDP_STAGE_GRID: their merge into the grid),
  * ms[SDP_STAGE_COUNT]. */
 enum { SDP_STAGE_PREDICT_FFT = 0, SDP_STAGE_DEGRID_SUBTRACT, SDP_STAGE_GRID, SDP_STAGE_INVERSE_FFT,
        SDP_STAGE_CLEAN, SDP_STAGE_COUNT };

@@ -53,9 +53,10 @@ static int enqueue_channel(sdp_batch *B, const channel *c, int profile)
                   SDP_FFT_REAL_IN | SDP_FFT_SHIFT_IN | SDP_FFT_SHIFT_OUT | SDP_FFT_CORRECT_IN, B->taper, B->fft_work,
                   B->s));
     if (profile) TRY(sdp_event_record(B->stage_ev[1], B->s));
-    TRY(sdp_degrid(c->plan, B->dk, B->grid, c->vis_meas, SDP_ORDER_PLAN, c->vis_resid, B->s));
+    TRY(sdp_degrid_grid_accumulate(c->plan, B->dk, B->gk, B->grid, c->vis_meas, SDP_ORDER_PLAN, c->vis_resid,
+                                   B->s));
     if (profile) TRY(sdp_event_record(B->stage_ev[2], B->s));
-    TRY(sdp_grid(c->plan, B->gk, c->vis_resid, SDP_ORDER_PLAN, B->grid, B->s));
+    TRY(sdp_grid_merge(c->plan, B->grid, B->s));
     if (profile) TRY(sdp_event_record(B->stage_ev[3], B->s));
     TRY(sdp_fft2d(B->grid, c->residual, G, SDP_FFT_INVERSE,
                   SDP_FFT_SHIFT_IN | SDP_FFT_SHIFT_OUT | SDP_FFT_REAL_OUT | SDP_FFT_CORRECT_OUT, B->taper,

@@ -2411,16 +2411,16 @@ int sdp_grid_autotune(const sdp_grid_params *pp, const int *supports, const floa
                 sdp_grid_plan *plan = nullptr;
                 if (sdp_grid_plan_create(&plan, &p, supports, d_uvw, n_vis, s) != SDP_OK) continue;   // shape not supported
                 float ms = 0.f;
-                // one untimed pass, then degrid + grid + merge timed together
-                rc = sdp_degrid(plan, d_degrid_kernels, (const float *)grid, (const float *)vis, SDP_ORDER_PLAN,
-                                (float *)vis, s);
-                if (rc == SDP_OK) rc = sdp_grid(plan, d_grid_kernels, (const float *)vis, SDP_ORDER_PLAN, (float *)grid, s);
+                // one untimed pass, then the major cycle's fused degrid +
+                // subtract + grid and the merge timed together
+                rc = sdp_degrid_grid_accumulate(plan, d_degrid_kernels, d_grid_kernels, (const float *)grid,
+                                                (const float *)vis, SDP_ORDER_PLAN, (float *)vis, s);
+                if (rc == SDP_OK) rc = sdp_grid_merge(plan, (float *)grid, s);
                 if (rc == SDP_OK) {
                     (void)hipEventRecord(e0, st);
-                    rc = sdp_degrid(plan, d_degrid_kernels, (const float *)grid, (const float *)vis, SDP_ORDER_PLAN,
-                                    (float *)vis, s);
-                    if (rc == SDP_OK)
-                        rc = sdp_grid(plan, d_grid_kernels, (const float *)vis, SDP_ORDER_PLAN, (float *)grid, s);
+                    rc = sdp_degrid_grid_accumulate(plan, d_degrid_kernels, d_grid_kernels, (const float *)grid,
+                                                    (const float *)vis, SDP_ORDER_PLAN, (float *)vis, s);
+                    if (rc == SDP_OK) rc = sdp_grid_merge(plan, (float *)grid, s);
                     (void)hipEventRecord(e1, st);
                     (void)hipEventSynchronize(e1);
                     (void)hipEventElapsedTime(&ms, e0, e1);

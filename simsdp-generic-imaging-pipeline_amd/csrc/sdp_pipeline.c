@@ -15,8 +15,10 @@
  *      fft_shift_complex_to_complex (fft_run.c:5-20,50-73,94-111), with the
  *      taper division of the inverse correction fused on the way in;
  *   2. fft_degrid + subtract_from_measurements_finegrid (degridgrid.c:127-202,
- *      gains_apply_run.c:120-144), kernel-weighted, one fused pass;
- *   3. grid (degridgrid.c:69-125);
+ *      gains_apply_run.c:120-144), kernel-weighted, and the gridding of the
+ *      residual into the work items' sub-grids (degridgrid.c:69-125): one
+ *      fused pass (sdp_degrid_grid_accumulate);
+ *   3. the sub-grids merged into the residual grid (sdp_grid_merge);
  *   4. inverse_fft_run + convolution_correction_run (fft_run.c:23-48,76-92,
  *      convolution_correction_run.c:4-37), one fused FFT;
  *   5. residual statistics;
@@ -85,9 +87,11 @@ static int enqueue_residual(sdp_pipeline *P)
     TRY(mark(P, SDP_STAGE_PREDICT_FFT));
     TRY(sdp_fft2d(P->model, P->grid, G, SDP_FFT_FORWARD, predict_flags, P->taper, P->fft_work, P->s));
     TRY(mark(P, SDP_STAGE_DEGRID_SUBTRACT));
-    TRY(sdp_degrid(P->plan, P->dk, P->grid, P->vis_meas, SDP_ORDER_PLAN, P->vis_resid, P->s));
+    /* the model grid is read by the fused pass before the merge overwrites it */
+    TRY(sdp_degrid_grid_accumulate(P->plan, P->dk, P->gk, P->grid, P->vis_meas, SDP_ORDER_PLAN, P->vis_resid,
+                                   P->s));
     TRY(mark(P, SDP_STAGE_GRID));
-    TRY(sdp_grid(P->plan, P->gk, P->vis_resid, SDP_ORDER_PLAN, P->grid, P->s));
+    TRY(sdp_grid_merge(P->plan, P->grid, P->s));
     TRY(mark(P, SDP_STAGE_INVERSE_FFT));
     TRY(sdp_fft2d(P->grid, P->residual, G, SDP_FFT_INVERSE,
                   SDP_FFT_SHIFT_IN | SDP_FFT_SHIFT_OUT | SDP_FFT_REAL_OUT | SDP_FFT_CORRECT_OUT, P->taper,

@@ -42,7 +42,7 @@ HOGBOM_PSF_AS_STORED = 1
 HOGBOM_DOUBLE_MEAN = 2
 # major-cycle pipeline (sdp_pipeline.h)
 PIPE_GRAPH, PIPE_REFERENCE_PREDICT, PIPE_PSF_MIRRORED, PIPE_PROFILE = 1, 2, 4, 8
-STAGES = ("predict_fft", "degrid_subtract", "grid", "inverse_fft", "clean")
+STAGES = ("predict_fft", "degrid_subtract_grid", "grid_merge", "inverse_fft", "clean")
 CNN_FP32, CNN_BF16 = 0, 1
 LEARNED_TILE_SCALE, LEARNED_GLOBAL_SCALE = 0, 1
 IMAGE_RESIDUAL, IMAGE_MODEL, IMAGE_PSF = 0, 1, 2
@@ -324,7 +324,7 @@ class _Params(ctypes.Structure):
 
 
 def autotune(grid_size, oversampling, supports, uv_scale, w_scale, d_uvw, n_vis, d_grid_kernels, d_degrid_kernels,
-             tiles=(4, 8, 16), chunks=(32768, 65536, 131072, 262144), snap_mode=SNAP_STD, accumulate=ACC_BLOCKED,
+             tiles=(4, 8, 16), chunks=(16384, 32768, 65536, 131072), snap_mode=SNAP_STD, accumulate=ACC_BLOCKED,
              stream=None):
     """Fastest (tile, taps per work item) for these coordinates: (tile, chunk, ms)."""
     sup = np.ascontiguousarray(np.asarray(supports, dtype=np.int32).reshape(-1, 2))
