@@ -34,7 +34,7 @@ def measured_traffic(phase):
     """HBM bytes per bench step of `phase` (its launches together), from the
     FETCH_SIZE / WRITE_SIZE counter summary committed under profiles/ for
     this workload (calibrated against a plain copy); None when absent."""
-    path = os.path.join(REPO, "profiles", "bench_config2_hbm_traffic.md")
+    path = os.path.join(REPO, "profiles", "degrid_grid_config2_hbm_traffic.md")
     try:
         text = open(path).read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
@@ -380,6 +380,8 @@ def main():
     ap.add_argument("--accumulate", choices=["blocked", "compensated"], default="blocked",
                     help="gridder accumulation inside a work item: float (blocked) or TwoSum; "
                          "work items are merged TwoSum-compensated either way")
+    ap.add_argument("--tile", type=int, default=0, help="uv tile of the plan (with --chunk: no autotuning)")
+    ap.add_argument("--chunk", type=int, default=0, help="taps per work item of the plan (with --tile)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -427,8 +429,11 @@ def main():
 
     t = time.perf_counter()
     acc_mode = hip.ACC_COMPENSATED if args.accumulate == "compensated" else hip.ACC_BLOCKED
-    tile, chunk, tune_ms = hip.autotune(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, d_uvw, N, d_gk, d_dk,
-                                        accumulate=acc_mode)
+    if args.tile > 0 and args.chunk > 0:   # a fixed plan shape (profiling runs)
+        tile, chunk, tune_ms = args.tile, args.chunk, 0.0
+    else:
+        tile, chunk, tune_ms = hip.autotune(G, d.oversampling, d.supports, d.uv_scale, d.w_scale, d_uvw, N, d_gk,
+                                            d_dk, accumulate=acc_mode)
     tune_s = time.perf_counter() - t
     log(f"autotune {tune_s:.1f} s: tile {tile}, {chunk} taps per work item ({tune_ms:.2f} ms per pass)")
     t = time.perf_counter()
