@@ -31,9 +31,9 @@
 #define PRECISION2 float2
 #define PRECISION3 float3
 
-/* HIP/CUDA compilers provide their own vector types under these names; a
+/* The HIP compiler provides its own vector types under these names; a
  * plain C (or non-GPU C++) includer gets the reference's struct layouts. */
-#if !(defined(__cplusplus) && (defined(__HIPCC__) || defined(__HIP__) || defined(__CUDACC__)))
+#if !(defined(__cplusplus) && (defined(__HIPCC__) || defined(__HIP__)))
 #ifndef SDP_HAVE_VECTOR_TYPES
 #define SDP_HAVE_VECTOR_TYPES
 typedef struct { int x; int y; } int2;

@@ -246,3 +246,27 @@ def test_kernel_csv_loaders(tmp_path):
         np.testing.assert_array_equal(ker[:total, 0], re.astype(np.float32))
         np.testing.assert_array_equal(ker[:total, 1], np.array([float(f"{v:.12e}") for v in im], np.float32))
         assert np.isnan(ker[total:]).all()                 # nothing written past the last plane
+
+
+def test_schedule_program_checks_its_case_directory(tmp_path):
+    """sdp_major_cycle (the generated single-core schedule as a C program
+    over libsdp_actors.so) is built, links, and rejects a missing argument,
+    a missing case file and an incomplete case before any actor runs."""
+    prog = os.path.join(os.path.dirname(sdp_hip.LIB_PATH), "sdp_major_cycle")
+    run = subprocess.run([prog], capture_output=True, text=True, timeout=60)
+    assert run.returncode == 2 and "usage" in run.stderr
+    run = subprocess.run([prog, str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert run.returncode == 2 and "case.txt" in run.stderr
+    (tmp_path / "case.txt").write_text("grid_size 64\nnum_vis 10\n")
+    run = subprocess.run([prog, str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert run.returncode == 2 and "does not set every key" in run.stderr
+    keys = dict(grid_size=64, num_vis=10, num_kernels=1, total_kernel_samples=16, num_receivers=5, num_baselines=10,
+                slices=4, num_major=1, num_minor=1, max_sources=1, oversampling=16)
+    (tmp_path / "case.txt").write_text("".join(f"{k} {v}\n" for k, v in keys.items()))
+    run = subprocess.run([prog, str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert run.returncode == 2 and "multiple of slices" in run.stderr
+    keys["slices"] = 5
+    (tmp_path / "case.txt").write_text("".join(f"{k} {v}\n" for k, v in keys.items()))
+    (tmp_path / "config.bin").write_bytes(bytes(100))
+    run = subprocess.run([prog, str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert run.returncode == 2 and "fewer than the 360 bytes" in run.stderr
