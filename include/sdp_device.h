@@ -129,6 +129,20 @@ typedef struct sdp_grid_plan sdp_grid_plan;
  * cells across, must fit a 64-lane wavefront), else SDP_E_UNSUPPORTED. */
 int sdp_grid_plan_create(sdp_grid_plan **plan, const sdp_grid_params *p, const int *supports,
                          const float *d_uvw, int n_vis, sdp_stream s);
+/* One plan over n_grids grids (e.g. frequency channels) that share the
+ * parameters and kernel tables: grid g's n_vis[g] coordinates are at
+ * d_uvw[g] (host array of device pointers).  Visibilities are indexed over
+ * all grids in grid order (grid g's first is sum_{k<g} n_vis[k]): the
+ * caller-order arrays the plan functions take hold them in that order, and
+ * every grid argument of the functions below then holds n_grids
+ * consecutive G x G grids.  Each pass covers all grids in the launches of
+ * one grid; a visibility's footprint is clipped to its own grid, and grid
+ * g's cells and visibilities come out as with a plan of grid g alone (same
+ * tile and work-item size, bit for bit while the sort key keeps the same
+ * low field: with very many grids and tiles the plane part of that field
+ * is dropped, which only reorders float sums inside a work item). */
+int sdp_grid_plan_create_multi(sdp_grid_plan **plan, const sdp_grid_params *p, const int *supports, int n_grids,
+                               const float *const *d_uvw, const int *n_vis, sdp_stream s);
 int sdp_grid_plan_destroy(sdp_grid_plan *plan);
 /* plan statistics (10 values): [0] visibilities kept, [1] dropped (outside
  * grid or beyond the support table), [2] tiles used, [3] work items, [4] tile
@@ -198,6 +212,13 @@ size_t sdp_fft_work_bytes(int grid_size);
  * d_taper: G/2+1 floats (taper with a trailing 0) when SDP_FFT_CORRECT_OUT. */
 int sdp_fft2d(const void *d_in, void *d_out, int grid_size, int direction, int flags,
               const float *d_taper, void *d_work, sdp_stream s);
+/* sdp_fft2d of n_batch consecutive grids (input, output and the work
+ * buffer, n_batch * sdp_fft_work_bytes(G) bytes, each hold n_batch grids)
+ * in the same three launches: results identical to n_batch sdp_fft2d calls.
+ * No reference counterpart (the reference transforms one grid per actor
+ * call); it serves the batched multi-channel pass. */
+int sdp_fft2d_batch(const void *d_in, void *d_out, int grid_size, int n_batch, int direction, int flags,
+                    const float *d_taper, void *d_work, sdp_stream s);
 
 /* ---- element-wise actors ------------------------------------------------ */
 int sdp_shift_c2c(int grid_size, const float *d_in, float *d_out, sdp_stream s);

@@ -140,15 +140,17 @@ const sdp_grid_plan *sdp_pipeline_plan(const sdp_pipeline *p);
  * No reference counterpart: the reference images one channel per MPI rank
  * (PN/generated/main.c:44-51), each running the cycle above.  Here
  * n_channels channels share one GPU: each has its own visibilities (any
- * count, e.g. after per-channel flagging), gridding plan, model and
- * residual images; all share the kernel tables, one complex grid and the
- * FFT work buffer, and are processed one after another on one stream.
- * sdp_batch_residuals runs, for every channel, the first four stages of the
- * major cycle (model -> predict FFT -> degrid and subtract -> grid ->
- * inverse FFT and correction -> residual image); with SDP_BATCH_GRAPH the
- * whole multi-channel pass is captured once (after one eager pass) and
+ * count, e.g. after per-channel flagging), model and residual images; one
+ * gridding plan covers all channels (sdp_grid_plan_create_multi), and the
+ * model images, complex grids and residual images are stacked.
+ * sdp_batch_residuals runs the first four stages of the major cycle for
+ * all channels at once (models -> batched predict FFT -> degrid and
+ * subtract -> grid -> batched inverse FFT and correction -> residual
+ * images), each stage in the launches one channel would take; with
+ * SDP_BATCH_GRAPH the pass is captured once (after one eager pass) and
  * replayed as one graph per cycle.  tile and chunk (0: defaults) apply to
- * every channel's plan (sdp_grid_autotune picks them). */
+ * the plan (sdp_grid_autotune picks them).  Channel k's images match what a
+ * plan of channel k alone gives (same tile and work-item size). */
 enum { SDP_BATCH_GRAPH = 1, SDP_BATCH_PROFILE = 2 };
 typedef struct sdp_batch_params {
     int n_channels, grid_size, oversampling, n_kernels;
@@ -165,11 +167,12 @@ int sdp_batch_create(sdp_batch **batch, const sdp_batch_params *params, const in
 int sdp_batch_destroy(sdp_batch *batch);
 /* one multi-channel pass; *ms: its device time.  With SDP_BATCH_PROFILE
  * (eager, events between stages) stage_ms[SDP_STAGE_COUNT] receives the
- * per-stage sums over channels (SDP_STAGE_CLEAN stays 0). */
+ * stage times of the pass (SDP_STAGE_CLEAN stays 0). */
 int sdp_batch_residuals(sdp_batch *batch, float *ms, float *stage_ms);
 /* device G x G float images of one channel (the model may be written) */
 float *sdp_batch_model(sdp_batch *batch, int channel);
 const float *sdp_batch_residual(const sdp_batch *batch, int channel);
+/* the plan over all channels (the same for every valid channel) */
 const sdp_grid_plan *sdp_batch_plan(const sdp_batch *batch, int channel);
 
 #ifdef __cplusplus

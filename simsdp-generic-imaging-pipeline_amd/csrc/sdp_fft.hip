@@ -21,6 +21,9 @@
 //
 // Twiddles come from one table per G of exp(2 pi i m / G) computed in double
 // precision on the host and rounded once to float.
+//
+// A batch of grids (consecutive G x G arrays) takes the same three launches,
+// the grid index in blockIdx.z.
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -154,6 +157,12 @@ fft_rows_kernel(const void *__restrict__ in, float2 *__restrict__ out, int G, in
                 float sign, const float2 *__restrict__ tw, const float *__restrict__ taper)
 {
     extern __shared__ float2 lds[];
+    const size_t gofs = (size_t)blockIdx.z * G * G;   // this grid's first element
+    if (flags & SDP_FFT_REAL_IN)
+        in = (const float *)in + gofs;
+    else
+        in = (const float2 *)in + gofs;
+    out += gofs;
     const int row0 = blockIdx.x * rows_per_block;
     const int n = rows_per_block * G;
     for (int i = threadIdx.x; i < n; i += kFftThreads) {
@@ -184,6 +193,7 @@ fft_cols1_kernel(float2 *__restrict__ data, int G, int G1, int G2, int CB, int t
                  const float2 *__restrict__ tw)
 {
     extern __shared__ float2 lds[];
+    data += (size_t)blockIdx.z * G * G;
     const int c0 = blockIdx.x * CB, j2 = blockIdx.y;
     const int n = G1 * CB;
     for (int i = threadIdx.x; i < n; i += kFftThreads) {
@@ -213,6 +223,12 @@ fft_cols2_kernel(const float2 *__restrict__ data, void *__restrict__ out, int G,
                  float sign, const float2 *__restrict__ tw, OutArgs o)
 {
     extern __shared__ float2 lds[];
+    const size_t gofs = (size_t)blockIdx.z * G * G;
+    data += gofs;
+    if (o.flags & SDP_FFT_REAL_OUT)
+        out = (float *)out + gofs;
+    else
+        out = (float2 *)out + gofs;
     const int c0 = blockIdx.x * CB, k1 = blockIdx.y;
     const int n = G2 * CB;
     for (int i = threadIdx.x; i < n; i += kFftThreads) {
@@ -290,7 +306,14 @@ size_t sdp_fft_work_bytes(int G) { return (G > 0) ? sizeof(float2) * (size_t)G *
 int sdp_fft2d(const void *d_in, void *d_out, int G, int direction, int flags, const float *d_taper, void *d_work,
               sdp_stream s)
 {
+    return sdp_fft2d_batch(d_in, d_out, G, 1, direction, flags, d_taper, d_work, s);
+}
+
+int sdp_fft2d_batch(const void *d_in, void *d_out, int G, int n_batch, int direction, int flags,
+                    const float *d_taper, void *d_work, sdp_stream s)
+{
     SDP_REQUIRE(G >= 16 && G <= 8192 && (G & (G - 1)) == 0, "FFT size %d unsupported (power of two in [16, 8192])", G);
+    SDP_REQUIRE(n_batch >= 1 && n_batch <= 65535, "FFT batch of %d grids unsupported (1 to 65535)", n_batch);
     SDP_REQUIRE(direction == SDP_FFT_FORWARD || direction == SDP_FFT_INVERSE, "bad FFT direction");
     SDP_REQUIRE(d_in && d_out && d_work, "sdp_fft2d: null pointer");
     SDP_REQUIRE(!(flags & SDP_FFT_CORRECT_OUT) || ((flags & SDP_FFT_REAL_OUT) && d_taper),
@@ -310,7 +333,7 @@ int sdp_fft2d(const void *d_in, void *d_out, int G, int direction, int flags, co
         const int rpb = G >= 2048 ? 1 : std::min(G, 2048 / G);   // rows per workgroup
         const size_t lds = sizeof(float2) * (size_t)rpb * G;
         const int maxb = butterflies_per_thread(G, rpb);
-        dim3 grid(G / rpb);
+        dim3 grid(G / rpb, 1, n_batch);
         if (maxb <= 4) {
             fft_rows_kernel<4><<<grid, kFftThreads, lds, st>>>(d_in, work, G, rpb, flags, sign, tw, d_taper);
         } else if (maxb <= 8) {
@@ -327,7 +350,7 @@ int sdp_fft2d(const void *d_in, void *d_out, int G, int direction, int flags, co
     const int CB = G >= 32 ? 32 : G;
     {
         const size_t lds = sizeof(float2) * (size_t)G1 * CB;
-        dim3 grid(G / CB, G2);
+        dim3 grid(G / CB, G2, n_batch);
         if (butterflies_per_thread(G1, CB) <= 4)
             fft_cols1_kernel<4><<<grid, kFftThreads, lds, st>>>(work, G, G1, G2, CB, ilog2(G / G1), sign, tw);
         else
@@ -336,7 +359,7 @@ int sdp_fft2d(const void *d_in, void *d_out, int G, int direction, int flags, co
     }
     {
         const size_t lds = sizeof(float2) * (size_t)G2 * CB;
-        dim3 grid(G / CB, G1);
+        dim3 grid(G / CB, G1, n_batch);
         OutArgs o{flags, d_taper};
         if (butterflies_per_thread(G2, CB) <= 4)
             fft_cols2_kernel<4><<<grid, kFftThreads, lds, st>>>(work, d_out, G, G1, G2, CB, ilog2(G / G2), sign, tw, o);

@@ -128,6 +128,9 @@ struct PlanArgs {
     int n, G, OS, n_kernels, snap_mode, tile, ntx, nty, ntiles, nkeys;
     int plane_bits;   // sort key = (class, tile) key << plane_bits | low field (plane, or t_u + t_v tile)
     double uv_scale, w_scale;
+    int grid, first;  // this launch's grid and the caller index of its first visibility
+    int ntiles_all;   // tiles over all grids: a key is rank * ntiles_all + grid * ntiles + tile
+    int class_rank[kAllClasses];   // key rank of each support class present (classes absent: -1)
 };
 
 // The exact-footprint test of one axis: first cell c0 (signed grid
@@ -153,9 +156,10 @@ __global__ void plan_records_kernel(const float *uvw, const int2 *supports, cons
                                     const int *plane_utab, PlanArgs a, VisRec *recs, uint2 *frecs, uint32_t *keys,
                                     uint32_t *vals)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    const float x = uvw[3 * (size_t)i], y = uvw[3 * (size_t)i + 1], z = uvw[3 * (size_t)i + 2];
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;   // within this grid's visibilities
+    if (li >= a.n) return;
+    const float x = uvw[3 * (size_t)li], y = uvw[3 * (size_t)li + 1], z = uvw[3 * (size_t)li + 2];
+    const int i = a.first + li;                             // caller index over all grids
     const int gc = a.G / 2;
     keys[i] = (uint32_t)a.nkeys << a.plane_bits;   // dropped: sorts last
     vals[i] = (uint32_t)i;
@@ -218,7 +222,8 @@ __global__ void plan_records_kernel(const float *uvw, const int2 *supports, cons
         }
     }
     frecs[i] = fr;
-    const uint32_t key = (uint32_t)(cls * a.ntiles + (va / a.tile) * a.ntx + (ua / a.tile));
+    const uint32_t key = (uint32_t)(a.class_rank[cls] * a.ntiles_all + a.grid * a.ntiles + (va / a.tile) * a.ntx +
+                                    (ua / a.tile));
     keys[i] = key << a.plane_bits | (low & ((1u << a.plane_bits) - 1u));
 }
 
@@ -335,7 +340,7 @@ __global__ void permute_kernel(const float2 *in, const uint32_t *perm, int n, fl
 // ---- chunk kernels -------------------------------------------------------
 
 struct ChunkArgs {
-    int G, OS, tile, ntx, chunk_base;
+    int G, OS, tile, ntx, ntiles, chunk_base;
     int lane_shift;   // log2 of the lanes per visibility (64 / copies)
     int copies;       // lane groups (and sub-grid copies) per wavefront
     int os_shift;     // log2 OS when OS is a power of two, else -1
@@ -574,12 +579,13 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
     extern __shared__ float4 lds4[];
     constexpr bool kGrid = MODE != MODE_DEGRID;
     constexpr bool kDegrid = MODE != MODE_GRID;
-    const int4 ch = chunks[a.chunk_base + blockIdx.x];   // tile, first record, count, window S
+    const int4 ch = chunks[a.chunk_base + blockIdx.x];   // tile (over all grids), first record, count, window S
     const int S = ch.w, stride = S + 1, cells = S * stride;
     const int lane = threadIdx.x;
     const int L = ONE ? kWave : 1 << a.lane_shift, C = ONE ? 1 : a.copies;
     const int g = ONE ? 0 : lane >> a.lane_shift, c = ONE ? lane : lane & (L - 1);
-    const int tile_u0 = (ch.x % a.ntx) * a.tile, tile_v0 = (ch.x / a.ntx) * a.tile;
+    const int grid_no = ch.x / a.ntiles, tl = ch.x - grid_no * a.ntiles;
+    const int tile_u0 = (tl % a.ntx) * a.tile, tile_v0 = (tl / a.ntx) * a.tile;
     const int gc = a.G / 2;
     const float os = (float)a.OS;
     // raw (stride 0) buffer resources over the tables; word 3 is gfx9's 32-bit data format
@@ -605,7 +611,7 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
             const int row = i / S, col = i - row * S;
             const int gy = tile_v0 + row, gx = tile_u0 + col;
             float2 v = make_float2(0.f, 0.f);
-            if (gy < a.G && gx < a.G) v = io.grid[(size_t)gy * a.G + gx];
+            if (gy < a.G && gx < a.G) v = io.grid[((size_t)grid_no * a.G + gy) * a.G + gx];
             win[row * stride + col] = v;
         }
         if (lane == 0) win[cells] = make_float2(0.f, 0.f);
@@ -1025,7 +1031,7 @@ template <bool COMP, int PMAX> constexpr size_t fused_lds()
 }
 
 struct FusedArgs {
-    int chunk_b, chunk_e, tile, G, ntx, tab_bytes;
+    int chunk_b, chunk_e, tile, G, ntx, ntiles, tab_bytes;
 };
 
 // One wavefront per work item: for each of its visibilities (in plan
@@ -1052,7 +1058,9 @@ fast_degrid_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restr
     Acc *cells = reinterpret_cast<Acc *>(lds4) + (size_t)wave * PMAX * 4 * kWave;
     const int4 ch = chunks[ci];
     const int S = ch.w, tile = fa.tile, W = S - tile + 1, n = W + 1, P = (W * W + kWave - 1) / kWave;
-    const int tile_u0 = (ch.x % fa.ntx) * tile, tile_v0 = (ch.x / fa.ntx) * tile;
+    const int grid_no = ch.x / fa.ntiles, tl = ch.x - grid_no * fa.ntiles;   // ch.x: tile over all grids
+    const int tile_u0 = (tl % fa.ntx) * tile, tile_v0 = (tl / fa.ntx) * tile;
+    model += (size_t)grid_no * fa.G * fa.G;
     const int lane = threadIdx.x % kWave;
     for (int q = 0; q < PMAX * 4; ++q) acc_zero(cells[q * kWave + lane]);
     const __amdgpu_buffer_rsrc_t rg =
@@ -1227,7 +1235,7 @@ constexpr int kRegionList = 512;    // slab descriptors staged per pass
 constexpr int kRegionCover = 1024;  // covering tiles per region (checked by the plan)
 
 struct RegionArgs {
-    int G, tile, ntx, nty, smax, R;
+    int G, tile, ntx, nty, ntiles, smax, R;   // blockIdx.z: the grid
 };
 
 template <int RG, int kRegionBatch>   // kRegionBatch: slabs whose loads are issued together
@@ -1245,6 +1253,8 @@ merge_region_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ k
     __shared__ int s_b[kRegionCover];           // (by + 512) | (bx + 512) << 10
     const int R = a.R;
     const int oy0 = blockIdx.y * R, ox0 = blockIdx.x * R;
+    tile_key_begin += (size_t)blockIdx.z * a.ntiles;
+    grid += (size_t)blockIdx.z * a.G * a.G;
     // At R = RG each of the 4 wavefronts owns a band of RG / 4 rows (64 / RG
     // rows of RG lanes at a time, its cells 64 / RG rows apart) and skips the
     // additions of slabs that miss its band; smaller regions (G < RG) take
@@ -1375,6 +1385,7 @@ __global__ void degrid_dropped_kernel(const uint32_t *perm, int first, int n, co
 struct sdp_grid_plan {
     sdp_grid_params p;
     int n = 0, n_kept = 0, tile = 0, chunk_taps = 0, ntx = 0, nty = 0, H = 0, smax = 0;
+    int n_grids = 1, ntiles = 0;   // grids of the plan (consecutive G x G grids), tiles per grid
     int n_chunks = 0, n_tiles_used = 0, n_class_launches = 0;
     long long total_taps = 0;
     int class_S[kAllClasses] = {};
@@ -1571,7 +1582,7 @@ int launch_class(const sdp_grid_plan *plan, const LaunchRange &rg, const ChunkIO
     int os_shift = -1;
     for (int k = 0; k < 12; ++k)
         if ((1 << k) == OS) os_shift = k;
-    ChunkArgs a{plan->p.grid_size, OS, plan->tile, plan->ntx, b, lane_shift, copies, os_shift,
+    ChunkArgs a{plan->p.grid_size, OS, plan->tile, plan->ntx, plan->ntiles, b, lane_shift, copies, os_shift,
                 (int)(8 * plan->table_len), lds_planes(plan->p.n_kernels),
                 (int)lds_window_bytes(MODE, S, copies, COMP)};
     const size_t lds = lds_bytes(MODE, S, copies, COMP, plan->p.n_kernels);
@@ -1632,7 +1643,7 @@ int launch_fused_group(const sdp_grid_plan *plan, int b, int e, const ChunkIO &i
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr_set = true;
     }
-    FusedArgs fa{b, e, plan->tile, plan->p.grid_size, plan->ntx, (int)(8 * plan->utab_len)};
+    FusedArgs fa{b, e, plan->tile, plan->p.grid_size, plan->ntx, plan->ntiles, (int)(8 * plan->utab_len)};
     fast_degrid_grid_kernel<COMP, PMAX><<<(e - b + waves - 1) / waves, kWave * waves, lds, st>>>(
         plan->frecs, perm, plan->chunks, plan->slab_off, io.measured, io.grid, io.vis_out, plan->utab, plan->utab_d,
         plan->conj_flag, fa, plan->slabs);
@@ -1768,13 +1779,27 @@ extern "C" {
 int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const int *supports,
                          const float *d_uvw, int n_vis, sdp_stream s)
 {
-    SDP_REQUIRE(out && pp && supports, "sdp_grid_plan_create: null argument");
+    return sdp_grid_plan_create_multi(out, pp, supports, 1, &d_uvw, &n_vis, s);
+}
+
+int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, const int *supports, int n_grids,
+                               const float *const *d_uvw_of, const int *n_vis_of, sdp_stream s)
+{
+    SDP_REQUIRE(out && pp && supports && d_uvw_of && n_vis_of, "sdp_grid_plan_create: null argument");
     *out = nullptr;
+    SDP_REQUIRE(n_grids >= 1, "grid count %d must be at least 1", n_grids);
+    std::vector<int> grid_first(n_grids + 1, 0);
+    for (int g = 0; g < n_grids; ++g) {
+        SDP_REQUIRE(n_vis_of[g] >= 0, "negative visibility count for grid %d", g);
+        SDP_REQUIRE(n_vis_of[g] == 0 || d_uvw_of[g], "null coordinates for grid %d", g);
+        SDP_REQUIRE((long long)grid_first[g] + n_vis_of[g] < (1LL << 31), "more than 2^31 - 1 visibilities");
+        grid_first[g + 1] = grid_first[g] + n_vis_of[g];
+    }
+    const int n_vis = grid_first[n_grids];
     const sdp_grid_params p = *pp;
     SDP_REQUIRE(p.grid_size >= 2 && p.grid_size % 2 == 0 && p.grid_size <= 32768,
                 "grid size %d must be even and in [2, 32768]", p.grid_size);
     SDP_REQUIRE(p.oversampling >= 1 && p.n_kernels >= 1 && p.n_kernels < 32768, "bad oversampling / n_kernels");
-    SDP_REQUIRE(n_vis >= 0, "negative visibility count");
     SDP_REQUIRE(p.snap_mode == SDP_SNAP_STD || p.snap_mode == SDP_SNAP_GRIDDING_ACTOR, "bad snap mode");
     SDP_REQUIRE(p.tile >= 0 && p.tile <= 32 && p.chunk >= 0, "tile must be in [1, 32] (0 = default)");
     SDP_REQUIRE(p.accumulate == SDP_ACC_BLOCKED || p.accumulate == SDP_ACC_COMPENSATED, "bad accumulate mode");
@@ -1824,6 +1849,7 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     sdp_grid_plan *plan = new sdp_grid_plan();
     plan->p = p;
     plan->n = n_vis;
+    plan->n_grids = n_grids;
     plan->H = H;
     plan->tile = std::min(p.tile > 0 ? p.tile : 8, p.grid_size);
     plan->chunk_taps = p.chunk > 0 ? p.chunk : (1 << 17);
@@ -1866,13 +1892,20 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
                                   tile, class_h[c]);
         }
     }
-    if ((long long)plan->ntx * plan->nty * kAllClasses >= (1LL << 31)) {
+    // sort keys rank the support classes present (in class order), so that
+    // absent classes take no key space
+    int class_rank[kAllClasses], n_ranked = 0;
+    for (int c = 0; c < kAllClasses; ++c) class_rank[c] = class_h[c] >= 0 ? n_ranked++ : -1;
+    if ((long long)plan->ntx * plan->nty * n_grids * std::max(n_ranked, 1) >= (1LL << 31)) {
         const int tile = plan->tile;
         delete plan;
-        return sdp::set_error(SDP_E_UNSUPPORTED, "tile %d too small for grid size %d", tile, p.grid_size);
+        return sdp::set_error(SDP_E_UNSUPPORTED, "tile %d too small for %d grids of size %d", tile, n_grids,
+                              p.grid_size);
     }
     const int ntiles = plan->ntx * plan->nty;
-    const int nkeys = kAllClasses * ntiles;
+    plan->ntiles = ntiles;
+    const int ntiles_all = ntiles * n_grids;
+    const int nkeys = std::max(n_ranked, 1) * ntiles_all;
     {
         // SDP_MERGE_REGION (16 or 32) overrides the region side for tuning
         const char *e = getenv("SDP_MERGE_REGION");
@@ -1948,7 +1981,8 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     std::vector<KeyRun> runs;
     if (n_vis > 0) {
         PlanArgs a{n_vis, p.grid_size, p.oversampling, p.n_kernels, p.snap_mode, plan->tile, plan->ntx, plan->nty,
-                   ntiles, nkeys, 0, p.uv_scale, p.w_scale};
+                   ntiles, nkeys, 0, p.uv_scale, p.w_scale, 0, 0, ntiles_all, {}};
+        for (int c = 0; c < kAllClasses; ++c) a.class_rank[c] = class_rank[c];
         // The low field of the sort key: the plane (slow classes; neighbours
         // then share a kernel table) and (t_u, t_v) (exact-footprint classes,
         // whose lanes keep their tap while it stays the same), when the
@@ -1963,9 +1997,17 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
         auto fits = [&](int bits) { return ((unsigned long long)nkeys << bits) < (1ull << 32); };
         if (fits(std::max(pb, tb))) a.plane_bits = std::max(pb, tb);
         else if (tb > 0 && fits(tb)) a.plane_bits = tb;
-        plan_records_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(d_uvw, d_supports, d_pclass, d_putab, a,
-                                                                    recs_tmp, frecs_tmp, keys, vals);
-        PLAN_TRY(hipGetLastError());
+        // one launch per grid, each writing its visibilities' slots
+        for (int g = 0; g < n_grids; ++g) {
+            if (n_vis_of[g] == 0) continue;
+            PlanArgs ag = a;
+            ag.n = n_vis_of[g];
+            ag.grid = g;
+            ag.first = grid_first[g];
+            plan_records_kernel<<<sdp_div_up(ag.n, 256), 256, 0, st>>>(d_uvw_of[g], d_supports, d_pclass, d_putab, ag,
+                                                                       recs_tmp, frecs_tmp, keys, vals);
+            PLAN_TRY(hipGetLastError());
+        }
         int end_bit = 1;
         while (end_bit < 32 && (1ull << end_bit) <= ((unsigned long long)nkeys << a.plane_bits)) ++end_bit;
         size_t temp_bytes = 0;
@@ -2054,9 +2096,11 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     size_t ri = 0;
     for (int c = 0; c < kAllClasses; ++c) {
         plan->class_chunk_begin[c] = (int)chunks.size();
-        for (; ri < runs.size() && (int)(runs[ri].key / (uint32_t)ntiles) == c; ++ri) {
+        for (; class_rank[c] >= 0 && ri < runs.size() && (int)(runs[ri].key / (uint32_t)ntiles_all) == class_rank[c];
+             ++ri) {
             const KeyRun &r = runs[ri];
-            const int t = (int)(r.key % (uint32_t)ntiles), cnt = (int)r.count, first = (int)r.first;
+            // t: the tile over all grids (grid * ntiles + tile)
+            const int t = (int)(r.key % (uint32_t)ntiles_all), cnt = (int)r.count, first = (int)r.first;
             long long pieces = (long long)((r.taps + plan->chunk_taps - 1) / plan->chunk_taps);
             pieces = std::max(1LL, std::min(pieces, (long long)cnt));
             key_tile.push_back(t);
@@ -2129,10 +2173,10 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
             extra += S2;
         }
     }
-    // each tile's keys in class order (runs are class-major), tiles in order
-    std::vector<int> tile_key_begin(ntiles + 1, 0), tile_key_list(nk);
+    // each tile's keys in class order (runs are class-major), tiles (grid-major) in order
+    std::vector<int> tile_key_begin(ntiles_all + 1, 0), tile_key_list(nk);
     for (int k = 0; k < nk; ++k) ++tile_key_begin[key_tile[k] + 1];
-    for (int t = 0; t < ntiles; ++t) {
+    for (int t = 0; t < ntiles_all; ++t) {
         plan->n_tiles_used += tile_key_begin[t + 1] > 0;
         tile_key_begin[t + 1] += tile_key_begin[t];
     }
@@ -2151,8 +2195,8 @@ int sdp_grid_plan_create(sdp_grid_plan **out, const sdp_grid_params *pp, const i
     plan->n_multi = (int)multi.size();
     plan->n_multi2 = (int)multi2.size();
 
-    PLAN_TRY(hipMalloc(&plan->tile_key_begin, sizeof(int) * (ntiles + 1)));
-    PLAN_TRY(hipMemcpyAsync(plan->tile_key_begin, tile_key_begin.data(), sizeof(int) * (ntiles + 1),
+    PLAN_TRY(hipMalloc(&plan->tile_key_begin, sizeof(int) * ((size_t)ntiles_all + 1)));
+    PLAN_TRY(hipMemcpyAsync(plan->tile_key_begin, tile_key_begin.data(), sizeof(int) * ((size_t)ntiles_all + 1),
                             hipMemcpyHostToDevice, st));
     if (plan->n_chunks > 0) {
         PLAN_TRY(hipMalloc(&plan->chunks, sizeof(int4) * plan->n_chunks));
@@ -2290,8 +2334,8 @@ int sdp_grid_merge(const sdp_grid_plan *plan, float *d_grid, sdp_stream s)
         }
     }
     const int R = plan->region;
-    RegionArgs ra{G, plan->tile, plan->ntx, plan->nty, std::max(plan->smax, 1), R};
-    dim3 rg((G + R - 1) / R, (G + R - 1) / R);
+    RegionArgs ra{G, plan->tile, plan->ntx, plan->nty, plan->ntiles, std::max(plan->smax, 1), R};
+    dim3 rg((G + R - 1) / R, (G + R - 1) / R, plan->n_grids);
     // 8 slabs' loads in flight per batch: 16 and 32 measured no faster
     if (plan->region_rg == 32)
         merge_region_kernel<32, 8><<<rg, kMergeThreads, 0, st>>>(plan->key_slabs, plan->key_S, plan->key_off,

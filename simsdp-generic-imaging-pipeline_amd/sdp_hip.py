@@ -147,6 +147,7 @@ def load() -> ctypes.CDLL:
         "sdp_finegrid_simplify": (c_int, [c_int, c_int, c_int, c_double, c_double, P, P, P, P,
                                           ctypes.POINTER(c_int), P]),
         "sdp_grid_plan_create": (c_int, [ctypes.POINTER(c_void_p), P, P, P, c_int, P]),
+        "sdp_grid_plan_create_multi": (c_int, [ctypes.POINTER(c_void_p), P, P, c_int, P, P, P]),
         "sdp_grid_plan_destroy": (c_int, [P]), "sdp_grid_plan_info": (c_int, [P, P]),
         "sdp_grid_plan_to_plan_order": (c_int, [P, P, P, P]),
         "sdp_grid_plan_to_original_order": (c_int, [P, P, P, P]),
@@ -161,6 +162,7 @@ def load() -> ctypes.CDLL:
         "sdp_degrid_nearest": (c_int, [c_int, c_int, c_int, c_double, P, P, P, P]),
         "sdp_fft_work_bytes": (c_size_t, [c_int]),
         "sdp_fft2d": (c_int, [P, P, c_int, c_int, c_int, P, P, P]),
+        "sdp_fft2d_batch": (c_int, [P, P, c_int, c_int, c_int, c_int, P, P, P]),
         "sdp_shift_c2c": (c_int, [c_int, P, P, P]), "sdp_shift_c2r": (c_int, [c_int, P, P, P]),
         "sdp_shift_r2c": (c_int, [c_int, P, P, P]),
         "sdp_convolution_correct": (c_int, [c_int, P, P, P, P]),
@@ -353,6 +355,30 @@ class GridPlan:
         self.ptr = h.value
         self.grid_size = grid_size
         self.n_vis = int(n_vis)
+        self.n_grids = 1
+
+    @classmethod
+    def multi(cls, grid_size, oversampling, supports, uv_scale, w_scale, d_uvws, n_vis, snap_mode=SNAP_STD,
+              tile=0, chunk=0, accumulate=ACC_BLOCKED, stream=None):
+        """One plan over len(d_uvws) grids (sdp_grid_plan_create_multi):
+        visibilities in grid order, grid arguments hold that many grids."""
+        lib = load()
+        check(lib.sdp_init())
+        sup = np.ascontiguousarray(np.asarray(supports, dtype=np.int32).reshape(-1, 2))
+        p = _Params(grid_size, oversampling, sup.shape[0], snap_mode, uv_scale, w_scale, tile, chunk, accumulate)
+        ng = len(d_uvws)
+        ptrs = (c_void_p * ng)(*[_ptr(d) for d in d_uvws])
+        counts = np.ascontiguousarray(np.asarray(n_vis, dtype=np.int32))
+        assert counts.size == ng
+        h = c_void_p()
+        check(lib.sdp_grid_plan_create_multi(ctypes.byref(h), ctypes.byref(p), sup.ctypes.data, ng,
+                                             ctypes.cast(ptrs, c_void_p), counts.ctypes.data, stream))
+        self = cls.__new__(cls)
+        self.ptr = h.value
+        self.grid_size = grid_size
+        self.n_vis = int(counts.sum())
+        self.n_grids = ng
+        return self
 
     def info(self) -> dict:
         a = np.zeros(10, dtype=np.int64)
@@ -416,6 +442,11 @@ def synchronize(stream=None):
 def fft2d(d_in, d_out, grid_size, direction, flags=0, d_taper=None, d_work=None, stream=None):
     check(load().sdp_fft2d(_ptr(d_in), _ptr(d_out), grid_size, direction, flags, _ptr(d_taper), _ptr(d_work),
                            stream))
+
+
+def fft2d_batch(d_in, d_out, grid_size, n_batch, direction, flags=0, d_taper=None, d_work=None, stream=None):
+    check(load().sdp_fft2d_batch(_ptr(d_in), _ptr(d_out), grid_size, n_batch, direction, flags, _ptr(d_taper),
+                                 _ptr(d_work), stream))
 
 
 def fft_work_bytes(grid_size: int) -> int:
@@ -523,7 +554,7 @@ class Batch:
         self.handle = h.value
 
     def residuals(self) -> tuple[float, list]:
-        """One multi-channel pass: (ms, per-stage ms summed over channels)."""
+        """One multi-channel pass: (ms, per-stage ms of the pass when profiling)."""
         ms = c_float()
         st = (c_float * len(STAGES))()
         check(load().sdp_batch_residuals(self.handle, ctypes.byref(ms), st))

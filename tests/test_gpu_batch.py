@@ -99,3 +99,77 @@ def test_batch_rejects_bad_input(hip, mc):
         hip.Batch(mc.grid_size, mc.oversampling, mc.supports, mc.uv_scale, mc.w_scale, [up(hip, d.uvw)],
                   [up(hip, vis)], [0], up(hip, mc.kernels), up(hip, mc.degrid_kernels),
                   up(hip, S.taper_padded(mc.grid_size)))
+
+
+def test_multi_grid_plan_equals_per_grid_plans(hip, mc):
+    """One plan over four channels (sdp_grid_plan_create_multi) against a
+    plan per channel with the same tile and work-item size: the fused
+    degrid + subtract + grid pass, the gridder alone and the degridder
+    alone give every channel's grid and visibilities bit for bit (the
+    channel is only a field of the sort key)."""
+    data = [channel_data(mc, c) for c in range(len(mc.freqs))]
+    G, nc = mc.grid_size, len(data)
+    rng = np.random.default_rng(9)
+    models = rng.normal(size=(nc, G * G, 2)).astype(np.float32)
+    kw = dict(tile=8, chunk=2048)
+    gk, dk = up(hip, mc.kernels), up(hip, mc.degrid_kernels)
+    uvws = [up(hip, d.uvw) for d, _ in data]
+    counts = [d.n_vis for d, _ in data]
+    multi = hip.GridPlan.multi(G, mc.oversampling, mc.supports, mc.uv_scale, mc.w_scale, uvws, counts, **kw)
+    assert multi.info()["chunks"] > 4 * nc
+    n = sum(counts)
+    meas = up(hip, np.concatenate([v for _, v in data]))
+    d_models = up(hip, models)
+    out_grid, out_vis = hip.DeviceBuffer(8 * G * G * nc), hip.DeviceBuffer(8 * n)
+    multi.degrid_grid(dk, gk, d_models, meas, out_grid, d_residual_vis=out_vis)
+    m_grid = out_grid.download((nc, G * G, 2), np.float32)
+    m_vis = out_vis.download((n, 2), np.float32)
+    multi.grid(gk, meas, out_grid)
+    g_grid = out_grid.download((nc, G * G, 2), np.float32)
+    multi.degrid(dk, d_models, out_vis)
+    d_vis = out_vis.download((n, 2), np.float32)
+    first = 0
+    for c, (d, vis) in enumerate(data):
+        one = hip.GridPlan(G, mc.oversampling, mc.supports, mc.uv_scale, mc.w_scale, uvws[c], d.n_vis, **kw)
+        mv, mo = up(hip, vis), up(hip, models[c])
+        og, ov = hip.DeviceBuffer(8 * G * G), hip.DeviceBuffer(8 * d.n_vis)
+        one.degrid_grid(dk, gk, mo, mv, og, d_residual_vis=ov)
+        part = slice(first, first + d.n_vis)
+        assert np.count_nonzero(m_grid[c]) > 1000
+        np.testing.assert_array_equal(og.download((G * G, 2), np.float32), m_grid[c])
+        np.testing.assert_array_equal(ov.download((d.n_vis, 2), np.float32), m_vis[part])
+        one.grid(gk, mv, og)
+        np.testing.assert_array_equal(og.download((G * G, 2), np.float32), g_grid[c])
+        one.degrid(dk, mo, ov)
+        np.testing.assert_array_equal(ov.download((d.n_vis, 2), np.float32), d_vis[part])
+        one.close()
+        first += d.n_vis
+    multi.close()
+
+
+@pytest.mark.parametrize("direction,flags", [("forward", "real_in"), ("inverse", "real_out")])
+def test_fft_batch_equals_one_grid_at_a_time(hip, direction, flags):
+    """sdp_fft2d_batch over three grids equals three sdp_fft2d calls bit for
+    bit, with the major cycle's fused shift / real / taper flags."""
+    G, nb = 256, 3
+    rng = np.random.default_rng(5)
+    d_tap = up(hip, S.taper_padded(G))
+    if flags == "real_in":
+        fl = hip.FFT_REAL_IN | hip.FFT_SHIFT_IN | hip.FFT_SHIFT_OUT | hip.FFT_CORRECT_IN
+        x = rng.normal(size=(nb, G * G)).astype(np.float32)
+        out_shape, out_bytes = (nb, G * G, 2), 8
+    else:
+        fl = hip.FFT_SHIFT_IN | hip.FFT_SHIFT_OUT | hip.FFT_REAL_OUT | hip.FFT_CORRECT_OUT
+        x = rng.normal(size=(nb, G * G, 2)).astype(np.float32)
+        out_shape, out_bytes = (nb, G * G), 4
+    sign = hip.FFT_FORWARD if direction == "forward" else hip.FFT_INVERSE
+    d_in = up(hip, x)
+    d_out = hip.DeviceBuffer(out_bytes * G * G * nb)
+    work = hip.DeviceBuffer(8 * G * G * nb)
+    hip.fft2d_batch(d_in, d_out, G, nb, sign, fl, d_tap, work)
+    got = d_out.download(out_shape, np.float32)
+    for b in range(nb):
+        one_in = up(hip, x[b])
+        one_out = hip.DeviceBuffer(out_bytes * G * G)
+        hip.fft2d(one_in, one_out, G, sign, fl, d_tap, hip.DeviceBuffer(8 * G * G))
+        np.testing.assert_array_equal(one_out.download(out_shape[1:], np.float32), got[b])
