@@ -327,19 +327,43 @@ __global__ void sum_splits_kernel(const float *part, int splits, size_t n, float
     }
 }
 
-// bias gradient: column sums of the output gradient (one workgroup per column)
-__global__ void bias_grad_kernel(const float *gout, int M, int N, float *gb)
+// bias gradient: column sums of the M x N output gradient, in two passes
+// with a fixed order.  Pass 1: workgroup b sums rows [b rows, (b + 1) rows)
+// into part[b][c]; when N divides the workgroup, thread t keeps column
+// t % N and steps kT / N rows at a time, so a wavefront reads whole
+// consecutive rows.  Pass 2: the partials in workgroup order.
+constexpr int kBiasBlocks = 256;
+__global__ void bias_part_kernel(const float *gout, int M, int N, int rows, float *part)
 {
     __shared__ float red[kT];
-    float s = 0.f;
-    for (int m = threadIdx.x; m < M; m += kT) s += gout[(size_t)m * N + blockIdx.x];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = kT / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    const int m0 = blockIdx.x * rows, m1 = min(M, m0 + rows);
+    if (N <= kT && kT % N == 0) {
+        const int c = threadIdx.x % N, step = kT / N;
+        float s = 0.f;
+        for (int m = m0 + threadIdx.x / N; m < m1; m += step) s += gout[(size_t)m * N + c];
+        red[threadIdx.x] = s;
         __syncthreads();
+        if (threadIdx.x < N) {
+            float t = 0.f;
+            for (int j = 0; j < step; ++j) t += red[j * N + threadIdx.x];
+            part[(size_t)blockIdx.x * N + threadIdx.x] = t;
+        }
+    } else {
+        for (int c = threadIdx.x; c < N; c += kT) {
+            float s = 0.f;
+            for (int m = m0; m < m1; ++m) s += gout[(size_t)m * N + c];
+            part[(size_t)blockIdx.x * N + c] = s;
+        }
     }
-    if (threadIdx.x == 0) gb[blockIdx.x] = red[0];
+}
+
+__global__ void bias_sum_kernel(const float *part, int blocks, int N, float *gb)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    float s = 0.f;
+    for (int b = 0; b < blocks; ++b) s += part[(size_t)b * N + c];
+    gb[c] = s;
 }
 
 // 2x2 average pool (NHWC), input H x W -> H/2 x W/2
@@ -649,7 +673,13 @@ int conv_backward(const sdp_cnn *n, int li, int B, int H, int W, const float *in
     const size_t nw = (size_t)g.M * g.N;
     sum_splits_kernel<<<ew_grid(nw), kT, 0, st>>>(n->work, sp, nw, n->grads + L.w_off);
     SDP_LAUNCH_CHECK();
-    bias_grad_kernel<<<L.cout, kT, 0, st>>>(gout, B * H * W, L.cout, n->grads + L.b_off);
+    // the weight partials are consumed: the bias partials reuse the workspace
+    // (at most kBiasBlocks * cout floats; it holds 1024 times the largest
+    // layer's 9 * cin * cout weights)
+    const int M = B * H * W, nb = std::min(kBiasBlocks, (M + 1023) / 1024), rows = (M + nb - 1) / nb;
+    bias_part_kernel<<<nb, kT, 0, st>>>(gout, M, L.cout, rows, n->work);
+    SDP_LAUNCH_CHECK();
+    bias_sum_kernel<<<(L.cout + kT - 1) / kT, kT, 0, st>>>(n->work, nb, L.cout, n->grads + L.b_off);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }
