@@ -159,16 +159,18 @@ def major_cycle(hip, sdp_synth, d, d_uvw, d_meas, d_gk, d_dk, tile, chunk, targe
             "components_last_cycle": n_src}
 
 
-def learned_vs_hogbom(hip, steps, seed=5):
+def learned_vs_hogbom(hip, nets, seed=5):
     """Major-cycle time to a fixed residual RMS, learned proposal + Hogbom
     against plain Hogbom, on the learned-proposal workload: config-1 array
     (516,096 visibilities, 1024^2 grid, OS 16, no w-term) observing 50
     power-law point sources and 20 elliptical Gaussians (FWHM 2-30 px) in a
     2 x 2 degree box, noise 0.01 Jy.  The plain path runs 5 major cycles x
     1000 Hogbom iterations (gain 0.1); its final residual RMS is the target.
-    The network (3 levels, 16 base channels, BF16 MFMA) is trained here by
-    the library's kernels on pairs made from the pipeline's own PSF; the
-    training time is reported, not counted in the major-cycle time."""
+    Each network of `nets` (dicts: levels, base channels, training pairs,
+    steps and optionally head_scale and lr; BF16 MFMA, FP32 accumulation)
+    is trained here by the library's kernels
+    on pairs the GPU makes from the pipeline's own PSF (make_pool_device);
+    the training time is reported, not counted in the major-cycle time."""
     import numpy as np
     import sdp_learned as L
 
@@ -182,6 +184,9 @@ def learned_vs_hogbom(hip, steps, seed=5):
         hip.synchronize()
         return ms, rows, (time.perf_counter() - t0) * 1e3
 
+    def fmt(rows):
+        return [[float(f"{a:.5g}"), float(f"{b:.5g}"), round(ms, 3)] for a, b, ms in rows]
+
     plain = w.pipeline(1000, flags=hip.PIPE_GRAPH)
     psf = plain.image(hip.IMAGE_PSF)
     _, hog_rows, hog_wall = run(plain, 0.0, 5)
@@ -192,29 +197,7 @@ def learned_vs_hogbom(hip, steps, seed=5):
     p300 = w.pipeline(300, flags=hip.PIPE_GRAPH)
     h300_ms, h300_rows, _ = run(p300, target, 40)
     p300.close()
-    t = time.perf_counter()
-    pool = L.make_pool(psf, 3072, seed=1, T=T, crops_per_sky=8, global_scale=True,
-                       crop_range=(G // 2 - T - 64, G // 2 + 64), n_points=(20, 80), n_gauss=(0, 20), box=w.box)
-    pool_s = time.perf_counter() - t
-    net = hip.Cnn(3, 16, T, T, 16, precision=hip.CNN_BF16)
-    L.init_params(net, 0)
-    t = time.perf_counter()
-    losses = L.train(net, pool, steps, 16, lr=3e-4, lr_final=3e-5, data_weight=1.0, psf=L.psf_patch(psf, T))
-    hip.synchronize()
-    train_s = time.perf_counter() - t
-    log(f"network trained: {steps} steps in {train_s:.1f} s, loss {losses[0]:.3g} -> {np.mean(losses[-50:]):.3g}")
-    lp = w.pipeline(300)
-    lp.set_learned(net, T, margin, gain=1.0, threshold=0.05, floor=0.0, max_cycles=3,
-                   scale=hip.LEARNED_GLOBAL_SCALE)
-    l_ms, l_rows, l_wall = run(lp, target, 14)
-    lp.close()
-    net.close()
-    log(f"learned + Hogbom: {'not reached' if l_ms is None else f'{l_ms:.1f} ms'} in {len(l_rows)} cycles")
-
-    def fmt(rows):
-        return [[float(f"{a:.5g}"), float(f"{b:.5g}"), round(ms, 3)] for a, b, ms in rows]
-
-    return {
+    out = {
         "workload": f"config-1 array ({w.ds.n_vis} vis), {G}^2 grid, OS 16, no w-term; sky of 50 power-law points "
                     f"+ 20 elliptical Gaussians FWHM 2-30 px in a 2 deg box (seed {seed}), noise 0.01 Jy",
         "target_rms": float(f"{target:.6g}"), "target": "residual RMS after 5 x 1000 Hogbom iterations",
@@ -222,16 +205,50 @@ def learned_vs_hogbom(hip, steps, seed=5):
                    "cycles": fmt(hog_rows)},
         "hogbom_n300": {"n_minor": 300, "ms": None if h300_ms is None else round(h300_ms, 3),
                         "n_cycles": len(h300_rows)},
-        "learned": {"n_minor": 300, "learned_cycles": 3, "gain": 1.0, "threshold": 0.05, "patch": T,
-                    "margin": margin, "scale": "global",
-                    "ms": None if l_ms is None else round(l_ms, 3), "wall_ms": round(l_wall, 3),
-                    "cycles": fmt(l_rows)},
-        "ratio": None if l_ms is None else round(l_ms / hog_ms, 4),
-        "net": {"levels": 3, "base": 16, "params": int(net_params(hip, 3, 16)), "precision": "bf16 MFMA, fp32 acc"},
-        "train": {"pairs": int(pool.inputs.shape[0]), "steps": steps, "batch": 16, "s": round(train_s, 2),
-                  "pool_s": round(pool_s, 2), "loss_first": float(f"{losses[0]:.4g}"),
-                  "loss_last50": float(f"{np.mean(losses[-50:]):.4g}")},
+        "networks": [],
     }
+    for spec in nets:
+        levels, base, n_pairs, steps = spec["levels"], spec["base"], spec["pairs"], spec["steps"]
+        head, lr = spec.get("head_scale", 1.0), spec.get("lr", 3e-4)
+        t = time.perf_counter()
+        pool = L.make_pool_device(psf, n_pairs, seed=1, T=T, crops_per_sky=8, with_dirty=True,
+                                  crop_range=(G // 2 - T - 64, G // 2 + 64), n_points=(20, 80), n_gauss=(0, 20),
+                                  box=w.box)
+        pool_s = time.perf_counter() - t
+        net = hip.Cnn(levels, base, T, T, 16, precision=hip.CNN_BF16)
+        L.init_params(net, 0, head_scale=head)
+        t = time.perf_counter()
+        losses = L.train(net, pool, steps, 16, lr=lr, lr_final=lr / 10, data_weight=1.0, psf=L.psf_patch(psf, T))
+        hip.synchronize()
+        train_s = time.perf_counter() - t
+        for b in (pool.d_x, pool.d_t, pool.d_d):
+            b.close()
+        log(f"network {levels}x{base}: {n_pairs} pairs in {pool_s:.1f} s, {steps} steps in {train_s:.1f} s, "
+            f"loss {losses[0]:.3g} -> {np.mean(losses[-50:]):.3g}")
+        lp = w.pipeline(300)
+        lp.set_learned(net, T, margin, gain=1.0, threshold=0.05, floor=0.0, max_cycles=3,
+                       scale=hip.LEARNED_GLOBAL_SCALE)
+        l_ms, l_rows, l_wall = run(lp, target, 14)
+        lp.close()
+        net.close()
+        log(f"learned ({levels}x{base}) + Hogbom: {'not reached' if l_ms is None else f'{l_ms:.1f} ms'} "
+            f"in {len(l_rows)} cycles")
+        out["networks"].append({
+            "net": {"levels": levels, "base": base, "params": int(net_params(hip, levels, base)),
+                    "precision": "bf16 MFMA, fp32 acc"},
+            "train": {"pairs": n_pairs, "steps": steps, "batch": 16, "lr": lr, "head_scale": head,
+                      "s": round(train_s, 2),
+                      "pool_s": round(pool_s, 2), "loss_first": float(f"{losses[0]:.4g}"),
+                      "loss_last50": float(f"{np.mean(losses[-50:]):.4g}")},
+            "learned": {"n_minor": 300, "learned_cycles": 3, "gain": 1.0, "threshold": 0.05, "patch": T,
+                        "margin": margin, "scale": "global",
+                        "ms": None if l_ms is None else round(l_ms, 3), "wall_ms": round(l_wall, 3),
+                        "cycles": fmt(l_rows)},
+            "ratio": None if l_ms is None else round(l_ms / hog_ms, 4),
+        })
+    # the configured network (the last) gives the headline ratio
+    out["ratio"] = out["networks"][-1]["ratio"] if out["networks"] else None
+    return out
 
 
 def dft_precision_variant(hip, d, n_src=4096):
@@ -377,7 +394,10 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-Hogbom major-cycle comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-channel workload")
-    ap.add_argument("--learned-steps", type=int, default=2000, help="training steps of the proposal network")
+    ap.add_argument("--learned-steps", type=int, default=2000, help="training steps of the 3x16 proposal network")
+    ap.add_argument("--learned-steps-4x32", type=int, default=3000,
+                    help="training steps of the configured 4-level, 32-channel proposal network")
+    ap.add_argument("--learned-pairs", type=int, default=49152, help="training pairs of the 4x32 network")
     ap.add_argument("--accumulate", choices=["blocked", "compensated"], default="blocked",
                     help="gridder accumulation inside a work item: float (blocked) or TwoSum; "
                          "work items are merged TwoSum-compensated either way")
@@ -559,7 +579,10 @@ def main():
             extras["batched_channels"] = batched_channels(hip)
         if not args.no_learned:
             log("learned vs Hogbom ...")
-            extras["learned_vs_hogbom"] = learned_vs_hogbom(hip, args.learned_steps)
+            extras["learned_vs_hogbom"] = learned_vs_hogbom(
+                hip, [dict(levels=3, base=16, pairs=3072, steps=args.learned_steps, head_scale=0.0),
+                      dict(levels=4, base=32, pairs=args.learned_pairs, steps=args.learned_steps_4x32,
+                           head_scale=0.0)])
 
     copy_gbs = copy_bandwidth(hip)
     out = {

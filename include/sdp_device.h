@@ -383,6 +383,23 @@ int sdp_cnn_gather_patches(int grid_size, int patch, int n, const int *d_origins
 int sdp_cnn_scatter_add(int grid_size, int patch, int margin, int n, const int *d_origins, const float *d_peaks,
                         const float *d_out, float scale, float floor, const float *d_batch_in, float in_floor,
                         float *d_model, sdp_stream s);
+/* sdp_cnn_crop_patches: one channel, out[p][i][j] = image[y0+i][x0+j] / peak[p]
+ * (zero outside the image): the training target and the dirty patch of a
+ * pair, beside sdp_cnn_gather_patches's network input. */
+int sdp_cnn_crop_patches(int grid_size, int patch, int n, const int *d_origins, const float *d_peaks,
+                         const float *d_image, float *d_out, sdp_stream s);
+/* A G x G sky image (overwritten) for training pairs: n_points point sources
+ * (d_points: y, x, flux doubles; pixel row and column) and n_gauss
+ * elliptical Gaussians (d_gauss: cy, cx, fwhm_a, fwhm_b, theta, peak
+ * doubles), each Gaussian over the square stamp of half side
+ * ceil(3 max(fwhm_a, fwhm_b)) pixels around (floor cy, floor cx), with
+ * peak exp(-(u^2 / (fwhm_a^2 k) + v^2 / (fwhm_b^2 k)) / 2), k = 1 / (8 ln 2),
+ * (u, v) the pixel offset rotated by theta.  Each pixel is summed in double
+ * (points first, then Gaussians, each in order) and rounded once.  At most
+ * 1024 points and 512 Gaussians.  No reference counterpart: it serves the
+ * learned proposal's training. */
+int sdp_sky_render(int grid_size, int n_points, const double *d_points, int n_gauss, const double *d_gauss,
+                   float *d_sky, sdp_stream s);
 
 #ifdef __cplusplus
 }

@@ -508,6 +508,56 @@ __global__ void gather_patches_kernel(int G, int T, int n, const int2 *origins, 
     }
 }
 
+// one channel of patches: out[p][i][j] = img[y0 + i][x0 + j] / peak[p]
+// (0 outside the image)
+__global__ void crop_patches_kernel(int G, int T, int n, const int2 *origins, const float *img, const float *peak,
+                                    float *out)
+{
+    const size_t total = (size_t)n * T * T;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const int x = i % T, y = (i / T) % T, p = (int)(i / ((size_t)T * T));
+        const int2 o = origins[p];
+        const int gy = o.y + y, gx = o.x + x;
+        out[i] = (gy >= 0 && gy < G && gx >= 0 && gx < G) ? img[(size_t)gy * G + gx] / peak[p] : 0.f;
+    }
+}
+
+// A sky image for training pairs: per pixel, in double, the point sources
+// (y, x, flux) at it in order, then each elliptical Gaussian (cy, cx,
+// fwhm_a, fwhm_b, theta, peak) whose stamp -- the square of half side
+// r = ceil(3 max(fwhm_a, fwhm_b)) around (floor cy, floor cx) -- holds it:
+// peak exp(-(u^2 / (fwhm_a^2 k) + v^2 / (fwhm_b^2 k)) / 2), k = 1 / (8 ln 2),
+// (u, v) the offset rotated by theta.  The parameters sit in LDS.
+constexpr int kSkyMax = 1024;   // points + Gaussians x 6 doubles per launch
+__global__ void sky_render_kernel(int G, int n_points, const double *points, int n_gauss, const double *gauss,
+                                  float *sky)
+{
+    __shared__ double sp[3 * kSkyMax], sg[6 * kSkyMax / 2];
+    for (int i = threadIdx.x; i < 3 * n_points; i += blockDim.x) sp[i] = points[i];
+    for (int i = threadIdx.x; i < 6 * n_gauss; i += blockDim.x) sg[i] = gauss[i];
+    __syncthreads();
+    const double k = 1.0 / (8.0 * log(2.0));
+    const size_t npix = (size_t)G * G;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += (size_t)gridDim.x * blockDim.x) {
+        const int y = (int)(i / G), x = (int)(i % G);
+        double v = 0.0;
+        for (int q = 0; q < n_points; ++q)
+            if ((int)sp[3 * q] == y && (int)sp[3 * q + 1] == x) v += sp[3 * q + 2];
+        for (int q = 0; q < n_gauss; ++q) {
+            const double *g = sg + 6 * q;
+            const double cy = g[0], cx = g[1], fa = g[2], fb = g[3], th = g[4];
+            const int r = (int)ceil(3.0 * fmax(fa, fb));
+            const int iy = (int)cy, ix = (int)cx;
+            if (y < iy - r || y > iy + r || x < ix - r || x > ix + r) continue;
+            const double dy = (double)y - cy, dx = (double)x - cx;
+            const double u = dx * cos(th) + dy * sin(th);
+            const double w = -dx * sin(th) + dy * cos(th);
+            v += g[5] * exp(-0.5 * (u * u / (fa * fa * k) + w * w / (fb * fb * k)));
+        }
+        sky[i] = (float)v;
+    }
+}
+
 // model[y0 + i][x0 + j] += scale * peak[p] * out[p][i][j] over each patch's
 // centre [margin, T - margin)^2 (centres of distinct patches do not overlap),
 // for outputs above floor only
@@ -934,6 +984,31 @@ int sdp_cnn_gather_patches(int G, int T, int n, const int *d_origins, const floa
     if (n == 0) return SDP_OK;
     gather_patches_kernel<<<ew_grid((size_t)n * T * T), kT, 0, sdp::stream_of(s)>>>(
         G, T, n, (const int2 *)d_origins, d_residual, d_peaks, d_psf, d_batch);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_cnn_crop_patches(int G, int T, int n, const int *d_origins, const float *d_peaks, const float *d_image,
+                         float *d_out, sdp_stream s)
+{
+    SDP_REQUIRE(G >= 1 && T >= 1 && n >= 0 && d_origins && d_peaks && d_image && d_out,
+                "sdp_cnn_crop_patches: bad argument");
+    if (n == 0) return SDP_OK;
+    crop_patches_kernel<<<ew_grid((size_t)n * T * T), kT, 0, sdp::stream_of(s)>>>(
+        G, T, n, (const int2 *)d_origins, d_image, d_peaks, d_out);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int sdp_sky_render(int G, int n_points, const double *d_points, int n_gauss, const double *d_gauss, float *d_sky,
+                   sdp_stream s)
+{
+    SDP_REQUIRE(G >= 1 && n_points >= 0 && n_gauss >= 0 && d_sky && (n_points == 0 || d_points) &&
+                    (n_gauss == 0 || d_gauss), "sdp_sky_render: bad argument");
+    SDP_REQUIRE(n_points <= kSkyMax && n_gauss <= kSkyMax / 2, "sdp_sky_render: at most %d points and %d Gaussians",
+                kSkyMax, kSkyMax / 2);
+    sky_render_kernel<<<ew_grid((size_t)G * G), kT, 0, sdp::stream_of(s)>>>(G, n_points, d_points, n_gauss, d_gauss,
+                                                                          d_sky);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }

@@ -217,6 +217,8 @@ def load() -> ctypes.CDLL:
         "sdp_cnn_patch_peaks": (c_int, [c_int, c_int, c_int, P, P, ctypes.POINTER(c_int), P]),
         "sdp_cnn_gather_patches": (c_int, [c_int, c_int, c_int, P, P, P, P, P, P]),
         "sdp_cnn_scatter_add": (c_int, [c_int, c_int, c_int, c_int, P, P, P, c_float, c_float, P, c_float, P, P]),
+        "sdp_cnn_crop_patches": (c_int, [c_int, c_int, c_int, P, P, P, P, P]),
+        "sdp_sky_render": (c_int, [c_int, c_int, P, c_int, P, P, P]),
         "sdp_complex_multiply": (c_int, [c_size_t, P, P, c_size_t, P, P]),
         "sdp_axpby": (c_int, [c_size_t, c_float, P, c_float, P, P, P]),
         "sdp_pipeline_set_learned": (c_int, [P, P, P]),
@@ -767,6 +769,20 @@ def axpby(n, a, d_x, b, d_y, d_out, stream=None):
 
 def complex_multiply(n, d_a, d_b, nb, d_out, stream=None):
     check(load().sdp_complex_multiply(n, _ptr(d_a), _ptr(d_b), nb, _ptr(d_out), stream))
+
+
+def cnn_gather_patches(grid_size, patch, n, d_origins, d_peaks, d_image, d_psf, d_batch, stream=None):
+    check(load().sdp_cnn_gather_patches(grid_size, patch, n, _ptr(d_origins), _ptr(d_peaks), _ptr(d_image),
+                                        _ptr(d_psf), _ptr(d_batch), stream))
+
+
+def cnn_crop_patches(grid_size, patch, n, d_origins, d_peaks, d_image, d_out, stream=None):
+    check(load().sdp_cnn_crop_patches(grid_size, patch, n, _ptr(d_origins), _ptr(d_peaks), _ptr(d_image),
+                                      _ptr(d_out), stream))
+
+
+def sky_render(grid_size, d_points, n_points, d_gauss, n_gauss, d_sky, stream=None):
+    check(load().sdp_sky_render(grid_size, n_points, _ptr(d_points), n_gauss, _ptr(d_gauss), _ptr(d_sky), stream))
 
 
 def cnn_mse(count, d_out, d_target, d_grad, stream=None) -> float:

@@ -28,14 +28,15 @@ def psf_patch(psf_image: np.ndarray, T: int) -> np.ndarray:
     return np.ascontiguousarray(psf_image[c:c + T, c:c + T], dtype=np.float32)
 
 
-def random_sky(T: int, rng: np.random.Generator, n_points=(1, 30), n_gauss=(0, 20), fwhm=(2.0, 30.0),
-               flux=(1e-3, 1.0), index=-1.5, margin=8, box: int | None = None) -> np.ndarray:
-    """Point sources with dN/dS ~ S^index on [flux] plus elliptical Gaussians
-    (FWHM per axis uniform in [fwhm], random orientation, peak 0.2 times a
-    flux drawn the same way), centred `margin` pixels away from the edges of
-    the T x T sky, or, with `box`, within `box` pixels of its centre pixel
-    (T/2, T/2) on either axis."""
-    sky = np.zeros((T, T), np.float64)
+def sky_components(T: int, rng: np.random.Generator, n_points=(1, 30), n_gauss=(0, 20), fwhm=(2.0, 30.0),
+                   flux=(1e-3, 1.0), index=-1.5, margin=8, box: int | None = None) -> tuple[np.ndarray, np.ndarray]:
+    """The random components of a T x T sky: point sources with
+    dN/dS ~ S^index on [flux] and elliptical Gaussians (FWHM per axis
+    uniform in [fwhm], random orientation, peak 0.2 times a flux drawn the
+    same way), centred `margin` pixels away from the edges, or, with `box`,
+    within `box` pixels of the centre pixel (T/2, T/2) on either axis.
+    Returns (points: n x (y, x, flux), gaussians: m x (cy, cx, fwhm_a,
+    fwhm_b, theta, peak)), float64."""
     a = index + 1.0
     lo, hi = (margin, T - margin) if box is None else (T // 2 - box, T // 2 + box)
 
@@ -46,11 +47,24 @@ def random_sky(T: int, rng: np.random.Generator, n_points=(1, 30), n_gauss=(0, 2
     n = int(rng.integers(n_points[0], n_points[1] + 1))
     ys = rng.integers(lo, hi, n)
     xs = rng.integers(lo, hi, n)
-    np.add.at(sky, (ys, xs), draw(n))
+    points = np.stack([ys, xs, draw(n)], axis=1).astype(np.float64)
     ng = int(rng.integers(n_gauss[0], n_gauss[1] + 1))
+    f = draw(ng)
+    cy, cx = rng.uniform(lo, hi, ng), rng.uniform(lo, hi, ng)
+    fa, fb = rng.uniform(*fwhm, ng), rng.uniform(*fwhm, ng)
+    th = rng.uniform(0, np.pi, ng)
+    gauss = np.stack([cy, cx, fa, fb, th, 0.2 * f], axis=1).astype(np.float64)
+    return points.reshape(-1, 3), gauss.reshape(-1, 6)
+
+
+def render_sky(T: int, points: np.ndarray, gauss: np.ndarray) -> np.ndarray:
+    """The T x T sky (float32) of sky_components' output, summed in double:
+    the points, then each Gaussian over the square stamp of half side
+    ceil(3 max FWHM) around its (floor) centre pixel."""
+    sky = np.zeros((T, T), np.float64)
+    np.add.at(sky, (points[:, 0].astype(int), points[:, 1].astype(int)), points[:, 2])
     k = 1.0 / (8.0 * math.log(2.0))   # sigma^2 = FWHM^2 k
-    for f, cy, cx, fa, fb, th in zip(draw(ng), rng.uniform(lo, hi, ng), rng.uniform(lo, hi, ng),
-                                     rng.uniform(*fwhm, ng), rng.uniform(*fwhm, ng), rng.uniform(0, np.pi, ng)):
+    for cy, cx, fa, fb, th, peak in gauss:
         r = int(math.ceil(3.0 * max(fa, fb)))   # stamp to 3 FWHM
         y0, y1 = max(0, int(cy) - r), min(T, int(cy) + r + 1)
         x0, x1 = max(0, int(cx) - r), min(T, int(cx) + r + 1)
@@ -58,17 +72,28 @@ def random_sky(T: int, rng: np.random.Generator, n_points=(1, 30), n_gauss=(0, 2
         dy, dx = yy - cy, xx - cx
         u = dx * math.cos(th) + dy * math.sin(th)
         v = -dx * math.sin(th) + dy * math.cos(th)
-        sky[y0:y1, x0:x1] += 0.2 * f * np.exp(-0.5 * (u * u / (fa * fa * k) + v * v / (fb * fb * k)))
+        sky[y0:y1, x0:x1] += peak * np.exp(-0.5 * (u * u / (fa * fa * k) + v * v / (fb * fb * k)))
     return sky.astype(np.float32)
 
 
-def init_params(net: "hip.Cnn", seed: int) -> np.ndarray:
+def random_sky(T: int, rng: np.random.Generator, **kw) -> np.ndarray:
+    """render_sky of sky_components (same keywords)."""
+    return render_sky(T, *sky_components(T, rng, **kw))
+
+
+def init_params(net: "hip.Cnn", seed: int, head_scale: float = 1.0) -> np.ndarray:
     """He-normal weights (standard deviation sqrt(2 / (9 cin))), zero biases,
-    as one parameter vector in the network's layer order."""
+    as one parameter vector in the network's layer order; the output
+    layer's weights are scaled by head_scale (0: the untrained network
+    proposes nothing)."""
     rng = np.random.default_rng(seed)
     parts = []
-    for name, ci, co in hip.cnn_layer_shapes(net.levels, net.base, net.in_ch, net.out_ch):
-        parts.append((rng.standard_normal(9 * ci * co) * math.sqrt(2.0 / (9 * ci))).astype(np.float32))
+    shapes = hip.cnn_layer_shapes(net.levels, net.base, net.in_ch, net.out_ch)
+    for li, (name, ci, co) in enumerate(shapes):
+        w = rng.standard_normal(9 * ci * co) * math.sqrt(2.0 / (9 * ci))
+        if li == len(shapes) - 1:
+            w = w * head_scale
+        parts.append(w.astype(np.float32))
         parts.append(np.zeros(co, np.float32))
     flat = np.concatenate(parts)
     net.set_params(flat)
@@ -94,10 +119,15 @@ class Convolver:
     def __call__(self, sky: np.ndarray) -> np.ndarray:
         T = self.T
         self.d_in.upload(np.ascontiguousarray(sky, np.float32))
+        self.run()
+        return self.d_out.download((T, T), np.float32)
+
+    def run(self):
+        """d_out = the convolution of d_in (both device buffers of T x T floats)."""
+        T = self.T
         hip.fft2d(self.d_in, self.d_c, T, hip.FFT_FORWARD, hip.FFT_REAL_IN, None, self.d_work)
         hip.complex_multiply(T * T, self.d_c, self.d_ft, T * T, self.d_c)
         hip.fft2d(self.d_c, self.d_out, T, hip.FFT_INVERSE, hip.FFT_REAL_OUT, None, self.d_work)
-        return self.d_out.download((T, T), np.float32)
 
 
 class DataTerm:
@@ -197,7 +227,70 @@ def make_pool(psf: np.ndarray, n: int, seed: int, T: int | None = None, crops_pe
     return Pool(x, y)
 
 
-def train(net: "hip.Cnn", pool: Pool, steps: int, batch: int, lr: float = 1e-3, seed: int = 0,
+@dataclass
+class DevicePool:
+    """Training pairs in device memory: inputs d_x (n, T, T, 2), targets
+    d_t (n, T, T) and, when made with_dirty, the scaled dirty patches d_d
+    (n, T, T) the data-domain loss compares against."""
+    n: int
+    T: int
+    d_x: "hip.DeviceBuffer"
+    d_t: "hip.DeviceBuffer"
+    d_d: "hip.DeviceBuffer | None"
+
+    def download(self) -> Pool:
+        T = self.T
+        return Pool(self.d_x.download((self.n, T, T, 2), np.float32),
+                    self.d_t.download((self.n, T, T, 1), np.float32))
+
+
+def make_pool_device(psf: np.ndarray, n: int, seed: int, T: int, crops_per_sky: int = 4,
+                     crop_range: tuple[int, int] | None = None, with_dirty: bool = False, **sky_kw) -> DevicePool:
+    """make_pool with global_scale and no target smoothing, built on the
+    GPU: each sky's components are drawn on the host (the same draws, in
+    the same order, as make_pool's), rendered by sdp_sky_render, convolved
+    with the PSF by the library's FFT, and its crops cut by
+    sdp_cnn_gather_patches / sdp_cnn_crop_patches straight into the
+    device pool; nothing of size G^2 or n T^2 passes through the host."""
+    G = psf.shape[0]
+    lo, hi = crop_range if crop_range is not None else (0, G - T + 1)
+    if lo < 0 or hi > G - T + 1 or lo >= hi:
+        raise ValueError("crop_range outside the field")
+    conv = Convolver(psf)
+    rng = np.random.default_rng(seed)
+    d_psf = hip.DeviceBuffer.from_array(np.ascontiguousarray(psf, np.float32))
+    pool = DevicePool(n, T, hip.DeviceBuffer(8 * n * T * T), hip.DeviceBuffer(4 * n * T * T),
+                      hip.DeviceBuffer(4 * n * T * T) if with_dirty else None)
+    d_pts, d_gs = hip.DeviceBuffer(8 * 3 * 1024), hip.DeviceBuffer(8 * 6 * 512)
+    d_orig = hip.DeviceBuffer(8 * crops_per_sky)
+    d_peak = hip.DeviceBuffer(4 * crops_per_sky)
+    i = 0
+    while i < n:
+        pts, gs = sky_components(G, rng, **sky_kw)
+        if len(pts):
+            d_pts.upload(np.ascontiguousarray(pts))
+        if len(gs):
+            d_gs.upload(np.ascontiguousarray(gs))
+        hip.sky_render(G, d_pts, len(pts), d_gs, len(gs), conv.d_in)
+        conv.run()
+        s = hip.image_stats(G * G, conv.d_out)[2] or 1.0
+        k = min(crops_per_sky, n - i)
+        orig = np.empty((k, 2), np.int32)
+        for j in range(k):
+            oy, ox = (int(v) for v in rng.integers(lo, hi, 2))
+            orig[j] = (ox, oy)
+        d_orig.upload(orig)
+        d_peak.upload(np.full(k, s, np.float32))
+        hip.cnn_gather_patches(G, T, k, d_orig, d_peak, conv.d_out, d_psf, pool.d_x.ptr + 8 * i * T * T)
+        hip.cnn_crop_patches(G, T, k, d_orig, d_peak, conv.d_in, pool.d_t.ptr + 4 * i * T * T)
+        if with_dirty:
+            hip.cnn_crop_patches(G, T, k, d_orig, d_peak, conv.d_out, pool.d_d.ptr + 4 * i * T * T)
+        i += k
+    hip.synchronize()
+    return pool
+
+
+def train(net: "hip.Cnn", pool: "Pool | DevicePool", steps: int, batch: int, lr: float = 1e-3, seed: int = 0,
           lr_final: float = 1e-4, log_every: int = 0, sky_weight: float = 1.0, data_weight: float = 0.0,
           psf: np.ndarray | None = None) -> list[float]:
     """Adam on batches drawn from the pool; learning rate decays
@@ -205,12 +298,19 @@ def train(net: "hip.Cnn", pool: Pool, steps: int, batch: int, lr: float = 1e-3, 
     mean square error against the true sky plus data_weight times the mean
     square of the proposal's data-domain residual (DataTerm, needs the PSF
     patch).  Returns the loss per step."""
-    n, T = pool.inputs.shape[0], pool.inputs.shape[1]
     rng = np.random.default_rng(seed)
     # the pool stays in device memory; batches are gathered there
-    d_pool_x = hip.DeviceBuffer.from_array(pool.inputs)
-    d_pool_t = hip.DeviceBuffer.from_array(pool.targets)
-    d_pool_d = hip.DeviceBuffer.from_array(np.ascontiguousarray(pool.inputs[..., 0])) if data_weight > 0 else None
+    if isinstance(pool, DevicePool):
+        n, T = pool.n, pool.T
+        d_pool_x, d_pool_t, d_pool_d = pool.d_x, pool.d_t, pool.d_d
+        if data_weight > 0 and d_pool_d is None:
+            raise ValueError("the data-domain loss needs a pool made with_dirty")
+    else:
+        n, T = pool.inputs.shape[0], pool.inputs.shape[1]
+        d_pool_x = hip.DeviceBuffer.from_array(pool.inputs)
+        d_pool_t = hip.DeviceBuffer.from_array(pool.targets)
+        d_pool_d = (hip.DeviceBuffer.from_array(np.ascontiguousarray(pool.inputs[..., 0])) if data_weight > 0
+                    else None)
     d_x = hip.DeviceBuffer(4 * batch * T * T * 2)
     d_t = hip.DeviceBuffer(4 * batch * T * T)
     d_y = hip.DeviceBuffer(4 * batch * T * T)

@@ -142,3 +142,39 @@ def test_trained_network_gains_over_hogbom(hip, small_workload):
     plain.close()
     lp.close()
     net.close()
+
+
+def test_sky_render_matches_the_host_sky(hip):
+    """sdp_sky_render against render_sky (numpy, double) for seeded skies
+    with stamps clipped at the field edge: equal to 1e-6 of the peak (the
+    two differ only by the last bits of exp, sin and cos in double)."""
+    import sdp_learned as L
+    G = 192
+    rng = np.random.default_rng(7)
+    d_sky = hip.DeviceBuffer(4 * G * G)
+    for _ in range(4):
+        pts, gs = L.sky_components(G, rng, n_points=(5, 40), n_gauss=(3, 12), margin=2)
+        hip.sky_render(G, up(hip, pts, np.float64), len(pts), up(hip, gs, np.float64), len(gs), d_sky)
+        got = d_sky.download((G, G), np.float32)
+        want = L.render_sky(G, pts, gs)
+        assert np.count_nonzero(want) > 1000
+        assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max()
+
+
+def test_device_pool_matches_the_host_pool(hip, rng):
+    """make_pool_device against make_pool (global scale) from the same seed:
+    the same pairs, inputs, targets and dirty patches, to 1e-6 of each
+    patch's scale."""
+    import sdp_learned as L
+    G, T = 256, 64
+    psf = L.smooth(np.pad(np.ones((1, 1), np.float32), ((G // 2, G // 2 - 1), (G // 2, G // 2 - 1))), 3.0)
+    psf = (psf / psf.max()).astype(np.float32)
+    kw = dict(crops_per_sky=4, crop_range=(G // 2 - T - 16, G // 2 + 16), n_points=(5, 20), n_gauss=(0, 8), box=40)
+    host = L.make_pool(psf, 22, seed=3, T=T, global_scale=True, **kw)
+    dev = L.make_pool_device(psf, 22, seed=3, T=T, with_dirty=True, **kw)
+    got = dev.download()
+    d = dev.d_d.download((22, T, T), np.float32)
+    assert np.abs(got.inputs - host.inputs).max() <= 1e-6
+    assert np.abs(got.targets - host.targets).max() <= 1e-6
+    np.testing.assert_array_equal(d, got.inputs[..., 0])
+    assert np.count_nonzero(host.targets) > 100 and np.abs(host.inputs[..., 0]).max() <= 1.0
