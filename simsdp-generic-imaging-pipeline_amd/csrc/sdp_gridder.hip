@@ -64,6 +64,7 @@
 #include <type_traits>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "sdp_internal.h"
@@ -282,12 +283,6 @@ __global__ void run_end_kernel(const uint32_t *keys, const uint32_t *flags, cons
     r->taps = tap_scan[i] - (f > 0 ? tap_scan[f - 1] : 0ull);
 }
 
-// the exact-footprint records follow the same permutation
-__global__ void gather_frecs_kernel(const uint2 *frecs, const uint32_t *order, int n, uint2 *out)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = frecs[order[i]];
-}
 
 // Runs of records (sorted order) with one (class, tile) key and one
 // (t_u, t_v, conj) field: each record's bits 11-31 receive the records left
@@ -321,10 +316,15 @@ __global__ void fast_run_rem_kernel(const uint32_t *run_id, const uint32_t *run_
     frecs[i].y = (frecs[i].y & kRunField) | rem << 11;
 }
 
-__global__ void gather_records_kernel(const VisRec *recs, const uint32_t *order, int n, VisRec *out)
+// the records and the exact-footprint records in sorted order
+__global__ void gather_records_kernel(const VisRec *recs, const uint2 *frecs, const uint32_t *order, int n,
+                                      VisRec *out, uint2 *fout)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = recs[order[i]];
+    if (i >= n) return;
+    const uint32_t o = order[i];
+    out[i] = recs[o];
+    fout[i] = frecs[o];
 }
 
 __global__ void permute_kernel(const float2 *in, const uint32_t *perm, int n, float2 *out, int to_plan)
@@ -939,10 +939,17 @@ __device__ __forceinline__ void fast_run(Acc &run, __amdgpu_buffer_rsrc_t rs, un
 // wavefronts share a workgroup (no barriers; each its own LDS cells), so the
 // CU's workgroup slots do not cap the resident wavefronts.
 constexpr int kFastWaves = 4;
+// The items of the exact-footprint classes, class by class: item i of
+// segment s (item_begin[s] <= i < item_begin[s + 1]) is part
+// (i - item_begin[s]) % parts[s] of chunk chunk_begin[s] + (i - item_begin[s]) / parts[s].
+struct FastItems {
+    int n_seg, n_items;
+    int item_begin[kFastMaxH + 1], chunk_begin[kFastMaxH], parts[kFastMaxH];
+};
 template <bool COMP, bool PERM>
 __global__ void __launch_bounds__(kWave * kFastWaves)
 fast_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restrict__ perm, const int4 *__restrict__ chunks,
-                 const int2 *__restrict__ items, const long long *__restrict__ slab_off,
+                 FastItems fi, const long long *__restrict__ slab_off,
                  const float2 *__restrict__ vis, const float2 *__restrict__ utab, int tab_bytes, int tile,
                  float2 *__restrict__ slabs)
 {
@@ -952,8 +959,12 @@ fast_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restrict__ p
     // then read through scalar loads
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     Acc *cells = wave_cells[wave];
-    const int2 item = items[blockIdx.x * kFastWaves + wave];
-    if (item.x < 0) return;             // padding after the last item
+    const int it = blockIdx.x * kFastWaves + wave;
+    if (it >= fi.n_items) return;       // past the last item (whole workgroups)
+    int seg = 0;
+    while (seg + 1 < fi.n_seg && it >= fi.item_begin[seg + 1]) ++seg;
+    const int2 item = make_int2(fi.chunk_begin[seg] + (it - fi.item_begin[seg]) / fi.parts[seg],
+                                (it - fi.item_begin[seg]) % fi.parts[seg]);
     const int4 ch = chunks[item.x];     // tile, first record, count, window S
     const int S = ch.w, W = S - tile + 1, n = W + 1;
     const int lane = threadIdx.x % kWave;
@@ -1400,7 +1411,7 @@ struct sdp_grid_plan {
     float2 *utab_d = nullptr;      // the unfolded degridding table (fused degrid + grid)
     int *conj_flag = nullptr;      // 1 when the degridding table is the conjugate of the gridding table
     int n_items = 0;               // register-gridder items: (chunk, 64-position part)
-    int2 *items = nullptr;
+    FastItems fast_items = {};
     long long n_fast = 0;          // visibilities in exact-footprint classes
     uint32_t *perm = nullptr;      // plan position -> caller index
     int2 *plane_tab = nullptr;     // (sample offset, h+1) per plane
@@ -1429,6 +1440,78 @@ struct sdp_grid_plan {
 
 namespace {
 
+// Set-up buffers of plan creation, kept per device from one plan to the
+// next (hipFree synchronises the device and costs about a quarter of a
+// millisecond a call): grown when a plan needs more, released after a plan
+// whose set-up holds more than kSetupKeep bytes.  Plan creations on one
+// device take turns (the cache's mutex).
+constexpr int kSetupSlots = 10, kHostSlots = 3;
+constexpr size_t kSetupKeep = 4ull << 30, kHostKeep = 256ull << 20;
+enum { SET_SUPPORTS, SET_PCLASS, SET_PUTAB, SET_RECS, SET_FRECS, SET_KEYS, SET_KEYS_SORTED, SET_VALS, SET_TEMP,
+       SET_RUNS };
+// pinned host staging of the work-item tables (their uploads then run at
+// full link speed)
+enum { HOST_CHUNKS, HOST_SLAB_OFF, HOST_TILE_KEYS };
+struct SetupCache {
+    std::mutex m;
+    void *buf[kSetupSlots] = {};
+    size_t cap[kSetupSlots] = {};
+    void *hbuf[kHostSlots] = {};
+    size_t hcap[kHostSlots] = {};
+    hipError_t hget(int slot, size_t bytes, void **out)
+    {
+        if (hcap[slot] < bytes) {
+            if (hbuf[slot]) (void)hipHostFree(hbuf[slot]);
+            hbuf[slot] = nullptr;
+            hcap[slot] = 0;
+            const size_t want = bytes + bytes / 4;
+            if (hipError_t e = hipHostMalloc(&hbuf[slot], want, hipHostMallocDefault)) {
+                hbuf[slot] = nullptr;
+                return e;
+            }
+            hcap[slot] = want;
+        }
+        *out = hbuf[slot];
+        return hipSuccess;
+    }
+    hipError_t get(int slot, size_t bytes, void **out)
+    {
+        if (cap[slot] < bytes) {
+            if (buf[slot]) (void)hipFree(buf[slot]);   // waits for the work that used it
+            buf[slot] = nullptr;
+            cap[slot] = 0;
+            const size_t want = bytes + bytes / 4;     // headroom for the next, slightly larger plan
+            if (hipError_t e = hipMalloc(&buf[slot], want)) {
+                buf[slot] = nullptr;
+                return e;
+            }
+            cap[slot] = want;
+        }
+        *out = buf[slot];
+        return hipSuccess;
+    }
+    void trim()
+    {
+        size_t total = 0, htotal = 0;
+        for (size_t c : cap) total += c;
+        for (size_t c : hcap) htotal += c;
+        if (total > kSetupKeep)
+            for (int i = 0; i < kSetupSlots; ++i) {
+                if (buf[i]) (void)hipFree(buf[i]);
+                buf[i] = nullptr;
+                cap[i] = 0;
+            }
+        if (htotal > kHostKeep)
+            for (int i = 0; i < kHostSlots; ++i) {
+                if (hbuf[i]) (void)hipHostFree(hbuf[i]);
+                hbuf[i] = nullptr;
+                hcap[i] = 0;
+            }
+    }
+};
+constexpr int kCacheDevices = 64;
+SetupCache g_setup_cache[kCacheDevices];
+
 void plan_free(sdp_grid_plan *p)
 {
     if (!p) return;
@@ -1438,7 +1521,6 @@ void plan_free(sdp_grid_plan *p)
     (void)hipFree(p->utab);
     (void)hipFree(p->utab_d);
     (void)hipFree(p->conj_flag);
-    (void)hipFree(p->items);
     (void)hipFree(p->perm);
     (void)hipFree(p->plane_tab);
     (void)hipFree(p->ktab);
@@ -1601,22 +1683,24 @@ int launch_fast(const sdp_grid_plan *plan, const float2 *vis, const uint32_t *pe
 {
     const bool comp = plan->p.accumulate == SDP_ACC_COMPENSATED;
     const int tb = (int)(8 * plan->utab_len);
+    const int nb = sdp_div_up(plan->n_items, kFastWaves);
+    const FastItems &fi = plan->fast_items;
     if (comp && perm)
-        fast_grid_kernel<true, true><<<plan->n_items / kFastWaves, kWave * kFastWaves, 0, st>>>(plan->frecs, perm, plan->chunks, plan->items,
-                                                                     plan->slab_off, vis, plan->utab, tb, plan->tile,
-                                                                     plan->slabs);
-    else if (comp)
-        fast_grid_kernel<true, false><<<plan->n_items / kFastWaves, kWave * kFastWaves, 0, st>>>(plan->frecs, perm, plan->chunks, plan->items,
-                                                                      plan->slab_off, vis, plan->utab, tb, plan->tile,
-                                                                      plan->slabs);
-    else if (perm)
-        fast_grid_kernel<false, true><<<plan->n_items / kFastWaves, kWave * kFastWaves, 0, st>>>(plan->frecs, perm, plan->chunks, plan->items,
-                                                                      plan->slab_off, vis, plan->utab, tb, plan->tile,
-                                                                      plan->slabs);
-    else
-        fast_grid_kernel<false, false><<<plan->n_items / kFastWaves, kWave * kFastWaves, 0, st>>>(plan->frecs, perm, plan->chunks, plan->items,
+        fast_grid_kernel<true, true><<<nb, kWave * kFastWaves, 0, st>>>(plan->frecs, perm, plan->chunks, fi,
                                                                        plan->slab_off, vis, plan->utab, tb,
                                                                        plan->tile, plan->slabs);
+    else if (comp)
+        fast_grid_kernel<true, false><<<nb, kWave * kFastWaves, 0, st>>>(plan->frecs, perm, plan->chunks, fi,
+                                                                        plan->slab_off, vis, plan->utab, tb,
+                                                                        plan->tile, plan->slabs);
+    else if (perm)
+        fast_grid_kernel<false, true><<<nb, kWave * kFastWaves, 0, st>>>(plan->frecs, perm, plan->chunks, fi,
+                                                                        plan->slab_off, vis, plan->utab, tb,
+                                                                        plan->tile, plan->slabs);
+    else
+        fast_grid_kernel<false, false><<<nb, kWave * kFastWaves, 0, st>>>(plan->frecs, perm, plan->chunks, fi,
+                                                                         plan->slab_off, vis, plan->utab, tb,
+                                                                         plan->tile, plan->slabs);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }
@@ -1946,37 +2030,41 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
     uint2 *frecs_tmp = nullptr;
     uint32_t *keys = nullptr, *keys_sorted = nullptr, *vals = nullptr;
     KeyRun *d_runs = nullptr;
-    // the set-up buffers above: freed once the key runs are read back, or by
-    // this guard when an error returns early
-    struct SetupBuffers {
-        void **slot[10];
-        void release()
-        {
-            for (void **q : slot)
-                if (*q) {
-                    (void)hipFree(*q);
-                    *q = nullptr;
-                }
-        }
-        ~SetupBuffers() { release(); }
-    } setup{{(void **)&d_supports, (void **)&d_pclass, (void **)&d_putab, (void **)&recs_tmp, (void **)&frecs_tmp,
-             (void **)&keys, (void **)&keys_sorted, (void **)&vals, (void **)&d_runs, (void **)&d_runs}};
-    PLAN_TRY(hipMalloc(&d_supports, sizeof(int2) * p.n_kernels));
+    // the set-up buffers above come from this device's cache, held until the
+    // plan is built (or an error returns)
+    int dev = 0;
+    PLAN_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kCacheDevices)
+        return fail(sdp::set_error(SDP_E_UNSUPPORTED, "device %d: at most %d devices", dev, kCacheDevices));
+    SetupCache &cache = g_setup_cache[dev];
+    std::lock_guard<std::mutex> cache_lock(cache.m);
+    struct TrimGuard {
+        SetupCache &c;
+        ~TrimGuard() { c.trim(); }
+    } trim_guard{cache};
+    auto setup_buf = [&](int slot, size_t bytes, auto **ptr) {
+        void *q = nullptr;
+        hipError_t e = cache.get(slot, bytes, &q);
+        using T = std::remove_pointer_t<std::remove_reference_t<decltype(*ptr)>>;
+        *ptr = static_cast<T *>(q);
+        return e;
+    };
+    PLAN_TRY(setup_buf(SET_SUPPORTS, sizeof(int2) * p.n_kernels, &d_supports));
     PLAN_TRY(hipMemcpyAsync(d_supports, sup.data(), sizeof(int2) * p.n_kernels, hipMemcpyHostToDevice, st));
-    PLAN_TRY(hipMalloc(&d_pclass, sizeof(int) * p.n_kernels));
+    PLAN_TRY(setup_buf(SET_PCLASS, sizeof(int) * p.n_kernels, &d_pclass));
     PLAN_TRY(hipMemcpyAsync(d_pclass, pclass.data(), sizeof(int) * p.n_kernels, hipMemcpyHostToDevice, st));
-    PLAN_TRY(hipMalloc(&d_putab, sizeof(int) * p.n_kernels));
+    PLAN_TRY(setup_buf(SET_PUTAB, sizeof(int) * p.n_kernels, &d_putab));
     PLAN_TRY(hipMemcpyAsync(d_putab, plane_utab.data(), sizeof(int) * p.n_kernels, hipMemcpyHostToDevice, st));
 
     const size_t n = (size_t)std::max(n_vis, 1);
-    PLAN_TRY(hipMalloc(&recs_tmp, sizeof(VisRec) * n));
-    PLAN_TRY(hipMalloc(&frecs_tmp, sizeof(uint2) * n));
+    PLAN_TRY(setup_buf(SET_RECS, sizeof(VisRec) * n, &recs_tmp));
+    PLAN_TRY(setup_buf(SET_FRECS, sizeof(uint2) * n, &frecs_tmp));
     PLAN_TRY(hipMalloc(&plan->recs, sizeof(VisRec) * n));
     PLAN_TRY(hipMalloc(&plan->frecs, sizeof(uint2) * n));
     PLAN_TRY(hipMalloc(&plan->perm, sizeof(uint32_t) * n));
-    PLAN_TRY(hipMalloc(&keys, sizeof(uint32_t) * n));
-    PLAN_TRY(hipMalloc(&keys_sorted, sizeof(uint32_t) * n));
-    PLAN_TRY(hipMalloc(&vals, sizeof(uint32_t) * n));
+    PLAN_TRY(setup_buf(SET_KEYS, sizeof(uint32_t) * n, &keys));
+    PLAN_TRY(setup_buf(SET_KEYS_SORTED, sizeof(uint32_t) * n, &keys_sorted));
+    PLAN_TRY(setup_buf(SET_VALS, sizeof(uint32_t) * n, &vals));
 
     std::vector<KeyRun> runs;
     if (n_vis > 0) {
@@ -2014,15 +2102,12 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
         PLAN_TRY(rocprim::radix_sort_pairs(nullptr, temp_bytes, keys, keys_sorted, vals, plan->perm, (size_t)n_vis,
                                            0, (unsigned)end_bit, st));
         void *temp = nullptr;
-        PLAN_TRY(hipMalloc(&temp, temp_bytes));
+        PLAN_TRY(setup_buf(SET_TEMP, temp_bytes, &temp));
         hipError_t se = rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_sorted, vals, plan->perm,
                                                   (size_t)n_vis, 0, (unsigned)end_bit, st);
-        (void)hipStreamSynchronize(st);
-        (void)hipFree(temp);
         PLAN_TRY(se);
-        gather_records_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(recs_tmp, plan->perm, n_vis, plan->recs);
-        PLAN_TRY(hipGetLastError());
-        gather_frecs_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(frecs_tmp, plan->perm, n_vis, plan->frecs);
+        gather_records_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(recs_tmp, frecs_tmp, plan->perm, n_vis,
+                                                                      plan->recs, plan->frecs);
         PLAN_TRY(hipGetLastError());
         if (plan->n_unfold > 0) {
             // run lengths of the sorted exact-footprint records; the unsorted
@@ -2033,10 +2118,8 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
             PLAN_TRY(hipGetLastError());
             size_t bb = 0;
             PLAN_TRY(rocprim::inclusive_scan(nullptr, bb, fl, rid, (size_t)n_vis, rocprim::plus<uint32_t>(), st));
-            PLAN_TRY(hipMalloc(&temp, bb));
+            PLAN_TRY(setup_buf(SET_TEMP, bb, &temp));
             se = rocprim::inclusive_scan(temp, bb, fl, rid, (size_t)n_vis, rocprim::plus<uint32_t>(), st);
-            (void)hipStreamSynchronize(st);
-            (void)hipFree(temp);
             PLAN_TRY(se);
             fast_run_ends_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(fl, rid, n_vis, rend);
             PLAN_TRY(hipGetLastError());
@@ -2058,7 +2141,7 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
                                          rocprim::plus<unsigned long long>(), st));
         PLAN_TRY(rocprim::exclusive_scan(nullptr, b2, flags, run_no, 0u, (size_t)n_vis, rocprim::plus<uint32_t>(),
                                          st));
-        PLAN_TRY(hipMalloc(&temp, std::max(b1, b2)));
+        PLAN_TRY(setup_buf(SET_TEMP, std::max(b1, b2), &temp));
         se = rocprim::inclusive_scan(temp, b1, tap_count, tap_scan, (size_t)n_vis,
                                      rocprim::plus<unsigned long long>(), st);
         if (se == hipSuccess)
@@ -2066,12 +2149,11 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
         uint32_t last[2] = {0, 0};
         if (se == hipSuccess) se = hipMemcpyAsync(&last[0], run_no + n_vis - 1, 4, hipMemcpyDeviceToHost, st);
         if (se == hipSuccess) se = hipMemcpyAsync(&last[1], flags + n_vis - 1, 4, hipMemcpyDeviceToHost, st);
-        (void)hipStreamSynchronize(st);
-        (void)hipFree(temp);
+        (void)hipStreamSynchronize(st);   // the run count is read below
         PLAN_TRY(se);
         const size_t n_runs = (size_t)last[0] + last[1];
         if (n_runs > 0) {
-            PLAN_TRY(hipMalloc(&d_runs, sizeof(KeyRun) * n_runs));
+            PLAN_TRY(setup_buf(SET_RUNS, sizeof(KeyRun) * n_runs, &d_runs));
             run_start_kernel<<<sdp_div_up(n_vis, 256), 256, 0, st>>>(keys_sorted, flags, run_no, n_vis,
                                                                      a.plane_bits, d_runs);
             PLAN_TRY(hipGetLastError());
@@ -2083,19 +2165,30 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
         }
     }
     PLAN_TRY(hipStreamSynchronize(st));
-    setup.release();
 
     // work items, class-major then tile-major, each (class, tile) list cut
     // into pieces of about chunk_taps taps
-    std::vector<int4> chunks;
-    std::vector<long long> slab_off;
+    // chunks and slab_off: pinned staging holding at most
+    // sum ceil(taps / chunk_taps) <= runs + total taps / chunk_taps + 1 entries
+    int4 *chunks = nullptr;
+    long long *slab_off = nullptr;
+    size_t n_ch = 0, max_ch = 0;
     std::vector<int> key_S, key_tile;
     std::vector<long long> key_first_chunk, key_pieces;
+    {
+        unsigned long long all_taps = 0;
+        for (const KeyRun &r : runs) all_taps += r.taps;
+        max_ch = runs.size() + (size_t)(all_taps / (unsigned long long)plan->chunk_taps) + 1;
+        PLAN_TRY(cache.hget(HOST_CHUNKS, sizeof(int4) * max_ch, (void **)&chunks));
+        PLAN_TRY(cache.hget(HOST_SLAB_OFF, sizeof(long long) * max_ch, (void **)&slab_off));
+        for (auto *v : {&key_S, &key_tile}) v->reserve(runs.size());
+        for (auto *v : {&key_first_chunk, &key_pieces}) v->reserve(runs.size());
+    }
     long long cells = 0;
     int kept = 0;
     size_t ri = 0;
     for (int c = 0; c < kAllClasses; ++c) {
-        plan->class_chunk_begin[c] = (int)chunks.size();
+        plan->class_chunk_begin[c] = (int)n_ch;
         for (; class_rank[c] >= 0 && ri < runs.size() && (int)(runs[ri].key / (uint32_t)ntiles_all) == class_rank[c];
              ++ri) {
             const KeyRun &r = runs[ri];
@@ -2105,12 +2198,14 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
             pieces = std::max(1LL, std::min(pieces, (long long)cnt));
             key_tile.push_back(t);
             key_S.push_back(plan->class_S[c]);
-            key_first_chunk.push_back((long long)chunks.size());
+            key_first_chunk.push_back((long long)n_ch);
+            if (n_ch + (size_t)pieces > max_ch)   // cannot happen (the bound above)
+                return fail(sdp::set_error(SDP_E_UNSUPPORTED, "work-item count past its bound"));
             key_pieces.push_back(pieces);
             for (long long q = 0; q < pieces; ++q) {
                 const int b = first + (int)(cnt * q / pieces), e = first + (int)(cnt * (q + 1) / pieces);
-                chunks.push_back(make_int4(t, b, e - b, plan->class_S[c]));
-                slab_off.push_back(cells);
+                chunks[n_ch] = make_int4(t, b, e - b, plan->class_S[c]);
+                slab_off[n_ch++] = cells;
                 cells += (long long)plan->class_S[c] * plan->class_S[c];
             }
             kept += cnt;
@@ -2118,21 +2213,31 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
             if (is_fast(c)) plan->n_fast += cnt;
         }
     }
-    plan->class_chunk_begin[kAllClasses] = (int)chunks.size();
+    plan->class_chunk_begin[kAllClasses] = (int)n_ch;
     for (int c = 0; c < kAllClasses; ++c)
         plan->n_class_launches += plan->class_chunk_begin[c + 1] > plan->class_chunk_begin[c];
     plan->n_kept = kept;
-    plan->n_chunks = (int)chunks.size();
+    plan->n_chunks = (int)n_ch;
     // register-gridder items: each exact-footprint work item in parts of 64
-    // footprint positions
-    std::vector<int2> items;
-    for (int c = kNumClasses; c < kAllClasses; ++c) {
-        const int W = 2 * plan->class_h[c], parts = (W * W + kWave - 1) / kWave;
-        for (int k = plan->class_chunk_begin[c]; k < plan->class_chunk_begin[c + 1]; ++k)
-            for (int w = 0; w < parts; ++w) items.push_back(make_int2(k, w));
+    // footprint positions, one segment per class present
+    {
+        FastItems &fi = plan->fast_items;
+        long long total = 0;
+        for (int c = kNumClasses; c < kAllClasses; ++c) {
+            const int b = plan->class_chunk_begin[c], e = plan->class_chunk_begin[c + 1];
+            if (e <= b) continue;
+            const int W = 2 * plan->class_h[c], parts = (W * W + kWave - 1) / kWave;
+            fi.item_begin[fi.n_seg] = (int)total;
+            fi.chunk_begin[fi.n_seg] = b;
+            fi.parts[fi.n_seg] = parts;
+            ++fi.n_seg;
+            total += (long long)(e - b) * parts;
+        }
+        if (total >= (1LL << 31)) return fail(sdp::set_error(SDP_E_UNSUPPORTED, "%lld register-gridder items", total));
+        fi.item_begin[fi.n_seg] = (int)total;
+        fi.n_items = (int)total;
+        plan->n_items = (int)total;
     }
-    while (items.size() % kFastWaves) items.push_back(make_int2(-1, 0));   // whole workgroups
-    plan->n_items = (int)items.size();
 
     // key slabs: single-piece keys use their chunk's slab; multi-piece keys
     // get a reduced slab placed after the chunk slabs.  A key of more than
@@ -2174,14 +2279,17 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
         }
     }
     // each tile's keys in class order (runs are class-major), tiles (grid-major) in order
-    std::vector<int> tile_key_begin(ntiles_all + 1, 0), tile_key_list(nk);
+    int *tile_key_begin = nullptr;
+    PLAN_TRY(cache.hget(HOST_TILE_KEYS, sizeof(int) * ((size_t)ntiles_all + 1), (void **)&tile_key_begin));
+    std::fill(tile_key_begin, tile_key_begin + ntiles_all + 1, 0);
+    std::vector<int> tile_key_list(nk);
     for (int k = 0; k < nk; ++k) ++tile_key_begin[key_tile[k] + 1];
     for (int t = 0; t < ntiles_all; ++t) {
         plan->n_tiles_used += tile_key_begin[t + 1] > 0;
         tile_key_begin[t + 1] += tile_key_begin[t];
     }
     {
-        std::vector<int> fill(tile_key_begin.begin(), tile_key_begin.end() - 1);
+        std::vector<int> fill(tile_key_begin, tile_key_begin + ntiles_all);
         for (int k = 0; k < nk; ++k) tile_key_list[fill[key_tile[k]]++] = k;
     }
     // key tables in tile order so that a tile's keys are contiguous
@@ -2196,13 +2304,13 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
     plan->n_multi2 = (int)multi2.size();
 
     PLAN_TRY(hipMalloc(&plan->tile_key_begin, sizeof(int) * ((size_t)ntiles_all + 1)));
-    PLAN_TRY(hipMemcpyAsync(plan->tile_key_begin, tile_key_begin.data(), sizeof(int) * ((size_t)ntiles_all + 1),
+    PLAN_TRY(hipMemcpyAsync(plan->tile_key_begin, tile_key_begin, sizeof(int) * ((size_t)ntiles_all + 1),
                             hipMemcpyHostToDevice, st));
     if (plan->n_chunks > 0) {
         PLAN_TRY(hipMalloc(&plan->chunks, sizeof(int4) * plan->n_chunks));
-        PLAN_TRY(hipMemcpyAsync(plan->chunks, chunks.data(), sizeof(int4) * plan->n_chunks, hipMemcpyHostToDevice, st));
+        PLAN_TRY(hipMemcpyAsync(plan->chunks, chunks, sizeof(int4) * plan->n_chunks, hipMemcpyHostToDevice, st));
         PLAN_TRY(hipMalloc(&plan->slab_off, sizeof(long long) * plan->n_chunks));
-        PLAN_TRY(hipMemcpyAsync(plan->slab_off, slab_off.data(), sizeof(long long) * plan->n_chunks,
+        PLAN_TRY(hipMemcpyAsync(plan->slab_off, slab_off, sizeof(long long) * plan->n_chunks,
                                 hipMemcpyHostToDevice, st));
         PLAN_TRY(hipMalloc(&plan->key_S, sizeof(int) * nk));
         PLAN_TRY(hipMemcpyAsync(plan->key_S, kS_t.data(), sizeof(int) * nk, hipMemcpyHostToDevice, st));
@@ -2227,11 +2335,6 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
             PLAN_TRY(hipMalloc(&plan->part_off, sizeof(long long) * part_off.size()));
             PLAN_TRY(hipMemcpyAsync(plan->part_off, part_off.data(), sizeof(long long) * part_off.size(),
                                     hipMemcpyHostToDevice, st));
-        }
-        if (plan->n_items > 0) {
-            PLAN_TRY(hipMalloc(&plan->items, sizeof(int2) * plan->n_items));
-            PLAN_TRY(hipMemcpyAsync(plan->items, items.data(), sizeof(int2) * plan->n_items, hipMemcpyHostToDevice,
-                                    st));
         }
         plan->slab_bytes = sizeof(float2) * (size_t)(cells + extra);
         PLAN_TRY(hipMalloc(&plan->slabs, plan->slab_bytes));
