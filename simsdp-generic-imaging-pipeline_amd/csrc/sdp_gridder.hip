@@ -1032,9 +1032,11 @@ __global__ void set_flag_kernel(int *flag, int v)
     if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v;
 }
 
-// waves per workgroup and LDS cells per wave of the fused kernel: a wave
-// covers all W^2 footprint positions of its work item in P <= PMAX parts of
-// 64, four cells per part and lane
+// waves per workgroup and LDS of the fused kernel: a wave covers all W^2
+// footprint positions of its work item in P <= PMAX parts of 64 and holds
+// the work item's S x S window in LDS.  An exact-footprint class has
+// 2h + 1 >= tile, so S = tile + 2h - 1 <= 4h and S^2 <= PMAX * 4 * 64: the
+// bound below.
 template <int PMAX> constexpr int fused_waves() { return PMAX <= 4 ? 4 : PMAX <= 8 ? 2 : 1; }
 template <bool COMP, int PMAX> constexpr size_t fused_lds()
 {
@@ -1043,6 +1045,7 @@ template <bool COMP, int PMAX> constexpr size_t fused_lds()
 
 struct FusedArgs {
     int chunk_b, chunk_e, tile, G, ntx, ntiles, tab_bytes;
+    int wave_cells;   // LDS cells per wave: the largest S^2 of the launch's classes
 };
 
 // One wavefront per work item: for each of its visibilities (in plan
@@ -1066,14 +1069,13 @@ fast_degrid_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restr
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int ci = fa.chunk_b + blockIdx.x * fused_waves<PMAX>() + wave;
     if (ci >= fa.chunk_e) return;
-    Acc *cells = reinterpret_cast<Acc *>(lds4) + (size_t)wave * PMAX * 4 * kWave;
+    Acc *cells = reinterpret_cast<Acc *>(lds4) + (size_t)wave * fa.wave_cells;   // the window, row-major
     const int4 ch = chunks[ci];
     const int S = ch.w, tile = fa.tile, W = S - tile + 1, n = W + 1, P = (W * W + kWave - 1) / kWave;
     const int grid_no = ch.x / fa.ntiles, tl = ch.x - grid_no * fa.ntiles;   // ch.x: tile over all grids
     const int tile_u0 = (tl % fa.ntx) * tile, tile_v0 = (tl / fa.ntx) * tile;
     model += (size_t)grid_no * fa.G * fa.G;
     const int lane = threadIdx.x % kWave;
-    for (int q = 0; q < PMAX * 4; ++q) acc_zero(cells[q * kWave + lane]);
     const __amdgpu_buffer_rsrc_t rg =
         __builtin_amdgcn_make_buffer_rsrc((void *)utab_g, (short)0, fa.tab_bytes, 0x00020000);
     const __amdgpu_buffer_rsrc_t rd =
@@ -1082,9 +1084,11 @@ fast_degrid_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restr
     const uint2 *fr = frecs + ch.y;
     Acc run[PMAX];
     unsigned voff[PMAX];
-    int cell[PMAX];
+    int cell[PMAX];           // window cell (row-major) of the lane's tap in the current run
     float2 g[PMAX];
     int pa[PMAX], pb[PMAX];   // the lane's footprint position (a, b) in each part; a = -1 when unused
+    // Window cell (x, y) is reached only from position (x mod W, y mod W):
+    // each cell has one owning lane, which alone zeroes, sums and writes it.
 #pragma unroll
     for (int k = 0; k < PMAX; ++k) {
         acc_zero(run[k]);
@@ -1095,6 +1099,13 @@ fast_degrid_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restr
         const bool act = k < P && p < W * W;
         pa[k] = act ? p % W : -1;
         pb[k] = act ? p / W : 0;
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int u = pa[k] + (q & 1) * W, v = pb[k] + (q >> 1) * W;
+                if (u < S && v < S) acc_zero(cells[v * S + u]);
+            }
+        }
     }
     bool started = false;
     int j = 0;
@@ -1105,12 +1116,12 @@ fast_degrid_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restr
         const bool conj = (r0.y & 0x400u) != 0u;
 #pragma unroll
         for (int k = 0; k < PMAX; ++k) {
-            if (started) acc_merge(cells[(k * 4 + cell[k]) * kWave + lane], run[k]);
             const bool act = pa[k] >= 0;
+            if (started && act) acc_merge(cells[cell[k]], run[k]);
             const int a = act ? pa[k] : 0, b = pb[k];
             const int du = a >= tu ? a - tu : a - tu + W, dv = b >= tv ? b - tv : b - tv + W;
             voff[k] = act ? (unsigned)(8 * (dv * n + du)) : (unsigned)kOob;
-            cell[k] = (a < tu ? 1 : 0) + (b < tv ? 2 : 0);
+            cell[k] = (b + (b < tv ? W : 0)) * S + a + (a < tu ? W : 0);
             // the model value at the lane's cell for this run
             const int gu = tile_u0 + a + (a < tu ? W : 0), gv = tile_v0 + b + (b < tv ? W : 0);
             g[k] = (act && gu < fa.G && gv < fa.G) ? model[(size_t)gv * fa.G + gu] : make_float2(0.f, 0.f);
@@ -1172,13 +1183,13 @@ fast_degrid_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restr
     float2 *slab = slabs + slab_off[ci];
 #pragma unroll
     for (int k = 0; k < PMAX; ++k) {
-        if (started) acc_merge(cells[(k * 4 + cell[k]) * kWave + lane], run[k]);
         if (pa[k] >= 0) {
+            if (started) acc_merge(cells[cell[k]], run[k]);
             const int a = pa[k], b = pb[k];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int u = a + (q & 1) * W, v = b + (q >> 1) * W;
-                if (u < S && v < S) slab[v * S + u] = acc_value(cells[(k * 4 + q) * kWave + lane]);
+                if (u < S && v < S) slab[v * S + u] = acc_value(cells[v * S + u]);
             }
         }
     }
@@ -1720,14 +1731,23 @@ int launch_fused_group(const sdp_grid_plan *plan, int b, int e, const ChunkIO &i
 {
     if (e <= b) return SDP_OK;
     constexpr int waves = fused_waves<PMAX>();
-    constexpr size_t lds = fused_lds<COMP, PMAX>();
+    constexpr size_t lds_max = fused_lds<COMP, PMAX>();
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute((const void *)fast_degrid_grid_kernel<COMP, PMAX>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max);
         attr_set = true;
     }
-    FusedArgs fa{b, e, plan->tile, plan->p.grid_size, plan->ntx, plan->ntiles, (int)(8 * plan->utab_len)};
+    int smax = 0;   // the largest window of the classes in [b, e)
+    for (int c = kNumClasses; c < kAllClasses; ++c)
+        if (plan->class_chunk_begin[c + 1] > plan->class_chunk_begin[c] && plan->class_chunk_begin[c] < e &&
+            plan->class_chunk_begin[c + 1] > b)
+            smax = std::max(smax, plan->class_S[c]);
+    const size_t lds = (size_t)waves * smax * smax * (COMP ? sizeof(float4) : sizeof(float2));
+    if (lds > lds_max)
+        return sdp::set_error(SDP_E_UNSUPPORTED, "window of %d cells exceeds the fused kernel's LDS", smax);
+    FusedArgs fa{b, e, plan->tile, plan->p.grid_size, plan->ntx, plan->ntiles, (int)(8 * plan->utab_len),
+                 smax * smax};
     fast_degrid_grid_kernel<COMP, PMAX><<<(e - b + waves - 1) / waves, kWave * waves, lds, st>>>(
         plan->frecs, perm, plan->chunks, plan->slab_off, io.measured, io.grid, io.vis_out, plan->utab, plan->utab_d,
         plan->conj_flag, fa, plan->slabs);
