@@ -105,7 +105,7 @@ enum { SDP_ORDER_ORIGINAL = 0, SDP_ORDER_PLAN = 1 };
 typedef struct sdp_grid_params {
     int grid_size;        /* G, even, <= 32768 */
     int oversampling;     /* OS */
-    int n_kernels;        /* number of w-planes in the support table */
+    int n_kernels;        /* number of w-planes in the support table, 1 to 8192 */
     int snap_mode;        /* SDP_SNAP_* */
     double uv_scale;      /* cell_size * G */
     double w_scale;       /* (n_kernels - 1)^2 / max_w */

@@ -402,14 +402,13 @@ __device__ __forceinline__ int col_part(int kx, int OS, int os_shift, int hp1)
     return fx * hp1 * hp1 + ix;
 }
 
-// one workgroup row per plane (blockIdx.y); tab holds (offset, h + 1)
-__global__ void relayout_kernel(const float2 *__restrict__ in, const int2 *__restrict__ tab, int OS,
-                                float2 *__restrict__ out)
+// the phase-major copy of one plane; t = (offset, h + 1); block bx of nbx
+__device__ void relayout_plane(const float2 *__restrict__ in, int2 t, int OS, float2 *__restrict__ out, int bx,
+                               int nbx)
 {
-    const int2 t = tab[blockIdx.y];
     const int n = t.y * OS;
     const int total = n * n;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    for (int i = bx * blockDim.x + threadIdx.x; i < total; i += nbx * blockDim.x) {
         const int ky = i / n, kx = i - ky * n;
         out[t.x + row_part(ky, OS, -1, t.y) + col_part(kx, OS, -1, t.y)] = in[t.x + i];
     }
@@ -823,17 +822,16 @@ chunk_kernel(const VisRec *__restrict__ recs, const uint32_t *__restrict__ perm,
 
 // ---- the register gridder of the exact-footprint classes -----------------
 
-// One workgroup row per plane (blockIdx.y); tab holds (quadrant offset, h,
-// unfolded offset, -).  Output sample ((fy OS + fx) n + jy + h) n + jx + h,
-// n = 2h + 1, is the quadrant sample (|jy OS + fy|, |jx OS + fx|): a
-// visibility's taps are then a (2h) x (2h) block of rows n samples apart.
-__global__ void unfold_kernel(const float2 *__restrict__ in, const int4 *__restrict__ tab, int OS,
-                              float2 *__restrict__ out)
+// The unfolded copy of one plane; t = (quadrant offset, h, unfolded offset,
+// -).  Output sample ((fy OS + fx) n + jy + h) n + jx + h, n = 2h + 1, is the
+// quadrant sample (|jy OS + fy|, |jx OS + fx|): a visibility's taps are then
+// a (2h) x (2h) block of rows n samples apart.
+__device__ void unfold_plane(const float2 *__restrict__ in, int4 t, int OS, float2 *__restrict__ out, int bx,
+                             int nbx)
 {
-    const int4 t = tab[blockIdx.y];
     const int h = t.y, n = 2 * h + 1, row = (h + 1) * OS;
     const int total = OS * OS * n * n;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    for (int i = bx * blockDim.x + threadIdx.x; i < total; i += nbx * blockDim.x) {
         const int jx = i % n - h;
         int r = i / n;
         const int jy = r % n - h;
@@ -1016,20 +1014,56 @@ fast_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restrict__ p
 
 // ---- fused degrid + subtract + grid of the exact-footprint classes --------
 
-// flag = 0 when some degridding sample is not the conjugate of its gridding
-// sample (the caller set it to 1 first)
-__global__ void conj_check_kernel(const float2 *__restrict__ kg, const float2 *__restrict__ kd, long long n,
-                                  int *flag)
-{
-    bool ok = true;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        ok = ok && kd[i].x == kg[i].x && kd[i].y == -kg[i].y;
-    if (!ok) atomicAnd(flag, 0);
-}
+// The kernel-table copies a pass needs, in one launch: workgroup rows
+// (blockIdx.y) are planes of the gridding and degridding tables' phase-major
+// copies, then planes of their unfolded copies, then one row whose first
+// kConjBlocks workgroups compare the two caller tables sample for sample:
+// workgroup b writes mismatch[b] = 1 when some degridding sample is not the
+// conjugate of its gridding sample, else 0, and the last of them to finish
+// (a counter at mismatch[kConjBlocks + 1], back to 0 after each call) writes
+// their OR to mismatch[kConjBlocks].  Every flag is written every call.
+constexpr int kPrepBlocks = 128, kConjBlocks = 32;
+struct PrepArgs {
+    const float2 *kg, *kd;          // the caller's tables (reference layout)
+    float2 *rel_g, *rel_d;          // phase-major copies
+    float2 *unf_g, *unf_d;          // unfolded copies
+    int rows_rel_g, rows_rel_d, rows_unf_g, rows_unf_d, conj;
+    int OS;
+    long long table_len;            // samples compared
+    int *mismatch;                  // kConjBlocks flags, their OR, the arrival counter
+};
 
-__global__ void set_flag_kernel(int *flag, int v)
+__global__ void __launch_bounds__(256) prep_tables_kernel(const int2 *__restrict__ plane_tab,
+                                                          const int4 *__restrict__ unfold_tab, PrepArgs a)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v;
+    int y = blockIdx.y;
+    if (y < a.rows_rel_g) return relayout_plane(a.kg, plane_tab[y], a.OS, a.rel_g, blockIdx.x, gridDim.x);
+    y -= a.rows_rel_g;
+    if (y < a.rows_rel_d) return relayout_plane(a.kd, plane_tab[y], a.OS, a.rel_d, blockIdx.x, gridDim.x);
+    y -= a.rows_rel_d;
+    if (y < a.rows_unf_g) return unfold_plane(a.kg, unfold_tab[y], a.OS, a.unf_g, blockIdx.x, gridDim.x);
+    y -= a.rows_unf_g;
+    if (y < a.rows_unf_d) return unfold_plane(a.kd, unfold_tab[y], a.OS, a.unf_d, blockIdx.x, gridDim.x);
+    if (!a.conj || blockIdx.x >= kConjBlocks) return;
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    bool ok = true;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < a.table_len; i += kConjBlocks * 256LL)
+        ok = ok && a.kd[i].x == a.kg[i].x && a.kd[i].y == -a.kg[i].y;
+    if (!ok) bad = 1;   // any lane: the same value
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.mismatch[blockIdx.x] = bad;
+        __threadfence();
+        if (atomicAdd(&a.mismatch[kConjBlocks + 1], 1) == kConjBlocks - 1) {   // the last to arrive
+            __threadfence();
+            int any = 0;
+            for (int q = 0; q < kConjBlocks; ++q) any |= __atomic_load_n(&a.mismatch[q], __ATOMIC_RELAXED);
+            a.mismatch[kConjBlocks] = any;
+            a.mismatch[kConjBlocks + 1] = 0;
+        }
+    }
 }
 
 // waves per workgroup and LDS of the fused kernel: a wave covers all W^2
@@ -1054,14 +1088,14 @@ struct FusedArgs {
 // reduced over the wavefront, subtracted from the measured value, and the
 // residual gridded with the same table samples into the lane's running sums
 // (as fast_grid_kernel).  When the degridding table is the conjugate of the
-// gridding table (*conj_flag) each tap reads one sample for both.
+// gridding table (no mismatch flag set) each tap reads one sample for both.
 template <bool COMP, int PMAX>
 __global__ void __launch_bounds__(kWave * fused_waves<PMAX>())
 fast_degrid_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restrict__ perm,
                         const int4 *__restrict__ chunks, const long long *__restrict__ slab_off,
                         const float2 *__restrict__ measured, const float2 *__restrict__ model,
                         float2 *__restrict__ vis_out, const float2 *__restrict__ utab_g,
-                        const float2 *__restrict__ utab_d, const int *__restrict__ conj_flag, FusedArgs fa,
+                        const float2 *__restrict__ utab_d, const int *__restrict__ mismatch, FusedArgs fa,
                         float2 *__restrict__ slabs)
 {
     using Acc = typename AccT<COMP>::T;
@@ -1080,7 +1114,7 @@ fast_degrid_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restr
         __builtin_amdgcn_make_buffer_rsrc((void *)utab_g, (short)0, fa.tab_bytes, 0x00020000);
     const __amdgpu_buffer_rsrc_t rd =
         __builtin_amdgcn_make_buffer_rsrc((void *)utab_d, (short)0, fa.tab_bytes, 0x00020000);
-    const bool own_d = *conj_flag == 0;   // a degridding table of its own
+    const bool own_d = mismatch[kConjBlocks] != 0;   // a degridding table of its own
     const uint2 *fr = frecs + ch.y;
     Acc run[PMAX];
     unsigned voff[PMAX];
@@ -1420,7 +1454,8 @@ struct sdp_grid_plan {
     long long utab_len = 0;        // samples of the unfolded gridding table
     float2 *utab = nullptr;
     float2 *utab_d = nullptr;      // the unfolded degridding table (fused degrid + grid)
-    int *conj_flag = nullptr;      // 1 when the degridding table is the conjugate of the gridding table
+    int *conj_flag = nullptr;      // per-block and overall "some degridding sample is not the conjugate
+                                   // gridding sample" flags, and an arrival counter (prep_tables_kernel)
     int n_items = 0;               // register-gridder items: (chunk, 64-position part)
     FastItems fast_items = {};
     long long n_fast = 0;          // visibilities in exact-footprint classes
@@ -1716,14 +1751,6 @@ int launch_fast(const sdp_grid_plan *plan, const float2 *vis, const uint32_t *pe
     return SDP_OK;
 }
 
-// an unfolded copy of a kernel table (the gridding table into utab, the
-// degridding table into utab_d)
-void unfold(const sdp_grid_plan *plan, const float *d_kernels, float2 *out, hipStream_t st)
-{
-    const int OS = plan->p.oversampling, n = 2 * kFastMaxH + 1;
-    const int bx = (int)std::min<long long>(128, ((long long)OS * OS * n * n + 255) / 256);
-    unfold_kernel<<<dim3(bx, plan->n_unfold), 256, 0, st>>>((const float2 *)d_kernels, plan->unfold_tab, OS, out);
-}
 
 template <bool COMP, int PMAX>
 int launch_fused_group(const sdp_grid_plan *plan, int b, int e, const ChunkIO &io, const uint32_t *perm,
@@ -1783,15 +1810,47 @@ int fused_group_chunks(const sdp_grid_plan *plan, int group)
            plan->class_chunk_begin[fast_class(kFusedLo[group])];
 }
 
-// phase-major copy of a kernel table into the plan (slot 0 gridding, 1 degridding)
-const float2 *relayout(const sdp_grid_plan *plan, const float *d_kernels, int slot, hipStream_t st)
+// The kernel-table copies a pass reads, in one launch (prep_tables_kernel):
+// the phase-major copies when the LDS kernels run (slot 0 gridding, 1
+// degridding, into plan->ktab), the unfolded copies (and the conjugate
+// check, with both tables) when exact-footprint classes are present.
+struct Prepared {
+    const float2 *rel_g = nullptr, *rel_d = nullptr;
+};
+Prepared prepare_tables(const sdp_grid_plan *plan, const float *d_kg, const float *d_kd, bool lds_kernels,
+                        bool fast_kernels, hipStream_t st)
 {
-    float2 *out = plan->ktab + (size_t)slot * (size_t)plan->table_len;
-    int maxn = 1;
-    for (int c = 0; c < kAllClasses; ++c) maxn = std::max(maxn, (plan->class_h[c] + 1) * plan->p.oversampling);
-    const int bx = (int)std::min<long long>(64, ((long long)maxn * maxn + 255) / 256);
-    relayout_kernel<<<dim3(bx, plan->p.n_kernels), 256, 0, st>>>((const float2 *)d_kernels, plan->plane_tab,
-                                                                  plan->p.oversampling, out);
+    Prepared out;
+    PrepArgs a{};
+    a.kg = (const float2 *)d_kg;
+    a.kd = (const float2 *)d_kd;
+    a.OS = plan->p.oversampling;
+    if (lds_kernels && d_kg) {
+        a.rel_g = plan->ktab;
+        a.rows_rel_g = plan->p.n_kernels;
+        out.rel_g = a.rel_g;
+    }
+    if (lds_kernels && d_kd) {
+        a.rel_d = plan->ktab + (size_t)plan->table_len;
+        a.rows_rel_d = plan->p.n_kernels;
+        out.rel_d = a.rel_d;
+    }
+    if (fast_kernels && plan->n_items > 0) {
+        if (d_kg) {
+            a.unf_g = plan->utab;
+            a.rows_unf_g = plan->n_unfold;
+        }
+        if (d_kd) {
+            a.unf_d = plan->utab_d;
+            a.rows_unf_d = plan->n_unfold;
+        }
+        a.conj = d_kg && d_kd;
+        a.table_len = plan->table_len;
+        a.mismatch = plan->conj_flag;
+    }
+    const int rows = a.rows_rel_g + a.rows_rel_d + a.rows_unf_g + a.rows_unf_d + (a.conj ? 1 : 0);
+    if (rows > 0)
+        prep_tables_kernel<<<dim3(kPrepBlocks, rows), 256, 0, st>>>(plan->plane_tab, plan->unfold_tab, a);
     return out;
 }
 
@@ -1903,7 +1962,8 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
     const sdp_grid_params p = *pp;
     SDP_REQUIRE(p.grid_size >= 2 && p.grid_size % 2 == 0 && p.grid_size <= 32768,
                 "grid size %d must be even and in [2, 32768]", p.grid_size);
-    SDP_REQUIRE(p.oversampling >= 1 && p.n_kernels >= 1 && p.n_kernels < 32768, "bad oversampling / n_kernels");
+    SDP_REQUIRE(p.oversampling >= 1 && p.n_kernels >= 1 && p.n_kernels <= 8192,
+                "bad oversampling / n_kernels (1 to 8192 planes)");
     SDP_REQUIRE(p.snap_mode == SDP_SNAP_STD || p.snap_mode == SDP_SNAP_GRIDDING_ACTOR, "bad snap mode");
     SDP_REQUIRE(p.tile >= 0 && p.tile <= 32 && p.chunk >= 0, "tile must be in [1, 32] (0 = default)");
     SDP_REQUIRE(p.accumulate == SDP_ACC_BLOCKED || p.accumulate == SDP_ACC_COMPENSATED, "bad accumulate mode");
@@ -2042,7 +2102,8 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
                                 hipMemcpyHostToDevice, st));
         PLAN_TRY(hipMalloc(&plan->utab, sizeof(float2) * (size_t)plan->utab_len));
         PLAN_TRY(hipMalloc(&plan->utab_d, sizeof(float2) * (size_t)plan->utab_len));
-        PLAN_TRY(hipMalloc(&plan->conj_flag, sizeof(int)));
+        PLAN_TRY(hipMalloc(&plan->conj_flag, sizeof(int) * (kConjBlocks + 2)));
+        PLAN_TRY(hipMemsetAsync(plan->conj_flag, 0, sizeof(int) * (kConjBlocks + 2), st));
     }
     int2 *d_supports = nullptr;
     int *d_pclass = nullptr, *d_putab = nullptr;
@@ -2418,8 +2479,7 @@ int sdp_grid_accumulate(const sdp_grid_plan *plan, const float *d_kernels, const
     hipStream_t st = sdp::stream_of(s);
     ChunkIO io{};
     // the LDS kernels read the phase-major table, the register gridder the unfolded one
-    if (plan->class_chunk_begin[kNumClasses] > 0) io.kg = relayout(plan, d_kernels, 0, st);
-    if (plan->n_items > 0) unfold(plan, d_kernels, plan->utab, st);
+    io.kg = prepare_tables(plan, d_kernels, nullptr, plan->class_chunk_begin[kNumClasses] > 0, true, st).rel_g;
     SDP_LAUNCH_CHECK();
     io.vis = (const float2 *)d_vis;
     io.slabs = plan->slabs;
@@ -2487,7 +2547,7 @@ int sdp_degrid(const sdp_grid_plan *plan, const float *d_kernels, const float *d
     if (plan->n_chunks > 0) {
         SDP_REQUIRE(d_kernels, "sdp_degrid: null kernels");
         ChunkIO io{};
-        io.kd = relayout(plan, d_kernels, 1, st);
+        io.kd = prepare_tables(plan, nullptr, d_kernels, true, false, st).rel_d;
         SDP_LAUNCH_CHECK();
         io.grid = (const float2 *)d_grid;
         io.measured = (const float2 *)d_vis_measured;
@@ -2513,17 +2573,12 @@ int sdp_degrid_grid_accumulate(const sdp_grid_plan *plan, const float *d_degrid_
     if (plan->n_chunks > 0) {
         SDP_REQUIRE(d_degrid_kernels && d_grid_kernels, "sdp_degrid_grid: null kernels");
         ChunkIO io{};
-        if (plan->class_chunk_begin[kNumClasses] > 0) {   // the LDS kernels' phase-major tables
-            io.kd = relayout(plan, d_degrid_kernels, 1, st);
-            io.kg = relayout(plan, d_grid_kernels, 0, st);
-        }
-        if (plan->n_items > 0) {
-            // the fused kernel's unfolded tables, and whether one serves both
-            unfold(plan, d_grid_kernels, plan->utab, st);
-            unfold(plan, d_degrid_kernels, plan->utab_d, st);
-            set_flag_kernel<<<1, 64, 0, st>>>(plan->conj_flag, 1);
-            conj_check_kernel<<<256, 256, 0, st>>>(plan->utab, plan->utab_d, plan->utab_len, plan->conj_flag);
-        }
+        // the LDS kernels' phase-major tables, the fused kernel's unfolded
+        // tables and whether one of those serves both
+        const Prepared pt = prepare_tables(plan, d_grid_kernels, d_degrid_kernels,
+                                           plan->class_chunk_begin[kNumClasses] > 0, true, st);
+        io.kg = pt.rel_g;
+        io.kd = pt.rel_d;
         SDP_LAUNCH_CHECK();
         io.grid = (const float2 *)d_model_grid;
         io.measured = (const float2 *)d_vis_measured;
