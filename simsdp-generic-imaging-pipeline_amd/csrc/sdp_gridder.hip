@@ -1082,15 +1082,244 @@ struct FusedArgs {
     int wave_cells;   // LDS cells per wave: the largest S^2 of the launch's classes
 };
 
-// One wavefront per work item: for each of its visibilities (in plan
+// ---- cross-lane sums of the fused kernel ----------------------------------
+// A step's U visibilities need 2U wavefront sums (real and imaginary
+// prediction).  Rather than one 64-lane butterfly per value, the values are
+// folded together: v_permlane32_swap and v_permlane16_swap exchange half of
+// two registers at a time, so each fold halves the lanes a value spans and
+// the number of registers, until every 16-lane row holds one value; a
+// 16-lane butterfly of DPP adds then leaves each row's total in all of its
+// lanes.
+// fold32(a, b): lanes 0-31 get a(l) + a(l + 32), lanes 32-63 b(l - 32) + b(l)
+__device__ __forceinline__ float fold32(float a, float b)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+// fold16(a, b): rows 0, 1, 2, 3 get a's rows 0 + 1, b's rows 0 + 1, a's rows
+// 2 + 3, b's rows 2 + 3
+__device__ __forceinline__ float fold16(float a, float b)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+// the total of the lane's 16-lane row, in every lane of the row
+__device__ __forceinline__ float row_total(float x)
+{
+    x = dpp_add<0xb1>(x);
+    x = dpp_add<0x4e>(x);
+    x = dpp_add<0x141>(x);
+    return dpp_add<0x140>(x);
+}
+__device__ __forceinline__ float lane_value(float x, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// vis * k (CONJ: vis * conj k) added to a lane's running sum; the same
+// roundings as tap_add with the conjugation sign applied to vr, vi
+template <bool CONJ> __device__ __forceinline__ void tap_add_s(float2 &t, float2 k, float vr, float vi)
+{
+    t.x = fmaf(k.x, vr, t.x);
+    t.x = fmaf(k.y, CONJ ? vi : -vi, t.x);
+    t.y = fmaf(k.x, vi, t.y);
+    t.y = fmaf(k.y, CONJ ? -vr : vr, t.y);
+}
+template <bool CONJ> __device__ __forceinline__ void tap_add_s(float4 &t, float2 k, float vr, float vi)
+{
+    two_sum_acc(t.x, t.z, fmaf(vr, k.x, (CONJ ? vi : -vi) * k.y));
+    two_sum_acc(t.y, t.w, fmaf(vi, k.x, (CONJ ? -vr : vr) * k.y));
+}
+
+// One work item of the fused pass: for each of its visibilities (in plan
 // order) the prediction sum over the footprint -- each lane's taps read the
 // model grid at the lane's own cells, held in registers for the run -- is
 // reduced over the wavefront, subtracted from the measured value, and the
 // residual gridded with the same table samples into the lane's running sums
-// (as fast_grid_kernel).  When the degridding table is the conjugate of the
-// gridding table (no mismatch flag set) each tap reads one sample for both.
+// (as fast_grid_kernel).  OWN_D: the degridding table is not the conjugate
+// of the gridding table, and each tap reads a sample of both.
+//
+// Visibilities go U at a time.  After the folds the lanes of row r hold
+// (U = 4) visibility r's real part in one register and its imaginary part in
+// another; (U = 2) one register with rows (re 0, im 0, re 1, im 1); (U = 1)
+// rows (re, re, im, im).  Each lane loads the measured value of its row, the
+// row's first lane stores the residual, and the residuals are then read into
+// scalar registers for the gridding products.
+template <bool COMP, int PMAX, bool OWN_D>
+__device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs, const uint32_t *__restrict__ perm,
+                                                const int4 ch, int ci, const long long *__restrict__ slab_off,
+                                                const float2 *__restrict__ measured,
+                                                const float2 *__restrict__ model, float2 *__restrict__ vis_out,
+                                                __amdgpu_buffer_rsrc_t rg, __amdgpu_buffer_rsrc_t rd,
+                                                const FusedArgs &fa, float2 *__restrict__ slabs,
+                                                typename AccT<COMP>::T *cells)
+{
+    using Acc = typename AccT<COMP>::T;
+    const int S = ch.w, tile = fa.tile, W = S - tile + 1, n = W + 1, P = (W * W + kWave - 1) / kWave;
+    const int grid_no = ch.x / fa.ntiles, tl = ch.x - grid_no * fa.ntiles;   // ch.x: tile over all grids
+    const int tile_u0 = (tl % fa.ntx) * tile, tile_v0 = (tl / fa.ntx) * tile;
+    model += (size_t)grid_no * fa.G * fa.G;
+    const int lane = threadIdx.x % kWave;
+    const int row = lane >> 4;
+    const uint2 *fr = frecs + ch.y;
+    const float *meas1 = reinterpret_cast<const float *>(measured);
+    float *out1 = reinterpret_cast<float *>(vis_out);
+    Acc run[PMAX];
+    unsigned voff[PMAX];
+    float2 g[PMAX];
+    int pab[PMAX];   // the lane's footprint position a | b << 8 in each part; -1 when unused
+    // Window cell (x, y) is reached only from position (x mod W, y mod W):
+    // each cell has one owning lane, which alone zeroes, sums and writes it;
+    // in a run with first cell (t_u, t_v) position (a, b) sums into window
+    // cell (a + (a < t_u ? W : 0), b + (b < t_v ? W : 0)).
+    auto cell_of = [&](int a, int b, int tu, int tv) { return (b + (b < tv ? W : 0)) * S + a + (a < tu ? W : 0); };
+#pragma unroll
+    for (int k = 0; k < PMAX; ++k) {
+        acc_zero(run[k]);
+        voff[k] = (unsigned)kOob;
+        g[k] = make_float2(0.f, 0.f);
+        const int p = k * kWave + lane;
+        const bool act = k < P && p < W * W;
+        pab[k] = act ? (p % W) | (p / W) << 8 : -1;
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int u = (pab[k] & 0xff) + (q & 1) * W, v = (pab[k] >> 8) + (q >> 1) * W;
+                if (u < S && v < S) acc_zero(cells[v * S + u]);
+            }
+        }
+    }
+    int ptu = -1, ptv = 0;   // the previous run's (t_u, t_v); -1: none yet
+    int j = 0;
+    while (j < ch.z) {
+        const uint2 r0 = fr[j];
+        const int len = max(1, min((int)(r0.y >> 11), ch.z - j));
+        const int tu = (int)(r0.y & 0x1fu), tv = (int)((r0.y >> 5) & 0x1fu);
+        const bool conj = (r0.y & 0x400u) != 0u;
+#pragma unroll
+        for (int k = 0; k < PMAX; ++k) {
+            // opaque to the optimiser: what is formed from the position is
+            // formed again per run, not hoisted into registers held all along
+            asm volatile("" : "+v"(pab[k]));
+            const bool act = pab[k] >= 0;
+            const int a = act ? pab[k] & 0xff : 0, b = act ? pab[k] >> 8 : 0;
+            if (ptu >= 0 && act) acc_merge(cells[cell_of(a, b, ptu, ptv)], run[k]);
+            const int du = a >= tu ? a - tu : a - tu + W, dv = b >= tv ? b - tv : b - tv + W;
+            voff[k] = act ? (unsigned)(8 * (dv * n + du)) : (unsigned)kOob;
+            // the model value at the lane's cell for this run
+            const int gu = tile_u0 + a + (a < tu ? W : 0), gv = tile_v0 + b + (b < tv ? W : 0);
+            g[k] = (act && gu < fa.G && gv < fa.G) ? model[(size_t)gv * fa.G + gu] : make_float2(0.f, 0.f);
+            acc_zero(run[k]);
+        }
+        ptu = tu;
+        ptv = tv;
+        auto run_body = [&](auto conj_tag) {
+            constexpr bool CONJ = decltype(conj_tag)::value;
+            auto step = [&](auto ucount, int i0) {
+                constexpr int U = decltype(ucount)::value;
+                const int q0 = j + i0;   // the step's first record, from the work item's first
+                // the visibility (rv) and component (rc: 0 real, 1 imaginary) of this lane's row
+                const int rv = U == 4 ? row : U == 2 ? row >> 1 : 0;
+                const int rc = U == 2 ? (row & 1) : U == 1 ? (row >> 1) : 0;
+                const uint32_t o = perm ? perm[ch.y + q0 + rv] : (uint32_t)(ch.y + q0 + rv);
+                float2 kk[U][PMAX], kd[U][PMAX];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t base = fr[q0 + u].x;
+#pragma unroll
+                    for (int k = 0; k < PMAX; ++k) kk[u][k] = utab_load(rg, voff[k], base);
+                    if (OWN_D) {
+#pragma unroll
+                        for (int k = 0; k < PMAX; ++k) kd[u][k] = utab_load(rd, voff[k], base);
+                    }
+                }
+                float2 m2 = make_float2(0.f, 0.f);
+                float m1 = 0.f;
+                if (U == 4) m2 = measured[o];
+                else m1 = meas1[2 * (size_t)o + rc];
+                // model * (degridding sample) over the lane's taps
+                float pr[U], pim[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    pr[u] = 0.f;
+                    pim[u] = 0.f;
+#pragma unroll
+                    for (int k = 0; k < PMAX; ++k) {
+                        const float2 q = OWN_D ? kd[u][k] : make_float2(kk[u][k].x, -kk[u][k].y);
+                        pr[u] = fmaf(g[k].x, q.x, pr[u]);
+                        pr[u] = fmaf(CONJ ? g[k].y : -g[k].y, q.y, pr[u]);
+                        pim[u] = fmaf(g[k].y, q.x, pim[u]);
+                        pim[u] = fmaf(CONJ ? -g[k].x : g[k].x, q.y, pim[u]);
+                    }
+                }
+                float vr[U], vi[U];
+                if constexpr (U == 4) {
+                    const float b0 = m2.x - row_total(fold16(fold32(pr[0], pr[2]), fold32(pr[1], pr[3])));
+                    const float b1 = m2.y - row_total(fold16(fold32(pim[0], pim[2]), fold32(pim[1], pim[3])));
+                    if (vis_out && (lane & 15) == 0) vis_out[o] = make_float2(b0, b1);
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        vr[u] = lane_value(b0, 16 * u);
+                        vi[u] = lane_value(b1, 16 * u);
+                    }
+                } else if constexpr (U == 2) {
+                    const float b = m1 - row_total(fold16(fold32(pr[0], pr[1]), fold32(pim[0], pim[1])));
+                    if (vis_out && (lane & 15) == 0) out1[2 * (size_t)o + rc] = b;
+                    vr[0] = lane_value(b, 0);
+                    vi[0] = lane_value(b, 16);
+                    vr[1] = lane_value(b, 32);
+                    vi[1] = lane_value(b, 48);
+                } else {
+                    const float a = fold32(pr[0], pim[0]);
+                    const float b = m1 - row_total(fold16(a, a));
+                    if (vis_out && (lane & 31) == 0) out1[2 * (size_t)o + rc] = b;
+                    vr[0] = lane_value(b, 0);
+                    vi[0] = lane_value(b, 32);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+#pragma unroll
+                    for (int k = 0; k < PMAX; ++k) tap_add_s<CONJ>(run[k], kk[u][k], vr[u], vi[u]);
+                }
+            };
+            // visibilities per step: the step's table samples stay in registers
+            // until the residuals are known (twice as many with OWN_D)
+            constexpr int UB = PMAX <= 2 ? 4 : PMAX <= 8 ? 2 : 1;
+            constexpr int UU = OWN_D && UB > 1 ? UB / 2 : UB;
+            int i = 0;
+            for (; i + UU <= len; i += UU) step(std::integral_constant<int, UU>{}, i);
+            for (; i < len; ++i) step(std::integral_constant<int, 1>{}, i);
+        };
+        if (conj) run_body(std::true_type{});
+        else run_body(std::false_type{});
+        j += len;
+    }
+    float2 *slab = slabs + slab_off[ci];
+#pragma unroll
+    for (int k = 0; k < PMAX; ++k) {
+        if (pab[k] >= 0) {
+            const int a = pab[k] & 0xff, b = pab[k] >> 8;
+            if (ptu >= 0) acc_merge(cells[cell_of(a, b, ptu, ptv)], run[k]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int u = a + (q & 1) * W, v = b + (q >> 1) * W;
+                if (u < S && v < S) slab[v * S + u] = acc_value(cells[v * S + u]);
+            }
+        }
+    }
+}
+
+// One wavefront per work item; each holds its window in LDS.  Whether the
+// degridding table is the conjugate of the gridding table is a device flag
+// (set by prep_tables_kernel), read once: the branch is uniform.
+// resident wavefronts per SIMD the register allocation aims at
+template <bool COMP, int PMAX> constexpr int fused_occupancy()
+{
+    return PMAX <= 4 ? 8 : PMAX <= 6 ? (COMP ? 5 : 6) : PMAX <= 8 ? (COMP ? 4 : 5) : PMAX <= 9 ? 4 : (COMP ? 2 : 3);
+}
 template <bool COMP, int PMAX>
 __global__ void __launch_bounds__(kWave * fused_waves<PMAX>())
+__attribute__((amdgpu_waves_per_eu(fused_occupancy<COMP, PMAX>())))
 fast_degrid_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restrict__ perm,
                         const int4 *__restrict__ chunks, const long long *__restrict__ slab_off,
                         const float2 *__restrict__ measured, const float2 *__restrict__ model,
@@ -1105,128 +1334,16 @@ fast_degrid_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restr
     if (ci >= fa.chunk_e) return;
     Acc *cells = reinterpret_cast<Acc *>(lds4) + (size_t)wave * fa.wave_cells;   // the window, row-major
     const int4 ch = chunks[ci];
-    const int S = ch.w, tile = fa.tile, W = S - tile + 1, n = W + 1, P = (W * W + kWave - 1) / kWave;
-    const int grid_no = ch.x / fa.ntiles, tl = ch.x - grid_no * fa.ntiles;   // ch.x: tile over all grids
-    const int tile_u0 = (tl % fa.ntx) * tile, tile_v0 = (tl / fa.ntx) * tile;
-    model += (size_t)grid_no * fa.G * fa.G;
-    const int lane = threadIdx.x % kWave;
     const __amdgpu_buffer_rsrc_t rg =
         __builtin_amdgcn_make_buffer_rsrc((void *)utab_g, (short)0, fa.tab_bytes, 0x00020000);
     const __amdgpu_buffer_rsrc_t rd =
         __builtin_amdgcn_make_buffer_rsrc((void *)utab_d, (short)0, fa.tab_bytes, 0x00020000);
-    const bool own_d = mismatch[kConjBlocks] != 0;   // a degridding table of its own
-    const uint2 *fr = frecs + ch.y;
-    Acc run[PMAX];
-    unsigned voff[PMAX];
-    int cell[PMAX];           // window cell (row-major) of the lane's tap in the current run
-    float2 g[PMAX];
-    int pa[PMAX], pb[PMAX];   // the lane's footprint position (a, b) in each part; a = -1 when unused
-    // Window cell (x, y) is reached only from position (x mod W, y mod W):
-    // each cell has one owning lane, which alone zeroes, sums and writes it.
-#pragma unroll
-    for (int k = 0; k < PMAX; ++k) {
-        acc_zero(run[k]);
-        voff[k] = (unsigned)kOob;
-        cell[k] = 0;
-        g[k] = make_float2(0.f, 0.f);
-        const int p = k * kWave + lane;
-        const bool act = k < P && p < W * W;
-        pa[k] = act ? p % W : -1;
-        pb[k] = act ? p / W : 0;
-        if (act) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int u = pa[k] + (q & 1) * W, v = pb[k] + (q >> 1) * W;
-                if (u < S && v < S) acc_zero(cells[v * S + u]);
-            }
-        }
-    }
-    bool started = false;
-    int j = 0;
-    while (j < ch.z) {
-        const uint2 r0 = fr[j];
-        const int len = max(1, min((int)(r0.y >> 11), ch.z - j));
-        const int tu = (int)(r0.y & 0x1fu), tv = (int)((r0.y >> 5) & 0x1fu);
-        const bool conj = (r0.y & 0x400u) != 0u;
-#pragma unroll
-        for (int k = 0; k < PMAX; ++k) {
-            const bool act = pa[k] >= 0;
-            if (started && act) acc_merge(cells[cell[k]], run[k]);
-            const int a = act ? pa[k] : 0, b = pb[k];
-            const int du = a >= tu ? a - tu : a - tu + W, dv = b >= tv ? b - tv : b - tv + W;
-            voff[k] = act ? (unsigned)(8 * (dv * n + du)) : (unsigned)kOob;
-            cell[k] = (b + (b < tv ? W : 0)) * S + a + (a < tu ? W : 0);
-            // the model value at the lane's cell for this run
-            const int gu = tile_u0 + a + (a < tu ? W : 0), gv = tile_v0 + b + (b < tv ? W : 0);
-            g[k] = (act && gu < fa.G && gv < fa.G) ? model[(size_t)gv * fa.G + gu] : make_float2(0.f, 0.f);
-            acc_zero(run[k]);
-        }
-        started = true;
-        const float cs = conj ? -1.f : 1.f;
-        // U visibilities at a time: their loads issued together and their
-        // wavefront reductions interleaved
-        auto step = [&](auto ucount, int i0) {
-            constexpr int U = decltype(ucount)::value;
-            float2 kk[U][PMAX], kd[U][PMAX];
-            uint32_t o[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t base = fr[j + i0 + u].x;
-                o[u] = perm ? perm[ch.y + j + i0 + u] : (uint32_t)(ch.y + j + i0 + u);
-#pragma unroll
-                for (int k = 0; k < PMAX; ++k) kk[u][k] = utab_load(rg, voff[k], base);
-                if (own_d) {
-#pragma unroll
-                    for (int k = 0; k < PMAX; ++k) kd[u][k] = utab_load(rd, voff[k], base);
-                }
-            }
-            float sr[U], si[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                // model * (kd.x, conj kd.y) over the lane's taps
-                float sxx = 0.f, syy = 0.f, syx = 0.f, sxy = 0.f;
-#pragma unroll
-                for (int k = 0; k < PMAX; ++k) {
-                    const float2 q = own_d ? kd[u][k] : make_float2(kk[u][k].x, -kk[u][k].y);
-                    sxx = fmaf(g[k].x, q.x, sxx);
-                    syy = fmaf(g[k].y, q.y, syy);
-                    syx = fmaf(g[k].y, q.x, syx);
-                    sxy = fmaf(g[k].x, q.y, sxy);
-                }
-                sr[u] = fmaf(-cs, syy, sxx);
-                si[u] = fmaf(cs, sxy, syx);
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) group_sum(sr[u], si[u], 6);
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const float2 m = measured[o[u]];
-                const float vr = m.x - sr[u], vi = m.y - si[u];
-                if (vis_out && lane == 0) vis_out[o[u]] = make_float2(vr, vi);
-                const float cvr = cs * vr, cvi = cs * vi;
-#pragma unroll
-                for (int k = 0; k < PMAX; ++k) tap_add(run[k], kk[u][k], vr, vi, cvr, cvi);
-            }
-        };
-        constexpr int UU = PMAX <= 4 ? 4 : PMAX <= 8 ? 2 : 1;
-        int i = 0;
-        for (; i + UU <= len; i += UU) step(std::integral_constant<int, UU>{}, i);
-        for (; i < len; ++i) step(std::integral_constant<int, 1>{}, i);
-        j += len;
-    }
-    float2 *slab = slabs + slab_off[ci];
-#pragma unroll
-    for (int k = 0; k < PMAX; ++k) {
-        if (pa[k] >= 0) {
-            if (started) acc_merge(cells[cell[k]], run[k]);
-            const int a = pa[k], b = pb[k];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int u = a + (q & 1) * W, v = b + (q >> 1) * W;
-                if (u < S && v < S) slab[v * S + u] = acc_value(cells[v * S + u]);
-            }
-        }
-    }
+    if (mismatch[kConjBlocks] != 0)
+        fused_work_item<COMP, PMAX, true>(frecs, perm, ch, ci, slab_off, measured, model, vis_out, rg, rd, fa, slabs,
+                                          cells);
+    else
+        fused_work_item<COMP, PMAX, false>(frecs, perm, ch, ci, slab_off, measured, model, vis_out, rg, rd, fa,
+                                           slabs, cells);
 }
 
 constexpr int kMergeThreads = 256;
@@ -1784,10 +1901,12 @@ int launch_fused_group(const sdp_grid_plan *plan, int b, int e, const ChunkIO &i
 
 // the fused kernel's launches: exact-footprint classes grouped by the parts
 // of 64 footprint positions their work items need, P = ceil((2h)^2 / 64):
-// h <= 4 one part, 5 two, 6 three, 7-8 four, 9 six, 10-11 eight, 12-16 up
-// to sixteen (a group's kernel runs all PMAX parts; unused ones read zeros)
-constexpr int kFusedGroups = 7;
-constexpr int kFusedLo[kFusedGroups] = {1, 5, 6, 7, 9, 10, 12}, kFusedHi[kFusedGroups] = {4, 5, 6, 8, 9, 11, 16};
+// h <= 4 one part, 5 two, 6 three, 7-8 four, 9 six, 10 seven, 11 eight, 12
+// nine, 13-16 up to sixteen (a group's kernel runs all PMAX parts; unused
+// ones read zeros, so only the rare h >= 13 group has any)
+constexpr int kFusedGroups = 9;
+constexpr int kFusedLo[kFusedGroups] = {1, 5, 6, 7, 9, 10, 11, 12, 13};
+constexpr int kFusedHi[kFusedGroups] = {4, 5, 6, 8, 9, 10, 11, 12, 16};
 template <bool COMP>
 int launch_fused(const sdp_grid_plan *plan, int group, const ChunkIO &io, const uint32_t *perm, hipStream_t st)
 {
@@ -1799,7 +1918,9 @@ int launch_fused(const sdp_grid_plan *plan, int group, const ChunkIO &io, const 
     case 2: return launch_fused_group<COMP, 3>(plan, b, e, io, perm, st);
     case 3: return launch_fused_group<COMP, 4>(plan, b, e, io, perm, st);
     case 4: return launch_fused_group<COMP, 6>(plan, b, e, io, perm, st);
-    case 5: return launch_fused_group<COMP, 8>(plan, b, e, io, perm, st);
+    case 5: return launch_fused_group<COMP, 7>(plan, b, e, io, perm, st);
+    case 6: return launch_fused_group<COMP, 8>(plan, b, e, io, perm, st);
+    case 7: return launch_fused_group<COMP, 9>(plan, b, e, io, perm, st);
     default: return launch_fused_group<COMP, 16>(plan, b, e, io, perm, st);
     }
 }
