@@ -1066,16 +1066,34 @@ __global__ void __launch_bounds__(256) prep_tables_kernel(const int2 *__restrict
     }
 }
 
-// waves per workgroup and LDS of the fused kernel: a wave covers all W^2
-// footprint positions of its work item in P <= PMAX parts of 64 and holds
-// the work item's S x S window in LDS.  An exact-footprint class has
-// 2h + 1 >= tile, so S = tile + 2h - 1 <= 4h and S^2 <= PMAX * 4 * 64: the
-// bound below.
-template <int PMAX> constexpr int fused_waves() { return PMAX <= 4 ? 4 : PMAX <= 8 ? 2 : 1; }
-template <bool COMP, int PMAX> constexpr size_t fused_lds()
-{
-    return (size_t)fused_waves<PMAX>() * PMAX * 4 * kWave * (COMP ? sizeof(float4) : sizeof(float2));
-}
+// ---- fused degrid + subtract + grid: tap ownership -------------------------
+// Lanes own footprint taps.  The W^2 taps (W = 2h) of a visibility are taken
+// row-major in "pair parts" and at most one "single part": in a pair part
+// lane l holds taps 2m and 2m + 1 (m = 64 k + l; W is even, so both sit in
+// one footprint row and in adjacent samples of the unfolded table) and reads
+// them with one 16-byte load; in the single part lane l holds one tap.  The
+// vector L1 serves a 64-lane load instruction in about the same time whether
+// each lane reads 8 or 16 bytes, so pairs halve the load time of a tap.  A
+// class uses q = W^2 / 128 pair parts and, for the r = W^2 mod 128 taps left,
+// one single part when r <= 64, else one more pair part.  Shape <Q, H64>: up
+// to Q pair parts and a single part.
+template <int Q, bool H64> struct FusedShape {
+    static constexpr int taps = 128 * Q + (H64 ? 64 : 0);   // the largest W^2 the shape holds
+    // waves per workgroup; each holds its work item's S x S window in LDS.  An
+    // exact-footprint class has 2h + 1 >= tile, so S = tile + W - 1 <= 2 W and
+    // S^2 <= 4 W^2 <= 4 taps
+    static constexpr int waves = taps <= 256 ? 4 : taps <= 512 ? 2 : 1;
+    template <bool COMP> static constexpr size_t lds()
+    {
+        return (size_t)waves * 4 * taps * (COMP ? sizeof(float4) : sizeof(float2));
+    }
+    // visibilities per step (their table samples stay in registers until
+    // the residuals are known; half as many when a degridding table of its
+    // own doubles the samples)
+    static constexpr int ub = Q <= 1 ? 4 : Q <= 4 ? 2 : 1;
+    // resident wavefronts per SIMD the register allocation aims at
+    template <bool COMP> static constexpr int occupancy() { return Q <= 1 ? 8 : Q <= 2 ? 6 : Q <= 4 ? (COMP ? 4 : 5) : (COMP ? 2 : 3); }
+};
 
 struct FusedArgs {
     int chunk_b, chunk_e, tile, G, ntx, ntiles, tab_bytes;
@@ -1130,14 +1148,38 @@ template <bool CONJ> __device__ __forceinline__ void tap_add_s(float4 &t, float2
     two_sum_acc(t.x, t.z, fmaf(vr, k.x, (CONJ ? vi : -vi) * k.y));
     two_sum_acc(t.y, t.w, fmaf(vi, k.x, (CONJ ? -vr : vr) * k.y));
 }
+// model * (degridding sample) added to the prediction partials; the sample
+// is the degridding table's own, or the conjugate of the gridding sample
+template <bool CONJ> __device__ __forceinline__ void pred_add(float &pr, float &pim, float2 g, float2 q)
+{
+    pr = fmaf(g.x, q.x, pr);
+    pr = fmaf(CONJ ? g.y : -g.y, q.y, pr);
+    pim = fmaf(g.y, q.x, pim);
+    pim = fmaf(CONJ ? -g.x : g.x, q.y, pim);
+}
+
+// two adjacent unfolded-table samples at byte offset voff + soff
+__device__ __forceinline__ float4 utab_load2(__amdgpu_buffer_rsrc_t rs, unsigned voff, unsigned soff)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
 
 // One work item of the fused pass: for each of its visibilities (in plan
 // order) the prediction sum over the footprint -- each lane's taps read the
-// model grid at the lane's own cells, held in registers for the run -- is
+// model grid at their window cells, held in registers for the run -- is
 // reduced over the wavefront, subtracted from the measured value, and the
-// residual gridded with the same table samples into the lane's running sums
-// (as fast_grid_kernel).  OWN_D: the degridding table is not the conjugate
-// of the gridding table, and each tap reads a sample of both.
+// residual gridded with the same table samples into the lanes' running sums.
+// OWN_D: the degridding table is not the conjugate of the gridding table,
+// and each tap reads a sample of both.
+//
+// Runs and cells.  A run's visibilities share their first window cell
+// (t_u, t_v), so tap (d_u, d_v) of each of them lands on window cell
+// (t_u + d_u, t_v + d_v): a lane's taps keep their cells, model values and
+// table offsets for the run and sum into registers, which are added into
+// the LDS window when the run ends.  Within a run the taps of a lane are
+// distinct cells from every other lane's, and runs are added one after the
+// other, so every cell receives its runs' sums in run order.
 //
 // Visibilities go U at a time.  After the folds the lanes of row r hold
 // (U = 4) visibility r's real part in one register and its imaginary part in
@@ -1145,7 +1187,7 @@ template <bool CONJ> __device__ __forceinline__ void tap_add_s(float4 &t, float2
 // rows (re, re, im, im).  Each lane loads the measured value of its row, the
 // row's first lane stores the residual, and the residuals are then read into
 // scalar registers for the gridding products.
-template <bool COMP, int PMAX, bool OWN_D>
+template <bool COMP, int Q, bool H64, bool OWN_D>
 __device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs, const uint32_t *__restrict__ perm,
                                                 const int4 ch, int ci, const long long *__restrict__ slab_off,
                                                 const float2 *__restrict__ measured,
@@ -1155,7 +1197,8 @@ __device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs,
                                                 typename AccT<COMP>::T *cells)
 {
     using Acc = typename AccT<COMP>::T;
-    const int S = ch.w, tile = fa.tile, W = S - tile + 1, n = W + 1, P = (W * W + kWave - 1) / kWave;
+    constexpr int NP = Q + (H64 ? 1 : 0);   // parts; part Q is the single part
+    const int S = ch.w, tile = fa.tile, W = S - tile + 1, n = W + 1, WW = W * W;
     const int grid_no = ch.x / fa.ntiles, tl = ch.x - grid_no * fa.ntiles;   // ch.x: tile over all grids
     const int tile_u0 = (tl % fa.ntx) * tile, tile_v0 = (tl / fa.ntx) * tile;
     model += (size_t)grid_no * fa.G * fa.G;
@@ -1164,52 +1207,65 @@ __device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs,
     const uint2 *fr = frecs + ch.y;
     const float *meas1 = reinterpret_cast<const float *>(measured);
     float *out1 = reinterpret_cast<float *>(vis_out);
-    Acc run[PMAX];
-    unsigned voff[PMAX];
-    float2 g[PMAX];
-    int pab[PMAX];   // the lane's footprint position a | b << 8 in each part; -1 when unused
-    // Window cell (x, y) is reached only from position (x mod W, y mod W):
-    // each cell has one owning lane, which alone zeroes, sums and writes it;
-    // in a run with first cell (t_u, t_v) position (a, b) sums into window
-    // cell (a + (a < t_u ? W : 0), b + (b < t_v ? W : 0)).
-    auto cell_of = [&](int a, int b, int tu, int tv) { return (b + (b < tv ? W : 0)) * S + a + (a < tu ? W : 0); };
+    // taps in pair parts; the rest (<= 64) in the single part
+    const int r128 = WW & 127;
+    const int n_pair = (r128 == 0 || r128 > 64) ? WW / 2 : WW / 128 * 64;
+    const int n_single = WW - 2 * n_pair;
+    Acc run[NP][2];
+    unsigned voff[NP];
+    float2 g[NP][2];
+    int tp[NP];   // the lane's first tap of each part, d_u | d_v << 8; -1 when unused
 #pragma unroll
-    for (int k = 0; k < PMAX; ++k) {
-        acc_zero(run[k]);
+    for (int k = 0; k < NP; ++k) {
+        int t = -1;
+        if (k < Q) {
+            const int m = k * kWave + lane;
+            if (m < n_pair) t = 2 * m;
+        } else if (lane < n_single) {
+            t = 2 * n_pair + lane;
+        }
+        tp[k] = t >= 0 ? (t % W) | (t / W) << 8 : -1;
         voff[k] = (unsigned)kOob;
-        g[k] = make_float2(0.f, 0.f);
-        const int p = k * kWave + lane;
-        const bool act = k < P && p < W * W;
-        pab[k] = act ? (p % W) | (p / W) << 8 : -1;
-        if (act) {
+        acc_zero(run[k][0]);
+        acc_zero(run[k][1]);
+        g[k][0] = g[k][1] = make_float2(0.f, 0.f);
+    }
+    for (int c = lane; c < S * S; c += kWave) acc_zero(cells[c]);
+    int ptu = -1, ptv = 0;   // the previous run's (t_u, t_v); -1: none yet
+    // the run's sums of a lane's taps into their window cells at (tu, tv)
+    auto merge_run = [&](int tu, int tv) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int u = (pab[k] & 0xff) + (q & 1) * W, v = (pab[k] >> 8) + (q >> 1) * W;
-                if (u < S && v < S) acc_zero(cells[v * S + u]);
+        for (int k = 0; k < NP; ++k) {
+            if (tp[k] >= 0) {
+                const int c = (tv + (tp[k] >> 8)) * S + tu + (tp[k] & 0xff);
+                acc_merge(cells[c], run[k][0]);
+                if (k < Q) acc_merge(cells[c + 1], run[k][1]);
             }
         }
-    }
-    int ptu = -1, ptv = 0;   // the previous run's (t_u, t_v); -1: none yet
+    };
     int j = 0;
     while (j < ch.z) {
         const uint2 r0 = fr[j];
         const int len = max(1, min((int)(r0.y >> 11), ch.z - j));
         const int tu = (int)(r0.y & 0x1fu), tv = (int)((r0.y >> 5) & 0x1fu);
         const bool conj = (r0.y & 0x400u) != 0u;
+        if (ptu >= 0) merge_run(ptu, ptv);
 #pragma unroll
-        for (int k = 0; k < PMAX; ++k) {
-            // opaque to the optimiser: what is formed from the position is
-            // formed again per run, not hoisted into registers held all along
-            asm volatile("" : "+v"(pab[k]));
-            const bool act = pab[k] >= 0;
-            const int a = act ? pab[k] & 0xff : 0, b = act ? pab[k] >> 8 : 0;
-            if (ptu >= 0 && act) acc_merge(cells[cell_of(a, b, ptu, ptv)], run[k]);
-            const int du = a >= tu ? a - tu : a - tu + W, dv = b >= tv ? b - tv : b - tv + W;
+        for (int k = 0; k < NP; ++k) {
+            // opaque to the optimiser: what is formed from the tap is formed
+            // again per run, not hoisted into registers held all along
+            asm volatile("" : "+v"(tp[k]));
+            const bool act = tp[k] >= 0;
+            const int du = act ? tp[k] & 0xff : 0, dv = act ? tp[k] >> 8 : 0;
             voff[k] = act ? (unsigned)(8 * (dv * n + du)) : (unsigned)kOob;
-            // the model value at the lane's cell for this run
-            const int gu = tile_u0 + a + (a < tu ? W : 0), gv = tile_v0 + b + (b < tv ? W : 0);
-            g[k] = (act && gu < fa.G && gv < fa.G) ? model[(size_t)gv * fa.G + gu] : make_float2(0.f, 0.f);
-            acc_zero(run[k]);
+            // the model values at the taps' cells for this run
+            const int gu = tile_u0 + tu + du, gv = tile_v0 + tv + dv;
+            const float2 *mrow = model + (size_t)gv * fa.G;
+            const bool in_v = act && gv < fa.G;
+            g[k][0] = (in_v && gu < fa.G) ? mrow[gu] : make_float2(0.f, 0.f);
+            if (k < Q) g[k][1] = (in_v && gu + 1 < fa.G) ? mrow[gu + 1] : make_float2(0.f, 0.f);
+            acc_zero(run[k][0]);
+            acc_zero(run[k][1]);
         }
         ptu = tu;
         ptv = tv;
@@ -1222,34 +1278,40 @@ __device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs,
                 const int rv = U == 4 ? row : U == 2 ? row >> 1 : 0;
                 const int rc = U == 2 ? (row & 1) : U == 1 ? (row >> 1) : 0;
                 const uint32_t o = perm ? perm[ch.y + q0 + rv] : (uint32_t)(ch.y + q0 + rv);
-                float2 kk[U][PMAX], kd[U][PMAX];
+                float4 kk[U][NP], kd[U][NP];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const uint32_t base = fr[q0 + u].x;
 #pragma unroll
-                    for (int k = 0; k < PMAX; ++k) kk[u][k] = utab_load(rg, voff[k], base);
-                    if (OWN_D) {
-#pragma unroll
-                        for (int k = 0; k < PMAX; ++k) kd[u][k] = utab_load(rd, voff[k], base);
+                    for (int k = 0; k < NP; ++k) {
+                        if (k < Q) {
+                            kk[u][k] = utab_load2(rg, voff[k], base);
+                            if (OWN_D) kd[u][k] = utab_load2(rd, voff[k], base);
+                        } else {
+                            const float2 a = utab_load(rg, voff[k], base);
+                            kk[u][k] = make_float4(a.x, a.y, 0.f, 0.f);
+                            if (OWN_D) {
+                                const float2 b = utab_load(rd, voff[k], base);
+                                kd[u][k] = make_float4(b.x, b.y, 0.f, 0.f);
+                            }
+                        }
                     }
                 }
                 float2 m2 = make_float2(0.f, 0.f);
                 float m1 = 0.f;
                 if (U == 4) m2 = measured[o];
                 else m1 = meas1[2 * (size_t)o + rc];
-                // model * (degridding sample) over the lane's taps
                 float pr[U], pim[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     pr[u] = 0.f;
                     pim[u] = 0.f;
 #pragma unroll
-                    for (int k = 0; k < PMAX; ++k) {
-                        const float2 q = OWN_D ? kd[u][k] : make_float2(kk[u][k].x, -kk[u][k].y);
-                        pr[u] = fmaf(g[k].x, q.x, pr[u]);
-                        pr[u] = fmaf(CONJ ? g[k].y : -g[k].y, q.y, pr[u]);
-                        pim[u] = fmaf(g[k].y, q.x, pim[u]);
-                        pim[u] = fmaf(CONJ ? -g[k].x : g[k].x, q.y, pim[u]);
+                    for (int k = 0; k < NP; ++k) {
+                        const float4 q = OWN_D ? kd[u][k]
+                                               : make_float4(kk[u][k].x, -kk[u][k].y, kk[u][k].z, -kk[u][k].w);
+                        pred_add<CONJ>(pr[u], pim[u], g[k][0], make_float2(q.x, q.y));
+                        if (k < Q) pred_add<CONJ>(pr[u], pim[u], g[k][1], make_float2(q.z, q.w));
                     }
                 }
                 float vr[U], vi[U];
@@ -1279,12 +1341,13 @@ __device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs,
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
 #pragma unroll
-                    for (int k = 0; k < PMAX; ++k) tap_add_s<CONJ>(run[k], kk[u][k], vr[u], vi[u]);
+                    for (int k = 0; k < NP; ++k) {
+                        tap_add_s<CONJ>(run[k][0], make_float2(kk[u][k].x, kk[u][k].y), vr[u], vi[u]);
+                        if (k < Q) tap_add_s<CONJ>(run[k][1], make_float2(kk[u][k].z, kk[u][k].w), vr[u], vi[u]);
+                    }
                 }
             };
-            // visibilities per step: the step's table samples stay in registers
-            // until the residuals are known (twice as many with OWN_D)
-            constexpr int UB = PMAX <= 2 ? 4 : PMAX <= 8 ? 2 : 1;
+            constexpr int UB = FusedShape<Q, H64>::ub;
             constexpr int UU = OWN_D && UB > 1 ? UB / 2 : UB;
             int i = 0;
             for (; i + UU <= len; i += UU) step(std::integral_constant<int, UU>{}, i);
@@ -1294,32 +1357,17 @@ __device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs,
         else run_body(std::false_type{});
         j += len;
     }
+    if (ptu >= 0) merge_run(ptu, ptv);
     float2 *slab = slabs + slab_off[ci];
-#pragma unroll
-    for (int k = 0; k < PMAX; ++k) {
-        if (pab[k] >= 0) {
-            const int a = pab[k] & 0xff, b = pab[k] >> 8;
-            if (ptu >= 0) acc_merge(cells[cell_of(a, b, ptu, ptv)], run[k]);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int u = a + (q & 1) * W, v = b + (q >> 1) * W;
-                if (u < S && v < S) slab[v * S + u] = acc_value(cells[v * S + u]);
-            }
-        }
-    }
+    for (int c = lane; c < S * S; c += kWave) slab[c] = acc_value(cells[c]);
 }
 
 // One wavefront per work item; each holds its window in LDS.  Whether the
 // degridding table is the conjugate of the gridding table is a device flag
 // (set by prep_tables_kernel), read once: the branch is uniform.
-// resident wavefronts per SIMD the register allocation aims at
-template <bool COMP, int PMAX> constexpr int fused_occupancy()
-{
-    return PMAX <= 4 ? 8 : PMAX <= 6 ? (COMP ? 5 : 6) : PMAX <= 8 ? (COMP ? 4 : 5) : PMAX <= 9 ? 4 : (COMP ? 2 : 3);
-}
-template <bool COMP, int PMAX>
-__global__ void __launch_bounds__(kWave * fused_waves<PMAX>())
-__attribute__((amdgpu_waves_per_eu(fused_occupancy<COMP, PMAX>())))
+template <bool COMP, int Q, bool H64>
+__global__ void __launch_bounds__((kWave * FusedShape<Q, H64>::waves))
+__attribute__((amdgpu_waves_per_eu(FusedShape<Q, H64>::template occupancy<COMP>())))
 fast_degrid_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restrict__ perm,
                         const int4 *__restrict__ chunks, const long long *__restrict__ slab_off,
                         const float2 *__restrict__ measured, const float2 *__restrict__ model,
@@ -1330,7 +1378,7 @@ fast_degrid_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restr
     using Acc = typename AccT<COMP>::T;
     extern __shared__ float4 lds4[];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int ci = fa.chunk_b + blockIdx.x * fused_waves<PMAX>() + wave;
+    const int ci = fa.chunk_b + blockIdx.x * FusedShape<Q, H64>::waves + wave;
     if (ci >= fa.chunk_e) return;
     Acc *cells = reinterpret_cast<Acc *>(lds4) + (size_t)wave * fa.wave_cells;   // the window, row-major
     const int4 ch = chunks[ci];
@@ -1339,11 +1387,11 @@ fast_degrid_grid_kernel(const uint2 *__restrict__ frecs, const uint32_t *__restr
     const __amdgpu_buffer_rsrc_t rd =
         __builtin_amdgcn_make_buffer_rsrc((void *)utab_d, (short)0, fa.tab_bytes, 0x00020000);
     if (mismatch[kConjBlocks] != 0)
-        fused_work_item<COMP, PMAX, true>(frecs, perm, ch, ci, slab_off, measured, model, vis_out, rg, rd, fa, slabs,
-                                          cells);
+        fused_work_item<COMP, Q, H64, true>(frecs, perm, ch, ci, slab_off, measured, model, vis_out, rg, rd, fa,
+                                            slabs, cells);
     else
-        fused_work_item<COMP, PMAX, false>(frecs, perm, ch, ci, slab_off, measured, model, vis_out, rg, rd, fa,
-                                           slabs, cells);
+        fused_work_item<COMP, Q, H64, false>(frecs, perm, ch, ci, slab_off, measured, model, vis_out, rg, rd, fa,
+                                             slabs, cells);
 }
 
 constexpr int kMergeThreads = 256;
@@ -1869,16 +1917,17 @@ int launch_fast(const sdp_grid_plan *plan, const float2 *vis, const uint32_t *pe
 }
 
 
-template <bool COMP, int PMAX>
+template <bool COMP, int Q, bool H64>
 int launch_fused_group(const sdp_grid_plan *plan, int b, int e, const ChunkIO &io, const uint32_t *perm,
                        hipStream_t st)
 {
     if (e <= b) return SDP_OK;
-    constexpr int waves = fused_waves<PMAX>();
-    constexpr size_t lds_max = fused_lds<COMP, PMAX>();
+    using Shape = FusedShape<Q, H64>;
+    constexpr int waves = Shape::waves;
+    constexpr size_t lds_max = Shape::template lds<COMP>();
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)fast_degrid_grid_kernel<COMP, PMAX>,
+        (void)hipFuncSetAttribute((const void *)fast_degrid_grid_kernel<COMP, Q, H64>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max);
         attr_set = true;
     }
@@ -1892,18 +1941,19 @@ int launch_fused_group(const sdp_grid_plan *plan, int b, int e, const ChunkIO &i
         return sdp::set_error(SDP_E_UNSUPPORTED, "window of %d cells exceeds the fused kernel's LDS", smax);
     FusedArgs fa{b, e, plan->tile, plan->p.grid_size, plan->ntx, plan->ntiles, (int)(8 * plan->utab_len),
                  smax * smax};
-    fast_degrid_grid_kernel<COMP, PMAX><<<(e - b + waves - 1) / waves, kWave * waves, lds, st>>>(
+    fast_degrid_grid_kernel<COMP, Q, H64><<<(e - b + waves - 1) / waves, kWave * waves, lds, st>>>(
         plan->frecs, perm, plan->chunks, plan->slab_off, io.measured, io.grid, io.vis_out, plan->utab, plan->utab_d,
         plan->conj_flag, fa, plan->slabs);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }
 
-// the fused kernel's launches: exact-footprint classes grouped by the parts
-// of 64 footprint positions their work items need, P = ceil((2h)^2 / 64):
-// h <= 4 one part, 5 two, 6 three, 7-8 four, 9 six, 10 seven, 11 eight, 12
-// nine, 13-16 up to sixteen (a group's kernel runs all PMAX parts; unused
-// ones read zeros, so only the rare h >= 13 group has any)
+// the fused kernel's launches: exact-footprint classes grouped by their
+// part shape (pair parts, single part) for W^2 = 4 h^2 taps: h <= 4 one
+// single part, 5 one pair part, 6 one pair part + single, 7-8 two pair
+// parts, 9 three, 10 three + single, 11 four, 12 four + single, 13-16 up to
+// eight + single (a group's kernel runs all its parts; unused ones read
+// zeros, so only the rare h >= 13 group has any)
 constexpr int kFusedGroups = 9;
 constexpr int kFusedLo[kFusedGroups] = {1, 5, 6, 7, 9, 10, 11, 12, 13};
 constexpr int kFusedHi[kFusedGroups] = {4, 5, 6, 8, 9, 10, 11, 12, 16};
@@ -1913,15 +1963,15 @@ int launch_fused(const sdp_grid_plan *plan, int group, const ChunkIO &io, const 
     const int b = plan->class_chunk_begin[fast_class(kFusedLo[group])];
     const int e = plan->class_chunk_begin[fast_class(kFusedHi[group]) + 1];
     switch (group) {
-    case 0: return launch_fused_group<COMP, 1>(plan, b, e, io, perm, st);
-    case 1: return launch_fused_group<COMP, 2>(plan, b, e, io, perm, st);
-    case 2: return launch_fused_group<COMP, 3>(plan, b, e, io, perm, st);
-    case 3: return launch_fused_group<COMP, 4>(plan, b, e, io, perm, st);
-    case 4: return launch_fused_group<COMP, 6>(plan, b, e, io, perm, st);
-    case 5: return launch_fused_group<COMP, 7>(plan, b, e, io, perm, st);
-    case 6: return launch_fused_group<COMP, 8>(plan, b, e, io, perm, st);
-    case 7: return launch_fused_group<COMP, 9>(plan, b, e, io, perm, st);
-    default: return launch_fused_group<COMP, 16>(plan, b, e, io, perm, st);
+    case 0: return launch_fused_group<COMP, 0, true>(plan, b, e, io, perm, st);
+    case 1: return launch_fused_group<COMP, 1, false>(plan, b, e, io, perm, st);
+    case 2: return launch_fused_group<COMP, 1, true>(plan, b, e, io, perm, st);
+    case 3: return launch_fused_group<COMP, 2, false>(plan, b, e, io, perm, st);
+    case 4: return launch_fused_group<COMP, 3, false>(plan, b, e, io, perm, st);
+    case 5: return launch_fused_group<COMP, 3, true>(plan, b, e, io, perm, st);
+    case 6: return launch_fused_group<COMP, 4, false>(plan, b, e, io, perm, st);
+    case 7: return launch_fused_group<COMP, 4, true>(plan, b, e, io, perm, st);
+    default: return launch_fused_group<COMP, 8, true>(plan, b, e, io, perm, st);
     }
 }
 
