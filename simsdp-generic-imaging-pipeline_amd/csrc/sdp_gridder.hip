@@ -2088,10 +2088,14 @@ int launch_chunks(const sdp_grid_plan *plan, const ChunkIO &io, const uint32_t *
     }
     int rc = SDP_OK;
     if (fast) rc = launch_fast(plan, io.vis, perm, st);
-    // the fused launches each fill the GPU: one after another on the caller's stream
-    for (int q = 0; q < kFusedGroups && rc == SDP_OK; ++q)
-        if (fused[q] > 0)
-            rc = comp ? launch_fused<true>(plan, q, io, perm, st) : launch_fused<false>(plan, q, io, perm, st);
+    // the fused launches alternate between the caller's stream and the
+    // helper stream, so that one launch's last wavefronts overlap the next
+    // launch's first instead of leaving the GPU part-idle at every boundary
+    for (int q = 0, nth = 0; q < kFusedGroups && rc == SDP_OK; ++q)
+        if (fused[q] > 0) {
+            hipStream_t s = (nth++ & 1) ? side : st;
+            rc = comp ? launch_fused<true>(plan, q, io, perm, s) : launch_fused<false>(plan, q, io, perm, s);
+        }
     for (size_t i = 0; i < ranges.size() && rc == SDP_OK; ++i) {
         hipStream_t s = (int)i == big ? st : side;
         rc = (comp || MODE == MODE_DEGRID) ? launch_class<MODE, true>(plan, ranges[i], io, perm, s)
