@@ -1211,6 +1211,14 @@ __device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs,
     const int r128 = WW & 127;
     const int n_pair = (r128 == 0 || r128 > 64) ? WW / 2 : WW / 128 * 64;
     const int n_single = WW - 2 * n_pair;
+    // Row mode (the one-pair-part + single-part shape, h = 6: 16 single
+    // taps): in a step of four visibilities the single part serves all four
+    // at once, row r of the wavefront holding visibility r's single taps.
+    // Row r's prediction partials join the folded sums of the same row, and
+    // its gridding products use the row's residual; the four rows' running
+    // sums are added together when the run ends.  Steps of fewer
+    // visibilities use row 0 only.
+    constexpr bool ROWS = !COMP && !OWN_D && Q == 1 && H64 && FusedShape<Q, H64>::ub == 4;
     Acc run[NP][2];
     unsigned voff[NP];
     float2 g[NP][2];
@@ -1221,8 +1229,8 @@ __device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs,
         if (k < Q) {
             const int m = k * kWave + lane;
             if (m < n_pair) t = 2 * m;
-        } else if (lane < n_single) {
-            t = 2 * n_pair + lane;
+        } else if (ROWS ? (lane & 15) < n_single : lane < n_single) {
+            t = 2 * n_pair + (ROWS ? lane & 15 : lane);
         }
         tp[k] = t >= 0 ? (t % W) | (t / W) << 8 : -1;
         voff[k] = (unsigned)kOob;
@@ -1234,8 +1242,15 @@ __device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs,
     int ptu = -1, ptv = 0;   // the previous run's (t_u, t_v); -1: none yet
     // the run's sums of a lane's taps into their window cells at (tu, tv)
     auto merge_run = [&](int tu, int tv) {
+        if constexpr (ROWS) {
+            // the four rows' single-part sums of each tap, added (every lane takes part)
+            Acc &t = run[Q][0];
+            t.x = fold16(fold32(t.x, t.x), fold32(t.x, t.x));
+            t.y = fold16(fold32(t.y, t.y), fold32(t.y, t.y));
+        }
 #pragma unroll
         for (int k = 0; k < NP; ++k) {
+            if (ROWS && k == Q && row != 0) continue;
             if (tp[k] >= 0) {
                 const int c = (tv + (tp[k] >> 8)) * S + tu + (tp[k] & 0xff);
                 acc_merge(cells[c], run[k][0]);
@@ -1278,20 +1293,32 @@ __device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs,
                 const int rv = U == 4 ? row : U == 2 ? row >> 1 : 0;
                 const int rc = U == 2 ? (row & 1) : U == 1 ? (row >> 1) : 0;
                 const uint32_t o = perm ? perm[ch.y + q0 + rv] : (uint32_t)(ch.y + q0 + rv);
+                // the single part serves the step's four rows at once (row mode)
+                constexpr bool RS = ROWS && U == 4;
                 float4 kk[U][NP], kd[U][NP];
+                uint32_t base[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) base[u] = fr[q0 + u].x;
+                float2 krow = make_float2(0.f, 0.f);   // row mode: the lane's row's single-part sample
+                if constexpr (RS) {
+                    const uint32_t brow = row == 0 ? base[0] : row == 1 ? base[1] : row == 2 ? base[2] : base[3];
+                    const auto v = __builtin_amdgcn_raw_buffer_load_b64(rg, voff[Q] + brow, 0, 0);
+                    krow = make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+                }
+                // outside row mode, a row-mode lane of rows 1-3 has no single tap
+                const unsigned vs = ROWS && !RS && row != 0 ? (unsigned)kOob : voff[NP - 1];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const uint32_t base = fr[q0 + u].x;
 #pragma unroll
                     for (int k = 0; k < NP; ++k) {
                         if (k < Q) {
-                            kk[u][k] = utab_load2(rg, voff[k], base);
-                            if (OWN_D) kd[u][k] = utab_load2(rd, voff[k], base);
-                        } else {
-                            const float2 a = utab_load(rg, voff[k], base);
+                            kk[u][k] = utab_load2(rg, voff[k], base[u]);
+                            if (OWN_D) kd[u][k] = utab_load2(rd, voff[k], base[u]);
+                        } else if (!RS) {
+                            const float2 a = utab_load(rg, vs, base[u]);
                             kk[u][k] = make_float4(a.x, a.y, 0.f, 0.f);
                             if (OWN_D) {
-                                const float2 b = utab_load(rd, voff[k], base);
+                                const float2 b = utab_load(rd, vs, base[u]);
                                 kd[u][k] = make_float4(b.x, b.y, 0.f, 0.f);
                             }
                         }
@@ -1308,6 +1335,7 @@ __device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs,
                     pim[u] = 0.f;
 #pragma unroll
                     for (int k = 0; k < NP; ++k) {
+                        if (RS && k == Q) continue;
                         const float4 q = OWN_D ? kd[u][k]
                                                : make_float4(kk[u][k].x, -kk[u][k].y, kk[u][k].z, -kk[u][k].w);
                         pred_add<CONJ>(pr[u], pim[u], g[k][0], make_float2(q.x, q.y));
@@ -1316,14 +1344,17 @@ __device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs,
                 }
                 float vr[U], vi[U];
                 if constexpr (U == 4) {
-                    const float b0 = m2.x - row_total(fold16(fold32(pr[0], pr[2]), fold32(pr[1], pr[3])));
-                    const float b1 = m2.y - row_total(fold16(fold32(pim[0], pim[2]), fold32(pim[1], pim[3])));
+                    float xr = 0.f, xi = 0.f;   // row mode: the row's single-part partials
+                    if constexpr (RS) pred_add<CONJ>(xr, xi, g[Q][0], make_float2(krow.x, -krow.y));
+                    const float b0 = m2.x - row_total(fold16(fold32(pr[0], pr[2]), fold32(pr[1], pr[3])) + xr);
+                    const float b1 = m2.y - row_total(fold16(fold32(pim[0], pim[2]), fold32(pim[1], pim[3])) + xi);
                     if (vis_out && (lane & 15) == 0) vis_out[o] = make_float2(b0, b1);
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
                         vr[u] = lane_value(b0, 16 * u);
                         vi[u] = lane_value(b1, 16 * u);
                     }
+                    if constexpr (RS) tap_add_s<CONJ>(run[Q][0], krow, b0, b1);
                 } else if constexpr (U == 2) {
                     const float b = m1 - row_total(fold16(fold32(pr[0], pr[1]), fold32(pim[0], pim[1])));
                     if (vis_out && (lane & 15) == 0) out1[2 * (size_t)o + rc] = b;
@@ -1342,6 +1373,7 @@ __device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs,
                 for (int u = 0; u < U; ++u) {
 #pragma unroll
                     for (int k = 0; k < NP; ++k) {
+                        if (RS && k == Q) continue;
                         tap_add_s<CONJ>(run[k][0], make_float2(kk[u][k].x, kk[u][k].y), vr[u], vi[u]);
                         if (k < Q) tap_add_s<CONJ>(run[k][1], make_float2(kk[u][k].z, kk[u][k].w), vr[u], vi[u]);
                     }
