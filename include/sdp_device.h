@@ -285,8 +285,14 @@ int sdp_hogbom(int grid_size, int n_minor, int max_sources, float *d_residual, c
  * differently only when the peak is within that drift of the threshold.
  * Both entry points also stop after max_sources components (the reference
  * keeps going and writes past its sources array); the first max_sources
- * iterations are the reference's. */
-enum { SDP_HOGBOM_PSF_AS_STORED = 1, SDP_HOGBOM_DOUBLE_MEAN = 2 };
+ * iterations are the reference's.
+ * When the side is a power of two up to 4096, the noise stop cannot fire
+ * and the stream is not being captured, all iterations run in one
+ * cooperative launch that keeps the residual in registers (the same results
+ * bit for bit); SDP_HOGBOM_PER_ITERATION forces one launch per iteration.
+ * A run that cannot complete its cross-workgroup exchange reports -1
+ * sources. */
+enum { SDP_HOGBOM_PSF_AS_STORED = 1, SDP_HOGBOM_DOUBLE_MEAN = 2, SDP_HOGBOM_PER_ITERATION = 4 };
 int sdp_hogbom_ex(int grid_size, int n_minor, int max_sources, float *d_residual, const float *d_psf,
                   double psf_max, double loop_gain, double noise_factor, double weak_source_percent,
                   double cell_size, const float *d_model_in, float *d_model_out, float *d_sources,

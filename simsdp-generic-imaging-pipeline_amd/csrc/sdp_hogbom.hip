@@ -45,7 +45,9 @@
 // which agrees with the reference except when the peak lies within that
 // drift of the threshold.
 #include <algorithm>
+#include <cstdint>
 #include <cstdlib>
+#include <type_traits>
 
 #include "sdp_internal.h"
 
@@ -400,6 +402,317 @@ hogbom_step_kernel(int iter, HogbomArgs a, float *__restrict__ r, const float *_
     }
 }
 
+// ---- one launch for all iterations: the residual held in registers ------
+//
+// A cooperative launch (every workgroup resident at once, one per CU at
+// most) runs all minor iterations.  Workgroup b owns the 16-byte groups
+// q = b * TPB NG + k * TPB + t (thread t, k < NG) and keeps them in
+// registers from the first iteration to the last, so an iteration reads
+// only the PSF window: the residual is read once and written once.
+//
+// Per iteration each workgroup finds its best pixel (largest |r|, first
+// index on ties: the largest |r| in one pass, then the first index holding
+// it), publishes it as one 64-bit word (|r| bits, index, sign) and arrives
+// at a counter; one thread per workgroup waits for the counter to reach
+// every workgroup's arrival, then every workgroup reduces all words the
+// same way, so all reach the same component and stop decision; workgroup 0
+// records the component and every workgroup subtracts it from its own
+// pixels.  Words are double-buffered by iteration parity: a workgroup can
+// publish iteration i + 2 into a buffer only after every workgroup has
+// arrived at iteration i + 1, that is after all have read iteration i.
+//
+// Waits are bounded: a counter that does not arrive within 20 ms (it cannot,
+// short of a fault elsewhere) sets an abort flag that every waiter also
+// checks, every workgroup leaves, and the run reports -1 sources.
+//
+// The arithmetic, tie order and stop tests are those of hogbom_step_kernel
+// (bit for bit); the noise stop is left out, so the form is taken only when
+// it cannot fire (noise_factor * gain <= 1/2).
+constexpr unsigned long long kWaitTicks = 2000000ull;   // 20 ms of the 100 MHz counter
+
+struct PersistArgs {
+    int G, n_minor, max_sources, nblocks, mirror, psf_pad_bytes, row_shift;   // row_shift: log2(G / 4)
+    double psf_max, gain, weak;
+};
+
+struct Best {
+    float v;     // |r|, > 0 to count
+    int i;       // linear index, INT_MAX when none
+    float val;   // r
+};
+
+__device__ __forceinline__ void take_better(Best &b, const Best &o)
+{
+    if (o.v > 0.f && better(o.v, o.i, b.v, b.i)) b = o;
+}
+
+// a 64-lane wavefront's largest float / smallest int in every lane: DPP
+// moves within 16-lane rows, then the gfx950 row swaps across them (no
+// lane-address registers)
+template <int CTRL> __device__ __forceinline__ int dpp_mov(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int swap_rows16(int x)
+{
+    return (int)__builtin_amdgcn_permlane16_swap((unsigned)x, (unsigned)x, false, false)[1];
+}
+__device__ __forceinline__ int swap_rows32(int x)
+{
+    return (int)__builtin_amdgcn_permlane32_swap((unsigned)x, (unsigned)x, false, false)[1];
+}
+__device__ __forceinline__ float wave_max(float x)
+{
+    x = fmaxf(x, __int_as_float(dpp_mov<0xb1>(__float_as_int(x))));    // quad: lanes ^ 1
+    x = fmaxf(x, __int_as_float(dpp_mov<0x4e>(__float_as_int(x))));    // quad: lanes ^ 2
+    x = fmaxf(x, __int_as_float(dpp_mov<0x141>(__float_as_int(x))));   // half-row mirror
+    x = fmaxf(x, __int_as_float(dpp_mov<0x140>(__float_as_int(x))));   // row mirror
+    x = fmaxf(x, __int_as_float(swap_rows16(__float_as_int(x))));
+    return fmaxf(x, __int_as_float(swap_rows32(__float_as_int(x))));
+}
+__device__ __forceinline__ int wave_min(int x)
+{
+    x = min(x, dpp_mov<0xb1>(x));
+    x = min(x, dpp_mov<0x4e>(x));
+    x = min(x, dpp_mov<0x141>(x));
+    x = min(x, dpp_mov<0x140>(x));
+    x = min(x, swap_rows16(x));
+    return min(x, swap_rows32(x));
+}
+// the workgroup's largest float / smallest int, in every thread (lds: TPB / 64 slots)
+template <int TPB> __device__ float block_max(float x, float *lds)
+{
+    x = wave_max(x);
+    __syncthreads();   // lds free from its last use
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
+    __syncthreads();
+    float r = lds[0];
+    for (int w = 1; w < TPB / 64; ++w) r = fmaxf(r, lds[w]);
+    return r;
+}
+template <int TPB> __device__ int block_min(int x, int *lds)
+{
+    x = wave_min(x);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
+    __syncthreads();
+    int r = lds[0];
+    for (int w = 1; w < TPB / 64; ++w) r = min(r, lds[w]);
+    return r;
+}
+
+// word: |r| bits << 32 | (index * 2 + sign) << 7 | iteration tag (7 bits;
+// index < 2^24); a workgroup with no |r| > 0 publishes |r| bits 0 and, for
+// workgroup 0, the sign of pixel 0 (the reference's peak when the whole
+// image is zero)
+__device__ __forceinline__ uint64_t pack_word(float absmax, int idx, unsigned sign, unsigned tag)
+{
+    return (uint64_t)__float_as_uint(absmax) << 32 | ((uint64_t)idx * 2 + sign) << 7 | tag;
+}
+
+// take ? a : b, as bit operations
+__device__ __forceinline__ float blend(unsigned take, float a, float b)
+{
+    const unsigned m = take ? ~0u : 0u;
+    return __uint_as_float((__float_as_uint(a) & m) | (__float_as_uint(b) & ~m));
+}
+
+template <int NG, int TPB, int B>
+__global__ void __launch_bounds__(TPB)
+hogbom_resident_kernel(PersistArgs a, float *__restrict__ r, const float *__restrict__ psf_pad,
+                       float *__restrict__ model, float *__restrict__ sources, int *__restrict__ num_sources,
+                       State *__restrict__ st_final, uint64_t *__restrict__ words, unsigned *__restrict__ sync)
+{
+    __shared__ float lds_f[TPB / 64];
+    __shared__ int lds_i[TPB / 64];
+    __shared__ int s_ok;
+    __shared__ Best s_comp;   // the iteration's component, from thread 0
+    unsigned *counter = sync, *abort_flag = sync + 64;   // 256 bytes apart
+    const int G = a.G, half = G / 2;
+    const int n4 = G * G / 4;
+    const int gmask = (1 << a.row_shift) - 1;
+    float4 *r4 = reinterpret_cast<float4 *>(r);
+    const int qb = blockIdx.x * (TPB * NG) + threadIdx.x;
+    float4 v[NG];
+#pragma unroll
+    for (int k = 0; k < NG; ++k) {
+        const int q = qb + k * TPB;
+        v[k] = q < n4 ? r4[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)psf_pad, (short)0, a.psf_pad_bytes, 0x00020000);
+    int nsrc = 0, stop = 0;
+    float src0 = 0.f;
+    for (int it = 0; it < a.n_minor; ++it) {
+        // opaque to the optimiser: the groups' coordinates are formed again
+        // per iteration, not hoisted into registers held all along
+        int qi = qb;
+        asm volatile("" : "+v"(qi));
+        // the workgroup's largest |r| (positive pixels only count: fmaxf
+        // over |r| >= 0 leaves 0 when there are none); per thread, its first
+        // group holding its own largest, and that group's values
+        float m = 0.f;
+        int kmax = 0;
+        float4 e4 = v[0];
+#pragma unroll
+        for (int k = 0; k < NG; ++k) {
+            const float gm = fmaxf(fmaxf(fabsf(v[k].x), fabsf(v[k].y)), fmaxf(fabsf(v[k].z), fabsf(v[k].w)));
+            const bool up = gm > m;
+            kmax = up ? k : kmax;
+            e4.x = up ? v[k].x : e4.x;
+            e4.y = up ? v[k].y : e4.y;
+            e4.z = up ? v[k].z : e4.z;
+            e4.w = up ? v[k].w : e4.w;
+            m = fmaxf(m, gm);
+        }
+        const float mwg = block_max<TPB>(m, lds_f);
+        // its first index (index * 2 + sign), from the threads that hold it
+        int first = 0x7fffffff;
+        if (mwg > 0.f && m == mwg) {
+            const int j = fabsf(e4.x) == mwg ? 0 : fabsf(e4.y) == mwg ? 1 : fabsf(e4.z) == mwg ? 2 : 3;
+            const float ej = j == 0 ? e4.x : j == 1 ? e4.y : j == 2 ? e4.z : e4.w;
+            first = ((qi + kmax * TPB) * 4 + j) * 2 + (int)(__float_as_uint(ej) >> 31);
+        }
+        const int fwg = block_min<TPB>(first, lds_i);
+        uint64_t *buf = words + (size_t)(it & 1) * a.nblocks;
+        const unsigned tag = (unsigned)it & 0x3fu;   // 0x7f never: the reset words carry it
+        if (threadIdx.x == 0) {
+            // pixel 0 lives in thread 0 of workgroup 0
+            const uint64_t w = mwg > 0.f ? pack_word(mwg, fwg >> 1, (unsigned)(fwg & 1), tag)
+                                         : pack_word(0.f, 0, __float_as_uint(v[0].x) >> 31, tag);
+            __hip_atomic_store(buf + blockIdx.x, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // every workgroup's arrival at this iteration
+            const unsigned target = (unsigned)a.nblocks * (unsigned)(it + 1);
+            const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+            int ok = 1;
+            while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                    __builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+                    __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            s_ok = ok;
+        }
+        __syncthreads();
+        // all words (each carries its iteration's tag; the counter says they
+        // were stored, the tag that this one has arrived)
+        float wv = 0.f;
+        int wi = 0x7fffffff;
+        uint64_t w0 = 0;
+        if (s_ok && (int)threadIdx.x < a.nblocks) {
+            const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+            uint64_t w;
+            while (((w = __hip_atomic_load(buf + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & 0x7fu) !=
+                   tag) {
+                if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+                    __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    s_ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            wv = __uint_as_float((uint32_t)(w >> 32));
+            wi = (int)((w >> 7) & 0x1ffffffu);   // index * 2 + sign
+            if (threadIdx.x == 0) w0 = w;
+        }
+        __syncthreads();
+        if (!s_ok) {
+            stop = 2;
+            break;   // every workgroup leaves: the abort flag is set
+        }
+        const float gmax = block_max<TPB>(wv, lds_f);
+        const int gfirst = block_min<TPB>(gmax > 0.f && wv == gmax ? wi : 0x7fffffff, lds_i);
+        if (threadIdx.x == 0) {
+            // nothing beat the initial 0: the reference keeps pixel (0, 0)
+            // with its value, a zero of pixel 0's sign (workgroup 0's word)
+            s_comp = gmax > 0.f ? Best{gmax, gfirst >> 1, (gfirst & 1) ? -gmax : gmax}
+                                : Best{0.f, 0, ((w0 >> 7) & 1u) ? -0.f : 0.f};
+        }
+        __syncthreads();
+        const float curr_max = s_comp.v, peak = s_comp.val;
+        const int max_idx = s_comp.i;
+        const bool weak_source = nsrc > 0 && (double)curr_max < (double)fabsf(src0) * a.weak;
+        if (weak_source || nsrc >= a.max_sources) {
+            stop = 1;
+            break;   // the same decision in every workgroup
+        }
+        const float max_val = (float)((double)peak / a.psf_max);
+        const int mx = max_idx % G, my = max_idx / G;
+        const float flux = (float)((double)max_val * a.gain);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sources[3 * nsrc] = (float)mx;
+            sources[3 * nsrc + 1] = (float)my;
+            sources[3 * nsrc + 2] = flux;
+            model[max_idx] = (float)((double)model[max_idx] + (double)max_val * a.gain);
+            *num_sources = nsrc + 1;
+        }
+        if (nsrc == 0) src0 = flux;
+        ++nsrc;
+        // subtraction window (deconvolution_run.c:259-264)
+        const int xlo = mx - min(mx, half), xhi = mx + min(G - mx, half);
+        const int ylo = my - min(my, half), yhi = my + min(G - my, half);
+        const double scale = a.gain * (double)max_val;
+        const double pm = a.psf_max;
+        {
+            auto sub = [&](float x, float p) { return subtract_one(x, scale, p, pm); };
+#pragma unroll
+            for (int k0 = 0; k0 < NG; k0 += B) {
+                v4f_t w[B];
+                unsigned lanes[B];
+#pragma unroll
+                for (int k = 0; k < B; ++k) {
+                    const int q = qi + (k0 + k) * TPB;
+                    const int y = q >> a.row_shift, x0 = (q & gmask) * 4;
+                    // bitwise tests (no short-circuit branches)
+                    const unsigned row = (unsigned)(q < n4) & (unsigned)(y >= ylo) & (unsigned)(y < yhi);
+                    unsigned mk = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        mk |= (row & (unsigned)(x0 + j >= xlo) & (unsigned)(x0 + j < xhi)) << j;
+                    lanes[k] = mk;
+                    const int prow = (y - my + half) * G;
+                    const int lo = a.mirror ? prow + (G - (x0 + 3 - mx + half)) : prow + (x0 - mx + half);
+                    const unsigned off = mk ? (unsigned)(lo + kPsfPad) * 4u : kOob;
+                    w[k] = __builtin_bit_cast(v4f_t, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+                }
+#pragma unroll
+                for (int k = 0; k < B; ++k) {
+                    const unsigned mk = lanes[k];
+                    const float p0 = a.mirror ? w[k].w : w[k].x, p1 = a.mirror ? w[k].z : w[k].y;
+                    const float p2 = a.mirror ? w[k].y : w[k].z, p3 = a.mirror ? w[k].x : w[k].w;
+                    // every lane computed, the window's kept by a bitwise blend
+                    // (no per-pixel branches)
+                    float4 e = v[k0 + k];
+                    e.x = blend(mk & 1u, sub(e.x, p0), e.x);
+                    e.y = blend(mk & 2u, sub(e.y, p1), e.y);
+                    e.z = blend(mk & 4u, sub(e.z, p2), e.z);
+                    e.w = blend(mk & 8u, sub(e.w, p3), e.w);
+                    v[k0 + k] = e;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NG; ++k) {
+        const int q = qb + k * TPB;
+        if (q < n4) r4[q] = v[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *st_final = State{stop ? 1 : 0, stop == 2 ? -1 : nsrc, src0, 0.f};
+}
+
+// the arrival counter and the abort flag to 0, the words of both parities
+// to a tag no iteration carries
+__global__ void hogbom_resident_init_kernel(unsigned *sync, uint64_t *words, int n)
+{
+    if (threadIdx.x == 0) sync[0] = 0u;
+    if (threadIdx.x == 64) sync[64] = 0u;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) words[i] = 0x7full;
+}
+
 // deconvolution_run.c:297-306: pixel coordinates to radians
 __global__ void hogbom_finish_kernel(const State *st, int G, double cell_size, float *sources, int *num_sources)
 {
@@ -419,6 +732,13 @@ size_t partial_count(size_t npix) { return (npix + kMinPixelsPerChunk - 1) / kMi
 size_t psf_pad_offset(size_t npix)
 {
     return (partial_offset() + 2 * partial_count(npix) * sizeof(Partial) + 255) & ~(size_t)255;
+}
+
+// then the one-launch form's arrival counter and abort flag (256 bytes
+// apart, 512 bytes in all)
+size_t sync_offset(size_t npix)
+{
+    return (psf_pad_offset(npix) + (npix + 2 * kPsfPad) * sizeof(float) + 255) & ~(size_t)255;
 }
 
 // 16-byte groups per thread per chunk (2, 4 or 8): the most that still
@@ -452,6 +772,85 @@ int compute_units()
     return n;
 }
 
+// All iterations in one cooperative launch (hogbom_resident_kernel) when it
+// applies: a power-of-two side whose pixels fit the registers of one
+// workgroup per CU, a noise stop that cannot fire, a stream that is not
+// being captured (a graph keeps per-iteration launches) and a device that
+// takes the cooperative launch.  Returns 1 when launched, 0 when the
+// caller should queue per-iteration launches.  SDP_HOGBOM_RESIDENT=0 in the
+// environment turns it off for the process.
+int launch_resident(int G, int n_minor, int max_sources, const HogbomArgs &h, double noise_gain, float *d_residual,
+                    const float *psf_pad, float *d_model, float *d_sources, int *d_num_sources, State *st_final,
+                    void *d_work, hipStream_t st)
+{
+    static int env = -1;
+    if (env < 0) {
+        const char *e = getenv("SDP_HOGBOM_RESIDENT");
+        env = (e && atoi(e) == 0) ? 0 : 1;
+    }
+    if (!env || h.exact_mean || noise_gain > 0.5 || n_minor < 1 || G < 16 || G > 4096 || (G & (G - 1)) != 0)
+        return 0;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return 0;
+    static int coop = -1;
+    if (coop < 0) {
+        int dev = 0, v = 0;
+        coop = (hipGetDevice(&dev) == hipSuccess &&
+                hipDeviceGetAttribute(&v, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess && v)
+                   ? 1
+                   : 0;
+    }
+    if (!coop) return 0;
+    const size_t npix = (size_t)G * G, n4 = npix / 4;
+    // (groups per thread, threads per workgroup): 1024 threads up to 8
+    // groups (128 registers a thread); beyond, 512 threads and the
+    // registers for more PSF loads in flight
+    struct Shape {
+        int ng, tpb;
+        const void *k;
+    };
+    const Shape shapes[5] = {{1, 1024, (const void *)hogbom_resident_kernel<1, 1024, 1>},
+                             {2, 1024, (const void *)hogbom_resident_kernel<2, 1024, 2>},
+                             {4, 1024, (const void *)hogbom_resident_kernel<4, 1024, 4>},
+                             {8, 1024, (const void *)hogbom_resident_kernel<8, 1024, 4>},
+                             {16, 1024, (const void *)hogbom_resident_kernel<16, 1024, 2>}};
+    for (const Shape &sh : shapes) {
+        const size_t nb = (n4 + (size_t)sh.tpb * sh.ng - 1) / ((size_t)sh.tpb * sh.ng);
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sh.k, sh.tpb, 0) != hipSuccess || per_cu < 1)
+            continue;
+        // one workgroup per CU at most; one waiting thread per word
+        if (nb > (size_t)compute_units() || nb > (size_t)sh.tpb) continue;
+        PersistArgs a;
+        a.G = G;
+        a.n_minor = n_minor;
+        a.max_sources = max_sources;
+        a.nblocks = (int)nb;
+        a.mirror = h.mirror;
+        a.psf_pad_bytes = h.psf_pad_bytes;
+        a.row_shift = 0;
+        while ((1 << a.row_shift) < G / 4) ++a.row_shift;
+        a.psf_max = h.psf_max;
+        a.gain = h.gain;
+        a.weak = h.weak;
+        uint64_t *words = (uint64_t *)((char *)d_work + partial_offset());
+        unsigned *sync = (unsigned *)((char *)d_work + sync_offset(npix));
+        hogbom_resident_init_kernel<<<1, 256, 0, st>>>(sync, words, (int)(2 * nb));
+        SDP_LAUNCH_CHECK();
+        void *args[] = {&a, &d_residual, (void *)&psf_pad, &d_model, &d_sources, &d_num_sources, &st_final,
+                        &words, &sync};
+        // A cooperative launch can be refused (another process may hold the
+        // device's cooperative resources); nothing is enqueued then, and the
+        // iterations go as separate launches.
+        if (hipLaunchCooperativeKernel(sh.k, dim3((unsigned)nb), dim3(sh.tpb), args, 0, st) != hipSuccess) {
+            (void)hipGetLastError();
+            return 0;
+        }
+        return 1;
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -460,8 +859,8 @@ size_t sdp_hogbom_work_bytes(int G)
 {
     if (G < 2) return 0;
     const size_t npix = (size_t)G * G;
-    // partials of two consecutive launches, then the padded PSF copy
-    return psf_pad_offset(npix) + (npix + 2 * kPsfPad) * sizeof(float);
+    // partials of two consecutive launches, the padded PSF copy, the counter and flag
+    return sync_offset(npix) + 512;
 }
 
 int sdp_hogbom(int G, int n_minor, int max_sources, float *d_residual, const float *d_psf, double psf_max,
@@ -518,6 +917,14 @@ int sdp_hogbom_ex(int G, int n_minor, int max_sources, float *d_residual, const 
     hogbom_init_kernel<<<sdp_div_up(npix, 256) > 4096 ? 4096 : sdp_div_up(npix, 256), 256, 0, st>>>(
         state, d_model_in, d_model_out, d_model_in == d_model_out ? 0 : npix, d_psf, psf_pad, npix);
     SDP_LAUNCH_CHECK();
+    if (!(flags & SDP_HOGBOM_PER_ITERATION) &&
+        launch_resident(G, n_minor, max_sources, a, noise_factor * loop_gain, d_residual, psf_pad, d_model_out,
+                        d_sources, d_num_sources, state + ((n_minor + 1) & 1), d_work, st)) {
+        hogbom_finish_kernel<<<4, 256, 0, st>>>(state + ((n_minor + 1) & 1), G, cell_size, d_sources,
+                                                d_num_sources);
+        SDP_LAUNCH_CHECK();
+        return SDP_OK;
+    }
     for (int it = 0; it <= n_minor; ++it) {
         if (a.exact_mean && it > 0) {
             // the residual the previous launch left, for launch it's decision

@@ -6,8 +6,9 @@
  * visibilities in the gridding plan's order, one complex grid (the model
  * grid during predict, then the residual grid), the residual, model and PSF
  * images.  A cycle is a fixed sequence of enqueued launches with no host
- * synchronisation inside, so after a first eager cycle it can be captured
- * once into a graph and replayed (SDP_PIPE_GRAPH).
+ * synchronisation inside, so after a first eager cycle its stages 1-5 can
+ * be captured once into a graph and replayed (SDP_PIPE_GRAPH); Hogbom is
+ * enqueued after the replay, where it can run as one cooperative launch.
  *
  * Stage order per cycle (preesm_pipelines/simsdp_fft_imaging_pipeline/
  * Algo/fft_delta.pi:61-320; actor sources under .../Code/src/CPU/):
@@ -380,7 +381,7 @@ int sdp_pipeline_cycle(sdp_pipeline *P, double *rms_residual, double *rms_after,
             /* the first cycle ran eagerly, so every lazily created resource
              * (twiddle tables, kernel attributes) exists before capture */
             TRY(sdp_graph_begin_capture(P->s));
-            int rc = enqueue_cycle(P);
+            int rc = enqueue_residual(P);
             sdp_graph g = NULL;
             int rc2 = sdp_graph_end_capture(P->s, &g);
             if (rc != SDP_OK) {
@@ -392,6 +393,7 @@ int sdp_pipeline_cycle(sdp_pipeline *P, double *rms_residual, double *rms_after,
             TRY(sdp_event_record(P->ev0, P->s));
         }
         TRY(sdp_graph_launch(P->graph, P->s));
+        TRY(enqueue_clean(P));
     } else {
         TRY(enqueue_cycle(P));
     }

@@ -765,3 +765,18 @@ def test_autotune_picks_a_working_candidate(hip, oracle, case):
     plan.grid(gk, vis, g)
     ref64 = oracle.std_gridding_f64(G, d.oversampling, d.kernels, d.supports, d.uvw, d.vis, d.uv_scale, d.w_scale)
     assert np.abs(g.download((G * G, 2), np.float32) - ref64).max() <= 1e-6 * np.abs(ref64).max()
+
+
+@pytest.mark.parametrize("G,n_minor", [(16, 30), (256, 300), (1024, 200), (2048, 100), (4096, 40)])
+def test_hogbom_one_launch_equals_per_iteration_launches(hip, rng, G, n_minor):
+    """All iterations in one cooperative launch (residual held in registers,
+    one word per workgroup and iteration as the exchange) against one launch
+    per iteration: residual, model and components bit for bit, at sizes that
+    take one to sixteen 16-byte groups per thread."""
+    res, psf = dirty_case(G, rng, n_src=12)
+    cfg = hip.make_config(G, 1e-4, psf_max_value=1.0)
+    a = run_device_hogbom(hip, G, n_minor, n_minor, res, psf, cfg)
+    b = run_device_hogbom(hip, G, n_minor, n_minor, res, psf, cfg, flags=hip.HOGBOM_PER_ITERATION)
+    assert a[0] == b[0] == n_minor
+    for x, y in zip(a[1:], b[1:]):
+        np.testing.assert_array_equal(x, y)
