@@ -412,18 +412,20 @@ hogbom_step_kernel(int iter, HogbomArgs a, float *__restrict__ r, const float *_
 //
 // Per iteration each workgroup finds its best pixel (largest |r|, first
 // index on ties: the largest |r| in one pass, then the first index holding
-// it), publishes it as one 64-bit word (|r| bits, index, sign) and arrives
-// at a counter; one thread per workgroup waits for the counter to reach
-// every workgroup's arrival, then every workgroup reduces all words the
-// same way, so all reach the same component and stop decision; workgroup 0
-// records the component and every workgroup subtracts it from its own
-// pixels.  Words are double-buffered by iteration parity: a workgroup can
-// publish iteration i + 2 into a buffer only after every workgroup has
-// arrived at iteration i + 1, that is after all have read iteration i.
+// it), publishes it as one 64-bit word (|r| bits, index, sign, iteration
+// tag) and arrives at a counter; one thread per workgroup waits for the
+// counter to reach every workgroup's arrival, then each workgroup reads all
+// words (re-reading any whose tag is not yet this iteration's: the counter
+// and the word are separate stores) and reduces them the same way, so all
+// reach the same component and stop decision; workgroup 0 records the
+// component and every workgroup subtracts it from its own pixels.  Words
+// are double-buffered by iteration parity: a workgroup can publish
+// iteration i + 2 into a buffer only after every workgroup has arrived at
+// iteration i + 1, that is after all have read iteration i.
 //
-// Waits are bounded: a counter that does not arrive within 20 ms (it cannot,
-// short of a fault elsewhere) sets an abort flag that every waiter also
-// checks, every workgroup leaves, and the run reports -1 sources.
+// Waits are bounded: a counter or word that does not arrive within 20 ms (it
+// cannot, short of a fault elsewhere) sets an abort flag that every waiter
+// also checks, every workgroup leaves, and the run reports -1 sources.
 //
 // The arithmetic, tie order and stop tests are those of hogbom_step_kernel
 // (bit for bit); the noise stop is left out, so the form is taken only when
