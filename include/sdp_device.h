@@ -291,8 +291,14 @@ int sdp_hogbom(int grid_size, int n_minor, int max_sources, float *d_residual, c
  * cooperative launch that keeps the residual in registers (the same results
  * bit for bit); SDP_HOGBOM_PER_ITERATION forces one launch per iteration.
  * A run that cannot complete its cross-workgroup exchange reports -1
- * sources. */
-enum { SDP_HOGBOM_PSF_AS_STORED = 1, SDP_HOGBOM_DOUBLE_MEAN = 2, SDP_HOGBOM_PER_ITERATION = 4 };
+ * sources; SDP_HOGBOM_TEST_ABORT raises that exit at the first exchange (a
+ * test hook: the one-launch form then ends at once with -1 sources). */
+enum {
+    SDP_HOGBOM_PSF_AS_STORED = 1,
+    SDP_HOGBOM_DOUBLE_MEAN = 2,
+    SDP_HOGBOM_PER_ITERATION = 4,
+    SDP_HOGBOM_TEST_ABORT = 8
+};
 int sdp_hogbom_ex(int grid_size, int n_minor, int max_sources, float *d_residual, const float *d_psf,
                   double psf_max, double loop_gain, double noise_factor, double weak_source_percent,
                   double cell_size, const float *d_model_in, float *d_model_out, float *d_sources,

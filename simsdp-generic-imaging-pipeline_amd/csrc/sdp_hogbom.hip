@@ -519,27 +519,54 @@ __device__ __forceinline__ float blend(unsigned take, float a, float b)
     return __uint_as_float((__float_as_uint(a) & m) | (__float_as_uint(b) & ~m));
 }
 
-template <int NG, int TPB, int B>
+// NG 16-byte groups per thread: the first NG - NL in registers, the last NL
+// in LDS (each thread reads and writes only its own LDS slots, so they need
+// no barriers); the scan for the next component runs in the same pass as
+// the subtraction, so a group is read and written once per iteration.
+template <int NG, int NL, int TPB, int B>
 __global__ void __launch_bounds__(TPB)
 hogbom_resident_kernel(PersistArgs a, float *__restrict__ r, const float *__restrict__ psf_pad,
                        float *__restrict__ model, float *__restrict__ sources, int *__restrict__ num_sources,
                        State *__restrict__ st_final, uint64_t *__restrict__ words, unsigned *__restrict__ sync)
 {
+    constexpr int NR = NG - NL;
+    static_assert(NR >= 1 && NG % B == 0 && NR % B == 0, "group split");
     __shared__ float lds_f[TPB / 64];
     __shared__ int lds_i[TPB / 64];
     __shared__ int s_ok;
     __shared__ Best s_comp;   // the iteration's component, from thread 0
+    __shared__ float4 lds_v[NL > 0 ? NL * TPB : 1];
     unsigned *counter = sync, *abort_flag = sync + 64;   // 256 bytes apart
     const int G = a.G, half = G / 2;
     const int n4 = G * G / 4;
     const int gmask = (1 << a.row_shift) - 1;
     float4 *r4 = reinterpret_cast<float4 *>(r);
     const int qb = blockIdx.x * (TPB * NG) + threadIdx.x;
-    float4 v[NG];
+    float4 v[NR];
+    auto lds_slot = [&](int k) -> float4 & { return lds_v[(k - NR) * TPB + threadIdx.x]; };
+    // the scan state: the thread's largest |r| (positive pixels only count:
+    // fmaxf over |r| >= 0 leaves 0 when there are none), its first group
+    // holding it, and that group's values
+    float m = 0.f;
+    int kmax = 0;
+    float4 e4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto scan_group = [&](int k, const float4 &g) {
+        const float gm = fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fmaxf(fabsf(g.z), fabsf(g.w)));
+        const bool up = gm > m;
+        kmax = up ? k : kmax;
+        e4.x = up ? g.x : e4.x;
+        e4.y = up ? g.y : e4.y;
+        e4.z = up ? g.z : e4.z;
+        e4.w = up ? g.w : e4.w;
+        m = fmaxf(m, gm);
+    };
 #pragma unroll
     for (int k = 0; k < NG; ++k) {
         const int q = qb + k * TPB;
-        v[k] = q < n4 ? r4[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 g = q < n4 ? r4[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (k < NR) v[k < NR ? k : 0] = g;
+        else lds_slot(k) = g;
+        scan_group(k, g);
     }
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc((void *)psf_pad, (short)0, a.psf_pad_bytes, 0x00020000);
@@ -550,23 +577,6 @@ hogbom_resident_kernel(PersistArgs a, float *__restrict__ r, const float *__rest
         // per iteration, not hoisted into registers held all along
         int qi = qb;
         asm volatile("" : "+v"(qi));
-        // the workgroup's largest |r| (positive pixels only count: fmaxf
-        // over |r| >= 0 leaves 0 when there are none); per thread, its first
-        // group holding its own largest, and that group's values
-        float m = 0.f;
-        int kmax = 0;
-        float4 e4 = v[0];
-#pragma unroll
-        for (int k = 0; k < NG; ++k) {
-            const float gm = fmaxf(fmaxf(fabsf(v[k].x), fabsf(v[k].y)), fmaxf(fabsf(v[k].z), fabsf(v[k].w)));
-            const bool up = gm > m;
-            kmax = up ? k : kmax;
-            e4.x = up ? v[k].x : e4.x;
-            e4.y = up ? v[k].y : e4.y;
-            e4.z = up ? v[k].z : e4.z;
-            e4.w = up ? v[k].w : e4.w;
-            m = fmaxf(m, gm);
-        }
         const float mwg = block_max<TPB>(m, lds_f);
         // its first index (index * 2 + sign), from the threads that hold it
         int first = 0x7fffffff;
@@ -579,7 +589,7 @@ hogbom_resident_kernel(PersistArgs a, float *__restrict__ r, const float *__rest
         uint64_t *buf = words + (size_t)(it & 1) * a.nblocks;
         const unsigned tag = (unsigned)it & 0x3fu;   // 0x7f never: the reset words carry it
         if (threadIdx.x == 0) {
-            // pixel 0 lives in thread 0 of workgroup 0
+            // pixel 0 lives in thread 0 of workgroup 0 (group 0: registers)
             const uint64_t w = mwg > 0.f ? pack_word(mwg, fwg >> 1, (unsigned)(fwg & 1), tag)
                                          : pack_word(0.f, 0, __float_as_uint(v[0].x) >> 31, tag);
             __hip_atomic_store(buf + blockIdx.x, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -587,10 +597,18 @@ hogbom_resident_kernel(PersistArgs a, float *__restrict__ r, const float *__rest
             // every workgroup's arrival at this iteration
             const unsigned target = (unsigned)a.nblocks * (unsigned)(it + 1);
             const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+            // the counter and the abort flag read together each round: an
+            // abort raised anywhere ends every workgroup at its next exchange
             int ok = 1;
-            while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-                    __builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+            for (;;) {
+                const unsigned c = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned ab = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (ab) {
+                    ok = 0;
+                    break;
+                }
+                if (c >= target) break;
+                if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
                     __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     ok = 0;
                     break;
@@ -654,64 +672,68 @@ hogbom_resident_kernel(PersistArgs a, float *__restrict__ r, const float *__rest
         }
         if (nsrc == 0) src0 = flux;
         ++nsrc;
-        // subtraction window (deconvolution_run.c:259-264)
+        // subtraction window (deconvolution_run.c:259-264), and in the same
+        // pass the scan for the next component
         const int xlo = mx - min(mx, half), xhi = mx + min(G - mx, half);
         const int ylo = my - min(my, half), yhi = my + min(G - my, half);
         const double scale = a.gain * (double)max_val;
         const double pm = a.psf_max;
-        {
-            auto sub = [&](float x, float p) { return subtract_one(x, scale, p, pm); };
+        m = 0.f;
+        kmax = 0;
 #pragma unroll
-            for (int k0 = 0; k0 < NG; k0 += B) {
-                v4f_t w[B];
-                unsigned lanes[B];
+        for (int k0 = 0; k0 < NG; k0 += B) {
+            v4f_t w[B];
+            unsigned lanes[B];
+            float4 e[B];
 #pragma unroll
-                for (int k = 0; k < B; ++k) {
-                    const int q = qi + (k0 + k) * TPB;
-                    const int y = q >> a.row_shift, x0 = (q & gmask) * 4;
-                    // bitwise tests (no short-circuit branches)
-                    const unsigned row = (unsigned)(q < n4) & (unsigned)(y >= ylo) & (unsigned)(y < yhi);
-                    unsigned mk = 0;
+            for (int k = 0; k < B; ++k) {
+                const int q = qi + (k0 + k) * TPB;
+                const int y = q >> a.row_shift, x0 = (q & gmask) * 4;
+                // bitwise tests (no short-circuit branches)
+                const unsigned row = (unsigned)(q < n4) & (unsigned)(y >= ylo) & (unsigned)(y < yhi);
+                unsigned mk = 0;
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        mk |= (row & (unsigned)(x0 + j >= xlo) & (unsigned)(x0 + j < xhi)) << j;
-                    lanes[k] = mk;
-                    const int prow = (y - my + half) * G;
-                    const int lo = a.mirror ? prow + (G - (x0 + 3 - mx + half)) : prow + (x0 - mx + half);
-                    const unsigned off = mk ? (unsigned)(lo + kPsfPad) * 4u : kOob;
-                    w[k] = __builtin_bit_cast(v4f_t, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
-                }
+                for (int j = 0; j < 4; ++j) mk |= (row & (unsigned)(x0 + j >= xlo) & (unsigned)(x0 + j < xhi)) << j;
+                lanes[k] = mk;
+                const int prow = (y - my + half) * G;
+                const int lo = a.mirror ? prow + (G - (x0 + 3 - mx + half)) : prow + (x0 - mx + half);
+                const unsigned off = mk ? (unsigned)(lo + kPsfPad) * 4u : kOob;
+                w[k] = __builtin_bit_cast(v4f_t, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+                e[k] = k0 + k < NR ? v[k0 + k < NR ? k0 + k : 0] : lds_slot(k0 + k);
+            }
 #pragma unroll
-                for (int k = 0; k < B; ++k) {
-                    const unsigned mk = lanes[k];
-                    const float p0 = a.mirror ? w[k].w : w[k].x, p1 = a.mirror ? w[k].z : w[k].y;
-                    const float p2 = a.mirror ? w[k].y : w[k].z, p3 = a.mirror ? w[k].x : w[k].w;
-                    // every lane computed, the window's kept by a bitwise blend
-                    // (no per-pixel branches)
-                    float4 e = v[k0 + k];
-                    e.x = blend(mk & 1u, sub(e.x, p0), e.x);
-                    e.y = blend(mk & 2u, sub(e.y, p1), e.y);
-                    e.z = blend(mk & 4u, sub(e.z, p2), e.z);
-                    e.w = blend(mk & 8u, sub(e.w, p3), e.w);
-                    v[k0 + k] = e;
-                }
+            for (int k = 0; k < B; ++k) {
+                const unsigned mk = lanes[k];
+                const float p0 = a.mirror ? w[k].w : w[k].x, p1 = a.mirror ? w[k].z : w[k].y;
+                const float p2 = a.mirror ? w[k].y : w[k].z, p3 = a.mirror ? w[k].x : w[k].w;
+                // every lane computed, the window's kept by a bitwise blend
+                // (no per-pixel branches)
+                float4 g = e[k];
+                g.x = blend(mk & 1u, subtract_one(g.x, scale, p0, pm), g.x);
+                g.y = blend(mk & 2u, subtract_one(g.y, scale, p1, pm), g.y);
+                g.z = blend(mk & 4u, subtract_one(g.z, scale, p2, pm), g.z);
+                g.w = blend(mk & 8u, subtract_one(g.w, scale, p3, pm), g.w);
+                if (k0 + k < NR) v[k0 + k < NR ? k0 + k : 0] = g;
+                else lds_slot(k0 + k) = g;
+                scan_group(k0 + k, g);
             }
         }
     }
 #pragma unroll
     for (int k = 0; k < NG; ++k) {
         const int q = qb + k * TPB;
-        if (q < n4) r4[q] = v[k];
+        if (q < n4) r4[q] = k < NR ? v[k < NR ? k : 0] : lds_slot(k);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *st_final = State{stop ? 1 : 0, stop == 2 ? -1 : nsrc, src0, 0.f};
 }
 
-// the arrival counter and the abort flag to 0, the words of both parities
-// to a tag no iteration carries
-__global__ void hogbom_resident_init_kernel(unsigned *sync, uint64_t *words, int n)
+// the arrival counter to 0, the abort flag to `abort` (1 only to exercise
+// the exit path: every workgroup then leaves at its first exchange), the
+// words of both parities to a tag no iteration carries
+__global__ void hogbom_resident_init_kernel(unsigned *sync, uint64_t *words, int n, unsigned abort)
 {
     if (threadIdx.x == 0) sync[0] = 0u;
-    if (threadIdx.x == 64) sync[64] = 0u;
+    if (threadIdx.x == 64) sync[64] = abort;
     for (int i = threadIdx.x; i < n; i += blockDim.x) words[i] = 0x7full;
 }
 
@@ -783,7 +805,7 @@ int compute_units()
 // environment turns it off for the process.
 int launch_resident(int G, int n_minor, int max_sources, const HogbomArgs &h, double noise_gain, float *d_residual,
                     const float *psf_pad, float *d_model, float *d_sources, int *d_num_sources, State *st_final,
-                    void *d_work, hipStream_t st)
+                    void *d_work, bool preset_abort, hipStream_t st)
 {
     static int env = -1;
     if (env < 0) {
@@ -811,11 +833,11 @@ int launch_resident(int G, int n_minor, int max_sources, const HogbomArgs &h, do
         int ng, tpb;
         const void *k;
     };
-    const Shape shapes[5] = {{1, 1024, (const void *)hogbom_resident_kernel<1, 1024, 1>},
-                             {2, 1024, (const void *)hogbom_resident_kernel<2, 1024, 2>},
-                             {4, 1024, (const void *)hogbom_resident_kernel<4, 1024, 4>},
-                             {8, 1024, (const void *)hogbom_resident_kernel<8, 1024, 4>},
-                             {16, 1024, (const void *)hogbom_resident_kernel<16, 1024, 2>}};
+    const Shape shapes[5] = {{1, 1024, (const void *)hogbom_resident_kernel<1, 0, 1024, 1>},
+                             {2, 1024, (const void *)hogbom_resident_kernel<2, 0, 1024, 2>},
+                             {4, 1024, (const void *)hogbom_resident_kernel<4, 0, 1024, 4>},
+                             {8, 1024, (const void *)hogbom_resident_kernel<8, 0, 1024, 4>},
+                             {16, 1024, (const void *)hogbom_resident_kernel<16, 8, 1024, 4>}};
     for (const Shape &sh : shapes) {
         const size_t nb = (n4 + (size_t)sh.tpb * sh.ng - 1) / ((size_t)sh.tpb * sh.ng);
         int per_cu = 0;
@@ -837,7 +859,7 @@ int launch_resident(int G, int n_minor, int max_sources, const HogbomArgs &h, do
         a.weak = h.weak;
         uint64_t *words = (uint64_t *)((char *)d_work + partial_offset());
         unsigned *sync = (unsigned *)((char *)d_work + sync_offset(npix));
-        hogbom_resident_init_kernel<<<1, 256, 0, st>>>(sync, words, (int)(2 * nb));
+        hogbom_resident_init_kernel<<<1, 256, 0, st>>>(sync, words, (int)(2 * nb), preset_abort ? 1u : 0u);
         SDP_LAUNCH_CHECK();
         void *args[] = {&a, &d_residual, (void *)&psf_pad, &d_model, &d_sources, &d_num_sources, &st_final,
                         &words, &sync};
@@ -921,7 +943,8 @@ int sdp_hogbom_ex(int G, int n_minor, int max_sources, float *d_residual, const 
     SDP_LAUNCH_CHECK();
     if (!(flags & SDP_HOGBOM_PER_ITERATION) &&
         launch_resident(G, n_minor, max_sources, a, noise_factor * loop_gain, d_residual, psf_pad, d_model_out,
-                        d_sources, d_num_sources, state + ((n_minor + 1) & 1), d_work, st)) {
+                        d_sources, d_num_sources, state + ((n_minor + 1) & 1), d_work,
+                        (flags & SDP_HOGBOM_TEST_ABORT) != 0, st)) {
         hogbom_finish_kernel<<<4, 256, 0, st>>>(state + ((n_minor + 1) & 1), G, cell_size, d_sources,
                                                 d_num_sources);
         SDP_LAUNCH_CHECK();

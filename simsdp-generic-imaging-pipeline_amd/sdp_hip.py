@@ -41,6 +41,7 @@ FFT_SHIFT_IN, FFT_SHIFT_OUT, FFT_REAL_IN, FFT_REAL_OUT, FFT_CORRECT_OUT, FFT_COR
 HOGBOM_PSF_AS_STORED = 1
 HOGBOM_DOUBLE_MEAN = 2
 HOGBOM_PER_ITERATION = 4
+HOGBOM_TEST_ABORT = 8
 # major-cycle pipeline (sdp_pipeline.h)
 PIPE_GRAPH, PIPE_REFERENCE_PREDICT, PIPE_PSF_MIRRORED, PIPE_PROFILE = 1, 2, 4, 8
 STAGES = ("predict_fft", "degrid_subtract_grid", "grid_merge", "inverse_fft", "clean")

@@ -780,3 +780,20 @@ def test_hogbom_one_launch_equals_per_iteration_launches(hip, rng, G, n_minor):
     assert a[0] == b[0] == n_minor
     for x, y in zip(a[1:], b[1:]):
         np.testing.assert_array_equal(x, y)
+
+
+def test_hogbom_one_launch_exit_path_ends_every_workgroup(hip, rng):
+    """The one-launch form's bounded-wait exit, raised from the host before the
+    launch: every workgroup leaves at its first exchange, the run ends (no
+    hang) and reports -1 sources; a normal run afterwards on the same work
+    buffer is unaffected."""
+    G = 1024
+    res, psf = dirty_case(G, rng)
+    cfg = hip.make_config(G, 1e-4, psf_max_value=1.0)
+    ns, _, _, _ = run_device_hogbom(hip, G, 50, 50, res, psf, cfg, flags=hip.HOGBOM_TEST_ABORT)
+    assert ns == -1
+    a = run_device_hogbom(hip, G, 50, 50, res, psf, cfg)
+    b = run_device_hogbom(hip, G, 50, 50, res, psf, cfg, flags=hip.HOGBOM_PER_ITERATION)
+    assert a[0] == b[0] == 50
+    for x, y in zip(a[1:], b[1:]):
+        np.testing.assert_array_equal(x, y)
