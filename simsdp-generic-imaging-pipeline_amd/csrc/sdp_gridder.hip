@@ -1477,12 +1477,14 @@ reduce_pieces_kernel(const float2 *__restrict__ slabs, const long long *__restri
 }
 
 // Merge by output region: one workgroup per R x R block of grid cells
-// (R = RG, 16 or 32, or G when smaller).  Its cells are covered by the key slabs (one
-// per (class, tile), pieces already summed) of the tiles whose windows
-// [tile origin, tile origin + S) reach into the block; every cell is the
-// compensated sum of those slabs' values in (tile row, tile column, class)
-// order, and every cell is written.  Consecutive lanes read consecutive
-// cells of a slab row, and the loads of 8 slabs are in flight together.
+// (R = RG, 16 or 32, or G when smaller) that some tile's window reaches (the
+// plan lists them; the grid is zeroed first, so the others stay zero).  Its
+// cells are covered by the key slabs (one per (class, tile), pieces already
+// summed) of the tiles whose windows [tile origin, tile origin + S) reach
+// into the block; every cell of the block is the compensated sum of those
+// slabs' values in (tile row, tile column, class) order, and every cell of
+// it is written.  Consecutive lanes read consecutive cells of a slab row,
+// and the loads of 8 slabs are in flight together.
 constexpr int kRegion = 16;         // default region side (cells)
 constexpr int kRegionList = 512;    // slab descriptors staged per pass
 constexpr int kRegionCover = 1024;  // covering tiles per region (checked by the plan)
@@ -1494,8 +1496,8 @@ struct RegionArgs {
 template <int RG, int kRegionBatch>   // kRegionBatch: slabs whose loads are issued together
 __global__ void __launch_bounds__(kMergeThreads)
 merge_region_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ key_S,
-                    const long long *__restrict__ key_off, const int *__restrict__ tile_key_begin, RegionArgs a,
-                    float2 *__restrict__ grid)
+                    const long long *__restrict__ key_off, const int *__restrict__ tile_key_begin,
+                    const int2 *__restrict__ regions, RegionArgs a, float2 *__restrict__ grid)
 {
     constexpr int kCells = RG * RG / kMergeThreads;   // cells per thread at R = RG
     static_assert(kCells >= 1 && RG % 4 == 0 && kWave % RG == 0, "region side");
@@ -1505,9 +1507,10 @@ merge_region_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ k
     __shared__ int s_k0[kRegionCover];          // its first key
     __shared__ int s_b[kRegionCover];           // (by + 512) | (bx + 512) << 10
     const int R = a.R;
-    const int oy0 = blockIdx.y * R, ox0 = blockIdx.x * R;
-    tile_key_begin += (size_t)blockIdx.z * a.ntiles;
-    grid += (size_t)blockIdx.z * a.G * a.G;
+    const int2 reg = regions[blockIdx.x];   // workgroup -> region (x | y << 16, grid)
+    const int oy0 = (reg.x >> 16) * R, ox0 = (reg.x & 0xffff) * R;
+    tile_key_begin += (size_t)reg.y * a.ntiles;
+    grid += (size_t)reg.y * a.G * a.G;
     // At R = RG each of the 4 wavefronts owns a band of RG / 4 rows (64 / RG
     // rows of RG lanes at a time, its cells 64 / RG rows apart) and skips the
     // additions of slabs that miss its band; smaller regions (G < RG) take
@@ -1622,6 +1625,13 @@ merge_region_kernel(const float2 *__restrict__ kslabs, const int *__restrict__ k
     }
 }
 
+// n cells to zero (grids only 8-byte aligned)
+__global__ void zero_cells_kernel(float2 *g, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        g[i] = make_float2(0.f, 0.f);
+}
+
 // visibilities the plan dropped (outside the grid): prediction 0
 __global__ void degrid_dropped_kernel(const uint32_t *perm, int first, int n, const float2 *measured, float2 *out)
 {
@@ -1673,6 +1683,8 @@ struct sdp_grid_plan {
     int *tile_key_begin = nullptr;
     int region = 0;                    // merge region edge (min(region_rg, G))
     int region_rg = kRegion;           // merge kernel's region side, 16 or 32
+    int n_regions = 0;                 // merge regions some tile's window reaches (over all grids)
+    int2 *region_list = nullptr;       // their (x | y << 16, grid): the merge runs on these only
     int4 *multi = nullptr;         // first reduce pass: first chunk, pieces, S*S, -, then their output starts
     int n_multi2 = 0;
     int4 *multi2 = nullptr;        // second pass (keys of more than kGroupPieces pieces): first partial, count,
@@ -1773,6 +1785,7 @@ void plan_free(sdp_grid_plan *p)
     (void)hipFree(p->key_S);
     (void)hipFree(p->key_off);
     (void)hipFree(p->tile_key_begin);
+    (void)hipFree(p->region_list);
     (void)hipFree(p->multi);
     (void)hipFree(p->multi2);
     (void)hipFree(p->part_off);
@@ -2594,6 +2607,32 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
     PLAN_TRY(hipMalloc(&plan->tile_key_begin, sizeof(int) * ((size_t)ntiles_all + 1)));
     PLAN_TRY(hipMemcpyAsync(plan->tile_key_begin, tile_key_begin, sizeof(int) * ((size_t)ntiles_all + 1),
                             hipMemcpyHostToDevice, st));
+    {
+        // the merge regions some tile's window [origin, origin + smax) reaches
+        // (the merge kernel's own covering test); the others are zero
+        const int G = p.grid_size, R = plan->region, nr = (G + R - 1) / R, sm = std::max(plan->smax, 1);
+        std::vector<unsigned char> hit((size_t)plan->n_grids * nr * nr, 0);
+        for (int z = 0; z < plan->n_grids; ++z)
+            for (int t = 0; t < plan->ntiles; ++t) {
+                const int ta = z * plan->ntiles + t;
+                if (tile_key_begin[ta + 1] == tile_key_begin[ta]) continue;
+                const int x0 = (t % plan->ntx) * plan->tile, y0 = (t / plan->ntx) * plan->tile;
+                const int x1 = std::min(x0 + sm, G) - 1, y1 = std::min(y0 + sm, G) - 1;
+                for (int ry = y0 / R; ry <= y1 / R; ++ry)
+                    for (int rx = x0 / R; rx <= x1 / R; ++rx) hit[((size_t)z * nr + ry) * nr + rx] = 1;
+            }
+        std::vector<int2> list;
+        for (int z = 0; z < plan->n_grids; ++z)
+            for (int ry = 0; ry < nr; ++ry)
+                for (int rx = 0; rx < nr; ++rx)
+                    if (hit[((size_t)z * nr + ry) * nr + rx]) list.push_back(make_int2(rx | ry << 16, z));
+        plan->n_regions = (int)list.size();
+        if (!list.empty()) {
+            PLAN_TRY(hipMalloc(&plan->region_list, sizeof(int2) * list.size()));
+            PLAN_TRY(hipMemcpyAsync(plan->region_list, list.data(), sizeof(int2) * list.size(),
+                                    hipMemcpyHostToDevice, st));
+        }
+    }
     if (plan->n_chunks > 0) {
         PLAN_TRY(hipMalloc(&plan->chunks, sizeof(int4) * plan->n_chunks));
         PLAN_TRY(hipMemcpyAsync(plan->chunks, chunks, sizeof(int4) * plan->n_chunks, hipMemcpyHostToDevice, st));
@@ -2725,14 +2764,25 @@ int sdp_grid_merge(const sdp_grid_plan *plan, float *d_grid, sdp_stream s)
     }
     const int R = plan->region;
     RegionArgs ra{G, plan->tile, plan->ntx, plan->nty, plan->ntiles, std::max(plan->smax, 1), R};
-    dim3 rg((G + R - 1) / R, (G + R - 1) / R, plan->n_grids);
+    // every cell zero, then the regions some window reaches summed over it
+    const size_t cells = (size_t)G * G * plan->n_grids;
+    if (((uintptr_t)d_grid & 15) == 0) {
+        if (int e = sdp_memset_zero(d_grid, sizeof(float2) * cells, s)) return e;
+    } else {
+        zero_cells_kernel<<<(unsigned)std::min<size_t>(sdp_div_up(cells, 256), 8192), 256, 0, st>>>(
+            (float2 *)d_grid, cells);
+        SDP_LAUNCH_CHECK();
+    }
+    if (plan->n_regions == 0) return SDP_OK;
     // 8 slabs' loads in flight per batch: 16 and 32 measured no faster
     if (plan->region_rg == 32)
-        merge_region_kernel<32, 8><<<rg, kMergeThreads, 0, st>>>(plan->key_slabs, plan->key_S, plan->key_off,
-                                                                plan->tile_key_begin, ra, (float2 *)d_grid);
+        merge_region_kernel<32, 8><<<plan->n_regions, kMergeThreads, 0, st>>>(
+            plan->key_slabs, plan->key_S, plan->key_off, plan->tile_key_begin, plan->region_list, ra,
+            (float2 *)d_grid);
     else
-        merge_region_kernel<16, 8><<<rg, kMergeThreads, 0, st>>>(plan->key_slabs, plan->key_S, plan->key_off,
-                                                                plan->tile_key_begin, ra, (float2 *)d_grid);
+        merge_region_kernel<16, 8><<<plan->n_regions, kMergeThreads, 0, st>>>(
+            plan->key_slabs, plan->key_S, plan->key_off, plan->tile_key_begin, plan->region_list, ra,
+            (float2 *)d_grid);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }
