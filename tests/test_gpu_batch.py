@@ -147,11 +147,13 @@ def test_multi_grid_plan_equals_per_grid_plans(hip, mc):
     multi.close()
 
 
+@pytest.mark.parametrize("G,nb", [(256, 3), (2048, 4)])
 @pytest.mark.parametrize("direction,flags", [("forward", "real_in"), ("inverse", "real_out")])
-def test_fft_batch_equals_one_grid_at_a_time(hip, direction, flags):
-    """sdp_fft2d_batch over three grids equals three sdp_fft2d calls bit for
-    bit, with the major cycle's fused shift / real / taper flags."""
-    G, nb = 256, 3
+def test_fft_batch_equals_one_grid_at_a_time(hip, direction, flags, G, nb):
+    """sdp_fft2d_batch over several grids equals one sdp_fft2d call per grid
+    bit for bit, with the major cycle's fused shift / real / taper flags.  At
+    2048^2 the batch of four takes the 16-column blocks of the column pass and
+    the single grids the 8-column blocks."""
     rng = np.random.default_rng(5)
     d_tap = up(hip, S.taper_padded(G))
     if flags == "real_in":
