@@ -412,20 +412,20 @@ hogbom_step_kernel(int iter, HogbomArgs a, float *__restrict__ r, const float *_
 //
 // Per iteration each workgroup finds its best pixel (largest |r|, first
 // index on ties: the largest |r| in one pass, then the first index holding
-// it), publishes it as one 64-bit word (|r| bits, index, sign, iteration
-// tag) and arrives at a counter; one thread per workgroup waits for the
-// counter to reach every workgroup's arrival, then each workgroup reads all
-// words (re-reading any whose tag is not yet this iteration's: the counter
-// and the word are separate stores) and reduces them the same way, so all
-// reach the same component and stop decision; workgroup 0 records the
-// component and every workgroup subtracts it from its own pixels.  Words
-// are double-buffered by iteration parity: a workgroup can publish
-// iteration i + 2 into a buffer only after every workgroup has arrived at
-// iteration i + 1, that is after all have read iteration i.
+// it) and publishes it as one 64-bit word (|r| bits, index, sign, iteration
+// tag); thread t of every workgroup reads workgroup t's word until it
+// carries this iteration's tag, and each workgroup reduces the words the
+// same way, so all reach the same component and stop decision; workgroup 0
+// records the component and every workgroup subtracts it from its own
+// pixels.  Words are double-buffered by iteration parity: a workgroup
+// publishes iteration i + 2 into a buffer only after it has read every
+// workgroup's word of iteration i + 1, each published after its workgroup
+// had read all of iteration i.
 //
-// Waits are bounded: a counter or word that does not arrive within 20 ms (it
-// cannot, short of a fault elsewhere) sets an abort flag that every waiter
-// also checks, every workgroup leaves, and the run reports -1 sources.
+// Waits are bounded: a word that does not arrive within 20 ms (it cannot,
+// short of a fault elsewhere) sets an abort flag, which thread 0 of every
+// workgroup reads at each exchange; every workgroup leaves and the run
+// reports -1 sources.
 //
 // The arithmetic, tie order and stop tests are those of hogbom_step_kernel
 // (bit for bit); the noise stop is left out, so the form is taken only when
@@ -536,7 +536,8 @@ hogbom_resident_kernel(PersistArgs a, float *__restrict__ r, const float *__rest
     __shared__ int s_ok;
     __shared__ Best s_comp;   // the iteration's component, from thread 0
     __shared__ float4 lds_v[NL > 0 ? NL * TPB : 1];
-    unsigned *counter = sync, *abort_flag = sync + 64;   // 256 bytes apart
+    unsigned *abort_flag = sync + 64;
+    if (threadIdx.x == 0) s_ok = 1;   // read first after block_max's barriers
     const int G = a.G, half = G / 2;
     const int n4 = G * G / 4;
     const int gmask = (1 << a.row_shift) - 1;
@@ -593,41 +594,29 @@ hogbom_resident_kernel(PersistArgs a, float *__restrict__ r, const float *__rest
             const uint64_t w = mwg > 0.f ? pack_word(mwg, fwg >> 1, (unsigned)(fwg & 1), tag)
                                          : pack_word(0.f, 0, __float_as_uint(v[0].x) >> 31, tag);
             __hip_atomic_store(buf + blockIdx.x, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            // every workgroup's arrival at this iteration
-            const unsigned target = (unsigned)a.nblocks * (unsigned)(it + 1);
-            const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-            // the counter and the abort flag read together each round: an
-            // abort raised anywhere ends every workgroup at its next exchange
-            int ok = 1;
-            for (;;) {
-                const unsigned c = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned ab = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (ab) {
-                    ok = 0;
-                    break;
-                }
-                if (c >= target) break;
-                if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
-                    __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    ok = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-            s_ok = ok;
         }
-        __syncthreads();
-        // all words (each carries its iteration's tag; the counter says they
-        // were stored, the tag that this one has arrived)
+        // all words: thread t < nblocks reads workgroup t's until it carries
+        // this iteration's tag; thread 0 reads the abort flag with every read
+        // of its word (an abort raised anywhere ends every workgroup at its
+        // next exchange) and passes it on through s_ok, which the other
+        // waiting threads read each round
         float wv = 0.f;
         int wi = 0x7fffffff;
         uint64_t w0 = 0;
-        if (s_ok && (int)threadIdx.x < a.nblocks) {
+        if ((int)threadIdx.x < a.nblocks) {
             const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
             uint64_t w;
-            while (((w = __hip_atomic_load(buf + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & 0x7fu) !=
-                   tag) {
+            for (;;) {
+                w = __hip_atomic_load(buf + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (threadIdx.x == 0) {
+                    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        s_ok = 0;
+                        break;
+                    }
+                } else if (!*(volatile int *)&s_ok) {
+                    break;
+                }
+                if ((w & 0x7fu) == tag) break;
                 if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
                     __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     s_ok = 0;
@@ -727,12 +716,11 @@ hogbom_resident_kernel(PersistArgs a, float *__restrict__ r, const float *__rest
     if (blockIdx.x == 0 && threadIdx.x == 0) *st_final = State{stop ? 1 : 0, stop == 2 ? -1 : nsrc, src0, 0.f};
 }
 
-// the arrival counter to 0, the abort flag to `abort` (1 only to exercise
-// the exit path: every workgroup then leaves at its first exchange), the
-// words of both parities to a tag no iteration carries
+// the abort flag to `abort` (1 only to exercise the exit path: every
+// workgroup then leaves at its first exchange), the words of both parities
+// to a tag no iteration carries
 __global__ void hogbom_resident_init_kernel(unsigned *sync, uint64_t *words, int n, unsigned abort)
 {
-    if (threadIdx.x == 0) sync[0] = 0u;
     if (threadIdx.x == 64) sync[64] = abort;
     for (int i = threadIdx.x; i < n; i += blockDim.x) words[i] = 0x7full;
 }
@@ -758,8 +746,7 @@ size_t psf_pad_offset(size_t npix)
     return (partial_offset() + 2 * partial_count(npix) * sizeof(Partial) + 255) & ~(size_t)255;
 }
 
-// then the one-launch form's arrival counter and abort flag (256 bytes
-// apart, 512 bytes in all)
+// then the one-launch form's abort flag (at byte 256 of a 512-byte block)
 size_t sync_offset(size_t npix)
 {
     return (psf_pad_offset(npix) + (npix + 2 * kPsfPad) * sizeof(float) + 255) & ~(size_t)255;
@@ -883,7 +870,7 @@ size_t sdp_hogbom_work_bytes(int G)
 {
     if (G < 2) return 0;
     const size_t npix = (size_t)G * G;
-    // partials of two consecutive launches, the padded PSF copy, the counter and flag
+    // partials of two consecutive launches, the padded PSF copy, the abort flag
     return sync_offset(npix) + 512;
 }
 
