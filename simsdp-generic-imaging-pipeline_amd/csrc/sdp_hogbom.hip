@@ -431,26 +431,20 @@ hogbom_step_kernel(int iter, HogbomArgs a, float *__restrict__ r, const float *_
 // (bit for bit); the noise stop is left out, so the form is taken only when
 // it cannot fire (noise_factor * gain <= 1/2).
 constexpr unsigned long long kWaitTicks = 2000000ull;   // 20 ms of the 100 MHz counter
+// Every word is stored in kWordReplicas copies, kWordStride words apart
+// (8.5 KB: different memory channels), and workgroup b polls copy
+// b % kWordReplicas: each copy's lines are polled by an eighth of the
+// workgroups instead of all of them.
+constexpr int kWordReplicas = 8;
+constexpr int kWordStride = 1088;   // >= the largest workgroup count (one per CU)
 
 struct PersistArgs {
     int G, n_minor, max_sources, nblocks, mirror, psf_pad_bytes, row_shift;   // row_shift: log2(G / 4)
     double psf_max, gain, weak;
 };
 
-struct Best {
-    float v;     // |r|, > 0 to count
-    int i;       // linear index, INT_MAX when none
-    float val;   // r
-};
-
-__device__ __forceinline__ void take_better(Best &b, const Best &o)
-{
-    if (o.v > 0.f && better(o.v, o.i, b.v, b.i)) b = o;
-}
-
-// a 64-lane wavefront's largest float / smallest int in every lane: DPP
-// moves within 16-lane rows, then the gfx950 row swaps across them (no
-// lane-address registers)
+// lane moves for wavefront reductions: DPP within 16-lane rows, then the
+// gfx950 row swaps across them (no lane-address registers)
 template <int CTRL> __device__ __forceinline__ int dpp_mov(int x)
 {
     return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xf, 0xf, false);
@@ -463,44 +457,50 @@ __device__ __forceinline__ int swap_rows32(int x)
 {
     return (int)__builtin_amdgcn_permlane32_swap((unsigned)x, (unsigned)x, false, false)[1];
 }
-__device__ __forceinline__ float wave_max(float x)
+// a 64-lane wavefront's largest 64-bit key in every lane
+template <int CTRL> __device__ __forceinline__ uint64_t dpp_mov64(uint64_t x)
 {
-    x = fmaxf(x, __int_as_float(dpp_mov<0xb1>(__float_as_int(x))));    // quad: lanes ^ 1
-    x = fmaxf(x, __int_as_float(dpp_mov<0x4e>(__float_as_int(x))));    // quad: lanes ^ 2
-    x = fmaxf(x, __int_as_float(dpp_mov<0x141>(__float_as_int(x))));   // half-row mirror
-    x = fmaxf(x, __int_as_float(dpp_mov<0x140>(__float_as_int(x))));   // row mirror
-    x = fmaxf(x, __int_as_float(swap_rows16(__float_as_int(x))));
-    return fmaxf(x, __int_as_float(swap_rows32(__float_as_int(x))));
+    const unsigned lo = (unsigned)dpp_mov<CTRL>((int)(unsigned)x);
+    const unsigned hi = (unsigned)dpp_mov<CTRL>((int)(unsigned)(x >> 32));
+    return (uint64_t)hi << 32 | lo;
 }
-__device__ __forceinline__ int wave_min(int x)
+__device__ __forceinline__ uint64_t swap64(uint64_t x, bool rows32)
 {
-    x = min(x, dpp_mov<0xb1>(x));
-    x = min(x, dpp_mov<0x4e>(x));
-    x = min(x, dpp_mov<0x141>(x));
-    x = min(x, dpp_mov<0x140>(x));
-    x = min(x, swap_rows16(x));
-    return min(x, swap_rows32(x));
+    const unsigned lo = (unsigned)(rows32 ? swap_rows32((int)(unsigned)x) : swap_rows16((int)(unsigned)x));
+    const unsigned hi = (unsigned)(rows32 ? swap_rows32((int)(unsigned)(x >> 32)) : swap_rows16((int)(unsigned)(x >> 32)));
+    return (uint64_t)hi << 32 | lo;
 }
-// the workgroup's largest float / smallest int, in every thread (lds: TPB / 64 slots)
-template <int TPB> __device__ float block_max(float x, float *lds)
+__device__ __forceinline__ uint64_t max64(uint64_t a, uint64_t b) { return a > b ? a : b; }
+__device__ __forceinline__ uint64_t wave_max64(uint64_t x)
 {
-    x = wave_max(x);
-    __syncthreads();   // lds free from its last use
+    x = max64(x, dpp_mov64<0xb1>(x));
+    x = max64(x, dpp_mov64<0x4e>(x));
+    x = max64(x, dpp_mov64<0x141>(x));
+    x = max64(x, dpp_mov64<0x140>(x));
+    x = max64(x, swap64(x, false));
+    return max64(x, swap64(x, true));
+}
+// the workgroup's largest key, in every thread: one barrier.  lds holds
+// TPB / 64 slots; consecutive calls must alternate between two such arrays
+// (a call's slots are rewritten two calls later, after the barrier of the
+// call in between has seen every thread finish reading them).
+template <int TPB> __device__ uint64_t block_max64(uint64_t x, uint64_t *lds)
+{
+    x = wave_max64(x);
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
     __syncthreads();
-    float r = lds[0];
-    for (int w = 1; w < TPB / 64; ++w) r = fmaxf(r, lds[w]);
+    uint64_t r = lds[0];
+#pragma unroll
+    for (int w = 1; w < TPB / 64; ++w) r = max64(r, lds[w]);
     return r;
 }
-template <int TPB> __device__ int block_min(int x, int *lds)
+
+// a pixel's reduction key: |r| bits above, the complement of index * 2 +
+// sign below, so the largest key is the largest |r| at its first index;
+// 0 for |r| = 0 (never a component unless every pixel is 0)
+__device__ __forceinline__ uint64_t pixel_key(float absr, unsigned idx2)
 {
-    x = wave_min(x);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = x;
-    __syncthreads();
-    int r = lds[0];
-    for (int w = 1; w < TPB / 64; ++w) r = min(r, lds[w]);
-    return r;
+    return absr > 0.f ? (uint64_t)__float_as_uint(absr) << 32 | (uint32_t)~idx2 : 0ull;
 }
 
 // word: |r| bits << 32 | (index * 2 + sign) << 7 | iteration tag (7 bits;
@@ -531,10 +531,8 @@ hogbom_resident_kernel(PersistArgs a, float *__restrict__ r, const float *__rest
 {
     constexpr int NR = NG - NL;
     static_assert(NR >= 1 && NG % B == 0 && NR % B == 0, "group split");
-    __shared__ float lds_f[TPB / 64];
-    __shared__ int lds_i[TPB / 64];
+    __shared__ uint64_t lds_k[2][TPB / 64];   // block_max64's slots, used alternately
     __shared__ int s_ok;
-    __shared__ Best s_comp;   // the iteration's component, from thread 0
     __shared__ float4 lds_v[NL > 0 ? NL * TPB : 1];
     unsigned *abort_flag = sync + 64;
     if (threadIdx.x == 0) s_ok = 1;   // read first after block_max's barriers
@@ -578,31 +576,34 @@ hogbom_resident_kernel(PersistArgs a, float *__restrict__ r, const float *__rest
         // per iteration, not hoisted into registers held all along
         int qi = qb;
         asm volatile("" : "+v"(qi));
-        const float mwg = block_max<TPB>(m, lds_f);
-        // its first index (index * 2 + sign), from the threads that hold it
-        int first = 0x7fffffff;
-        if (mwg > 0.f && m == mwg) {
-            const int j = fabsf(e4.x) == mwg ? 0 : fabsf(e4.y) == mwg ? 1 : fabsf(e4.z) == mwg ? 2 : 3;
+        // the workgroup's best pixel: the largest key over its threads, each
+        // the thread's largest |r| (m) at the first index holding it
+        uint64_t key = 0;
+        if (m > 0.f) {
+            const int j = fabsf(e4.x) == m ? 0 : fabsf(e4.y) == m ? 1 : fabsf(e4.z) == m ? 2 : 3;
             const float ej = j == 0 ? e4.x : j == 1 ? e4.y : j == 2 ? e4.z : e4.w;
-            first = ((qi + kmax * TPB) * 4 + j) * 2 + (int)(__float_as_uint(ej) >> 31);
+            key = pixel_key(m, (unsigned)(((qi + kmax * TPB) * 4 + j) * 2) + (__float_as_uint(ej) >> 31));
         }
-        const int fwg = block_min<TPB>(first, lds_i);
-        uint64_t *buf = words + (size_t)(it & 1) * a.nblocks;
+        key = block_max64<TPB>(key, lds_k[0]);
+        // copy c of parity p at words + (c * 2 + p) * kWordStride
+        uint64_t *par = words + (size_t)(it & 1) * kWordStride;
+        uint64_t *buf = par + (size_t)(blockIdx.x % kWordReplicas) * 2 * kWordStride;   // the copy this workgroup reads
         const unsigned tag = (unsigned)it & 0x3fu;   // 0x7f never: the reset words carry it
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < 64) {
             // pixel 0 lives in thread 0 of workgroup 0 (group 0: registers)
-            const uint64_t w = mwg > 0.f ? pack_word(mwg, fwg >> 1, (unsigned)(fwg & 1), tag)
-                                         : pack_word(0.f, 0, __float_as_uint(v[0].x) >> 31, tag);
-            __hip_atomic_store(buf + blockIdx.x, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned sign0 = (unsigned)__builtin_amdgcn_readfirstlane((int)__float_as_uint(v[0].x)) >> 31;
+            const uint64_t w = key ? (key & 0xffffffff00000000ull) | (uint64_t)(~(uint32_t)key) << 7 | tag
+                                   : pack_word(0.f, 0, sign0, tag);
+            if (threadIdx.x < kWordReplicas)
+                __hip_atomic_store(par + (size_t)threadIdx.x * 2 * kWordStride + blockIdx.x, w, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
         // all words: thread t < nblocks reads workgroup t's until it carries
         // this iteration's tag; thread 0 reads the abort flag with every read
         // of its word (an abort raised anywhere ends every workgroup at its
         // next exchange) and passes it on through s_ok, which the other
         // waiting threads read each round
-        float wv = 0.f;
-        int wi = 0x7fffffff;
-        uint64_t w0 = 0;
+        uint64_t wkey = 0;
         if ((int)threadIdx.x < a.nblocks) {
             const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
             uint64_t w;
@@ -622,28 +623,34 @@ hogbom_resident_kernel(PersistArgs a, float *__restrict__ r, const float *__rest
                     s_ok = 0;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(1);
+                __builtin_amdgcn_s_sleep(3);
             }
-            wv = __uint_as_float((uint32_t)(w >> 32));
-            wi = (int)((w >> 7) & 0x1ffffffu);   // index * 2 + sign
-            if (threadIdx.x == 0) w0 = w;
+            wkey = pixel_key(__uint_as_float((uint32_t)(w >> 32)), (unsigned)(w >> 7) & 0x1ffffffu);
         }
-        __syncthreads();
+        // the same reduction over the words in every workgroup (its barrier
+        // also publishes s_ok)
+        const uint64_t gkey = block_max64<TPB>(wkey, lds_k[1]);
         if (!s_ok) {
             stop = 2;
             break;   // every workgroup leaves: the abort flag is set
         }
-        const float gmax = block_max<TPB>(wv, lds_f);
-        const int gfirst = block_min<TPB>(gmax > 0.f && wv == gmax ? wi : 0x7fffffff, lds_i);
-        if (threadIdx.x == 0) {
+        float curr_max, peak;
+        int max_idx;
+        if (gkey) {
+            const unsigned idx2 = ~(uint32_t)gkey;
+            curr_max = __uint_as_float((uint32_t)(gkey >> 32));
+            max_idx = (int)(idx2 >> 1);
+            peak = (idx2 & 1u) ? -curr_max : curr_max;
+        } else {
             // nothing beat the initial 0: the reference keeps pixel (0, 0)
-            // with its value, a zero of pixel 0's sign (workgroup 0's word)
-            s_comp = gmax > 0.f ? Best{gmax, gfirst >> 1, (gfirst & 1) ? -gmax : gmax}
-                                : Best{0.f, 0, ((w0 >> 7) & 1u) ? -0.f : 0.f};
+            // with its value, a zero of pixel 0's sign (workgroup 0's word,
+            // this iteration's: workgroup 0 publishes no later one before
+            // this workgroup's next word)
+            const uint64_t w0 = __hip_atomic_load(buf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            curr_max = 0.f;
+            max_idx = 0;
+            peak = ((w0 >> 7) & 1u) ? -0.f : 0.f;
         }
-        __syncthreads();
-        const float curr_max = s_comp.v, peak = s_comp.val;
-        const int max_idx = s_comp.i;
         const bool weak_source = nsrc > 0 && (double)curr_max < (double)fabsf(src0) * a.weak;
         if (weak_source || nsrc >= a.max_sources) {
             stop = 1;
@@ -718,11 +725,12 @@ hogbom_resident_kernel(PersistArgs a, float *__restrict__ r, const float *__rest
 
 // the abort flag to `abort` (1 only to exercise the exit path: every
 // workgroup then leaves at its first exchange), the words of both parities
-// to a tag no iteration carries
+// in every copy to a tag no iteration carries
 __global__ void hogbom_resident_init_kernel(unsigned *sync, uint64_t *words, int n, unsigned abort)
 {
     if (threadIdx.x == 64) sync[64] = abort;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) words[i] = 0x7full;
+    for (int c = 0; c < 2 * kWordReplicas; ++c)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) words[(size_t)c * kWordStride + i] = 0x7full;
 }
 
 // deconvolution_run.c:297-306: pixel coordinates to radians
@@ -751,6 +759,11 @@ size_t sync_offset(size_t npix)
 {
     return (psf_pad_offset(npix) + (npix + 2 * kPsfPad) * sizeof(float) + 255) & ~(size_t)255;
 }
+
+// then its words: both parities in every copy
+size_t words_offset(size_t npix) { return sync_offset(npix) + 512; }
+
+size_t words_bytes() { return sizeof(uint64_t) * 2 * kWordReplicas * kWordStride; }
 
 // 16-byte groups per thread per chunk (2, 4 or 8): the most that still
 // gives every CU two chunks, so small images spread over the whole GPU;
@@ -831,7 +844,7 @@ int launch_resident(int G, int n_minor, int max_sources, const HogbomArgs &h, do
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sh.k, sh.tpb, 0) != hipSuccess || per_cu < 1)
             continue;
         // one workgroup per CU at most; one waiting thread per word
-        if (nb > (size_t)compute_units() || nb > (size_t)sh.tpb) continue;
+        if (nb > (size_t)compute_units() || nb > (size_t)sh.tpb || nb > (size_t)kWordStride) continue;
         PersistArgs a;
         a.G = G;
         a.n_minor = n_minor;
@@ -844,9 +857,9 @@ int launch_resident(int G, int n_minor, int max_sources, const HogbomArgs &h, do
         a.psf_max = h.psf_max;
         a.gain = h.gain;
         a.weak = h.weak;
-        uint64_t *words = (uint64_t *)((char *)d_work + partial_offset());
+        uint64_t *words = (uint64_t *)((char *)d_work + words_offset(npix));
         unsigned *sync = (unsigned *)((char *)d_work + sync_offset(npix));
-        hogbom_resident_init_kernel<<<1, 256, 0, st>>>(sync, words, (int)(2 * nb), preset_abort ? 1u : 0u);
+        hogbom_resident_init_kernel<<<1, 256, 0, st>>>(sync, words, (int)nb, preset_abort ? 1u : 0u);
         SDP_LAUNCH_CHECK();
         void *args[] = {&a, &d_residual, (void *)&psf_pad, &d_model, &d_sources, &d_num_sources, &st_final,
                         &words, &sync};
@@ -870,8 +883,9 @@ size_t sdp_hogbom_work_bytes(int G)
 {
     if (G < 2) return 0;
     const size_t npix = (size_t)G * G;
-    // partials of two consecutive launches, the padded PSF copy, the abort flag
-    return sync_offset(npix) + 512;
+    // partials of two consecutive launches, the padded PSF copy, the abort
+    // flag, the one-launch form's words
+    return words_offset(npix) + words_bytes();
 }
 
 int sdp_hogbom(int G, int n_minor, int max_sources, float *d_residual, const float *d_psf, double psf_max,
