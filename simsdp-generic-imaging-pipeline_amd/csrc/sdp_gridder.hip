@@ -1344,10 +1344,16 @@ __device__ __forceinline__ void fused_work_item(const uint2 *__restrict__ frecs,
                 }
                 float vr[U], vi[U];
                 if constexpr (U == 4) {
-                    float xr = 0.f, xi = 0.f;   // row mode: the row's single-part partials
-                    if constexpr (RS) pred_add<CONJ>(xr, xi, g[Q][0], make_float2(krow.x, -krow.y));
-                    const float b0 = m2.x - row_total(fold16(fold32(pr[0], pr[2]), fold32(pr[1], pr[3])) + xr);
-                    const float b1 = m2.y - row_total(fold16(fold32(pim[0], pim[2]), fold32(pim[1], pim[3])) + xi);
+                    float s0 = fold16(fold32(pr[0], pr[2]), fold32(pr[1], pr[3]));
+                    float s1 = fold16(fold32(pim[0], pim[2]), fold32(pim[1], pim[3]));
+                    if constexpr (RS) {
+                        float xr = 0.f, xi = 0.f;   // row mode: the row's single-part partials
+                        pred_add<CONJ>(xr, xi, g[Q][0], make_float2(krow.x, -krow.y));
+                        s0 += xr;
+                        s1 += xi;
+                    }
+                    const float b0 = m2.x - row_total(s0);
+                    const float b1 = m2.y - row_total(s1);
                     if (vis_out && (lane & 15) == 0) vis_out[o] = make_float2(b0, b1);
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
@@ -2765,6 +2771,8 @@ int sdp_grid_merge(const sdp_grid_plan *plan, float *d_grid, sdp_stream s)
     const int R = plan->region;
     RegionArgs ra{G, plan->tile, plan->ntx, plan->nty, plan->ntiles, std::max(plan->smax, 1), R};
     // every cell zero, then the regions some window reaches summed over it
+    // (a clear beside the piece reduction, on the helper stream, measured no
+    // faster: both are bound by HBM)
     const size_t cells = (size_t)G * G * plan->n_grids;
     if (((uintptr_t)d_grid & 15) == 0) {
         if (int e = sdp_memset_zero(d_grid, sizeof(float2) * cells, s)) return e;
