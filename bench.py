@@ -405,6 +405,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="taps per work item of the plan (with --tile)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (HBM counter calibration)")
+    ap.add_argument("--fft-only", action="store_true",
+                    help="only run the major cycle's 4096^2 inverse FFT (shift / real / taper) 5 times "
+                         "(counter runs on the FFT kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -422,6 +425,19 @@ def main():
             hip.check(hip.load().sdp_device_copy(b.ptr, a.ptr, n, None))
         hip.synchronize()
         log(f"copied {n} bytes x 5")
+        return
+    if args.fft_only:
+        G = 4096
+        d_grid = hip.DeviceBuffer(8 * G * G)
+        d_grid.zero()
+        d_dirty = hip.DeviceBuffer(4 * G * G)
+        d_work = hip.DeviceBuffer(hip.fft_work_bytes(G))
+        d_tap = hip.DeviceBuffer.from_array(sdp_synth.taper_padded(G))
+        flags = hip.FFT_SHIFT_IN | hip.FFT_SHIFT_OUT | hip.FFT_REAL_OUT | hip.FFT_CORRECT_OUT
+        for _ in range(5):
+            hip.fft2d(d_grid, d_dirty, G, hip.FFT_INVERSE, flags, d_tap, d_work)
+        hip.synchronize()
+        log(f"{G}^2 inverse FFT x 5")
         return
 
     t = time.perf_counter()
