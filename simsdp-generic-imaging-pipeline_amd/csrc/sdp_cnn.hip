@@ -317,6 +317,182 @@ __global__ void __launch_bounds__(kT) gemm_fwd_narrow_kernel(GemmArgs g, int cin
                          0);
 }
 
+// Forward 3x3 convolution on the matrix cores with the input tile staged
+// once: a workgroup computes a TH-row x 16-column tile of output pixels for
+// BN output channels.  Per 32-channel chunk of the input it stages the
+// tile's (TH + 2) x 18 halo (BF16, channels contiguous) and the chunk's
+// weights for all nine taps in LDS, then every tap's product reads its A
+// fragments from the halo at the tap's offset, so an input value is read
+// from HBM once per chunk instead of once per tap.  Wave w takes image rows
+// 2w and 2w + 1 of the tile: two 16-pixel row blocks x BN / 16 column
+// blocks of v_mfma_f32_16x16x32_bf16 per tap.  Weights come pre-rounded to
+// BF16 in [Cout_pad][9][Cin] order (refresh_bf16_weights; rows from Cout to
+// Cout_pad are zero and their outputs are not stored); the epilogue adds the
+// bias and the skip input and applies the ReLU, as the general kernels do.
+// Needs Cin % 32 == 0, H % TH == 0, W % 16 == 0 and BN | Cout_pad.
+constexpr int kHaloW = 18;
+constexpr int kLdA = 40;   // LDS row of one pixel's (or weight column's) 32 BF16 K values, padded
+
+// two floats rounded to BF16 (nearest even), the first in the low half
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b)
+{
+    const __bf16 lo = (__bf16)a, hi = (__bf16)b;
+    return (uint32_t)__builtin_bit_cast(uint16_t, lo) | (uint32_t)__builtin_bit_cast(uint16_t, hi) << 16;
+}
+
+struct ConvArgs {
+    int B, H, W, Cin, Cout;
+    const float *x;          // NHWC input
+    const __bf16 *wt;        // the layer's weights, [Cout_pad][9][Cin]
+    const float *bias, *skip;
+    float *out;
+    int relu;
+};
+
+template <int BN, int TH>
+__global__ void __launch_bounds__(32 * TH) conv_fwd_tile_kernel(ConvArgs c)
+{
+    constexpr int NJ = BN / 16, NT = 32 * TH, HALO = (TH + 2) * kHaloW;
+    __shared__ __attribute__((aligned(16))) __bf16 As[HALO][kLdA];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[9][BN][kLdA];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_x = c.W >> 4, tiles_y = c.H / TH;
+    int t = blockIdx.x;
+    const int tx = t % tiles_x;
+    t /= tiles_x;
+    const int ty = t % tiles_y, b = t / tiles_y;
+    const int x0 = tx * 16, y0 = ty * TH, n0 = blockIdx.y * BN;
+    f32x4 acc[2][NJ];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const float *xb = c.x + (size_t)b * c.H * c.W * c.Cin;
+    for (int c0 = 0; c0 < c.Cin; c0 += kBK) {
+        // halo: pixel h of the (TH + 2) x 18 tile, channels 4q .. 4q + 3 of the chunk
+        for (int i = tid; i < HALO * 8; i += NT) {
+            const int h = i >> 3, q = i & 7;
+            const int yy = y0 - 1 + h / kHaloW, xx = x0 - 1 + h % kHaloW;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (yy >= 0 && yy < c.H && xx >= 0 && xx < c.W)
+                v = *reinterpret_cast<const float4 *>(xb + ((size_t)yy * c.W + xx) * c.Cin + c0 + 4 * q);
+            *reinterpret_cast<uint2 *>(&As[h][4 * q]) = make_uint2(pack_bf16x2(v.x, v.y), pack_bf16x2(v.z, v.w));
+        }
+        // weights: output channel n, tap k, K values 8 p .. 8 p + 7 of the chunk
+        for (int i = tid; i < 9 * BN * 4; i += NT) {
+            const int p4 = i & 3, nk = i >> 2, n = nk % BN, k = nk / BN;
+            const uint4 w = *reinterpret_cast<const uint4 *>(c.wt + ((size_t)(n0 + n) * 9 + k) * c.Cin + c0 + 8 * p4);
+            *reinterpret_cast<uint4 *>(&Bs[k][n][8 * p4]) = w;
+        }
+        __syncthreads();
+        const int kq = (lane >> 4) * 8;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int ky = k / 3, kx = k % 3;   // halo offsets of the tap (the halo starts one pixel out)
+            bf16x8 bw[NJ];
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) bw[j] = *reinterpret_cast<const bf16x8 *>(&Bs[k][16 * j + (lane & 15)][kq]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int h = (2 * wave + i + ky) * kHaloW + (lane & 15) + kx;
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&As[h][kq]);
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[j], acc[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // accumulator (i, j): output channel n0 + 16 j + lane % 16, pixels 4 (lane / 16) .. + 3 of row 2 wave + i
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int y = y0 + 2 * wave + i;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int n = n0 + 16 * j + (lane & 15);
+            if (n >= c.Cout) continue;   // a padding row of the weights
+            const float bn = c.bias ? c.bias[n] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const size_t m = ((size_t)(b * c.H + y) * c.W + x0 + 4 * (lane >> 4) + r);
+                float v = acc[i][j][r] + bn;
+                if (c.skip) v += c.skip[m * c.Cout + n];
+                if (c.relu) v = fmaxf(v, 0.f);
+                c.out[m * c.Cout + n] = v;
+            }
+        }
+    }
+}
+
+// Forward 3x3 convolution of a network input with few channels (Cin < 8,
+// the first layer) on the vector ALUs: a thread computes four output
+// channels of one pixel.  Operands are rounded to BF16 as the matrix-core
+// path rounds them (their products are then exact in FP32); the sums are
+// FP32, taps then channels in order.
+__global__ void __launch_bounds__(kT) conv_fwd_small_cin_kernel(ConvArgs c, const float *w)
+{
+    extern __shared__ float ws[];   // [9][Cin][Cout], BF16-rounded
+    const int nw = 9 * c.Cin * c.Cout;
+    for (int i = threadIdx.x; i < nw; i += kT) ws[i] = (float)(__bf16)w[i];
+    __syncthreads();
+    const int groups = c.Cout >> 2;
+    const long long total = (long long)c.B * c.H * c.W * groups;
+    for (long long i = blockIdx.x * (long long)kT + threadIdx.x; i < total; i += (long long)gridDim.x * kT) {
+        const int g = (int)(i % groups);
+        const long long m = i / groups;
+        const int x = (int)(m % c.W), y = (int)((m / c.W) % c.H);
+        const long long bh = m / c.W - y;   // b * H
+        const float4 a = c.bias ? make_float4(c.bias[4 * g], c.bias[4 * g + 1], c.bias[4 * g + 2], c.bias[4 * g + 3])
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int k = 0; k < 9; ++k) {
+            const int yy = y + k / 3 - 1, xx = x + k % 3 - 1;
+            if (yy < 0 || yy >= c.H || xx < 0 || xx >= c.W) continue;
+            const float *px = c.x + ((bh + yy) * c.W + xx) * c.Cin;
+            for (int ci = 0; ci < c.Cin; ++ci) {
+                const float v = (float)(__bf16)px[ci];
+                const float4 wk = *reinterpret_cast<const float4 *>(&ws[(k * c.Cin + ci) * c.Cout + 4 * g]);
+                s.x = fmaf(v, wk.x, s.x);
+                s.y = fmaf(v, wk.y, s.y);
+                s.z = fmaf(v, wk.z, s.z);
+                s.w = fmaf(v, wk.w, s.w);
+            }
+        }
+        float4 v = make_float4(s.x + a.x, s.y + a.y, s.z + a.z, s.w + a.w);
+        if (c.skip) {
+            const float4 k4 = *reinterpret_cast<const float4 *>(c.skip + m * c.Cout + 4 * g);
+            v = make_float4(v.x + k4.x, v.y + k4.y, v.z + k4.z, v.w + k4.w);
+        }
+        if (c.relu) v = make_float4(fmaxf(v.x, 0.f), fmaxf(v.y, 0.f), fmaxf(v.z, 0.f), fmaxf(v.w, 0.f));
+        *reinterpret_cast<float4 *>(c.out + m * c.Cout + 4 * g) = v;
+    }
+}
+
+// The layers' weights rounded to BF16 in [Cout_pad][9][Cin] order, from the
+// FP32 parameters ([3][3][Cin][Cout] per layer), for conv_fwd_tile_kernel;
+// the rows from Cout to Cout_pad stay as created (zero).
+constexpr int kMaxLayers = 32;   // 4 levels - 1 layers at most 6 levels
+struct Bf16Layers {
+    int n;
+    int cin[kMaxLayers], cout[kMaxLayers];
+    long long off[kMaxLayers + 1];   // first weight of each layer in the parameter vector; off[n]: the end
+    long long dst[kMaxLayers];       // its first element in the BF16 array
+};
+
+__global__ void bf16_weights_kernel(const float *params, Bf16Layers L, __bf16 *wt)
+{
+    const long long total = L.off[L.n];
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
+        int l = 0;
+        while (i >= L.off[l + 1]) ++l;
+        if (i >= L.off[l] + 9LL * L.cin[l] * L.cout[l]) continue;   // a bias
+        const long long r = i - L.off[l];   // (k * Cin + ci) * Cout + co
+        const int co = (int)(r % L.cout[l]);
+        const long long kc = r / L.cout[l];
+        const int ci = (int)(kc % L.cin[l]), k = (int)(kc / L.cin[l]);
+        wt[L.dst[l] + ((long long)co * 9 + k) * L.cin[l] + ci] = (__bf16)params[i];
+    }
+}
+
 // weight gradient: sum the K-split partials in split order
 __global__ void sum_splits_kernel(const float *part, int splits, size_t n, float *out)
 {
@@ -614,10 +790,15 @@ struct sdp_cnn {
     struct Layer {
         int cin, cout;
         size_t w_off, b_off;   // into the parameter vector
+        int cout_pad;          // output channels of its BF16 weights (16 when cout < 16)
+        size_t w16_off;        // into wt16
     };
     std::vector<Layer> layers;   // enc0a, enc0b, ..., dec(L-2)u, dec(L-2)d, ..., dec0u, dec0d, head
     size_t n_params = 0;
     float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    __bf16 *wt16 = nullptr;       // BF16 weights in [Cout_pad][9][Cin] order (conv_fwd_tile_kernel)
+    size_t n_wt16 = 0;
+    int wt16_stale = 1;           // params changed since wt16 was formed
     int adam_t = 0;
     // activations per level (B_max x H_l x W_l x C)
     std::vector<float *> x, a, e, up, s, d, genc, g1, g2;
@@ -695,6 +876,31 @@ int conv_forward(const sdp_cnn *n, int li, int B, int H, int W, const float *in,
                  int relu, hipStream_t st)
 {
     const auto &L = n->layers[li];
+    if (n->p.precision == SDP_CNN_BF16 && L.cin % kBK == 0 &&
+        (L.cout_pad == 16 || L.cout_pad == 32 || L.cout_pad % 64 == 0) && H % 8 == 0 && W % 16 == 0) {
+        ConvArgs c{B, H, W, L.cin, L.cout, in, n->wt16 + L.w16_off, n->params + L.b_off, skip, out, relu};
+        const int BN = std::min(L.cout_pad, 64), TH = H % 16 == 0 ? 16 : 8;
+        const dim3 grid((unsigned)(B * (H / TH) * (W / 16)), (unsigned)(L.cout_pad / BN));
+        if (TH == 16) {
+            if (BN == 16) conv_fwd_tile_kernel<16, 16><<<grid, 512, 0, st>>>(c);
+            else if (BN == 32) conv_fwd_tile_kernel<32, 16><<<grid, 512, 0, st>>>(c);
+            else conv_fwd_tile_kernel<64, 16><<<grid, 512, 0, st>>>(c);
+        } else {
+            if (BN == 16) conv_fwd_tile_kernel<16, 8><<<grid, 256, 0, st>>>(c);
+            else if (BN == 32) conv_fwd_tile_kernel<32, 8><<<grid, 256, 0, st>>>(c);
+            else conv_fwd_tile_kernel<64, 8><<<grid, 256, 0, st>>>(c);
+        }
+        SDP_LAUNCH_CHECK();
+        return SDP_OK;
+    }
+    if (n->p.precision == SDP_CNN_BF16 && L.cin < 8 && L.cout % 4 == 0 && (size_t)9 * L.cin * L.cout <= 8192) {
+        ConvArgs c{B, H, W, L.cin, L.cout, in, nullptr, n->params + L.b_off, skip, out, relu};
+        const size_t items = (size_t)B * H * W * (L.cout / 4);
+        conv_fwd_small_cin_kernel<<<ew_grid(items), kT, sizeof(float) * 9 * L.cin * L.cout, st>>>(
+            c, n->params + L.w_off);
+        SDP_LAUNCH_CHECK();
+        return SDP_OK;
+    }
     GemmArgs g = geom(B, H, W, L.cin, L.cout);
     g.M = B * H * W; g.N = L.cout; g.K = 9 * L.cin;
     g.a_src = in; g.b_src = n->params + L.w_off; g.out = out;
@@ -734,6 +940,26 @@ int conv_backward(const sdp_cnn *n, int li, int B, int H, int W, const float *in
     return SDP_OK;
 }
 
+// wt16 from the FP32 parameters when they changed since it was formed
+int refresh_bf16_weights(sdp_cnn *n, hipStream_t st)
+{
+    if (!n->wt16_stale) return SDP_OK;
+    Bf16Layers L{};
+    SDP_REQUIRE(n->layers.size() <= (size_t)kMaxLayers, "too many layers for the BF16 weight table");
+    L.n = (int)n->layers.size();
+    for (int i = 0; i < L.n; ++i) {
+        L.cin[i] = n->layers[i].cin;
+        L.cout[i] = n->layers[i].cout;
+        L.off[i] = (long long)n->layers[i].w_off;
+        L.dst[i] = (long long)n->layers[i].w16_off;
+    }
+    L.off[L.n] = (long long)n->n_params;
+    bf16_weights_kernel<<<ew_grid(n->n_params), kT, 0, st>>>(n->params, L, n->wt16);
+    SDP_LAUNCH_CHECK();
+    n->wt16_stale = 0;
+    return SDP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -755,8 +981,10 @@ int sdp_cnn_create(sdp_cnn **out, const sdp_cnn_params *pp)
     sdp_cnn *n = new sdp_cnn();
     n->p = p;
     auto add = [&](int cin, int cout) {
-        sdp_cnn::Layer L{cin, cout, n->n_params, n->n_params + (size_t)9 * cin * cout};
+        const int cp = cout < 16 ? 16 : cout;
+        sdp_cnn::Layer L{cin, cout, n->n_params, n->n_params + (size_t)9 * cin * cout, cp, n->n_wt16};
         n->n_params += (size_t)9 * cin * cout + cout;
+        n->n_wt16 += (size_t)9 * cin * cp;
         n->layers.push_back(L);
     };
     int prev = p.in_ch;
@@ -777,6 +1005,8 @@ int sdp_cnn_create(sdp_cnn **out, const sdp_cnn_params *pp)
         if (hipMalloc((void **)ptr, sizeof(float) * std::max<size_t>(count, 1)) != hipSuccess) ok = false;
     };
     alloc(&n->params, n->n_params);
+    if (ok && hipMalloc((void **)&n->wt16, sizeof(__bf16) * std::max<size_t>(n->n_wt16, 1)) != hipSuccess) ok = false;
+    if (ok && hipMemset(n->wt16, 0, sizeof(__bf16) * std::max<size_t>(n->n_wt16, 1)) != hipSuccess) ok = false;
     alloc(&n->grads, n->n_params);
     alloc(&n->adam_m, n->n_params);
     alloc(&n->adam_v, n->n_params);
@@ -824,6 +1054,7 @@ int sdp_cnn_destroy(sdp_cnn *n)
     if (!n) return SDP_OK;
     float *bufs[] = {n->params, n->grads, n->adam_m, n->adam_v, n->work};
     for (float *b : bufs) (void)hipFree(b);
+    (void)hipFree(n->wt16);
     for (auto *v : {&n->x, &n->a, &n->e, &n->up, &n->s, &n->d, &n->genc, &n->g1, &n->g2})
         for (float *b : *v) (void)hipFree(b);
     (void)hipFree(n->loss_part);
@@ -844,6 +1075,7 @@ int sdp_cnn_set_params(sdp_cnn *n, const float *h_params)
 {
     SDP_REQUIRE(n && h_params, "sdp_cnn_set_params: null argument");
     SDP_HIP_TRY(hipMemcpy(n->params, h_params, sizeof(float) * n->n_params, hipMemcpyHostToDevice));
+    n->wt16_stale = 1;
     n->adam_t = 0;
     SDP_HIP_TRY(hipMemset(n->adam_m, 0, sizeof(float) * n->n_params));
     SDP_HIP_TRY(hipMemset(n->adam_v, 0, sizeof(float) * n->n_params));
@@ -869,6 +1101,8 @@ int sdp_cnn_forward(sdp_cnn *n, const float *d_in, float *d_out, int B, sdp_stre
     SDP_REQUIRE(n && d_in && d_out, "sdp_cnn_forward: null argument");
     SDP_REQUIRE(B >= 1 && B <= n->p.max_batch, "batch %d outside [1, %d]", B, n->p.max_batch);
     hipStream_t st = sdp::stream_of(s);
+    if (n->p.precision == SDP_CNN_BF16)
+        if (int e = refresh_bf16_weights(n, st)) return e;
     const int L = n->p.levels;
     const float *xin = d_in;
     for (int l = 0; l < L; ++l) {
@@ -1050,6 +1284,7 @@ int sdp_cnn_adam(sdp_cnn *n, float lr, float beta1, float beta2, float eps, sdp_
     adam_kernel<<<ew_grid(n->n_params), kT, 0, sdp::stream_of(s)>>>(n->params, n->grads, n->adam_m, n->adam_v,
                                                                      n->n_params, lr, beta1, beta2, eps, c1, c2);
     SDP_LAUNCH_CHECK();
+    n->wt16_stale = 1;
     return SDP_OK;
 }
 

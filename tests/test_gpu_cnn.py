@@ -85,6 +85,54 @@ def test_bf16_full_size_network_matches_fp32(hip, levels, base):
     assert a @ b / (np.linalg.norm(a) * np.linalg.norm(b)) > 0.999
 
 
+def bf16_round(a):
+    """Round to BF16 (nearest, ties to even), returned as float32."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+    r = (u + np.uint32(0x7FFF) + ((u >> 16) & np.uint32(1))) & np.uint32(0xFFFF0000)
+    return r.view(np.float32)
+
+
+def forward_bf16_operands(params, x, levels):
+    """The oracle's network with every convolution's input and weights
+    rounded to BF16 and activations kept in FP32, as the matrix-core kernels
+    compute it (their BF16 products are exact; their sums FP32)."""
+    f32 = lambda a: np.asarray(a, np.float32)
+    conv = lambda a, name: f32(C.conv3x3(bf16_round(a), bf16_round(params[name + ".w"]),
+                                         np.asarray(params[name + ".b"], np.float64)))
+    xl, enc = f32(x), []
+    for l in range(levels):
+        e = C.relu(conv(C.relu(conv(xl, f"enc{l}a")), f"enc{l}b"))
+        enc.append(e)
+        if l < levels - 1:
+            xl = f32(C.avgpool2(e))
+    d = enc[-1]
+    for l in range(levels - 2, -1, -1):
+        s = C.relu(f32(conv(f32(C.upsample2(d)), f"dec{l}u") + enc[l]))
+        d = C.relu(conv(s, f"dec{l}d"))
+    return conv(d, "head")
+
+
+@pytest.mark.parametrize("levels,base,bar", [(1, 32, 3e-3), (4, 32, 2e-2), (3, 16, 2e-2)])
+def test_bf16_forward_matches_bf16_operand_restatement(hip, levels, base, bar):
+    """The BF16 forward pass (the tiled matrix-core convolutions for layers
+    of 32k input channels, the padded 16-channel tile for the 1-channel head,
+    the vector-ALU first layer) against the oracle with the same BF16
+    rounding of every convolution's operands, at 128^2 (every level's tile
+    fits).  Only summation order separates them, amplified where an
+    activation lands on the other side of a BF16 rounding step in the next
+    layer: the same restatement summed in FP32 instead of FP64 moves by
+    9e-4 of the peak for the 3-layer network and by about 1e-2 for the deep
+    ones, which sets the bars (an indexing error is of order 1)."""
+    net, params, x, _ = setup(hip, levels, base, 128, 1, hip.CNN_BF16, seed=11)
+    d_x, d_y = up(hip, x), hip.DeviceBuffer(4 * 128 * 128)
+    net.forward(d_x, d_y, 1)
+    y = d_y.download((1, 128, 128, 1), np.float32)
+    ref = forward_bf16_operands(params, x, levels)
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    assert err <= bar, err
+    net.close()
+
+
 def test_adam_step_matches_oracle(hip):
     net, params, x, t = setup(hip, 2, 4, 16, 2, hip.CNN_FP32)
     _, _, grads, _ = run(hip, net, x, t)
