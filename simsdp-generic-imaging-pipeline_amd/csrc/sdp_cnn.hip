@@ -330,6 +330,12 @@ __global__ void __launch_bounds__(kT) gemm_fwd_narrow_kernel(GemmArgs g, int cin
 // Cout_pad are zero and their outputs are not stored); the epilogue adds the
 // bias and the skip input and applies the ReLU, as the general kernels do.
 // Needs Cin % 32 == 0, H % TH == 0, W % 16 == 0 and BN | Cout_pad.
+//
+// IN: the input as stored.  IN_DIRECT: H x W.  IN_UPSAMPLE: H/2 x W/2, read
+// through the nearest 2x upsampling (upsample2_vec_kernel's map).
+// IN_POOL: 2H x 2W, read through the 2x2 average (avgpool2_vec_kernel's
+// arithmetic, so the values are the same as the pooled tensor's).
+enum { IN_DIRECT = 0, IN_UPSAMPLE = 1, IN_POOL = 2 };
 constexpr int kHaloW = 18;
 constexpr int kLdA = 40;   // LDS row of one pixel's (or weight column's) 32 BF16 K values, padded
 
@@ -349,7 +355,7 @@ struct ConvArgs {
     int relu;
 };
 
-template <int BN, int TH>
+template <int BN, int TH, int IN>
 __global__ void __launch_bounds__(32 * TH) conv_fwd_tile_kernel(ConvArgs c)
 {
     constexpr int NJ = BN / 16, NT = 32 * TH, HALO = (TH + 2) * kHaloW;
@@ -367,15 +373,31 @@ __global__ void __launch_bounds__(32 * TH) conv_fwd_tile_kernel(ConvArgs c)
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < NJ; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const float *xb = c.x + (size_t)b * c.H * c.W * c.Cin;
+    // the stored input's width and its image's first element
+    const int sw = IN == IN_UPSAMPLE ? c.W / 2 : IN == IN_POOL ? 2 * c.W : c.W;
+    const size_t img = (size_t)c.H * c.W * c.Cin;
+    const float *xb = c.x + (IN == IN_UPSAMPLE ? img / 4 : IN == IN_POOL ? 4 * img : img) * b;
     for (int c0 = 0; c0 < c.Cin; c0 += kBK) {
         // halo: pixel h of the (TH + 2) x 18 tile, channels 4q .. 4q + 3 of the chunk
         for (int i = tid; i < HALO * 8; i += NT) {
             const int h = i >> 3, q = i & 7;
             const int yy = y0 - 1 + h / kHaloW, xx = x0 - 1 + h % kHaloW;
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (yy >= 0 && yy < c.H && xx >= 0 && xx < c.W)
-                v = *reinterpret_cast<const float4 *>(xb + ((size_t)yy * c.W + xx) * c.Cin + c0 + 4 * q);
+            if (yy >= 0 && yy < c.H && xx >= 0 && xx < c.W) {
+                if (IN == IN_DIRECT) {
+                    v = *reinterpret_cast<const float4 *>(xb + ((size_t)yy * sw + xx) * c.Cin + c0 + 4 * q);
+                } else if (IN == IN_UPSAMPLE) {
+                    v = *reinterpret_cast<const float4 *>(xb + ((size_t)(yy >> 1) * sw + (xx >> 1)) * c.Cin + c0 + 4 * q);
+                } else {
+                    const float *p = xb + ((size_t)(2 * yy) * sw + 2 * xx) * c.Cin + c0 + 4 * q;
+                    const float4 a = *reinterpret_cast<const float4 *>(p);
+                    const float4 e = *reinterpret_cast<const float4 *>(p + c.Cin);
+                    const float4 u = *reinterpret_cast<const float4 *>(p + (size_t)sw * c.Cin);
+                    const float4 w = *reinterpret_cast<const float4 *>(p + (size_t)sw * c.Cin + c.Cin);
+                    v = make_float4(0.25f * ((a.x + e.x) + (u.x + w.x)), 0.25f * ((a.y + e.y) + (u.y + w.y)),
+                                    0.25f * ((a.z + e.z) + (u.z + w.z)), 0.25f * ((a.w + e.w) + (u.w + w.w)));
+                }
+            }
             *reinterpret_cast<uint2 *>(&As[h][4 * q]) = make_uint2(pack_bf16x2(v.x, v.y), pack_bf16x2(v.z, v.w));
         }
         // weights: output channel n, tap k, K values 8 p .. 8 p + 7 of the chunk
@@ -424,46 +446,60 @@ __global__ void __launch_bounds__(32 * TH) conv_fwd_tile_kernel(ConvArgs c)
 }
 
 // Forward 3x3 convolution of a network input with few channels (Cin < 8,
-// the first layer) on the vector ALUs: a thread computes four output
-// channels of one pixel.  Operands are rounded to BF16 as the matrix-core
-// path rounds them (their products are then exact in FP32); the sums are
-// FP32, taps then channels in order.
+// the first layer) on the vector ALUs: a thread computes all COUT output
+// channels of one pixel, so each input value is read once.  Operands are
+// rounded to BF16 as the matrix-core path rounds them (their products are
+// then exact in FP32); the sums are FP32, taps then channels in order.
+template <int COUT>
 __global__ void __launch_bounds__(kT) conv_fwd_small_cin_kernel(ConvArgs c, const float *w)
 {
-    extern __shared__ float ws[];   // [9][Cin][Cout], BF16-rounded
-    const int nw = 9 * c.Cin * c.Cout;
+    extern __shared__ float4 ws4[];   // [9][Cin][COUT / 4], BF16-rounded
+    const int nw = 9 * c.Cin * COUT;
+    float *ws = reinterpret_cast<float *>(ws4);
     for (int i = threadIdx.x; i < nw; i += kT) ws[i] = (float)(__bf16)w[i];
     __syncthreads();
-    const int groups = c.Cout >> 2;
-    const long long total = (long long)c.B * c.H * c.W * groups;
-    for (long long i = blockIdx.x * (long long)kT + threadIdx.x; i < total; i += (long long)gridDim.x * kT) {
-        const int g = (int)(i % groups);
-        const long long m = i / groups;
+    const long long total = (long long)c.B * c.H * c.W;
+    for (long long m = blockIdx.x * (long long)kT + threadIdx.x; m < total; m += (long long)gridDim.x * kT) {
         const int x = (int)(m % c.W), y = (int)((m / c.W) % c.H);
         const long long bh = m / c.W - y;   // b * H
-        const float4 a = c.bias ? make_float4(c.bias[4 * g], c.bias[4 * g + 1], c.bias[4 * g + 2], c.bias[4 * g + 3])
-                                : make_float4(0.f, 0.f, 0.f, 0.f);
-        float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 acc[COUT / 4];
+#pragma unroll
+        for (int g = 0; g < COUT / 4; ++g) acc[g] = make_float4(0.f, 0.f, 0.f, 0.f);
         for (int k = 0; k < 9; ++k) {
             const int yy = y + k / 3 - 1, xx = x + k % 3 - 1;
             if (yy < 0 || yy >= c.H || xx < 0 || xx >= c.W) continue;
             const float *px = c.x + ((bh + yy) * c.W + xx) * c.Cin;
             for (int ci = 0; ci < c.Cin; ++ci) {
                 const float v = (float)(__bf16)px[ci];
-                const float4 wk = *reinterpret_cast<const float4 *>(&ws[(k * c.Cin + ci) * c.Cout + 4 * g]);
-                s.x = fmaf(v, wk.x, s.x);
-                s.y = fmaf(v, wk.y, s.y);
-                s.z = fmaf(v, wk.z, s.z);
-                s.w = fmaf(v, wk.w, s.w);
+                const float4 *wk = &ws4[(k * c.Cin + ci) * (COUT / 4)];
+#pragma unroll
+                for (int g = 0; g < COUT / 4; ++g) {
+                    const float4 q = wk[g];
+                    acc[g].x = fmaf(v, q.x, acc[g].x);
+                    acc[g].y = fmaf(v, q.y, acc[g].y);
+                    acc[g].z = fmaf(v, q.z, acc[g].z);
+                    acc[g].w = fmaf(v, q.w, acc[g].w);
+                }
             }
         }
-        float4 v = make_float4(s.x + a.x, s.y + a.y, s.z + a.z, s.w + a.w);
-        if (c.skip) {
-            const float4 k4 = *reinterpret_cast<const float4 *>(c.skip + m * c.Cout + 4 * g);
-            v = make_float4(v.x + k4.x, v.y + k4.y, v.z + k4.z, v.w + k4.w);
+        float4 *o = reinterpret_cast<float4 *>(c.out + m * COUT);
+        const float4 *sk = c.skip ? reinterpret_cast<const float4 *>(c.skip + m * COUT) : nullptr;
+#pragma unroll
+        for (int g = 0; g < COUT / 4; ++g) {
+            float4 v = acc[g];
+            if (c.bias) {
+                v.x += c.bias[4 * g];
+                v.y += c.bias[4 * g + 1];
+                v.z += c.bias[4 * g + 2];
+                v.w += c.bias[4 * g + 3];
+            }
+            if (sk) {
+                const float4 k4 = sk[g];
+                v = make_float4(v.x + k4.x, v.y + k4.y, v.z + k4.z, v.w + k4.w);
+            }
+            if (c.relu) v = make_float4(fmaxf(v.x, 0.f), fmaxf(v.y, 0.f), fmaxf(v.z, 0.f), fmaxf(v.w, 0.f));
+            o[g] = v;
         }
-        if (c.relu) v = make_float4(fmaxf(v.x, 0.f), fmaxf(v.y, 0.f), fmaxf(v.z, 0.f), fmaxf(v.w, 0.f));
-        *reinterpret_cast<float4 *>(c.out + m * c.Cout + 4 * g) = v;
     }
 }
 
@@ -806,6 +842,8 @@ struct sdp_cnn {
     size_t work_floats = 0;
     double *loss_part = nullptr;
     int batch = 0;                // batch of the last forward
+    int pooled_mem[8] = {};       // the last forward formed x[l] in memory
+    int upsampled_mem[8] = {};    // ... and up[l]
     const float *last_in = nullptr;
 
     int ch(int l) const { return p.base << l; }
@@ -871,33 +909,52 @@ GemmArgs geom(int B, int H, int W, int cin, int cout)
     return g;
 }
 
-// out = act(conv(in) + bias (+ skip))
-int conv_forward(const sdp_cnn *n, int li, int B, int H, int W, const float *in, float *out, const float *skip,
-                 int relu, hipStream_t st)
+// whether layer li at H x W takes conv_fwd_tile_kernel (which can also read
+// its input through the 2x upsampling or the 2x2 pooling)
+bool tile_path(const sdp_cnn *n, int li, int H, int W)
 {
     const auto &L = n->layers[li];
-    if (n->p.precision == SDP_CNN_BF16 && L.cin % kBK == 0 &&
-        (L.cout_pad == 16 || L.cout_pad == 32 || L.cout_pad % 64 == 0) && H % 8 == 0 && W % 16 == 0) {
+    return n->p.precision == SDP_CNN_BF16 && L.cin % kBK == 0 &&
+           (L.cout_pad == 16 || L.cout_pad == 32 || L.cout_pad % 64 == 0) && H % 8 == 0 && W % 16 == 0;
+}
+
+template <int IN>
+void launch_tile(const ConvArgs &c, int BN, int TH, dim3 grid, hipStream_t st)
+{
+    if (TH == 16) {
+        if (BN == 16) conv_fwd_tile_kernel<16, 16, IN><<<grid, 512, 0, st>>>(c);
+        else if (BN == 32) conv_fwd_tile_kernel<32, 16, IN><<<grid, 512, 0, st>>>(c);
+        else conv_fwd_tile_kernel<64, 16, IN><<<grid, 512, 0, st>>>(c);
+    } else {
+        if (BN == 16) conv_fwd_tile_kernel<16, 8, IN><<<grid, 256, 0, st>>>(c);
+        else if (BN == 32) conv_fwd_tile_kernel<32, 8, IN><<<grid, 256, 0, st>>>(c);
+        else conv_fwd_tile_kernel<64, 8, IN><<<grid, 256, 0, st>>>(c);
+    }
+}
+
+// out = act(conv(in) + bias (+ skip)) at H x W; `in_mode` (tile path only):
+// in is H x W (IN_DIRECT), H/2 x W/2 read upsampled, or 2H x 2W read pooled
+int conv_forward(const sdp_cnn *n, int li, int B, int H, int W, const float *in, float *out, const float *skip,
+                 int relu, hipStream_t st, int in_mode = IN_DIRECT)
+{
+    const auto &L = n->layers[li];
+    if (tile_path(n, li, H, W)) {
         ConvArgs c{B, H, W, L.cin, L.cout, in, n->wt16 + L.w16_off, n->params + L.b_off, skip, out, relu};
         const int BN = std::min(L.cout_pad, 64), TH = H % 16 == 0 ? 16 : 8;
         const dim3 grid((unsigned)(B * (H / TH) * (W / 16)), (unsigned)(L.cout_pad / BN));
-        if (TH == 16) {
-            if (BN == 16) conv_fwd_tile_kernel<16, 16><<<grid, 512, 0, st>>>(c);
-            else if (BN == 32) conv_fwd_tile_kernel<32, 16><<<grid, 512, 0, st>>>(c);
-            else conv_fwd_tile_kernel<64, 16><<<grid, 512, 0, st>>>(c);
-        } else {
-            if (BN == 16) conv_fwd_tile_kernel<16, 8><<<grid, 256, 0, st>>>(c);
-            else if (BN == 32) conv_fwd_tile_kernel<32, 8><<<grid, 256, 0, st>>>(c);
-            else conv_fwd_tile_kernel<64, 8><<<grid, 256, 0, st>>>(c);
-        }
+        if (in_mode == IN_UPSAMPLE) launch_tile<IN_UPSAMPLE>(c, BN, TH, grid, st);
+        else if (in_mode == IN_POOL) launch_tile<IN_POOL>(c, BN, TH, grid, st);
+        else launch_tile<IN_DIRECT>(c, BN, TH, grid, st);
         SDP_LAUNCH_CHECK();
         return SDP_OK;
     }
-    if (n->p.precision == SDP_CNN_BF16 && L.cin < 8 && L.cout % 4 == 0 && (size_t)9 * L.cin * L.cout <= 8192) {
+    SDP_REQUIRE(in_mode == IN_DIRECT, "conv_forward: a resampled input needs the tiled kernel");
+    if (n->p.precision == SDP_CNN_BF16 && L.cin < 8 && (L.cout == 16 || L.cout == 32 || L.cout == 64)) {
         ConvArgs c{B, H, W, L.cin, L.cout, in, nullptr, n->params + L.b_off, skip, out, relu};
-        const size_t items = (size_t)B * H * W * (L.cout / 4);
-        conv_fwd_small_cin_kernel<<<ew_grid(items), kT, sizeof(float) * 9 * L.cin * L.cout, st>>>(
-            c, n->params + L.w_off);
+        const size_t items = (size_t)B * H * W, lds = sizeof(float) * 9 * L.cin * L.cout;
+        if (L.cout == 16) conv_fwd_small_cin_kernel<16><<<ew_grid(items), kT, lds, st>>>(c, n->params + L.w_off);
+        else if (L.cout == 32) conv_fwd_small_cin_kernel<32><<<ew_grid(items), kT, lds, st>>>(c, n->params + L.w_off);
+        else conv_fwd_small_cin_kernel<64><<<ew_grid(items), kT, lds, st>>>(c, n->params + L.w_off);
         SDP_LAUNCH_CHECK();
         return SDP_OK;
     }
@@ -906,6 +963,33 @@ int conv_forward(const sdp_cnn *n, int li, int B, int H, int W, const float *in,
     g.a_src = in; g.b_src = n->params + L.w_off; g.out = out;
     g.bias = n->params + L.b_off; g.skip = skip; g.relu = relu;
     return run_gemm(n, G_FWD, g, st);
+}
+
+// x[l + 1] = avgpool2(e[l]) and up[l] = upsample2(d[l + 1]) (e at the top)
+int pool_level(const sdp_cnn *n, int l, int B, hipStream_t st)
+{
+    const int H = n->hl(l), W = n->wl(l);
+    const size_t np = (size_t)B * (H / 2) * (W / 2) * n->ch(l);
+    if (n->ch(l) % 4 == 0 && (size_t)B * H * W * n->ch(l) / 4 < (1u << 31))
+        avgpool2_vec_kernel<<<ew_grid(np / 4), kT, 0, st>>>((const float4 *)n->e[l], B, H, W, n->ch(l) / 4,
+                                                           (float4 *)n->x[l + 1]);
+    else
+        avgpool2_kernel<<<ew_grid(np), kT, 0, st>>>(n->e[l], B, H, W, n->ch(l), n->x[l + 1]);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
+}
+
+int upsample_level(const sdp_cnn *n, int l, int B, const float *low, hipStream_t st)
+{
+    const int H = n->hl(l), W = n->wl(l);
+    const size_t nu = (size_t)B * H * W * n->ch(l + 1);
+    if (n->ch(l + 1) % 4 == 0 && nu / 4 < (1u << 31))
+        upsample2_vec_kernel<<<ew_grid(nu / 4), kT, 0, st>>>((const float4 *)low, B, H / 2, W / 2, n->ch(l + 1) / 4,
+                                                            (float4 *)n->up[l]);
+    else
+        upsample2_kernel<<<ew_grid(nu), kT, 0, st>>>(low, B, H / 2, W / 2, n->ch(l + 1), n->up[l]);
+    SDP_LAUNCH_CHECK();
+    return SDP_OK;
 }
 
 // gin = conv backward data of gout (masked by mask > 0 when mask is given);
@@ -1104,33 +1188,37 @@ int sdp_cnn_forward(sdp_cnn *n, const float *d_in, float *d_out, int B, sdp_stre
     if (n->p.precision == SDP_CNN_BF16)
         if (int e = refresh_bf16_weights(n, st)) return e;
     const int L = n->p.levels;
+    // A level's pooled input and a decoder's upsampled input are read
+    // through the resampling by the tiled kernel when it takes the layer;
+    // they are formed in memory otherwise, and for the backward pass when it
+    // needs them (pooled_mem / upsampled_mem say which are formed).
     const float *xin = d_in;
     for (int l = 0; l < L; ++l) {
         const int H = n->hl(l), W = n->wl(l);
-        if (int e = conv_forward(n, enc_a(l), B, H, W, xin, n->a[l], nullptr, 1, st)) return e;
-        if (int e = conv_forward(n, enc_b(l), B, H, W, n->a[l], n->e[l], nullptr, 1, st)) return e;
-        if (l < L - 1) {
-            const size_t np = (size_t)B * (H / 2) * (W / 2) * n->ch(l);
-            if (n->ch(l) % 4 == 0 && (size_t)B * H * W * n->ch(l) / 4 < (1u << 31))
-                avgpool2_vec_kernel<<<ew_grid(np / 4), kT, 0, st>>>((const float4 *)n->e[l], B, H, W, n->ch(l) / 4,
-                                                                   (float4 *)n->x[l + 1]);
-            else
-                avgpool2_kernel<<<ew_grid(np), kT, 0, st>>>(n->e[l], B, H, W, n->ch(l), n->x[l + 1]);
-            SDP_LAUNCH_CHECK();
-            xin = n->x[l + 1];
+        int mode = IN_DIRECT;
+        if (l > 0) {
+            n->pooled_mem[l] = !tile_path(n, enc_a(l), H, W);
+            if (n->pooled_mem[l]) {
+                if (int e = pool_level(n, l - 1, B, st)) return e;
+                xin = n->x[l];
+            } else {
+                xin = n->e[l - 1];
+                mode = IN_POOL;
+            }
         }
+        if (int e = conv_forward(n, enc_a(l), B, H, W, xin, n->a[l], nullptr, 1, st, mode)) return e;
+        if (int e = conv_forward(n, enc_b(l), B, H, W, n->a[l], n->e[l], nullptr, 1, st)) return e;
     }
     const float *dcur = n->e[L - 1];
     for (int l = L - 2; l >= 0; --l) {
         const int H = n->hl(l), W = n->wl(l);
-        const size_t nu = (size_t)B * H * W * n->ch(l + 1);
-        if (n->ch(l + 1) % 4 == 0 && nu / 4 < (1u << 31))
-            upsample2_vec_kernel<<<ew_grid(nu / 4), kT, 0, st>>>((const float4 *)dcur, B, H / 2, W / 2,
-                                                                n->ch(l + 1) / 4, (float4 *)n->up[l]);
-        else
-            upsample2_kernel<<<ew_grid(nu), kT, 0, st>>>(dcur, B, H / 2, W / 2, n->ch(l + 1), n->up[l]);
-        SDP_LAUNCH_CHECK();
-        if (int e = conv_forward(n, dec_u(n, l), B, H, W, n->up[l], n->s[l], n->e[l], 1, st)) return e;
+        n->upsampled_mem[l] = !tile_path(n, dec_u(n, l), H, W);
+        if (n->upsampled_mem[l]) {
+            if (int e = upsample_level(n, l, B, dcur, st)) return e;
+            if (int e = conv_forward(n, dec_u(n, l), B, H, W, n->up[l], n->s[l], n->e[l], 1, st)) return e;
+        } else {
+            if (int e = conv_forward(n, dec_u(n, l), B, H, W, dcur, n->s[l], n->e[l], 1, st, IN_UPSAMPLE)) return e;
+        }
         if (int e = conv_forward(n, dec_d(n, l), B, H, W, n->s[l], n->d[l], nullptr, 1, st)) return e;
         dcur = n->d[l];
     }
@@ -1155,7 +1243,9 @@ int sdp_cnn_backward(sdp_cnn *n, const float *d_grad_out, sdp_stream s)
         const int H = n->hl(l), W = n->wl(l);
         // d_l = relu(conv(s_l)): gradient into s_l, masked by s_l > 0 (s_l = relu(u_l + e_l))
         if (int e = conv_backward(n, dec_d(n, l), B, H, W, n->s[l], gcur, n->genc[l], n->s[l], st)) return e;
-        // u_l = conv(up_l): gradient into up_l
+        // u_l = conv(up_l): gradient into up_l (up_l formed now if the forward read it upsampled)
+        if (!n->upsampled_mem[l])
+            if (int e = upsample_level(n, l, B, l + 1 <= L - 2 ? n->d[l + 1] : n->e[L - 1], st)) return e;
         if (int e = conv_backward(n, dec_u(n, l), B, H, W, n->up[l], n->genc[l], n->g2[l], nullptr, st)) return e;
         // up_l = upsample2(d_{l+1}); d_{l+1} is the next decoder output or, at the top, e_top
         const float *mask = (l + 1 <= L - 2) ? n->d[l + 1] : n->e[L - 1];
@@ -1178,7 +1268,10 @@ int sdp_cnn_backward(sdp_cnn *n, const float *d_grad_out, sdp_stream s)
         }
         // e_l = relu(conv(a_l)): gradient into a_l, masked by a_l > 0
         if (int e = conv_backward(n, enc_b(l), B, H, W, n->a[l], n->g1[l], n->g2[l], n->a[l], st)) return e;
-        // a_l = relu(conv(x_l)): gradient into x_l (not needed at level 0)
+        // a_l = relu(conv(x_l)): gradient into x_l (not needed at level 0); x_l
+        // formed now if the forward read it pooled
+        if (l > 0 && !n->pooled_mem[l])
+            if (int e = pool_level(n, l - 1, B, st)) return e;
         const float *xl = l == 0 ? n->last_in : n->x[l];
         if (int e = conv_backward(n, enc_a(l), B, H, W, xl, n->g2[l], l == 0 ? nullptr : n->g1[l], nullptr, st))
             return e;
