@@ -109,8 +109,10 @@ typedef struct sdp_grid_params {
     int snap_mode;        /* SDP_SNAP_* */
     double uv_scale;      /* cell_size * G */
     double w_scale;       /* (n_kernels - 1)^2 / max_w */
-    int tile;             /* uv tile edge in cells, 0 = default */
-    int chunk;            /* taps per work item, 0 = default */
+    int tile;             /* uv tile edge in cells; 0: 8, or the largest power of two
+                             within 2h + 1 of the smallest support when narrower */
+    int chunk;            /* taps per work item; 0: about 8 work items per CU, within
+                             4096 and 131072 taps */
     int accumulate;       /* SDP_ACC_*: how a work item sums into its sub-grid */
 } sdp_grid_params;
 /* Grid-cell accumulation.  Every cell's value is a TwoSum-compensated sum of

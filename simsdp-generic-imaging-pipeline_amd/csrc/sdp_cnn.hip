@@ -791,24 +791,35 @@ __global__ void scatter_add_kernel(int G, int T, int margin, int n, const int2 *
     }
 }
 
-// largest |residual| over each candidate patch (the whole T x T tile)
-__global__ void patch_peak_kernel(int G, int T, int margin, int stride, int npx, const float *res, float *peak)
+// largest |residual| over each candidate patch (the whole T x T tile): a
+// 1024-thread workgroup per patch, thread (r, c) taking columns c, c + 256,
+// ... of rows r, r + 4, ... (row-contiguous reads, no divisions)
+constexpr int kPeakThreads = 1024;
+__global__ void __launch_bounds__(kPeakThreads)
+patch_peak_kernel(int G, int T, int margin, int stride, int npx, const float *res, float *peak)
 {
-    __shared__ float red[kT];
+    __shared__ float red[kPeakThreads / 64];
     const int p = blockIdx.x;
     const int x0 = (p % npx) * stride - margin, y0 = (p / npx) * stride - margin;
+    const int c = threadIdx.x & 255, r = threadIdx.x >> 8;
     float m = 0.f;
-    for (int i = threadIdx.x; i < T * T; i += kT) {
-        const int gy = y0 + i / T, gx = x0 + i % T;
-        if (gy >= 0 && gy < G && gx >= 0 && gx < G) m = fmaxf(m, fabsf(res[(size_t)gy * G + gx]));
+    for (int y = r; y < T; y += kPeakThreads / 256) {
+        const int gy = y0 + y;
+        if (gy < 0 || gy >= G) continue;
+        const float *row = res + (size_t)gy * G;
+        for (int x = c; x < T; x += 256) {
+            const int gx = x0 + x;
+            if (gx >= 0 && gx < G) m = fmaxf(m, fabsf(row[gx]));
+        }
     }
-    red[threadIdx.x] = m;
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
-    for (int off = kT / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + off]);
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        float v = red[0];
+        for (int w = 1; w < kPeakThreads / 64; ++w) v = fmaxf(v, red[w]);
+        peak[p] = v;
     }
-    if (threadIdx.x == 0) peak[p] = red[0];
 }
 
 unsigned int ew_grid(size_t n)
@@ -1363,7 +1374,7 @@ int sdp_cnn_patch_peaks(int G, int T, int margin, const float *d_residual, float
     const int npx = (G + stride - 1) / stride;
     *n_patches = npx * npx;
     if (!d_peaks) return SDP_OK;
-    patch_peak_kernel<<<npx * npx, kT, 0, sdp::stream_of(s)>>>(G, T, margin, stride, npx, d_residual, d_peaks);
+    patch_peak_kernel<<<npx * npx, kPeakThreads, 0, sdp::stream_of(s)>>>(G, T, margin, stride, npx, d_residual, d_peaks);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }

@@ -208,19 +208,35 @@ __global__ void image_stats_kernel(const float *img, size_t n, double *part)
     }
 }
 
-// combines the per-block partials in block order: out = {sum_sq, max, max_abs}
-__global__ void image_stats_final_kernel(const double *part, int nparts, double *out)
+// combines the per-block partials: out = {sum_sq, max, max_abs}.  One
+// workgroup of 256 threads: thread t sums partials t, t + 256, ... in that
+// order, then a fixed tree over the threads (the same order every call)
+__global__ void __launch_bounds__(256) image_stats_final_kernel(const double *part, int nparts, double *out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double s_sq[256], s_mx[256], s_amx[256];
     double sq = 0.0, mx = -INFINITY, amx = 0.0;
-    for (int b = 0; b < nparts; ++b) {
+    for (int b = threadIdx.x; b < nparts; b += 256) {
         sq += part[3 * b];
         mx = fmax(mx, part[3 * b + 1]);
         amx = fmax(amx, part[3 * b + 2]);
     }
-    out[0] = sq;
-    out[1] = mx;
-    out[2] = amx;
+    s_sq[threadIdx.x] = sq;
+    s_mx[threadIdx.x] = mx;
+    s_amx[threadIdx.x] = amx;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            s_sq[threadIdx.x] += s_sq[threadIdx.x + off];
+            s_mx[threadIdx.x] = fmax(s_mx[threadIdx.x], s_mx[threadIdx.x + off]);
+            s_amx[threadIdx.x] = fmax(s_amx[threadIdx.x], s_amx[threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s_sq[0];
+        out[1] = s_mx[0];
+        out[2] = s_amx[0];
+    }
 }
 
 constexpr unsigned int kStatsBlocks = 1024;
@@ -373,7 +389,7 @@ int sdp_image_stats_async(size_t n, const float *d_img, double *d_stats, void *d
     const unsigned int blocks = std::min<unsigned int>(kStatsBlocks, sdp_div_up(n, 256));
     image_stats_kernel<<<blocks, 256, 0, st>>>(d_img, n, (double *)d_work);
     SDP_LAUNCH_CHECK();
-    image_stats_final_kernel<<<1, 64, 0, st>>>((const double *)d_work, (int)blocks, d_stats);
+    image_stats_final_kernel<<<1, 256, 0, st>>>((const double *)d_work, (int)blocks, d_stats);
     SDP_LAUNCH_CHECK();
     return SDP_OK;
 }

@@ -2241,8 +2241,15 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
     plan->n = n_vis;
     plan->n_grids = n_grids;
     plan->H = H;
-    plan->tile = std::min(p.tile > 0 ? p.tile : 8, p.grid_size);
-    plan->chunk_taps = p.chunk > 0 ? p.chunk : (1 << 17);
+    // Default tile: 8, or the largest power of two within 2h + 1 of the
+    // smallest support when that is narrower, so that every plane's
+    // footprint spans the tile and can take the exact-footprint classes.
+    int tile_default = 8;
+    if (p.snap_mode == SDP_SNAP_STD)
+        for (int k = 0; k < p.n_kernels; ++k)
+            while (supports[2 * k] >= 1 && tile_default > 2 * supports[2 * k] + 1) tile_default >>= 1;
+    plan->tile = std::min(p.tile > 0 ? p.tile : tile_default, p.grid_size);
+    plan->chunk_taps = p.chunk > 0 ? p.chunk : (1 << 17);   // the default is settled once the taps are counted
     plan->ntx = (p.grid_size + plan->tile - 1) / plan->tile;
     plan->nty = plan->ntx;
     // Exact-footprint classes (register gridder, std_gridding footprints
@@ -2485,6 +2492,20 @@ int sdp_grid_plan_create_multi(sdp_grid_plan **out, const sdp_grid_params *pp, c
     {
         unsigned long long all_taps = 0;
         for (const KeyRun &r : runs) all_taps += r.taps;
+        if (p.chunk <= 0) {
+            // default work-item size: about eight work items per CU, within
+            // 4096 and 131072 taps
+            static int cus = 0;
+            if (cus == 0) {
+                int dev = 0, v = 0;
+                cus = (hipGetDevice(&dev) == hipSuccess &&
+                       hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+                          ? v
+                          : 256;
+            }
+            plan->chunk_taps =
+                (int)std::min<unsigned long long>(1ull << 17, std::max<unsigned long long>(4096, all_taps / (8ull * cus)));
+        }
         max_ch = runs.size() + (size_t)(all_taps / (unsigned long long)plan->chunk_taps) + 1;
         PLAN_TRY(cache.hget(HOST_CHUNKS, sizeof(int4) * max_ch, (void **)&chunks));
         PLAN_TRY(cache.hget(HOST_SLAB_OFF, sizeof(long long) * max_ch, (void **)&slab_off));
