@@ -133,6 +133,26 @@ def test_bf16_forward_matches_bf16_operand_restatement(hip, levels, base, bar):
     net.close()
 
 
+def test_bf16_forward_non_square_patch(hip):
+    """A 24 x 32 patch: the full-resolution layers take the tiled kernel with
+    8-row tiles (24 is not a multiple of 16), the half-resolution level (12 x 16)
+    the general kernels; against the BF16-operand restatement."""
+    levels, base, H, W = 2, 32, 24, 32
+    rng = np.random.default_rng(21)
+    net = hip.Cnn(levels, base, H, W, 2, precision=hip.CNN_BF16)
+    params = C.init_params(levels, base, seed=21)
+    params = {k: (v + 0.05 * rng.standard_normal(v.shape)).astype(np.float32) for k, v in params.items()}
+    net.set_params(params)
+    x = rng.standard_normal((2, H, W, 2)).astype(np.float32)
+    d_x, d_y = up(hip, x), hip.DeviceBuffer(4 * 2 * H * W)
+    net.forward(d_x, d_y, 2)
+    y = d_y.download((2, H, W, 1), np.float32)
+    ref = forward_bf16_operands(params, x, levels)
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    assert err <= 1e-2, err
+    net.close()
+
+
 def test_adam_step_matches_oracle(hip):
     net, params, x, t = setup(hip, 2, 4, 16, 2, hip.CNN_FP32)
     _, _, grads, _ = run(hip, net, x, t)
