@@ -405,6 +405,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="taps per work item of the plan (with --tile)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (HBM counter calibration)")
+    ap.add_argument("--cnn-only", action="store_true",
+                    help="only run the 4-level, 32-channel network's BF16 forward pass on 16 patches of 256^2 "
+                         "10 times (kernel traces of the learned path's convolutions)")
     ap.add_argument("--fft-only", action="store_true",
                     help="only run the major cycle's 4096^2 inverse FFT (shift / real / taper) 5 times "
                          "(counter runs on the FFT kernels)")
@@ -425,6 +428,19 @@ def main():
             hip.check(hip.load().sdp_device_copy(b.ptr, a.ptr, n, None))
         hip.synchronize()
         log(f"copied {n} bytes x 5")
+        return
+    if args.cnn_only:
+        import numpy as np
+        import sdp_learned as L
+        net = hip.Cnn(4, 32, 256, 256, 16, precision=hip.CNN_BF16)
+        L.init_params(net, 1)
+        x = np.random.default_rng(0).standard_normal((16, 256, 256, 2)).astype(np.float32)
+        d_x, d_y = hip.DeviceBuffer.from_array(x), hip.DeviceBuffer(4 * 16 * 256 * 256)
+        for _ in range(10):
+            net.forward(d_x, d_y, 16)
+        hip.synchronize()
+        net.close()
+        log("4x32 forward x 10 on 16 x 256^2")
         return
     if args.fft_only:
         G = 4096

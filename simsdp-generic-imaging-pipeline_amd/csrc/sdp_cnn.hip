@@ -349,11 +349,20 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b)
 struct ConvArgs {
     int B, H, W, Cin, Cout;
     const float *x;          // NHWC input
-    const __bf16 *wt;        // the layer's weights, [Cout_pad][9][Cin]
+    const __bf16 *wt;        // the layer's weights, [Cout_pad][9][Cin_pad] (Cin_pad: Cin rounded up to 32)
     const float *bias, *skip;
     float *out;
     int relu;
+    int cinw;                // Cin_pad
 };
+
+// four channels ch .. ch + 3 of a pixel's Cin (zero beyond Cin)
+__device__ __forceinline__ float4 load_ch4(const float *px, int ch, int cin)
+{
+    if ((cin & 3) == 0) return ch < cin ? *reinterpret_cast<const float4 *>(px + ch) : make_float4(0.f, 0.f, 0.f, 0.f);
+    return make_float4(ch < cin ? px[ch] : 0.f, ch + 1 < cin ? px[ch + 1] : 0.f, ch + 2 < cin ? px[ch + 2] : 0.f,
+                       ch + 3 < cin ? px[ch + 3] : 0.f);
+}
 
 template <int BN, int TH, int IN>
 __global__ void __launch_bounds__(32 * TH) conv_fwd_tile_kernel(ConvArgs c)
@@ -377,23 +386,23 @@ __global__ void __launch_bounds__(32 * TH) conv_fwd_tile_kernel(ConvArgs c)
     const int sw = IN == IN_UPSAMPLE ? c.W / 2 : IN == IN_POOL ? 2 * c.W : c.W;
     const size_t img = (size_t)c.H * c.W * c.Cin;
     const float *xb = c.x + (IN == IN_UPSAMPLE ? img / 4 : IN == IN_POOL ? 4 * img : img) * b;
-    for (int c0 = 0; c0 < c.Cin; c0 += kBK) {
+    for (int c0 = 0; c0 < c.cinw; c0 += kBK) {
         // halo: pixel h of the (TH + 2) x 18 tile, channels 4q .. 4q + 3 of the chunk
         for (int i = tid; i < HALO * 8; i += NT) {
-            const int h = i >> 3, q = i & 7;
+            const int h = i >> 3, q = i & 7, ch = c0 + 4 * q;
             const int yy = y0 - 1 + h / kHaloW, xx = x0 - 1 + h % kHaloW;
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (yy >= 0 && yy < c.H && xx >= 0 && xx < c.W) {
+            if (yy >= 0 && yy < c.H && xx >= 0 && xx < c.W && ch < c.Cin) {
                 if (IN == IN_DIRECT) {
-                    v = *reinterpret_cast<const float4 *>(xb + ((size_t)yy * sw + xx) * c.Cin + c0 + 4 * q);
+                    v = load_ch4(xb + ((size_t)yy * sw + xx) * c.Cin, ch, c.Cin);
                 } else if (IN == IN_UPSAMPLE) {
-                    v = *reinterpret_cast<const float4 *>(xb + ((size_t)(yy >> 1) * sw + (xx >> 1)) * c.Cin + c0 + 4 * q);
+                    v = load_ch4(xb + ((size_t)(yy >> 1) * sw + (xx >> 1)) * c.Cin, ch, c.Cin);
                 } else {
-                    const float *p = xb + ((size_t)(2 * yy) * sw + 2 * xx) * c.Cin + c0 + 4 * q;
-                    const float4 a = *reinterpret_cast<const float4 *>(p);
-                    const float4 e = *reinterpret_cast<const float4 *>(p + c.Cin);
-                    const float4 u = *reinterpret_cast<const float4 *>(p + (size_t)sw * c.Cin);
-                    const float4 w = *reinterpret_cast<const float4 *>(p + (size_t)sw * c.Cin + c.Cin);
+                    const float *p = xb + ((size_t)(2 * yy) * sw + 2 * xx) * c.Cin;
+                    const float4 a = load_ch4(p, ch, c.Cin);
+                    const float4 e = load_ch4(p + c.Cin, ch, c.Cin);
+                    const float4 u = load_ch4(p + (size_t)sw * c.Cin, ch, c.Cin);
+                    const float4 w = load_ch4(p + (size_t)sw * c.Cin + c.Cin, ch, c.Cin);
                     v = make_float4(0.25f * ((a.x + e.x) + (u.x + w.x)), 0.25f * ((a.y + e.y) + (u.y + w.y)),
                                     0.25f * ((a.z + e.z) + (u.z + w.z)), 0.25f * ((a.w + e.w) + (u.w + w.w)));
                 }
@@ -403,7 +412,7 @@ __global__ void __launch_bounds__(32 * TH) conv_fwd_tile_kernel(ConvArgs c)
         // weights: output channel n, tap k, K values 8 p .. 8 p + 7 of the chunk
         for (int i = tid; i < 9 * BN * 4; i += NT) {
             const int p4 = i & 3, nk = i >> 2, n = nk % BN, k = nk / BN;
-            const uint4 w = *reinterpret_cast<const uint4 *>(c.wt + ((size_t)(n0 + n) * 9 + k) * c.Cin + c0 + 8 * p4);
+            const uint4 w = *reinterpret_cast<const uint4 *>(c.wt + ((size_t)(n0 + n) * 9 + k) * c.cinw + c0 + 8 * p4);
             *reinterpret_cast<uint4 *>(&Bs[k][n][8 * p4]) = w;
         }
         __syncthreads();
@@ -445,71 +454,13 @@ __global__ void __launch_bounds__(32 * TH) conv_fwd_tile_kernel(ConvArgs c)
     }
 }
 
-// Forward 3x3 convolution of a network input with few channels (Cin < 8,
-// the first layer) on the vector ALUs: a thread computes all COUT output
-// channels of one pixel, so each input value is read once.  Operands are
-// rounded to BF16 as the matrix-core path rounds them (their products are
-// then exact in FP32); the sums are FP32, taps then channels in order.
-template <int COUT>
-__global__ void __launch_bounds__(kT) conv_fwd_small_cin_kernel(ConvArgs c, const float *w)
-{
-    extern __shared__ float4 ws4[];   // [9][Cin][COUT / 4], BF16-rounded
-    const int nw = 9 * c.Cin * COUT;
-    float *ws = reinterpret_cast<float *>(ws4);
-    for (int i = threadIdx.x; i < nw; i += kT) ws[i] = (float)(__bf16)w[i];
-    __syncthreads();
-    const long long total = (long long)c.B * c.H * c.W;
-    for (long long m = blockIdx.x * (long long)kT + threadIdx.x; m < total; m += (long long)gridDim.x * kT) {
-        const int x = (int)(m % c.W), y = (int)((m / c.W) % c.H);
-        const long long bh = m / c.W - y;   // b * H
-        float4 acc[COUT / 4];
-#pragma unroll
-        for (int g = 0; g < COUT / 4; ++g) acc[g] = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int k = 0; k < 9; ++k) {
-            const int yy = y + k / 3 - 1, xx = x + k % 3 - 1;
-            if (yy < 0 || yy >= c.H || xx < 0 || xx >= c.W) continue;
-            const float *px = c.x + ((bh + yy) * c.W + xx) * c.Cin;
-            for (int ci = 0; ci < c.Cin; ++ci) {
-                const float v = (float)(__bf16)px[ci];
-                const float4 *wk = &ws4[(k * c.Cin + ci) * (COUT / 4)];
-#pragma unroll
-                for (int g = 0; g < COUT / 4; ++g) {
-                    const float4 q = wk[g];
-                    acc[g].x = fmaf(v, q.x, acc[g].x);
-                    acc[g].y = fmaf(v, q.y, acc[g].y);
-                    acc[g].z = fmaf(v, q.z, acc[g].z);
-                    acc[g].w = fmaf(v, q.w, acc[g].w);
-                }
-            }
-        }
-        float4 *o = reinterpret_cast<float4 *>(c.out + m * COUT);
-        const float4 *sk = c.skip ? reinterpret_cast<const float4 *>(c.skip + m * COUT) : nullptr;
-#pragma unroll
-        for (int g = 0; g < COUT / 4; ++g) {
-            float4 v = acc[g];
-            if (c.bias) {
-                v.x += c.bias[4 * g];
-                v.y += c.bias[4 * g + 1];
-                v.z += c.bias[4 * g + 2];
-                v.w += c.bias[4 * g + 3];
-            }
-            if (sk) {
-                const float4 k4 = sk[g];
-                v = make_float4(v.x + k4.x, v.y + k4.y, v.z + k4.z, v.w + k4.w);
-            }
-            if (c.relu) v = make_float4(fmaxf(v.x, 0.f), fmaxf(v.y, 0.f), fmaxf(v.z, 0.f), fmaxf(v.w, 0.f));
-            o[g] = v;
-        }
-    }
-}
-
-// The layers' weights rounded to BF16 in [Cout_pad][9][Cin] order, from the
-// FP32 parameters ([3][3][Cin][Cout] per layer), for conv_fwd_tile_kernel;
-// the rows from Cout to Cout_pad stay as created (zero).
+// The layers' weights rounded to BF16 in [Cout_pad][9][Cin_pad] order, from
+// the FP32 parameters ([3][3][Cin][Cout] per layer), for conv_fwd_tile_kernel
+// (the padding rows and channels stay as created: zero).
 constexpr int kMaxLayers = 32;   // 4 levels - 1 layers at most 6 levels
 struct Bf16Layers {
     int n;
-    int cin[kMaxLayers], cout[kMaxLayers];
+    int cin[kMaxLayers], cout[kMaxLayers], cinw[kMaxLayers];
     long long off[kMaxLayers + 1];   // first weight of each layer in the parameter vector; off[n]: the end
     long long dst[kMaxLayers];       // its first element in the BF16 array
 };
@@ -525,7 +476,7 @@ __global__ void bf16_weights_kernel(const float *params, Bf16Layers L, __bf16 *w
         const int co = (int)(r % L.cout[l]);
         const long long kc = r / L.cout[l];
         const int ci = (int)(kc % L.cin[l]), k = (int)(kc / L.cin[l]);
-        wt[L.dst[l] + ((long long)co * 9 + k) * L.cin[l] + ci] = (__bf16)params[i];
+        wt[L.dst[l] + ((long long)co * 9 + k) * L.cinw[l] + ci] = (__bf16)params[i];
     }
 }
 
@@ -838,12 +789,13 @@ struct sdp_cnn {
         int cin, cout;
         size_t w_off, b_off;   // into the parameter vector
         int cout_pad;          // output channels of its BF16 weights (16 when cout < 16)
+        int cin_pad;           // input channels of its BF16 weights (cin rounded up to 32)
         size_t w16_off;        // into wt16
     };
     std::vector<Layer> layers;   // enc0a, enc0b, ..., dec(L-2)u, dec(L-2)d, ..., dec0u, dec0d, head
     size_t n_params = 0;
     float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
-    __bf16 *wt16 = nullptr;       // BF16 weights in [Cout_pad][9][Cin] order (conv_fwd_tile_kernel)
+    __bf16 *wt16 = nullptr;       // BF16 weights in [Cout_pad][9][Cin_pad] order (conv_fwd_tile_kernel)
     size_t n_wt16 = 0;
     int wt16_stale = 1;           // params changed since wt16 was formed
     int adam_t = 0;
@@ -925,7 +877,7 @@ GemmArgs geom(int B, int H, int W, int cin, int cout)
 bool tile_path(const sdp_cnn *n, int li, int H, int W)
 {
     const auto &L = n->layers[li];
-    return n->p.precision == SDP_CNN_BF16 && L.cin % kBK == 0 &&
+    return n->p.precision == SDP_CNN_BF16 && (L.cin % kBK == 0 || L.cin < kBK) &&
            (L.cout_pad == 16 || L.cout_pad == 32 || L.cout_pad % 64 == 0) && H % 8 == 0 && W % 16 == 0;
 }
 
@@ -950,7 +902,7 @@ int conv_forward(const sdp_cnn *n, int li, int B, int H, int W, const float *in,
 {
     const auto &L = n->layers[li];
     if (tile_path(n, li, H, W)) {
-        ConvArgs c{B, H, W, L.cin, L.cout, in, n->wt16 + L.w16_off, n->params + L.b_off, skip, out, relu};
+        ConvArgs c{B, H, W, L.cin, L.cout, in, n->wt16 + L.w16_off, n->params + L.b_off, skip, out, relu, L.cin_pad};
         const int BN = std::min(L.cout_pad, 64), TH = H % 16 == 0 ? 16 : 8;
         const dim3 grid((unsigned)(B * (H / TH) * (W / 16)), (unsigned)(L.cout_pad / BN));
         if (in_mode == IN_UPSAMPLE) launch_tile<IN_UPSAMPLE>(c, BN, TH, grid, st);
@@ -960,15 +912,6 @@ int conv_forward(const sdp_cnn *n, int li, int B, int H, int W, const float *in,
         return SDP_OK;
     }
     SDP_REQUIRE(in_mode == IN_DIRECT, "conv_forward: a resampled input needs the tiled kernel");
-    if (n->p.precision == SDP_CNN_BF16 && L.cin < 8 && (L.cout == 16 || L.cout == 32 || L.cout == 64)) {
-        ConvArgs c{B, H, W, L.cin, L.cout, in, nullptr, n->params + L.b_off, skip, out, relu};
-        const size_t items = (size_t)B * H * W, lds = sizeof(float) * 9 * L.cin * L.cout;
-        if (L.cout == 16) conv_fwd_small_cin_kernel<16><<<ew_grid(items), kT, lds, st>>>(c, n->params + L.w_off);
-        else if (L.cout == 32) conv_fwd_small_cin_kernel<32><<<ew_grid(items), kT, lds, st>>>(c, n->params + L.w_off);
-        else conv_fwd_small_cin_kernel<64><<<ew_grid(items), kT, lds, st>>>(c, n->params + L.w_off);
-        SDP_LAUNCH_CHECK();
-        return SDP_OK;
-    }
     GemmArgs g = geom(B, H, W, L.cin, L.cout);
     g.M = B * H * W; g.N = L.cout; g.K = 9 * L.cin;
     g.a_src = in; g.b_src = n->params + L.w_off; g.out = out;
@@ -1045,6 +988,7 @@ int refresh_bf16_weights(sdp_cnn *n, hipStream_t st)
     for (int i = 0; i < L.n; ++i) {
         L.cin[i] = n->layers[i].cin;
         L.cout[i] = n->layers[i].cout;
+        L.cinw[i] = n->layers[i].cin_pad;
         L.off[i] = (long long)n->layers[i].w_off;
         L.dst[i] = (long long)n->layers[i].w16_off;
     }
@@ -1076,10 +1020,10 @@ int sdp_cnn_create(sdp_cnn **out, const sdp_cnn_params *pp)
     sdp_cnn *n = new sdp_cnn();
     n->p = p;
     auto add = [&](int cin, int cout) {
-        const int cp = cout < 16 ? 16 : cout;
-        sdp_cnn::Layer L{cin, cout, n->n_params, n->n_params + (size_t)9 * cin * cout, cp, n->n_wt16};
+        const int cp = cout < 16 ? 16 : cout, cip = (cin + 31) / 32 * 32;
+        sdp_cnn::Layer L{cin, cout, n->n_params, n->n_params + (size_t)9 * cin * cout, cp, cip, n->n_wt16};
         n->n_params += (size_t)9 * cin * cout + cout;
-        n->n_wt16 += (size_t)9 * cin * cp;
+        n->n_wt16 += (size_t)9 * cip * cp;
         n->layers.push_back(L);
     };
     int prev = p.in_ch;

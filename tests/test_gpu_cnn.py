@@ -115,8 +115,9 @@ def forward_bf16_operands(params, x, levels):
 @pytest.mark.parametrize("levels,base,bar", [(1, 32, 3e-3), (4, 32, 2e-2), (3, 16, 2e-2)])
 def test_bf16_forward_matches_bf16_operand_restatement(hip, levels, base, bar):
     """The BF16 forward pass (the tiled matrix-core convolutions for layers
-    of 32k input channels, the padded 16-channel tile for the 1-channel head,
-    the vector-ALU first layer) against the oracle with the same BF16
+    of 32k input channels, the zero-padded input channels of the 2-channel
+    first layer, the padded 16-channel tile for the 1-channel head) against
+    the oracle with the same BF16
     rounding of every convolution's operands, at 128^2 (every level's tile
     fits).  Only summation order separates them, amplified where an
     activation lands on the other side of a BF16 rounding step in the next
