@@ -32,6 +32,7 @@ constexpr int kT = 256;        // threads per GEMM workgroup
 constexpr int kBM = 64, kBN = 64, kBK = 32;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 enum { G_FWD = 0, G_BWD_DATA = 1, G_BWD_W = 2 };
 
@@ -326,10 +327,12 @@ __global__ void __launch_bounds__(kT) gemm_fwd_narrow_kernel(GemmArgs g, int cin
 // from HBM once per chunk instead of once per tap.  Wave w takes image rows
 // 2w and 2w + 1 of the tile: two 16-pixel row blocks x BN / 16 column
 // blocks of v_mfma_f32_16x16x32_bf16 per tap.  Weights come pre-rounded to
-// BF16 in [Cout_pad][9][Cin] order (refresh_bf16_weights; rows from Cout to
-// Cout_pad are zero and their outputs are not stored); the epilogue adds the
-// bias and the skip input and applies the ReLU, as the general kernels do.
-// Needs Cin % 32 == 0, H % TH == 0, W % 16 == 0 and BN | Cout_pad.
+// BF16 in [Cout_pad][9][Cin_pad] order (refresh_bf16_weights; rows from Cout
+// to Cout_pad and channels from Cin to Cin_pad are zero, padding rows'
+// outputs are not stored); the epilogue adds the bias and the skip input and
+// applies the ReLU, as the general kernels do.  Needs Cin % 32 == 0 or
+// Cin < 32 (the halo's channels from Cin on are zero), H % TH == 0,
+// W % 16 == 0 and BN | Cout_pad.
 //
 // IN: the input as stored.  IN_DIRECT: H x W.  IN_UPSAMPLE: H/2 x W/2, read
 // through the nearest 2x upsampling (upsample2_vec_kernel's map).
@@ -386,34 +389,71 @@ __global__ void __launch_bounds__(32 * TH) conv_fwd_tile_kernel(ConvArgs c)
     const int sw = IN == IN_UPSAMPLE ? c.W / 2 : IN == IN_POOL ? 2 * c.W : c.W;
     const size_t img = (size_t)c.H * c.W * c.Cin;
     const float *xb = c.x + (IN == IN_UPSAMPLE ? img / 4 : IN == IN_POOL ? 4 * img : img) * b;
+    // Halo item i: pixel i / 8 of the (TH + 2) x 18 tile, channels 4 (i % 8)
+    // .. + 3 of the chunk; weight item i: output channel (i / 4) % BN, tap
+    // i / (4 BN), K values 8 (i % 4) .. + 7.  Direct and upsampled inputs of
+    // 32 or more output channels load all of a thread's items of a chunk
+    // before writing any (one memory latency per chunk instead of one per
+    // item); pooled inputs (four loads per item) and the 16-channel head
+    // load item by item, which keeps their registers, and so the resident
+    // workgroups, as they are.
+    constexpr int HN = HALO * 8, HI = (HN + NT - 1) / NT, WN = 9 * BN * 4, WI = (WN + NT - 1) / NT;
+    constexpr bool BATCH = IN != IN_POOL && BN >= 32;
+    constexpr int HB = BATCH ? HI : 1, WB = BATCH ? WI : 1;
+    auto halo_item = [&](int i, int c0) __attribute__((always_inline)) {
+        const int h = i >> 3, ch = c0 + 4 * (i & 7);
+        const int yy = y0 - 1 + h / kHaloW, xx = x0 - 1 + h % kHaloW;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < HN && yy >= 0 && yy < c.H && xx >= 0 && xx < c.W && ch < c.Cin) {
+            if (IN == IN_DIRECT) {
+                v = load_ch4(xb + ((size_t)yy * sw + xx) * c.Cin, ch, c.Cin);
+            } else if (IN == IN_UPSAMPLE) {
+                v = load_ch4(xb + ((size_t)(yy >> 1) * sw + (xx >> 1)) * c.Cin, ch, c.Cin);
+            } else {
+                const float *p = xb + ((size_t)(2 * yy) * sw + 2 * xx) * c.Cin;
+                const float4 a = load_ch4(p, ch, c.Cin);
+                const float4 e = load_ch4(p + c.Cin, ch, c.Cin);
+                const float4 q = load_ch4(p + (size_t)sw * c.Cin, ch, c.Cin);
+                const float4 w = load_ch4(p + (size_t)sw * c.Cin + c.Cin, ch, c.Cin);
+                v = make_float4(0.25f * ((a.x + e.x) + (q.x + w.x)), 0.25f * ((a.y + e.y) + (q.y + w.y)),
+                                0.25f * ((a.z + e.z) + (q.z + w.z)), 0.25f * ((a.w + e.w) + (q.w + w.w)));
+            }
+        }
+        return v;
+    };
     for (int c0 = 0; c0 < c.cinw; c0 += kBK) {
-        // halo: pixel h of the (TH + 2) x 18 tile, channels 4q .. 4q + 3 of the chunk
-        for (int i = tid; i < HALO * 8; i += NT) {
-            const int h = i >> 3, q = i & 7, ch = c0 + 4 * q;
-            const int yy = y0 - 1 + h / kHaloW, xx = x0 - 1 + h % kHaloW;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (yy >= 0 && yy < c.H && xx >= 0 && xx < c.W && ch < c.Cin) {
-                if (IN == IN_DIRECT) {
-                    v = load_ch4(xb + ((size_t)yy * sw + xx) * c.Cin, ch, c.Cin);
-                } else if (IN == IN_UPSAMPLE) {
-                    v = load_ch4(xb + ((size_t)(yy >> 1) * sw + (xx >> 1)) * c.Cin, ch, c.Cin);
-                } else {
-                    const float *p = xb + ((size_t)(2 * yy) * sw + 2 * xx) * c.Cin;
-                    const float4 a = load_ch4(p, ch, c.Cin);
-                    const float4 e = load_ch4(p + c.Cin, ch, c.Cin);
-                    const float4 u = load_ch4(p + (size_t)sw * c.Cin, ch, c.Cin);
-                    const float4 w = load_ch4(p + (size_t)sw * c.Cin + c.Cin, ch, c.Cin);
-                    v = make_float4(0.25f * ((a.x + e.x) + (u.x + w.x)), 0.25f * ((a.y + e.y) + (u.y + w.y)),
-                                    0.25f * ((a.z + e.z) + (u.z + w.z)), 0.25f * ((a.w + e.w) + (u.w + w.w)));
+#pragma unroll 1
+        for (int u0 = 0; u0 < HI; u0 += HB) {
+            float4 hv[HB];
+#pragma unroll
+            for (int u = 0; u < HB; ++u) hv[u] = halo_item(tid + (u0 + u) * NT, c0);
+#pragma unroll
+            for (int u = 0; u < HB; ++u) {
+                const int i = tid + (u0 + u) * NT;
+                if (i < HN)
+                    *reinterpret_cast<uint2 *>(&As[i >> 3][4 * (i & 7)]) =
+                        make_uint2(pack_bf16x2(hv[u].x, hv[u].y), pack_bf16x2(hv[u].z, hv[u].w));
+            }
+        }
+#pragma unroll 1
+        for (int u0 = 0; u0 < WI; u0 += WB) {
+            u32x4 wv[WB];
+#pragma unroll
+            for (int u = 0; u < WB; ++u) {
+                const int i = tid + (u0 + u) * NT;
+                if (i < WN) {
+                    const int p4 = i & 3, nk = i >> 2, n = nk % BN, k = nk / BN;
+                    wv[u] = *reinterpret_cast<const u32x4 *>(c.wt + ((size_t)(n0 + n) * 9 + k) * c.cinw + c0 + 8 * p4);
                 }
             }
-            *reinterpret_cast<uint2 *>(&As[h][4 * q]) = make_uint2(pack_bf16x2(v.x, v.y), pack_bf16x2(v.z, v.w));
-        }
-        // weights: output channel n, tap k, K values 8 p .. 8 p + 7 of the chunk
-        for (int i = tid; i < 9 * BN * 4; i += NT) {
-            const int p4 = i & 3, nk = i >> 2, n = nk % BN, k = nk / BN;
-            const uint4 w = *reinterpret_cast<const uint4 *>(c.wt + ((size_t)(n0 + n) * 9 + k) * c.cinw + c0 + 8 * p4);
-            *reinterpret_cast<uint4 *>(&Bs[k][n][8 * p4]) = w;
+#pragma unroll
+            for (int u = 0; u < WB; ++u) {
+                const int i = tid + (u0 + u) * NT;
+                if (i < WN) {
+                    const int p4 = i & 3, nk = i >> 2, n = nk % BN, k = nk / BN;
+                    *reinterpret_cast<u32x4 *>(&Bs[k][n][8 * p4]) = wv[u];
+                }
+            }
         }
         __syncthreads();
         const int kq = (lane >> 4) * 8;
