@@ -371,8 +371,14 @@ template <int BN, int TH, int IN>
 __global__ void __launch_bounds__(32 * TH) conv_fwd_tile_kernel(ConvArgs c)
 {
     constexpr int NJ = BN / 16, NT = 32 * TH, HALO = (TH + 2) * kHaloW;
-    __shared__ __attribute__((aligned(16))) __bf16 As[HALO][kLdA];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[9][BN][kLdA];
+    // the halo and weight tiles, and after the last chunk each wave's
+    // accumulators in pixel-major order (epilogue)
+    constexpr size_t kAsBytes = sizeof(__bf16) * HALO * kLdA, kBsBytes = sizeof(__bf16) * 9 * BN * kLdA;
+    constexpr int BNP = BN + 4;   // a pixel's row of accumulators, padded (4 pixels 16 banks apart)
+    static_assert((size_t)(NT / 64) * 32 * BNP * sizeof(float) <= kAsBytes + kBsBytes, "epilogue tile exceeds LDS");
+    __shared__ __attribute__((aligned(16))) unsigned char smem[kAsBytes + kBsBytes];
+    auto &As = *reinterpret_cast<__bf16(*)[HALO][kLdA]>(smem);
+    auto &Bs = *reinterpret_cast<__bf16(*)[9][BN][kLdA]>(smem + kAsBytes);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_x = c.W >> 4, tiles_y = c.H / TH;
     int t = blockIdx.x;
@@ -473,7 +479,40 @@ __global__ void __launch_bounds__(32 * TH) conv_fwd_tile_kernel(ConvArgs c)
         }
         __syncthreads();
     }
-    // accumulator (i, j): output channel n0 + 16 j + lane % 16, pixels 4 (lane / 16) .. + 3 of row 2 wave + i
+    // accumulator (i, j): output channel n0 + 16 j + lane % 16, pixels 4 (lane / 16) .. + 3 of row 2 wave + i.
+    // Where a skip input is added and the layer's BN channels are all real
+    // and 16-byte aligned, each wave turns its two rows' accumulators into
+    // pixel-major order through LDS, then reads the skip input and writes
+    // four channels per lane (16-byte accesses instead of 4-byte ones; for
+    // the stores alone the round trip through LDS measured slower).
+    if (BN >= 32 && c.skip && c.Cout % 4 == 0 && n0 + BN <= c.Cout &&
+        ((reinterpret_cast<uintptr_t>(c.out) | reinterpret_cast<uintptr_t>(c.skip)) & 15) == 0) {
+        float *Cs = reinterpret_cast<float *>(smem) + wave * 32 * BNP;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Cs[(16 * i + 4 * (lane >> 4) + r) * BNP + 16 * j + (lane & 15)] = acc[i][j][r];
+        __syncthreads();
+        constexpr int QG = BN / 4;
+#pragma unroll
+        for (int it = lane; it < 32 * QG; it += 64) {
+            const int px = it / QG, q = it % QG, n = n0 + 4 * q;
+            const float4 a = *reinterpret_cast<const float4 *>(&Cs[px * BNP + 4 * q]);
+            const size_t m = ((size_t)(b * c.H + y0 + 2 * wave + (px >> 4)) * c.W + x0 + (px & 15)) * c.Cout + n;
+            float v[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] += c.bias ? c.bias[n + e] : 0.f;
+            const float4 sk = *reinterpret_cast<const float4 *>(c.skip + m);
+            v[0] += sk.x; v[1] += sk.y; v[2] += sk.z; v[3] += sk.w;
+            if (c.relu)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = fmaxf(v[e], 0.f);
+            *reinterpret_cast<float4 *>(c.out + m) = make_float4(v[0], v[1], v[2], v[3]);
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         const int y = y0 + 2 * wave + i;
