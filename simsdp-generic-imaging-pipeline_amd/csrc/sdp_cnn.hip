@@ -397,15 +397,16 @@ __global__ void __launch_bounds__(32 * TH) conv_fwd_tile_kernel(ConvArgs c)
     const float *xb = c.x + (IN == IN_UPSAMPLE ? img / 4 : IN == IN_POOL ? 4 * img : img) * b;
     // Halo item i: pixel i / 8 of the (TH + 2) x 18 tile, channels 4 (i % 8)
     // .. + 3 of the chunk; weight item i: output channel (i / 4) % BN, tap
-    // i / (4 BN), K values 8 (i % 4) .. + 7.  Direct and upsampled inputs of
-    // 32 or more output channels load all of a thread's items of a chunk
-    // before writing any (one memory latency per chunk instead of one per
-    // item); pooled inputs (four loads per item) and the 16-channel head
-    // load item by item, which keeps their registers, and so the resident
-    // workgroups, as they are.
+    // i / (4 BN), K values 8 (i % 4) .. + 7.  For 32 or more output
+    // channels a thread issues all its weight loads of a chunk, and all its
+    // halo loads of direct and upsampled inputs, before writing any (one
+    // memory latency per chunk instead of one per item); pooled inputs (four
+    // loads per item) go two items at a time and the 16-channel head item by
+    // item, which keeps their registers, and so the resident workgroups, as
+    // they are.
     constexpr int HN = HALO * 8, HI = (HN + NT - 1) / NT, WN = 9 * BN * 4, WI = (WN + NT - 1) / NT;
     constexpr bool BATCH = IN != IN_POOL && BN >= 32;
-    constexpr int HB = BATCH ? HI : 1, WB = BATCH ? WI : 1;
+    constexpr int HB = BATCH ? HI : (IN == IN_POOL && BN >= 32) ? 2 : 1, WB = BN >= 32 ? WI : 1;
     auto halo_item = [&](int i, int c0) __attribute__((always_inline)) {
         const int h = i >> 3, ch = c0 + 4 * (i & 7);
         const int yy = y0 - 1 + h / kHaloW, xx = x0 - 1 + h % kHaloW;
