@@ -411,6 +411,9 @@ def main():
     ap.add_argument("--fft-only", action="store_true",
                     help="only run the major cycle's 4096^2 inverse FFT (shift / real / taper) 5 times "
                          "(counter runs on the FFT kernels)")
+    ap.add_argument("--dft-only", action="store_true",
+                    help="only run the array DFT predict of 4096 sources onto the 16.7M-visibility array "
+                         "(BF16 and FP16 hi + lo split), 5 times each (counter runs on the matrix-core kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -454,6 +457,21 @@ def main():
             hip.fft2d(d_grid, d_dirty, G, hip.FFT_INVERSE, flags, d_tap, d_work)
         hip.synchronize()
         log(f"{G}^2 inverse FFT x 5")
+        return
+
+    if args.dft_only:
+        enu = sdp_synth.log_radial_layout(256, 50.0, 5000.0, seed=11)
+        ant = sdp_synth.antenna_uvw(enu, -30.0, -30.0, np.linspace(-4.0, 4.0, 512), 150e6)
+        src = sdp_synth.config3_sources(4096)
+        n_time, n_ant, n_src = ant.shape[0], ant.shape[1], src.shape[0]
+        d_ant, d_src = hip.DeviceBuffer.from_array(ant), hip.DeviceBuffer.from_array(src)
+        d_out = hip.DeviceBuffer(8 * n_time * (n_ant * (n_ant - 1) // 2))
+        work = hip.DeviceBuffer(hip.dft_array_work_bytes(n_src))
+        for flags in (hip.DFT_ARRAY_BF16, hip.DFT_ARRAY_FP16):
+            for _ in range(5):
+                hip.dft_predict_array(n_time, n_ant, d_ant, n_src, d_src, d_out, flags, work)
+        hip.synchronize()
+        log(f"array DFT {n_src} sources x {n_ant} antennas x {n_time} instants, BF16 and FP16 split, x 5 each")
         return
 
     t = time.perf_counter()
