@@ -15,7 +15,10 @@
 // each tap.  Two arithmetic modes: FP32 FMA on the vector ALUs (exact enough
 // for gradient checks), and BF16 operands on the gfx950 matrix cores
 // (v_mfma_f32_16x16x32_bf16) with FP32 accumulation.  Weights are kept in
-// FP32 (master copy for Adam) and rounded to BF16 as tiles are staged.
+// FP32 (master copy for Adam) and rounded to BF16 as tiles are staged; the
+// BF16 forward convolutions run on a halo-tiled kernel instead
+// (conv_fwd_tile_kernel), from a BF16 copy of the weights formed once per
+// parameter change.
 //
 // Element-wise epilogues are fused: bias, the decoder's skip addition and
 // ReLU after a forward GEMM; the ReLU mask of the producing layer after an
