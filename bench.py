@@ -159,6 +159,9 @@ def major_cycle(hip, sdp_synth, d, d_uvw, d_meas, d_gk, d_dk, tile, chunk, targe
             "components_last_cycle": n_src}
 
 
+LEARNED_N_MINOR = 50   # Hogbom iterations per major cycle on the learned path
+
+
 def learned_vs_hogbom(hip, nets, seed=5):
     """Major-cycle time to a fixed residual RMS, learned proposal + Hogbom
     against plain Hogbom, on the learned-proposal workload: config-1 array
@@ -170,7 +173,9 @@ def learned_vs_hogbom(hip, nets, seed=5):
     steps and optionally head_scale and lr; BF16 MFMA, FP32 accumulation)
     is trained here by the library's kernels
     on pairs the GPU makes from the pipeline's own PSF (make_pool_device);
-    the training time is reported, not counted in the major-cycle time."""
+    the training time is reported, not counted in the major-cycle time.
+    The learned path runs LEARNED_N_MINOR Hogbom iterations per major cycle
+    after the proposals (also timed at 300, 100 and 25 iterations)."""
     import numpy as np
     import sdp_learned as L
 
@@ -225,11 +230,20 @@ def learned_vs_hogbom(hip, nets, seed=5):
             b.close()
         log(f"network {levels}x{base}: {n_pairs} pairs in {pool_s:.1f} s, {steps} steps in {train_s:.1f} s, "
             f"loss {losses[0]:.3g} -> {np.mean(losses[-50:]):.3g}")
-        lp = w.pipeline(300)
-        lp.set_learned(net, T, margin, gain=1.0, threshold=0.05, floor=0.0, max_cycles=3,
-                       scale=hip.LEARNED_GLOBAL_SCALE)
-        l_ms, l_rows, l_wall = run(lp, target, 14)
-        lp.close()
+        # Hogbom iterations per learned cycle: the proposals remove most of
+        # the flux, so the minor cycle after them is short; the other counts
+        # are timed too and reported beside it
+        sweep = {}
+        for n_minor in (LEARNED_N_MINOR, 300, 100, 25):
+            lp = w.pipeline(n_minor)
+            lp.set_learned(net, T, margin, gain=1.0, threshold=0.05, floor=0.0, max_cycles=3,
+                           scale=hip.LEARNED_GLOBAL_SCALE)
+            ms_n, rows_n, wall_n = run(lp, target, 14)
+            lp.close()
+            if n_minor == LEARNED_N_MINOR and n_minor not in sweep:
+                l_ms, l_rows, l_wall = ms_n, rows_n, wall_n
+            sweep[n_minor] = {"ms": None if ms_n is None else round(ms_n, 3), "n_cycles": len(rows_n),
+                              "ratio": None if ms_n is None else round(ms_n / hog_ms, 4)}
         net.close()
         log(f"learned ({levels}x{base}) + Hogbom: {'not reached' if l_ms is None else f'{l_ms:.1f} ms'} "
             f"in {len(l_rows)} cycles")
@@ -240,10 +254,11 @@ def learned_vs_hogbom(hip, nets, seed=5):
                       "s": round(train_s, 2),
                       "pool_s": round(pool_s, 2), "loss_first": float(f"{losses[0]:.4g}"),
                       "loss_last50": float(f"{np.mean(losses[-50:]):.4g}")},
-            "learned": {"n_minor": 300, "learned_cycles": 3, "gain": 1.0, "threshold": 0.05, "patch": T,
-                        "margin": margin, "scale": "global",
+            "learned": {"n_minor": LEARNED_N_MINOR, "learned_cycles": 3, "gain": 1.0, "threshold": 0.05,
+                        "patch": T, "margin": margin, "scale": "global",
                         "ms": None if l_ms is None else round(l_ms, 3), "wall_ms": round(l_wall, 3),
                         "cycles": fmt(l_rows)},
+            "n_minor_sweep": {str(k): v for k, v in sorted(sweep.items(), reverse=True)},
             "ratio": None if l_ms is None else round(l_ms / hog_ms, 4),
         })
     # the configured network (the last) gives the headline ratio
